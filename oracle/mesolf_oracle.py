@@ -1,0 +1,662 @@
+"""FP64 CPU restatement of MesoLF's phase-space Richardson-Lucy deconvolution.
+
+TEST INFRASTRUCTURE ONLY.  Imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py`` -- as the checker or the timed CPU
+baseline, never as the product.  The product path is the HIP library under
+``mesolf_amd/``.
+
+Two families of functions live here:
+
+* ``*_literal`` functions follow the reference ``.m`` code line by line
+  (dense 5-D PSF arrays, full-resolution ``conv2(..., 'same')`` through
+  ``scipy.signal.convolve2d``, FFT-based ``sample_filter``).  They are only
+  feasible on tiny shapes.
+* ``*_compact`` functions compute the same mathematics on the compact
+  phase-space tables (lenslet offsets x 15x15 phases), i.e. without the 225x
+  redundant full-resolution outputs.  They are the oracle at mid sizes.
+
+Pinning (each function is checked by something independent in ``tests/``):
+
+==========================  ==================================================
+function                    pinned by
+==========================  ==================================================
+gen_spiral                  brute-force clockwise ring walk; known entries
+gen_phasespace_literal      closed form (A.2); round trip with the inverse
+phase_space_closed_form     literal reshape/permute emulation
+view_weights                hand evaluation on a disc aperture (177 views)
+filter_bank / sample_        separable circular convolution with real kernels
+  filter_literal            (independent derivation), energy conservation
+conv2_same                  brute-force loops on tiny inputs
+forward/backward_literal    compact forms; adjointness <Fx,r> = <x,F^T r>
+forward/backward_compact    literal forms (tiny), brute-force loops
+rl_literal / rl_compact     each other at tiny sizes
+mex_op                      brute-force loop over the kernel's own clipping
+                            rules; adjoint of forward_scatter (odd kw)
+==========================  ==================================================
+
+There is no reference test data to pin against: the reference ships no
+golden outputs (its only fixture, ``src/cuda/vols_slice.mat``, is an input).
+The MATLAB/cuFFT rounding of the reference is therefore not reproduced
+bit-for-bit; parity is by tolerance (see ``tests/``).
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+from scipy.signal import convolve2d
+
+# ---------------------------------------------------------------------------
+# small MATLAB semantics helpers
+# ---------------------------------------------------------------------------
+
+
+def matlab_round(x: float) -> int:
+    """MATLAB ``round``: half away from zero (numpy rounds half to even)."""
+    return int(math.floor(abs(x) + 0.5)) * (1 if x >= 0 else -1)
+
+
+# ---------------------------------------------------------------------------
+# spiral view order -- recon_theory_phase_space_LFM_with_filter.m:156-188
+# ---------------------------------------------------------------------------
+
+
+def gen_spiral(N: int):
+    """Return the 1-based (i_index, j_index) view tables of recon:156-188.
+
+    ``mode(Nnum) == 0`` at recon:158 is the statistical mode of a scalar,
+    i.e. ``Nnum`` itself, so the test is false for every N > 0 and
+    ``loop_n = (Nnum + 1) / 2`` always; ``for k = 1:loop_n`` runs to
+    ``floor(loop_n)``.
+    """
+    loop_n = (N + 1) / 2.0
+    i_index = np.zeros(N * N, dtype=np.int64)
+    j_index = np.zeros(N * N, dtype=np.int64)
+    start = 1
+    for k in range(1, int(math.floor(loop_n)) + 1):
+        cs = N - 2 * (k - 1)
+        bias = k - 1
+        if cs > 1:
+            i_new = np.concatenate([np.arange(1, cs + 1), cs * np.ones(cs - 1, np.int64),
+                                    np.arange(cs - 1, 0, -1), np.ones(cs - 2, np.int64)])
+            j_new = np.concatenate([np.ones(cs, np.int64), np.arange(2, cs + 1),
+                                    cs * np.ones(cs - 1, np.int64), np.arange(cs - 1, 1, -1)])
+            n = 4 * (cs - 1)
+            i_index[start - 1:start - 1 + n] = i_new + bias
+            j_index[start - 1:start - 1 + n] = j_new + bias
+            start += n
+        else:
+            i_index[start - 1] = 1 + bias
+            j_index[start - 1] = 1 + bias
+    return i_index, j_index
+
+
+# ---------------------------------------------------------------------------
+# PSF preparation -- recon:13-17 and gen_phasespace_PSF_theory.m
+# ---------------------------------------------------------------------------
+
+
+def normalize_depth(H: np.ndarray) -> np.ndarray:
+    """recon:13-16: divide each depth slab H(:,:,:,:,z) by its total sum."""
+    H = np.array(H, dtype=np.float64, copy=True)
+    for z in range(H.shape[4]):
+        H[:, :, :, :, z] /= H[:, :, :, :, z].sum()
+    return H
+
+
+def phase_space_crop(S: int, N: int):
+    """gen_phasespace_PSF_theory.m:6-8 -> (IMG, first 0-based index)."""
+    IMG = S - ((S - N) % (2 * N))
+    index1 = matlab_round(S / 2.0) - int(math.trunc(IMG / 2.0))  # 1-based
+    return IMG, index1 - 1
+
+
+def im_shift3(img: np.ndarray, sx: int, sy: int) -> np.ndarray:
+    """gen_phasespace_PSF_theory.m:34-57: integer shift with zero fill."""
+    out = np.zeros_like(img)
+    X, Y = img.shape[:2]
+    xs_dst = slice(max(sx, 0), X + min(sx, 0))
+    xs_src = slice(max(-sx, 0), X - max(sx, 0))
+    ys_dst = slice(max(sy, 0), Y + min(sy, 0))
+    ys_src = slice(max(-sy, 0), Y - max(sy, 0))
+    out[xs_dst, ys_dst] = img[xs_src, ys_src]
+    return out
+
+
+def gen_phasespace_literal(H: np.ndarray, Ht: np.ndarray):
+    """Line-by-line emulation of gen_phasespace_PSF_theory.m:1-31.
+
+    MATLAB ``reshape`` is column-major (numpy ``order='F'``) and ``permute``
+    takes 1-based axes.  Returns (H_ps [IMG,IMG,N,N,Z], Ht_crop).
+    """
+    S = H.shape[0]
+    N = H.shape[2]
+    Z = H.shape[4]
+    IMG, i0 = phase_space_crop(S, N)
+    c0 = (N + 1) / 2.0
+    Hd = np.zeros((IMG, IMG, N, N, Z), dtype=np.float64)
+    for z in range(Z):
+        for ii in range(N):
+            for jj in range(N):
+                Hd[:, :, ii, jj, z] = im_shift3(H[i0:i0 + IMG, i0:i0 + IMG, ii, jj, z],
+                                                int(round((ii + 1) - c0)), int(round((jj + 1) - c0)))
+    m = IMG // N
+    Hd = np.reshape(Hd, (N, m, N, m, N, N, Z), order="F")
+    Hd = np.transpose(Hd, (0, 2, 1, 3, 4, 5, 6))
+    Hd = Hd[:, :, :, :, ::-1, ::-1, :]
+    Hd = np.transpose(Hd, (0, 1, 4, 2, 5, 3, 6))
+    Hd = np.reshape(Hd, (N, N, IMG, IMG, Z), order="F")
+    Hd = np.transpose(Hd, (2, 3, 0, 1, 4))
+    Ht_c = np.array(Ht[i0:i0 + IMG, i0:i0 + IMG], dtype=np.float64)
+    return np.ascontiguousarray(Hd), Ht_c
+
+
+def phase_space_closed_form(H: np.ndarray):
+    """Closed form of the permutation in gen_phasespace_PSF_theory.m:22-27.
+
+    ``H_ps[X,Y,u,v,z] = Hs[u + N*(X//N), v + N*(Y//N), N-1-X%N, N-1-Y%N, z]``
+    where ``Hs`` is the cropped, de-shifted PSF (:6-20), 0-based.
+    """
+    S, N, Z = H.shape[0], H.shape[2], H.shape[4]
+    IMG, i0 = phase_space_crop(S, N)
+    c0 = (N - 1) // 2
+    Hs = np.zeros((IMG, IMG, N, N, Z))
+    for a in range(N):
+        for b in range(N):
+            Hs[:, :, a, b, :] = im_shift3(H[i0:i0 + IMG, i0:i0 + IMG, a, b, :], a - c0, b - c0)
+    X = np.arange(IMG)
+    out = np.empty((IMG, IMG, N, N, Z))
+    for u in range(N):
+        for v in range(N):
+            rows = u + N * (X // N)
+            cols = v + N * (X // N)
+            pa = N - 1 - X % N
+            out[:, :, u, v, :] = Hs[rows[:, None], cols[None, :], pa[:, None], pa[None, :], :]
+    return out
+
+
+def inverse_phasespace(H_ps: np.ndarray) -> np.ndarray:
+    """Raw PSF H [S,S,N,N,Z] (S = IMG) whose gen_phasespace is ``H_ps``.
+
+    Exact when every de-shifted entry stays inside the array (true for kernels
+    supported within 7 px of the IMG border).
+    """
+    IMG, N, Z = H_ps.shape[0], H_ps.shape[2], H_ps.shape[4]
+    c0 = (N - 1) // 2
+    X = np.arange(IMG)
+    Hs = np.zeros_like(H_ps)
+    for u in range(N):
+        for v in range(N):
+            rows = u + N * (X // N)
+            cols = v + N * (X // N)
+            pa = N - 1 - X % N
+            Hs[rows[:, None], cols[None, :], pa[:, None], pa[None, :], :] = H_ps[:, :, u, v, :]
+    H = np.zeros_like(H_ps)
+    for a in range(N):
+        for b in range(N):
+            H[:, :, a, b, :] = im_shift3(Hs[:, :, a, b, :], -(a - c0), -(b - c0))
+    return H
+
+
+# ---------------------------------------------------------------------------
+# view weights -- recon:19-40
+# ---------------------------------------------------------------------------
+
+
+def view_weights(H_ps: np.ndarray, weight_cut: float = 0.2) -> np.ndarray:
+    """recon:30-38.  ``weight(u,v) == 0`` marks an inactive view (recon:104)."""
+    w = H_ps.sum(axis=(0, 1, 4))
+    w[np.isnan(w)] = 0
+    w = w / w.sum()
+    w = w - w.min()
+    w = w / w.max()
+    w[w < weight_cut] = 0
+    return w * 0.3
+
+
+def view_weights_from_sums(wsum: np.ndarray, weight_cut: float = 0.2) -> np.ndarray:
+    """Same arithmetic as :func:`view_weights` starting from per-view sums."""
+    w = np.array(wsum, dtype=np.float64, copy=True)
+    w[np.isnan(w)] = 0
+    w = w / w.sum()
+    w = w - w.min()
+    w = w / w.max()
+    w[w < weight_cut] = 0
+    return w * 0.3
+
+
+def active_views(weight: np.ndarray):
+    """Views visited by recon:98-105, in spiral order: list of (u, v) 0-based."""
+    N = weight.shape[0]
+    i1, i2 = gen_spiral(N)
+    return [(int(i1[k] - 1), int(i2[k] - 1)) for k in range(N * N)
+            if not (weight[i1[k] - 1, i2[k] - 1] == 0)]
+
+
+# ---------------------------------------------------------------------------
+# object-space low-pass filter bank -- recon:20-26, 44-61, 222-240, 251-254
+# ---------------------------------------------------------------------------
+
+
+def nip_range(Z: int) -> int:
+    r = int(math.ceil(Z / 3.0))
+    return r + 1 if r % 2 == 1 else r
+
+
+def omega_array(Z: int, N: int):
+    R = nip_range(Z)
+    axis = np.arange(1, 3 * R + 1) - 3 * R / 2.0
+    om = (N - 7) * np.exp(-(axis / 100.0 * 2 * np.sqrt(np.log(2))) ** 2)
+    om[om < 0.2] = 0.2
+    return om
+
+
+def filter_bank(I1: int, I2: int, Z: int, N: int) -> np.ndarray:
+    """recon:45-61: obj_space_ft_filter_bank, [I1, I2, 3R] FP64."""
+    x = np.arange(1, I1 + 1) - I1 / 2.0
+    y = np.arange(1, I2 + 1) - I2 / 2.0
+    xx = x[:, None] * np.ones((1, I2))
+    yy = np.ones((I1, 1)) * y[None, :]
+    om = omega_array(Z, N)
+    bank = np.zeros((I1, I2, om.size))
+    for i, w in enumerate(om):
+        g = np.exp(-((xx / w) ** 2 + (yy / w) ** 2))
+        buf = np.abs(np.fft.fftshift(np.fft.fft2(g)))
+        bank[:, :, i] = buf / buf.max()
+    return bank
+
+
+def filter_plane_index(Z: int, nf: int):
+    """sample_filter (recon:224-231): 0-based filter index per plane or None."""
+    half_z = matlab_round(Z / 2.0)
+    out = []
+    for z in range(1, Z + 1):
+        if half_z - nf / 2.0 + 1 <= z <= half_z + nf / 2.0:
+            out.append(int(z - half_z + nf / 2.0) - 1)
+        else:
+            out.append(None)
+    return out
+
+
+def sample_filter_literal(X: np.ndarray, bank: np.ndarray) -> np.ndarray:
+    """recon:222-240 with freq_filter recon:251-254, FP64 FFTs."""
+    out = np.zeros_like(X)
+    idx = filter_plane_index(X.shape[2], bank.shape[2])
+    for z in range(X.shape[2]):
+        img = X[:, :, z]
+        k = idx[z]
+        if k is None:
+            out[:, :, z] = img
+            continue
+        s_before = img.sum()
+        f = np.fft.fftshift(np.fft.fft2(img))
+        buf = np.real(np.fft.ifft2(np.fft.ifftshift(f * bank[:, :, k])))
+        out[:, :, z] = s_before * buf / buf.sum()
+    return out
+
+
+def filter_kernel_1d(I: int, omega: float) -> np.ndarray:
+    """Real-space circular kernel equivalent to one axis of the filter bank.
+
+    The bank is separable: F = (|G_x|/max) (x) (|G_y|/max) with G = fft(g),
+    g(x) = exp(-((x - I/2)/omega)^2), x = 1..I; freq_filter therefore equals a
+    circular convolution with kappa_x (x) kappa_y, kappa = real(ifft(|G|/max)).
+    Returned with kappa[0] at index 0 (circular indexing).
+    """
+    x = np.arange(1, I + 1) - I / 2.0
+    G = np.abs(np.fft.fft(np.exp(-(x / omega) ** 2)))
+    return np.real(np.fft.ifft(G / G.max()))
+
+
+def sample_filter_separable(X: np.ndarray, Z: int, N: int) -> np.ndarray:
+    """sample_filter computed as separable circular convolutions (FP64)."""
+    I1, I2 = X.shape[0], X.shape[1]
+    om = omega_array(Z, N)
+    idx = filter_plane_index(Z, om.size)
+    out = np.empty_like(X)
+    for z in range(Z):
+        k = idx[z]
+        if k is None:
+            out[:, :, z] = X[:, :, z]
+            continue
+        k1 = filter_kernel_1d(I1, om[k])
+        k2 = filter_kernel_1d(I2, om[k])
+        # circular convolution via explicit circulant products (exact algebra)
+        C1 = np.stack([np.roll(k1, s) for s in range(I1)], axis=1)  # C1[p,q] = k1[p-q]
+        C2 = np.stack([np.roll(k2, s) for s in range(I2)], axis=1)
+        buf = C1 @ X[:, :, z] @ C2.T
+        out[:, :, z] = X[:, :, z].sum() * buf / buf.sum()
+    return out
+
+
+# ---------------------------------------------------------------------------
+# literal operators -- recon:191-208 (CPU branch)
+# ---------------------------------------------------------------------------
+
+
+def conv2_same(x: np.ndarray, h: np.ndarray) -> np.ndarray:
+    """MATLAB ``conv2(x, h, 'same')`` for odd-sized h (zero padding)."""
+    return convolve2d(x, h, mode="same")
+
+
+def forward_literal(X: np.ndarray, H_ps: np.ndarray, u: int, v: int, N: int) -> np.ndarray:
+    """forwardProjectACC_phasespace_theory, recon:191-198 (u, v 0-based)."""
+    proj = np.zeros(X.shape[:2])
+    for z in range(X.shape[2]):
+        proj += conv2_same(X[:, :, z], H_ps[:, :, u, v, z])
+    c0 = (N - 1) // 2
+    return proj[c0::N, c0::N]
+
+
+def backward_literal(Ht: np.ndarray, R: np.ndarray, u: int, v: int, N: int) -> np.ndarray:
+    """backwardProject_phasespace_theory, recon:201-208 (u, v 0-based)."""
+    big = np.zeros((N * R.shape[0], N * R.shape[1]))
+    big[u::N, v::N] = R
+    out = np.zeros(big.shape + (Ht.shape[4],))
+    for z in range(Ht.shape[4]):
+        out[:, :, z] = conv2_same(big, Ht[:, :, u, v, z])
+    return out
+
+
+def ratio_update(LFsub, y):
+    """recon:116-117."""
+    with np.errstate(divide="ignore", invalid="ignore"):
+        r = LFsub / y
+    r[~np.isfinite(r)] = 0
+    return r
+
+
+def multiplicative_update(X, B, B1, w):
+    """recon:130-137 (left-to-right evaluation of X .* B ./ B1)."""
+    with np.errstate(divide="ignore", invalid="ignore", over="ignore"):
+        C = X * B / B1
+    C[np.isnan(C)] = 0
+    C[np.isinf(C)] = 0
+    C[C < 0] = 0
+    Xn = C * w + (1 - w) * X
+    Xn[Xn < 0] = 0
+    return Xn
+
+
+def initial_volume(LF: np.ndarray, Z: int) -> np.ndarray:
+    """recon:72-73."""
+    X = np.ones((LF.shape[0], LF.shape[1], Z))
+    return X / X.sum() * LF.sum()
+
+
+def rl_literal(LF, H, Ht, max_iter, views_limit=None):
+    """recon:10-151, CPU branch (``Xguessv`` typo at :112 read as ``Xguess``,
+    host zeros in ``sample_filter``).  Dense arrays; tiny sizes only."""
+    Hn = normalize_depth(H)
+    Htn = normalize_depth(Ht)
+    H_ps, Ht_c = gen_phasespace_literal(Hn, Htn)
+    N = H_ps.shape[2]
+    Z = H_ps.shape[4]
+    weight = view_weights(H_ps)
+    bank = filter_bank(LF.shape[0], LF.shape[1], Z, N)
+    X = initial_volume(LF, Z)
+    views = active_views(weight)
+    if views_limit is not None:
+        views = views[:views_limit]
+    ones = np.ones((LF.shape[0] // N, LF.shape[1] // N))
+    for _ in range(max_iter):
+        for (u, v) in views:
+            y = forward_literal(X, H_ps, u, v, N)
+            R = ratio_update(LF[u::N, v::N], y)
+            B = backward_literal(Ht_c, R, u, v, N)
+            B1 = backward_literal(Ht_c, ones, u, v, N)
+            X = multiplicative_update(X, B, B1, weight[u, v])
+        X = sample_filter_literal(X, bank)
+    return X
+
+
+# ---------------------------------------------------------------------------
+# compact phase-space tables and operators
+# ---------------------------------------------------------------------------
+
+
+@dataclass
+class CompactPSF:
+    """Per active view (spiral order) and plane: compact lenslet-offset kernels.
+
+    ``fwd[k][z]`` has shape (2rf+1, 2rf+1, N, N) indexed [d1+rf, d2+rf, s1, s2]:
+    ``y[i] = sum_z sum_{L,s} X[N*L+s, z] * fwd[k][z][i-L+rf, s]``.
+    ``bwd[k][z]`` has shape (2rb+1, 2rb+1, N, N):
+    ``B[N*a+s, z] = sum_i R[i] * bwd[k][z][a-i+rb, s]``.
+    """
+
+    N: int
+    Z: int
+    views: list
+    weights: list
+    rf: list
+    rb: list
+    fwd: list = field(default_factory=list)
+    bwd: list = field(default_factory=list)
+
+
+def _trim_radius(tab: np.ndarray, center: int) -> int:
+    """Smallest symmetric lenslet radius containing all nonzero entries.
+
+    ``tab`` has shape (V, K, K, N, N) (views stacked on axis 0)."""
+    nz = np.nonzero(np.any(tab != 0, axis=(0, 3, 4)))
+    if nz[0].size == 0:
+        return 0
+    return int(max(np.abs(nz[0] - center).max(), np.abs(nz[1] - center).max()))
+
+
+def compact_from_dense(H: np.ndarray, Ht: np.ndarray) -> CompactPSF:
+    """Build compact tables from the raw PSFs exactly as recon:13-40 prepares
+    them (depth normalisation, gen_phasespace, weights, spiral order)."""
+    Hn = normalize_depth(H)
+    Htn = normalize_depth(Ht)
+    H_ps, Ht_c = gen_phasespace_literal(Hn, Htn)
+    return compact_from_phase_space(H_ps, Ht_c)
+
+
+def compact_from_phase_space(H_ps: np.ndarray, Ht_c: np.ndarray) -> CompactPSF:
+    IMG, N, Z = H_ps.shape[0], H_ps.shape[2], H_ps.shape[4]
+    R = (IMG // N - 1) // 2
+    c0 = (N - 1) // 2
+    weight = view_weights(H_ps)
+    views = active_views(weight)
+    V = len(views)
+    s = np.arange(N)
+    # forward: fwd[d+R, s] = H_ps[N*(d+R) + N-1-s]
+    kf = np.arange(2 * R + 1)
+    xf = N * kf[:, None] + (N - 1) - s[None, :]          # (2R+1, N)
+    # backward: bwd[d+R+1, s] = Ht[N*(d+R) + c0 + s - u]  (zero outside)
+    kb = np.arange(2 * R + 3)
+    cp = CompactPSF(N=N, Z=Z, views=views, weights=[float(weight[u, v]) for u, v in views],
+                    rf=[], rb=[])
+    F = np.zeros((V, Z, 2 * R + 1, 2 * R + 1, N, N))
+    B = np.zeros((V, Z, 2 * R + 3, 2 * R + 3, N, N))
+    for k, (u, v) in enumerate(views):
+        hp = H_ps[:, :, u, v, :]
+        F[k] = np.transpose(hp[xf[:, None, :, None], xf[None, :, None, :], :], (4, 0, 1, 2, 3))
+        xb1 = N * (kb[:, None] - 1) + c0 + s[None, :] - u
+        xb2 = N * (kb[:, None] - 1) + c0 + s[None, :] - v
+        ok1 = (xb1 >= 0) & (xb1 < IMG)
+        ok2 = (xb2 >= 0) & (xb2 < IMG)
+        ht = Ht_c[:, :, u, v, :]
+        vals = ht[np.clip(xb1, 0, IMG - 1)[:, None, :, None], np.clip(xb2, 0, IMG - 1)[None, :, None, :], :]
+        mask = ok1[:, None, :, None] & ok2[None, :, None, :]
+        B[k] = np.transpose(vals * mask[..., None], (4, 0, 1, 2, 3))
+    for z in range(Z):
+        rf = _trim_radius(F[:, z], R)
+        rb = _trim_radius(B[:, z], R + 1)
+        cp.rf.append(rf)
+        cp.rb.append(rb)
+    cp.fwd = [[np.ascontiguousarray(F[k, z, R - cp.rf[z]:R + cp.rf[z] + 1, R - cp.rf[z]:R + cp.rf[z] + 1])
+               for z in range(Z)] for k in range(V)]
+    cp.bwd = [[np.ascontiguousarray(B[k, z, R + 1 - cp.rb[z]:R + 2 + cp.rb[z], R + 1 - cp.rb[z]:R + 2 + cp.rb[z]])
+               for z in range(Z)] for k in range(V)]
+    return cp
+
+
+def to_lenslet(X: np.ndarray, N: int) -> np.ndarray:
+    """[I1, I2, Z] -> [n1, n2, Z, N*N] with phase index s1 + N*s2."""
+    I1, I2, Z = X.shape
+    n1, n2 = I1 // N, I2 // N
+    Xr = X.reshape(n1, N, n2, N, Z)  # [L1, s1, L2, s2, z]
+    return np.ascontiguousarray(np.transpose(Xr, (0, 2, 4, 3, 1)).reshape(n1, n2, Z, N * N))
+
+
+def from_lenslet(Xl: np.ndarray, N: int) -> np.ndarray:
+    n1, n2, Z, _ = Xl.shape
+    Xr = Xl.reshape(n1, n2, Z, N, N)  # [L1, L2, z, s2, s1]
+    return np.ascontiguousarray(np.transpose(Xr, (0, 4, 1, 3, 2)).reshape(n1 * N, n2 * N, Z))
+
+
+def _flat(tab: np.ndarray) -> np.ndarray:
+    """(K, K, N, N) [d1, d2, s1, s2] -> (K*K, N*N) with phase index s1 + N*s2."""
+    K, N = tab.shape[0], tab.shape[2]
+    return np.transpose(tab, (0, 1, 3, 2)).reshape(K * K, N * N)
+
+
+def forward_compact(Xl: np.ndarray, fwd_v: list, rf: list) -> np.ndarray:
+    """Strided forward for one view: y[i] = sum_{z,d,s} X[i-d, z, s] W[z][d, s]."""
+    n1, n2, Z, P = Xl.shape
+    y = np.zeros((n1, n2))
+    for z in range(Z):
+        r = rf[z]
+        K = 2 * r + 1
+        Q = Xl[:, :, z, :].reshape(n1 * n2, P) @ _flat(fwd_v[z]).T  # [(L1,L2), (d1,d2)]
+        Q = Q.reshape(n1, n2, K, K)
+        for d1 in range(-r, r + 1):
+            i1a, i1b = max(0, d1), min(n1, n1 + d1)
+            if i1a >= i1b:
+                continue
+            for d2 in range(-r, r + 1):
+                i2a, i2b = max(0, d2), min(n2, n2 + d2)
+                if i2a >= i2b:
+                    continue
+                y[i1a:i1b, i2a:i2b] += Q[i1a - d1:i1b - d1, i2a - d2:i2b - d2, d1 + r, d2 + r]
+    return y
+
+
+def _im2col(Rimg: np.ndarray, r: int) -> np.ndarray:
+    """cols[(a1,a2), (d1,d2)] = R[a1-d1, a2-d2] (zero outside)."""
+    n1, n2 = Rimg.shape
+    K = 2 * r + 1
+    pad = np.zeros((n1 + 2 * r, n2 + 2 * r))
+    pad[r:r + n1, r:r + n2] = Rimg
+    cols = np.empty((n1, n2, K, K))
+    for d1 in range(-r, r + 1):
+        for d2 in range(-r, r + 1):
+            cols[:, :, d1 + r, d2 + r] = pad[r - d1:r - d1 + n1, r - d2:r - d2 + n2]
+    return cols.reshape(n1 * n2, K * K)
+
+
+def backward_compact(Rimg: np.ndarray, bwd_v: list, rb: list) -> np.ndarray:
+    """B[a, z, s] = sum_i R[i] Wt[z][a-i, s]  -> [n1, n2, Z, N*N]."""
+    n1, n2 = Rimg.shape
+    Z = len(bwd_v)
+    P = bwd_v[0].shape[2] ** 2
+    out = np.empty((n1, n2, Z, P))
+    cache = {}
+    for z in range(Z):
+        r = rb[z]
+        if r not in cache:
+            cache[r] = _im2col(Rimg, r)
+        out[:, :, z, :] = (cache[r] @ _flat(bwd_v[z])).reshape(n1, n2, P)
+    return out
+
+
+def rl_compact(LF: np.ndarray, cp: CompactPSF, max_iter: int, X0=None, views_limit=None,
+               filter_planes=True, callback=None):
+    """Full phase-space RL (recon:72-146) on compact tables, FP64."""
+    N, Z = cp.N, cp.Z
+    I1, I2 = LF.shape
+    n1, n2 = I1 // N, I2 // N
+    X = initial_volume(LF, Z) if X0 is None else np.array(X0, dtype=np.float64)
+    Xl = to_lenslet(X, N)
+    ones = np.ones((n1, n2))
+    nv = len(cp.views) if views_limit is None else views_limit
+    om = omega_array(Z, N)
+    fidx = filter_plane_index(Z, om.size)
+    kern = {}
+    for it in range(max_iter):
+        for k in range(nv):
+            u, v = cp.views[k]
+            y = forward_compact(Xl, cp.fwd[k], cp.rf)
+            Rimg = ratio_update(LF[u::N, v::N], y)
+            B = backward_compact(Rimg, cp.bwd[k], cp.rb)
+            B1 = backward_compact(ones, cp.bwd[k], cp.rb)
+            Xl = multiplicative_update(Xl, B, B1, cp.weights[k])
+        if filter_planes:
+            X = from_lenslet(Xl, N)
+            for z in range(Z):
+                f = fidx[z]
+                if f is None:
+                    continue
+                if f not in kern:
+                    kern[f] = (np.fft.fft(filter_kernel_1d(I1, om[f])), np.fft.fft(filter_kernel_1d(I2, om[f])))
+                k1, k2 = kern[f]
+                img = X[:, :, z]
+                buf = np.real(np.fft.ifft2(np.fft.fft2(img) * (k1[:, None] * k2[None, :])))
+                X[:, :, z] = img.sum() * buf / buf.sum()
+            Xl = to_lenslet(X, N)
+        if callback is not None:
+            callback(it, from_lenslet(Xl, N))
+    return from_lenslet(Xl, N)
+
+
+# ---------------------------------------------------------------------------
+# MEX operator -- src/cuda/mex_lfm_convolution_vec.cu:43-266
+# ---------------------------------------------------------------------------
+
+
+def mex_op(bitmap: np.ndarray, K: np.ndarray, nphase: int = 15) -> np.ndarray:
+    """Race-free definition of lfm_convolution_kernel_vec (mex:43-266).
+
+    ``bitmap`` [W, H] or [W, H, n] (x = first, contiguous axis),
+    ``K`` [kw, kw, 15, 15].  For output (x, y):
+    ``out = sum_{ky,kx} in[x-h+kx, y-h+ky] * K[kx, ky, x%15, y%15]``, h = kw//2,
+    taps clipped by mex:55-78 (for even kw the tap landing on the last
+    column/row is clipped too, as the kernel does).
+    """
+    squeeze = bitmap.ndim == 2
+    b = bitmap[:, :, None] if squeeze else bitmap
+    W, Hh, n = b.shape
+    kw = K.shape[0]
+    h = kw // 2
+    out = np.zeros((W, Hh, n))
+    xs = np.arange(W)
+    ys = np.arange(Hh)
+    kxs = np.maximum(0, h - xs)
+    kxe = kw - np.maximum(0, xs + h + 1 - W)
+    kys = np.maximum(0, h - ys)
+    kye = kw - np.maximum(0, ys + h + 1 - Hh)
+    px = xs % nphase
+    py = ys % nphase
+    bd = b.astype(np.float64)
+    for ky in range(kw):
+        yok = (ky >= kys) & (ky < kye)
+        yin = ys - h + ky
+        for kx in range(kw):
+            xok = (kx >= kxs) & (kx < kxe)
+            xin = xs - h + kx
+            m = xok[:, None] & yok[None, :]
+            if not m.any():
+                continue
+            kv = K[kx, ky][px[:, None], py[None, :]].astype(np.float64)  # [W, H]
+            src = np.zeros((W, Hh, n))
+            xi = np.clip(xin, 0, W - 1)
+            yi = np.clip(yin, 0, Hh - 1)
+            src = bd[xi[:, None], yi[None, :], :]
+            out += (m * kv)[:, :, None] * src
+    return out[:, :, 0] if squeeze else out
+
+
+def forward_scatter(X: np.ndarray, K: np.ndarray, nphase: int = 15) -> np.ndarray:
+    """forwardProjectGPU.m:7-18 for one plane: sum_{a,b} conv2(X masked to
+    phase (a,b), K[:,:,a,b], 'same')."""
+    out = np.zeros(X.shape)
+    for a in range(nphase):
+        for b in range(nphase):
+            t = np.zeros(X.shape)
+            t[a::nphase, b::nphase] = X[a::nphase, b::nphase]
+            out += conv2_same(t, K[:, :, a, b])
+    return out

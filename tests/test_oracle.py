@@ -1,0 +1,295 @@
+"""Pins for the FP64 oracle (CPU only).
+
+Each oracle function is checked against something independent of it: a
+brute-force loop, a library routine (scipy), a closed form derived separately,
+or an algebraic identity (adjointness, energy conservation, round trips).
+"""
+import numpy as np
+import pytest
+
+from mesolf_amd import synth as S
+from oracle import mesolf_oracle as O
+
+
+# --------------------------------------------------------------------- spiral
+def _ring_walk(N):
+    """Independent spiral: walk each ring clockwise starting at its top-left
+    corner, first down the first column, then along the last row, up the last
+    column and back along the first row (recon:171-175 in words)."""
+    out = []
+    lo, hi = 0, N - 1
+    while lo <= hi:
+        if lo == hi:
+            out.append((lo, lo))
+            break
+        for i in range(lo, hi + 1):
+            out.append((i, lo))
+        for j in range(lo + 1, hi + 1):
+            out.append((hi, j))
+        for i in range(hi - 1, lo - 1, -1):
+            out.append((i, hi))
+        for j in range(hi - 1, lo, -1):
+            out.append((lo, j))
+        lo, hi = lo + 1, hi - 1
+    return out
+
+
+@pytest.mark.parametrize("N", [1, 3, 4, 5, 7, 15])
+def test_spiral_matches_ring_walk(N):
+    i1, i2 = O.gen_spiral(N)
+    got = [(int(a) - 1, int(b) - 1) for a, b in zip(i1, i2)]
+    assert got == _ring_walk(N)
+    assert sorted(got) == [(a, b) for a in range(N) for b in range(N)]
+
+
+def test_spiral_known_entries_n15():
+    i1, i2 = O.gen_spiral(15)
+    pairs = list(zip(i1.tolist(), i2.tolist()))
+    assert pairs[:3] == [(1, 1), (2, 1), (3, 1)]
+    assert pairs[14:17] == [(15, 1), (15, 2), (15, 3)]
+    assert pairs[-5:] == [(9, 9), (8, 9), (7, 9), (7, 8), (8, 8)]
+    w = np.zeros((15, 15))
+    for (u, v) in S.aperture_views():
+        w[u, v] = 1
+    act = [(a, b) for a, b in pairs if w[a - 1, b - 1] != 0]
+    assert act[:6] == [(6, 1), (7, 1), (8, 1), (9, 1), (10, 1), (15, 6)]
+
+
+# -------------------------------------------------------------- phase space
+def _random_psf(rng, S_, N, Z):
+    return rng.random((S_, S_, N, N, Z))
+
+
+@pytest.mark.parametrize("S_,N", [(45, 3), (47, 3), (75, 5), (75, 15)])
+def test_phasespace_literal_equals_closed_form(S_, N):
+    rng = np.random.default_rng(S_ + N)
+    H = _random_psf(rng, S_, N, 2)
+    a, _ = O.gen_phasespace_literal(H, H)
+    b = O.phase_space_closed_form(H)
+    assert a.shape == b.shape
+    assert np.array_equal(a, b)
+
+
+def test_phasespace_crop_rule():
+    assert O.phase_space_crop(75, 15) == (75, 0)
+    # 80 - mod(80-15, 30) = 75, round(40) - fix(37.5) = 3 (1-based)
+    assert O.phase_space_crop(80, 15) == (75, 2)
+    assert O.phase_space_crop(525, 15) == (525, 0)
+
+
+def test_inverse_phasespace_roundtrip():
+    Hps = S.dense_phase_space(S.TINY)
+    H = O.inverse_phasespace(Hps)
+    back, _ = O.gen_phasespace_literal(H, H)
+    assert np.array_equal(back, Hps)
+
+
+# ----------------------------------------------------------------- weights
+def test_disc_aperture_has_177_views():
+    Hps = S.dense_phase_space(S.TINY)
+    w = O.view_weights(O.normalize_depth(Hps))
+    act = w != 0
+    assert act.sum() == 177
+    ref = np.zeros((15, 15), bool)
+    for (u, v) in S.aperture_views():
+        ref[u, v] = True
+    assert np.array_equal(act, ref)
+    # synthetic kernels all sum to 1 -> every active view has weight 0.3
+    assert np.allclose(w[act], 0.3)
+
+
+def test_weights_threshold_by_hand():
+    s = np.zeros((3, 3))
+    s[0, 0], s[1, 1], s[2, 2], s[0, 1] = 10.0, 2.0, 3.0, 1.0
+    w = O.view_weights_from_sums(s)
+    # /sum -> /16; -min(0); /max(10/16) -> 1, .2, .3, .1 ; <0.2 -> 0 ; *0.3
+    exp = np.zeros((3, 3))
+    exp[0, 0], exp[1, 1], exp[2, 2] = 0.3, 0.2 * 0.3, 0.3 * 0.3
+    assert np.allclose(w, exp)
+    assert w[0, 1] == 0
+
+
+def test_weights_degenerate_all_equal_is_nan():
+    w = O.view_weights_from_sums(np.ones((3, 3)))
+    assert np.all(np.isnan(w))          # NaN views stay "active" (NaN == 0 false)
+
+
+# ------------------------------------------------------------------- conv2
+def test_conv2_same_bruteforce():
+    rng = np.random.default_rng(3)
+    x = rng.random((7, 9))
+    h = rng.random((5, 3))
+    got = O.conv2_same(x, h)
+    m1, m2 = 2, 1
+    exp = np.zeros_like(x)
+    for p in range(7):
+        for q in range(9):
+            acc = 0.0
+            for a in range(7):
+                for b in range(9):
+                    i, j = p - a + m1, q - b + m2
+                    if 0 <= i < 5 and 0 <= j < 3:
+                        acc += x[a, b] * h[i, j]
+            exp[p, q] = acc
+    assert np.allclose(got, exp, rtol=1e-13, atol=1e-13)
+
+
+# ---------------------------------------------------- forward / backward
+@pytest.fixture(scope="module")
+def tiny():
+    wl = S.TINY
+    Hps = S.dense_phase_space(wl)
+    Ht = S.adjoint_ht(Hps)
+    cp = O.compact_from_phase_space(Hps, Ht)
+    return wl, Hps, Ht, cp
+
+
+def test_forward_compact_equals_literal(tiny):
+    wl, Hps, Ht, cp = tiny
+    rng = np.random.default_rng(4)
+    X = rng.random((wl.I1, wl.I2, wl.Z))
+    Xl = O.to_lenslet(X, 15)
+    for k in (0, 17, 88, len(cp.views) - 1):
+        u, v = cp.views[k]
+        a = O.forward_literal(X, Hps, u, v, 15)
+        b = O.forward_compact(Xl, cp.fwd[k], cp.rf)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+
+
+def test_backward_compact_equals_literal_and_adjoint(tiny):
+    wl, Hps, Ht, cp = tiny
+    rng = np.random.default_rng(5)
+    X = rng.random((wl.I1, wl.I2, wl.Z))
+    for k in (3, 60, 150):
+        u, v = cp.views[k]
+        R = rng.random((wl.n1, wl.n2))
+        a = O.backward_literal(Ht, R, u, v, 15)
+        b = O.from_lenslet(O.backward_compact(R, cp.bwd[k], cp.rb), 15)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-14)
+        # <F x, r> == <x, F^T r> for the exact-adjoint Ht
+        y = O.forward_literal(X, Hps, u, v, 15)
+        assert abs((y * R).sum() - (X * a).sum()) <= 1e-12 * abs((y * R).sum())
+
+
+def test_compact_table_matches_generator(tiny):
+    wl, Hps, Ht, cp = tiny
+    for k in (0, 40, 176):
+        u, v = cp.views[k]
+        for zi, z in enumerate(wl.z_um):
+            t = S.compact_fwd_table(u, v, z)
+            assert np.allclose(cp.fwd[k][zi], t, rtol=1e-15, atol=1e-18)
+            # exact adjoint: backward table is the point reflection
+            assert np.array_equal(cp.bwd[k][zi], cp.fwd[k][zi][::-1, ::-1])
+
+
+# ---------------------------------------------------------- sample_filter
+def test_sample_filter_literal_equals_separable():
+    rng = np.random.default_rng(6)
+    for (I1, I2, Z) in [(45, 60, 3), (30, 30, 11)]:
+        Y = rng.random((I1, I2, Z))
+        bank = O.filter_bank(I1, I2, Z, 15)
+        a = O.sample_filter_literal(Y, bank)
+        b = O.sample_filter_separable(Y, Z, 15)
+        assert np.allclose(a, b, rtol=1e-12, atol=1e-13)
+        # renormalisation: every filtered plane keeps its sum
+        assert np.allclose(a.sum(axis=(0, 1)), Y.sum(axis=(0, 1)), rtol=1e-13)
+
+
+def test_filter_kernel_is_real_even_unit_sum():
+    for I in (45, 60, 495, 1515):
+        for om in (0.2, 6.5, 8.0):
+            k = O.filter_kernel_1d(I, om)
+            assert abs(k.sum() - 1) < 1e-12
+            assert np.allclose(k[1:], k[1:][::-1], atol=1e-15)
+
+
+def test_half_z_uses_matlab_round():
+    assert O.matlab_round(2.5) == 3 and O.matlab_round(25.5) == 26 and O.matlab_round(-2.5) == -3
+    # Z=51: half_z = 26, nf = 54 -> k = z + 1 (1-based), every plane filtered
+    idx = O.filter_plane_index(51, 54)
+    assert idx[0] == 1 and idx[-1] == 51
+    idx = O.filter_plane_index(11, 12)
+    assert idx == list(range(11))
+
+
+def test_omega_array_cfg2():
+    om = O.omega_array(51, 15)
+    assert om.size == 54
+    assert om.min() > 6.4 and om.max() <= 8.0
+
+
+# ------------------------------------------------------------ update rules
+def test_update_special_values():
+    X = np.array([1.0, 2.0, 0.0, 3.0, 4.0, 5.0])
+    B = np.array([2.0, 0.0, 1.0, 1.0, -1.0, 1.0])
+    B1 = np.array([1.0, 0.0, 0.0, 0.0, 1.0, 2.0])
+    w = 0.3
+    got = O.multiplicative_update(X, B, B1, w)
+    C = np.array([2.0, 0.0, 0.0, 0.0, 0.0, 2.5])  # 0/0->0, 0*1/0=NaN->0, 3/0=inf->0, neg->0
+    assert np.allclose(got, C * w + (1 - w) * X)
+    r = O.ratio_update(np.array([1.0, 0.0, 2.0]), np.array([0.0, 0.0, 4.0]))
+    assert np.array_equal(r, [0.0, 0.0, 0.5])
+
+
+# ------------------------------------------------------------------ full RL
+@pytest.mark.slow
+def test_rl_compact_equals_literal_small():
+    wl = S.Workload("t", 3, 4, (-4, 4), 1, 1)
+    Hps = S.dense_phase_space(wl)
+    H = O.inverse_phasespace(Hps)
+    Ht = S.adjoint_ht(Hps)
+    rng = np.random.default_rng(7)
+    LF = rng.poisson(50.0, (wl.I1, wl.I2)).astype(np.float64)
+    a = O.rl_literal(LF, H, Ht, 1, views_limit=6)
+    cp = O.compact_from_dense(H, Ht)
+    b = O.rl_compact(LF, cp, 1, views_limit=6)
+    assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(a)
+
+
+# ------------------------------------------------------------------- MEX op
+def _mex_loop(bitmap, K):
+    """Per-output-pixel loop following mex:50-78 and mex:194-251 literally."""
+    W, Hh = bitmap.shape
+    kw = K.shape[0]
+    out = np.zeros((W, Hh))
+    for y in range(Hh):
+        for x in range(W):
+            bsx, bsy = x - kw // 2, y - kw // 2
+            bex, bey = x + kw // 2 + 1, y + kw // 2 + 1
+            ksx = -bsx if bsx < 0 else 0
+            ksy = -bsy if bsy < 0 else 0
+            kex = kw - (bex - W) if bex > W else kw
+            key = kw - (bey - Hh) if bey > Hh else kw
+            acc = 0.0
+            for ky in range(ksy, key):
+                for kx in range(ksx, kex):
+                    acc += bitmap[bsx + kx, bsy + ky] * K[kx, ky, x % 15, y % 15]
+            out[x, y] = acc
+    return out
+
+
+@pytest.mark.parametrize("W,Hh,kw", [(17, 13, 5), (20, 31, 7), (16, 16, 4), (9, 40, 31)])
+def test_mex_op_equals_loop(W, Hh, kw):
+    rng = np.random.default_rng(W * Hh + kw)
+    b = rng.random((W, Hh))
+    K = rng.random((kw, kw, 15, 15))
+    assert np.allclose(O.mex_op(b, K), _mex_loop(b, K), rtol=1e-12, atol=1e-12)
+
+
+def test_mex_op_is_adjoint_of_scatter_forward():
+    rng = np.random.default_rng(9)
+    X = rng.random((30, 45))
+    Y = rng.random((30, 45))
+    K = rng.random((11, 11, 15, 15))
+    lhs = (O.forward_scatter(X, K) * Y).sum()
+    rhs = (X * O.mex_op(Y, K)).sum()
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs)
+
+
+def test_mex_op_batch_axis():
+    rng = np.random.default_rng(10)
+    b = rng.random((12, 14, 3))
+    K = rng.random((5, 5, 15, 15))
+    out = O.mex_op(b, K)
+    for n in range(3):
+        assert np.allclose(out[:, :, n], O.mex_op(b[:, :, n], K))
