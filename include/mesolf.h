@@ -1,0 +1,160 @@
+/*
+ * mesolf.h -- C ABI of libmesolf, an MI355X (gfx950) implementation of
+ * MesoLF's phase-space Richardson-Lucy light-field deconvolution and of its
+ * CUDA MEX operator.
+ *
+ * Arrays cross this boundary as plain pointers in MATLAB column-major order
+ * (the first index is contiguous).  No entry point takes or returns a
+ * framework type.  Every call is thread safe; calls on one handle are
+ * serialised by that handle's mutex.
+ *
+ * Drop-in targets in the reference (vazirilab/MesoLF):
+ *   mlf_recon_phase_space       <- src/recon/recon_theory_phase_space_LFM_with_filter.m:10
+ *   mlf_lfm_convolution_vec     <- src/cuda/mex_lfm_convolution_vec.cu:276 (mexFunction)
+ * The handle API (context / psf / patch / batch / server) has no single
+ * counterpart: it exposes the stages of recon:10-151 so that the PSF and the
+ * volumes stay resident on the device between calls.
+ */
+#ifndef MESOLF_H
+#define MESOLF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MLF_ABI_VERSION 1
+
+typedef enum {
+  MLF_OK = 0,
+  MLF_ERR_INVALID_ARGUMENT = 1, /* maps to the MEX error id parallel:gpu:mexGPUExample:InvalidInput */
+  MLF_ERR_UNSUPPORTED = 2,      /* valid in the reference, not supported here (message says why) */
+  MLF_ERR_DEVICE = 3,           /* HIP runtime error (message carries hipGetErrorString) */
+  MLF_ERR_OUT_OF_MEMORY = 4,
+  MLF_ERR_BUDGET = 5,           /* request refused by the HBM admission budget */
+  MLF_ERR_INTERNAL = 6
+} mlf_status;
+
+typedef enum {
+  MLF_PRECISION_FP32 = 0, /* FP32 operands, FP32 accumulation (the drop-in)            */
+  MLF_PRECISION_BF16 = 1, /* BF16 MFMA operands, FP32 accumulation, FP32 update        */
+  MLF_PRECISION_FP16 = 2  /* FP16 MFMA operands (PSF pre-scaled per view and plane)    */
+} mlf_precision;
+
+typedef struct mlf_context_s* mlf_context;
+typedef struct mlf_psf_s* mlf_psf;
+typedef struct mlf_patch_s* mlf_patch;
+
+/* Per-call statistics of a reconstruction (recon:99,144-145 prints the
+ * per-iteration time; this struct returns it). */
+typedef struct {
+  int32_t iterations;
+  int32_t views_per_iteration;
+  double total_seconds;
+  double iteration_seconds[64];   /* first 64 iterations */
+  double relative_change[64];     /* ||X_k - X_{k-1}|| / ||X_{k-1}||, FP64, if requested */
+} mlf_recon_stats;
+
+/* ------------------------------------------------------------------ errors */
+/* Thread-local message of the last failing call on this thread. */
+const char* mlf_last_error(void);
+const char* mlf_status_string(mlf_status s);
+int32_t mlf_abi_version(void);
+
+/* ----------------------------------------------------------------- context */
+/* One context per device (device is 0-based; the MATLAB gateway converts the
+ * 1-based gpu_id of recon:76).  Owns one HIP stream. */
+mlf_status mlf_context_create(int32_t device, mlf_context* out);
+mlf_status mlf_context_destroy(mlf_context ctx);
+mlf_status mlf_context_synchronize(mlf_context ctx);
+/* Free/total device memory as seen by the context. */
+mlf_status mlf_context_memory(mlf_context ctx, uint64_t* free_bytes, uint64_t* total_bytes);
+
+/* --------------------------------------------------------------------- PSF */
+/* From the raw LFM PSFs exactly as recon:13-40 consumes them:
+ * H, Ht are [S, S, N, N, Z] single (column-major).  Performs the per-depth
+ * normalisation (recon:13-16), gen_phasespace_PSF_theory (crop + de-shift +
+ * phase-space permutation), the view weights / aperture (recon:30-40) and the
+ * spiral visiting order (recon:95-105), then uploads compact tables.
+ * N must be 15 (the phase count of the MEX and of MesoLF's optics). */
+mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht,
+                                int64_t S, int64_t N, int64_t Z, mlf_psf* out);
+
+/* From compact phase-space tables (already normalised, views already in
+ * visiting order).  For view k and plane z:
+ *   fwd block: (2*rf[z]+1)^2 * N*N floats, index ((d2+rf)*(2rf+1) + (d1+rf))*N*N + s2*N + s1
+ *              y[i] = sum_z sum_{L,s} X[N*L+s, z] * fwd[i-L+rf, s]
+ *   bwd block: same with rb[z]:   B[N*a+s, z] = sum_i R[i] * bwd[a-i+rb, s]
+ * Blocks are concatenated view-major, then plane.  bwd may be NULL: then the
+ * exact adjoint of fwd is used (rb = rf, bwd[d] = fwd[-d]). */
+mlf_status mlf_psf_create_compact(mlf_context ctx, int64_t N, int64_t Z, int64_t nviews,
+                                  const int32_t* view_uv, const double* view_weight,
+                                  const int32_t* rf, const float* fwd,
+                                  const int32_t* rb, const float* bwd, mlf_psf* out);
+/* nviews, Z, and per-plane radii (arrays of length Z may be NULL). */
+mlf_status mlf_psf_info(mlf_psf psf, int64_t* nviews, int64_t* Z, int32_t* rf, int32_t* rb);
+/* Active views in visiting order: uv[2*k], uv[2*k+1] (0-based), weight[k]. */
+mlf_status mlf_psf_views(mlf_psf psf, int32_t* uv, double* weight);
+mlf_status mlf_psf_destroy(mlf_psf psf);
+
+/* ------------------------------------------------------------------- patch */
+/* One reconstruction problem: n1 x n2 lenslets (image 15*n1 x 15*n2 px),
+ * all planes of the PSF.  Holds the volume, the image and work buffers. */
+mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision prec, mlf_patch* out);
+mlf_status mlf_patch_destroy(mlf_patch p);
+/* LFIMG [15*n1, 15*n2] single, host memory. */
+mlf_status mlf_patch_set_image(mlf_patch p, const float* lfimg);
+/* X = sum(LFIMG) / (I1*I2*Z) everywhere (recon:72-73). */
+mlf_status mlf_patch_init_volume(mlf_patch p);
+/* X [15*n1, 15*n2, Z] single, host memory (warm start). */
+mlf_status mlf_patch_set_volume(mlf_patch p, const float* X);
+mlf_status mlf_patch_get_volume(mlf_patch p, float* X);
+mlf_status mlf_patch_get_volume_f64(mlf_patch p, double* X);
+/* Run RL iterations (recon:98-146) asynchronously on the context stream.
+ * views_limit <= 0 visits every active view; otherwise the first views_limit.
+ * filter != 0 applies sample_filter at the end of each iteration (recon:142). */
+mlf_status mlf_patch_run(mlf_patch p, int32_t iterations, int32_t views_limit, int32_t filter);
+/* Forward projection of the current volume for active view k (recon:109,
+ * 243-249): y [n1, n2] single, host.  Synchronous. */
+mlf_status mlf_patch_forward_view(mlf_patch p, int32_t view, float* y);
+/* Forward projection of every active view interleaved into a sensor image
+ * LF[u + 15*i1, v + 15*i2] (zero at inactive views), host [15*n1, 15*n2]. */
+mlf_status mlf_patch_project_all(mlf_patch p, float* sensor);
+/* sample_filter (recon:222-240) on the current volume, FP64 plane energies. */
+mlf_status mlf_patch_sample_filter(mlf_patch p);
+/* Time of the last mlf_patch_run on the device, milliseconds (HIP events). */
+mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms);
+
+/* ------------------------------------------------------------ drop-ins */
+/* recon_theory_phase_space_LFM_with_filter(LFIMG,H,Ht,maxIter,gpuFlag,gpu_id)
+ * (recon:10).  LFIMG [I1, I2] single; H, Ht [S,S,N,N,Z] single; X_out
+ * [I1, I2, Z] double (the reference returns a gathered double array, recon:147).
+ * device is 0-based.  gpu_flag == 0 returns MLF_ERR_UNSUPPORTED: the
+ * reference CPU branch cannot run as written (undefined Xguessv at recon:112).
+ * stats may be NULL. */
+mlf_status mlf_recon_phase_space(const float* lfimg, int64_t I1, int64_t I2,
+                                 const float* H, const float* Ht, int64_t S, int64_t N, int64_t Z,
+                                 int32_t max_iter, int32_t gpu_flag, int32_t device,
+                                 mlf_precision prec, double* X_out, mlf_recon_stats* stats);
+
+/* mex_lfm_convolution_vec(bitmap, kernels) (mex:276-421):
+ *   out[x,y,n] = sum_{kx,ky<kw} bitmap[x-kw/2+kx, y-kw/2+ky, n] * K[kx,ky,x%15,y%15]
+ * with zero outside the bitmap (taps clipped as mex:55-78 clips them).
+ * bitmap [width, height, n] single, kernels [kw, kw, 15, 15] single, out like
+ * bitmap.  Host pointers, synchronous.  No kernel-width limit (the reference
+ * stops at 341, mex:41). */
+mlf_status mlf_lfm_convolution_vec(const float* bitmap, int64_t width, int64_t height, int64_t n,
+                                   const float* kernels, int64_t kw, int64_t nphase,
+                                   float* out, int32_t device);
+/* Same on device pointers, asynchronous on ctx's stream. */
+mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap, int64_t width,
+                                          int64_t height, int64_t n, const float* d_kernels,
+                                          int64_t kw, int64_t nphase, float* d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MESOLF_H */

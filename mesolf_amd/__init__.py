@@ -1,0 +1,17 @@
+"""MI355X (gfx950) phase-space Richardson-Lucy light-field deconvolution.
+
+Native code: ``libmesolf.so`` (HIP kernels + C ABI, see ``include/mesolf.h``),
+built in-tree by ``python -m mesolf_amd.build``.  This package is the thin
+host-side mirror of the reference's MATLAB-facing interface.
+"""
+from .lib import (  # noqa: F401
+    BudgetExceeded, Context, InvalidInput, MesolfError, Patch, PSF, Unsupported,
+    compact_block_layout, exported_symbols, load_library, mex_lfm_convolution_vec,
+    recon_theory_phase_space_LFM_with_filter,
+)
+
+__all__ = [
+    "Context", "PSF", "Patch", "MesolfError", "InvalidInput", "Unsupported", "BudgetExceeded",
+    "load_library", "exported_symbols", "compact_block_layout",
+    "recon_theory_phase_space_LFM_with_filter", "mex_lfm_convolution_vec",
+]

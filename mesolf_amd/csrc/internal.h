@@ -1,0 +1,122 @@
+// Internal declarations shared by the host runtime and the HIP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/mesolf.h"
+
+namespace mlf {
+
+constexpr int kN = 15;          // phases per lenslet axis (MesoLF optics, mex:35)
+constexpr int kP = kN * kN;     // phases per lenslet
+constexpr int kC0 = kN / 2;     // centre phase, 0-based
+
+// ------------------------------------------------------------ error state
+mlf_status set_error(mlf_status s, const char* fmt, ...);
+mlf_status clear_error();
+
+#define MLF_HIP(call)                                                             \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) {                                                       \
+      return ::mlf::set_error(e_ == hipErrorOutOfMemory ? MLF_ERR_OUT_OF_MEMORY   \
+                                                        : MLF_ERR_DEVICE,         \
+                              "%s failed: %s (%s:%d)", #call,                     \
+                              hipGetErrorString(e_), __FILE__, __LINE__);         \
+    }                                                                             \
+  } while (0)
+
+#define MLF_CHECK(expr)                 \
+  do {                                  \
+    mlf_status s_ = (expr);             \
+    if (s_ != MLF_OK) return s_;        \
+  } while (0)
+
+// --------------------------------------------------------- device buffer
+template <typename T>
+struct DevBuf {
+  T* ptr = nullptr;
+  size_t count = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() { release(); }
+  mlf_status alloc(size_t n) {
+    release();
+    if (n == 0) return MLF_OK;
+    hipError_t e = hipMalloc(&ptr, n * sizeof(T));
+    if (e != hipSuccess) {
+      ptr = nullptr;
+      return set_error(MLF_ERR_OUT_OF_MEMORY, "hipMalloc(%zu bytes) failed: %s", n * sizeof(T),
+                       hipGetErrorString(e));
+    }
+    count = n;
+    return MLF_OK;
+  }
+  void release() {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    count = 0;
+  }
+  size_t bytes() const { return count * sizeof(T); }
+};
+
+// ----------------------------------------------------------------- launchers
+// All launchers enqueue on `st` and return hipGetLastError().
+
+// MATLAB [I1,I2,Z] (column-major) <-> lenslet-blocked [z][L2][L1][s2][s1].
+hipError_t launch_to_blocked(const float* src, float* dst, int n1, int n2, int Z, hipStream_t st);
+hipError_t launch_from_blocked_f32(const float* src, float* dst, int n1, int n2, int Z, hipStream_t st);
+hipError_t launch_from_blocked_f64(const float* src, double* dst, int n1, int n2, int Z, hipStream_t st);
+hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t st);
+
+// Per-plane geometry of the compact tables of one view.
+struct PlaneTables {
+  const int32_t* r;        // [Z] lenslet radius per plane
+  const int64_t* off;      // [Z] float offset of each plane's block inside one view
+};
+
+// Forward projection of one view into per-plane partial images part[z][n1*n2].
+hipError_t launch_forward_fp32(const float* X, const float* W, PlaneTables t, int n1, int n2, int Z,
+                               float* part, hipStream_t st);
+// y = sum_z part, R = LF(u::15, v::15) ./ y with non-finite -> 0.
+hipError_t launch_ratio(const float* part, int Z, int n1, int n2, const float* LF, int u, int v,
+                        float* R, float* y_out, hipStream_t st);
+// B = backprojection of R, B1 = backprojection of ones; multiplicative update.
+hipError_t launch_backward_update_fp32(float* X, const float* Wb, PlaneTables t, const float* R,
+                                       int n1, int n2, int Z, float w, hipStream_t st);
+// Scatter a view image into the sensor image at (u + 15 i1, v + 15 i2).
+hipError_t launch_scatter_view(const float* part, int Z, int n1, int n2, int u, int v, float* sensor,
+                               hipStream_t st);
+
+// sample_filter: separable circular convolution per filtered plane.
+struct FilterPlan {
+  int nplanes = 0;                 // planes with a filter
+  const int32_t* plane = nullptr;  // [nplanes] plane index z
+  const int32_t* n0_1 = nullptr;   // [nplanes] first tap offset along axis 0
+  const int32_t* len_1 = nullptr;  // [nplanes] number of taps along axis 0
+  const int64_t* off_1 = nullptr;  // [nplanes] offset into taps_1
+  const float* taps_1 = nullptr;
+  const int32_t* n0_2 = nullptr;
+  const int32_t* len_2 = nullptr;
+  const int64_t* off_2 = nullptr;
+  const float* taps_2 = nullptr;
+};
+hipError_t launch_filter_pass1(const float* X, float* tmp, FilterPlan fp, int n1, int n2,
+                               double* part_before, int nblk, hipStream_t st);
+hipError_t launch_filter_pass2(const float* tmp, float* X, FilterPlan fp, int n1, int n2,
+                               double* part_after, int nblk, hipStream_t st);
+hipError_t launch_filter_scale(float* X, FilterPlan fp, int n1, int n2, const double* part_before,
+                               const double* part_after, int nblk, hipStream_t st);
+int filter_partial_blocks(int n1, int n2);
+
+// MEX operator (mex:43-266).
+hipError_t launch_mex_conv(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
+                           float* out, hipStream_t st);
+
+}  // namespace mlf

@@ -1,0 +1,337 @@
+// Phase-space Richardson-Lucy kernels (FP32 reference kernels).
+//
+// Volume layout in HBM ("lenslet-blocked"): X[z][L2][L1][s2][s1], i.e. the
+// 225 phases of one lenslet are contiguous; pixel (p1, p2) of the MATLAB
+// array is (15*L1 + s1, 15*L2 + s2).  The compact PSF tables are
+// [view][z][e][phase] with e = (d2 + r)*(2r+1) + (d1 + r).
+#include "internal.h"
+
+namespace mlf {
+
+namespace {
+
+__device__ __forceinline__ size_t blk_index(int p1, int p2, int z, int n1, int n2) {
+  const int L1 = p1 / kN, s1 = p1 - L1 * kN;
+  const int L2 = p2 / kN, s2 = p2 - L2 * kN;
+  return ((((size_t)z * n2 + L2) * n1 + L1) * kN + s2) * kN + s1;
+}
+
+__global__ void k_to_blocked(const float* __restrict__ src, float* __restrict__ dst, int n1, int n2,
+                             int Z) {
+  const size_t total = (size_t)n1 * n2 * Z * kP;
+  const int I1 = n1 * kN, I2 = n2 * kN;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total;
+       o += (size_t)gridDim.x * blockDim.x) {
+    size_t t = o;
+    const int s1 = t % kN; t /= kN;
+    const int s2 = t % kN; t /= kN;
+    const int L1 = t % n1; t /= n1;
+    const int L2 = t % n2; t /= n2;
+    const int z = (int)t;
+    const int p1 = L1 * kN + s1, p2 = L2 * kN + s2;
+    dst[o] = src[(size_t)p1 + (size_t)I1 * (p2 + (size_t)I2 * z)];
+  }
+}
+
+template <typename T>
+__global__ void k_from_blocked(const float* __restrict__ src, T* __restrict__ dst, int n1, int n2,
+                               int Z) {
+  const int I1 = n1 * kN, I2 = n2 * kN;
+  const size_t total = (size_t)I1 * I2 * Z;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int p1 = o % I1;
+    const size_t r = o / I1;
+    const int p2 = r % I2;
+    const int z = (int)(r / I2);
+    dst[o] = (T)src[blk_index(p1, p2, z, n1, n2)];
+  }
+}
+
+__global__ void k_fill(float* dst, size_t n, float v) {
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n;
+       o += (size_t)gridDim.x * blockDim.x)
+    dst[o] = v;
+}
+
+// One thread per (output lenslet i, plane z):
+//   part[z][i] = sum_{d,phase} X[z][i-d][phase] * W[z][d][phase].
+__global__ void k_forward_fp32(const float* __restrict__ X, const float* __restrict__ W,
+                               PlaneTables t, int n1, int n2, float* __restrict__ part) {
+  const int z = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n1 * n2) return;
+  const int i1 = i % n1, i2 = i / n1;
+  const int r = t.r[z];
+  const int K = 2 * r + 1;
+  const float* Wz = W + t.off[z];
+  float acc = 0.f;
+  for (int d2 = -r; d2 <= r; ++d2) {
+    const int L2 = i2 - d2;
+    if (L2 < 0 || L2 >= n2) continue;
+    for (int d1 = -r; d1 <= r; ++d1) {
+      const int L1 = i1 - d1;
+      if (L1 < 0 || L1 >= n1) continue;
+      const float* x = X + (((size_t)z * n2 + L2) * n1 + L1) * kP;
+      const float* w = Wz + ((size_t)(d2 + r) * K + (d1 + r)) * kP;
+#pragma unroll 5
+      for (int p = 0; p < kP; ++p) acc = fmaf(x[p], w[p], acc);
+    }
+  }
+  part[(size_t)z * n1 * n2 + i] = acc;
+}
+
+__global__ void k_ratio(const float* __restrict__ part, int Z, int n1, int n2,
+                        const float* __restrict__ LF, int u, int v, float* __restrict__ R,
+                        float* __restrict__ y_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = n1 * n2;
+  if (i >= n) return;
+  double y = 0.0;
+  for (int z = 0; z < Z; ++z) y += (double)part[(size_t)z * n + i];
+  const float yf = (float)y;
+  const int i1 = i % n1, i2 = i / n1;
+  const float lf = LF[(size_t)(u + kN * i1) + (size_t)(kN * n1) * (v + kN * i2)];
+  float q = lf / yf;
+  if (!isfinite(q)) q = 0.f;
+  R[i] = q;
+  if (y_out) y_out[i] = yf;
+}
+
+__global__ void k_scatter_view(const float* __restrict__ part, int Z, int n1, int n2, int u, int v,
+                               float* __restrict__ sensor) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = n1 * n2;
+  if (i >= n) return;
+  double y = 0.0;
+  for (int z = 0; z < Z; ++z) y += (double)part[(size_t)z * n + i];
+  const int i1 = i % n1, i2 = i / n1;
+  sensor[(size_t)(u + kN * i1) + (size_t)(kN * n1) * (v + kN * i2)] = (float)y;
+}
+
+// One block per (lenslet a, plane z), one thread per phase:
+//   B  = sum_{d} R[a-d] * Wb[z][d][phase]      (backprojection of the ratio)
+//   B1 = sum_{d: a-d inside} Wb[z][d][phase]   (backprojection of ones)
+//   C = X*B/B1 (NaN/Inf -> 0, <0 -> 0);  X = w*C + (1-w)*X;  X<0 -> 0.
+__global__ void k_backward_update_fp32(float* __restrict__ X, const float* __restrict__ Wb,
+                                       PlaneTables t, const float* __restrict__ R, int n1, int n2,
+                                       float w) {
+  const int p = threadIdx.x;
+  if (p >= kP) return;
+  const int a = blockIdx.x;
+  const int z = blockIdx.y;
+  const int a1 = a % n1, a2 = a / n1;
+  const int r = t.r[z];
+  const int K = 2 * r + 1;
+  const float* Wz = Wb + t.off[z];
+  float B = 0.f, B1 = 0.f;
+  for (int d2 = -r; d2 <= r; ++d2) {
+    const int j2 = a2 - d2;
+    if (j2 < 0 || j2 >= n2) continue;
+    for (int d1 = -r; d1 <= r; ++d1) {
+      const int j1 = a1 - d1;
+      if (j1 < 0 || j1 >= n1) continue;
+      const float wv = Wz[((size_t)(d2 + r) * K + (d1 + r)) * kP + p];
+      B = fmaf(R[j1 + n1 * j2], wv, B);
+      B1 += wv;
+    }
+  }
+  const size_t idx = (((size_t)z * n2 + a2) * n1 + a1) * kP + p;
+  const float x = X[idx];
+  float C = x * B / B1;
+  if (isnan(C) || isinf(C) || C < 0.f) C = 0.f;
+  float xn = C * w + (1.f - w) * x;
+  if (xn < 0.f) xn = 0.f;
+  X[idx] = xn;
+}
+
+// ---------------------------------------------------------------- filter
+constexpr int kFilterThreads = 256;
+
+__device__ __forceinline__ double block_sum_f64(double v, double* sh) {
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double s = 0.0;
+  if (threadIdx.x == 0)
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+  __syncthreads();
+  return s;
+}
+
+// Pass 1: tmp[q][p2][p1] = sum_j k1[j] * X[(p1 - n0 - j) mod I1, p2] for each
+// filtered plane q; also per-block FP64 partial sums of the input plane.
+__global__ void k_filter_pass1(const float* __restrict__ X, float* __restrict__ tmp, FilterPlan fp,
+                               int n1, int n2, double* __restrict__ part_before) {
+  __shared__ double sh[kFilterThreads / 64];
+  const int q = blockIdx.y;
+  const int z = fp.plane[q];
+  const int I1 = n1 * kN, I2 = n2 * kN;
+  const size_t plane = (size_t)I1 * I2;
+  const int n0 = fp.n0_1[q], len = fp.len_1[q];
+  const float* k1 = fp.taps_1 + fp.off_1[q];
+  double own = 0.0;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int p1 = o % I1, p2 = (int)(o / I1);
+    own += (double)X[blk_index(p1, p2, z, n1, n2)];
+    int src = p1 - n0;
+    src %= I1;
+    if (src < 0) src += I1;
+    float acc = 0.f;
+    for (int j = 0; j < len; ++j) {
+      acc = fmaf(k1[j], X[blk_index(src, p2, z, n1, n2)], acc);
+      src = (src == 0) ? I1 - 1 : src - 1;
+    }
+    tmp[(size_t)q * plane + o] = acc;
+  }
+  const double s = block_sum_f64(own, sh);
+  if (threadIdx.x == 0) part_before[(size_t)q * gridDim.x + blockIdx.x] = s;
+}
+
+// Pass 2: X[z][p1,p2] = sum_j k2[j] * tmp[q][p1, (p2 - n0 - j) mod I2].
+__global__ void k_filter_pass2(const float* __restrict__ tmp, float* __restrict__ X, FilterPlan fp,
+                               int n1, int n2, double* __restrict__ part_after) {
+  __shared__ double sh[kFilterThreads / 64];
+  const int q = blockIdx.y;
+  const int z = fp.plane[q];
+  const int I1 = n1 * kN, I2 = n2 * kN;
+  const size_t plane = (size_t)I1 * I2;
+  const int n0 = fp.n0_2[q], len = fp.len_2[q];
+  const float* k2 = fp.taps_2 + fp.off_2[q];
+  const float* tq = tmp + (size_t)q * plane;
+  double own = 0.0;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int p1 = o % I1, p2 = (int)(o / I1);
+    int src = p2 - n0;
+    src %= I2;
+    if (src < 0) src += I2;
+    float acc = 0.f;
+    for (int j = 0; j < len; ++j) {
+      acc = fmaf(k2[j], tq[(size_t)p1 + (size_t)I1 * src], acc);
+      src = (src == 0) ? I2 - 1 : src - 1;
+    }
+    X[blk_index(p1, p2, z, n1, n2)] = acc;
+    own += (double)acc;
+  }
+  const double s = block_sum_f64(own, sh);
+  if (threadIdx.x == 0) part_after[(size_t)q * gridDim.x + blockIdx.x] = s;
+}
+
+// Pass 3: X[z] *= sum_before / sum_after (FP64 plane energies, fixed order).
+__global__ void k_filter_scale(float* __restrict__ X, FilterPlan fp, int n1, int n2,
+                               const double* __restrict__ pb, const double* __restrict__ pa,
+                               int nblk) {
+  __shared__ double scale;
+  const int q = blockIdx.y;
+  const int z = fp.plane[q];
+  if (threadIdx.x == 0) {
+    double b = 0.0, a = 0.0;
+    for (int k = 0; k < nblk; ++k) {
+      b += pb[(size_t)q * nblk + k];
+      a += pa[(size_t)q * nblk + k];
+    }
+    scale = b / a;
+  }
+  __syncthreads();
+  const double s = scale;
+  const size_t plane = (size_t)n1 * n2 * kP;
+  float* Xz = X + (size_t)z * plane;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane;
+       o += (size_t)gridDim.x * blockDim.x)
+    Xz[o] = (float)((double)Xz[o] * s);
+}
+
+inline int grid_for(size_t n, int threads, int cap = 65536) {
+  size_t g = (n + threads - 1) / threads;
+  if (g > (size_t)cap) g = cap;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+}  // namespace
+
+hipError_t launch_to_blocked(const float* src, float* dst, int n1, int n2, int Z, hipStream_t st) {
+  const size_t n = (size_t)n1 * n2 * Z * kP;
+  hipLaunchKernelGGL(k_to_blocked, dim3(grid_for(n, 256)), dim3(256), 0, st, src, dst, n1, n2, Z);
+  return hipGetLastError();
+}
+
+hipError_t launch_from_blocked_f32(const float* src, float* dst, int n1, int n2, int Z,
+                                   hipStream_t st) {
+  const size_t n = (size_t)n1 * n2 * Z * kP;
+  hipLaunchKernelGGL(k_from_blocked<float>, dim3(grid_for(n, 256)), dim3(256), 0, st, src, dst, n1,
+                     n2, Z);
+  return hipGetLastError();
+}
+
+hipError_t launch_from_blocked_f64(const float* src, double* dst, int n1, int n2, int Z,
+                                   hipStream_t st) {
+  const size_t n = (size_t)n1 * n2 * Z * kP;
+  hipLaunchKernelGGL(k_from_blocked<double>, dim3(grid_for(n, 256)), dim3(256), 0, st, src, dst,
+                     n1, n2, Z);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t st) {
+  hipLaunchKernelGGL(k_fill, dim3(grid_for(n, 256)), dim3(256), 0, st, dst, n, value);
+  return hipGetLastError();
+}
+
+hipError_t launch_forward_fp32(const float* X, const float* W, PlaneTables t, int n1, int n2, int Z,
+                               float* part, hipStream_t st) {
+  dim3 grid((n1 * n2 + 127) / 128, Z);
+  hipLaunchKernelGGL(k_forward_fp32, grid, dim3(128), 0, st, X, W, t, n1, n2, part);
+  return hipGetLastError();
+}
+
+hipError_t launch_ratio(const float* part, int Z, int n1, int n2, const float* LF, int u, int v,
+                        float* R, float* y_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ratio, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, part, Z, n1, n2, LF, u,
+                     v, R, y_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_scatter_view(const float* part, int Z, int n1, int n2, int u, int v, float* sensor,
+                               hipStream_t st) {
+  hipLaunchKernelGGL(k_scatter_view, dim3((n1 * n2 + 255) / 256), dim3(256), 0, st, part, Z, n1, n2,
+                     u, v, sensor);
+  return hipGetLastError();
+}
+
+hipError_t launch_backward_update_fp32(float* X, const float* Wb, PlaneTables t, const float* R,
+                                       int n1, int n2, int Z, float w, hipStream_t st) {
+  dim3 grid(n1 * n2, Z);
+  hipLaunchKernelGGL(k_backward_update_fp32, grid, dim3(256), 0, st, X, Wb, t, R, n1, n2, w);
+  return hipGetLastError();
+}
+
+int filter_partial_blocks(int n1, int n2) {
+  const size_t plane = (size_t)n1 * n2 * kP;
+  return grid_for(plane, kFilterThreads, 512);
+}
+
+hipError_t launch_filter_pass1(const float* X, float* tmp, FilterPlan fp, int n1, int n2,
+                               double* part_before, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter_pass1, dim3(nblk, fp.nplanes), dim3(kFilterThreads), 0, st, X, tmp,
+                     fp, n1, n2, part_before);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_pass2(const float* tmp, float* X, FilterPlan fp, int n1, int n2,
+                               double* part_after, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter_pass2, dim3(nblk, fp.nplanes), dim3(kFilterThreads), 0, st, tmp, X,
+                     fp, n1, n2, part_after);
+  return hipGetLastError();
+}
+
+hipError_t launch_filter_scale(float* X, FilterPlan fp, int n1, int n2, const double* part_before,
+                               const double* part_after, int nblk, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter_scale, dim3(256, fp.nplanes), dim3(256), 0, st, X, fp, n1, n2,
+                     part_before, part_after, nblk);
+  return hipGetLastError();
+}
+
+}  // namespace mlf

@@ -1,0 +1,953 @@
+// Host runtime of libmesolf: error state, contexts, PSF preparation and
+// residency, patches (reconstruction problems), the RL driver and the two
+// drop-in entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "internal.h"
+
+namespace mlf {
+
+// ================================================================ errors
+static thread_local std::string g_last_error;
+
+mlf_status set_error(mlf_status s, const char* fmt, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(buf, sizeof(buf), fmt, ap);
+  va_end(ap);
+  g_last_error = buf;
+  return s;
+}
+
+mlf_status clear_error() {
+  g_last_error.clear();
+  return MLF_OK;
+}
+
+const char* last_error() { return g_last_error.c_str(); }
+
+// ======================================================== MATLAB helpers
+static long matlab_round(double x) {
+  return (long)(std::floor(std::fabs(x) + 0.5)) * (x >= 0 ? 1 : -1);
+}
+
+// recon:156-188.  mode(Nnum)==0 is false for every N>0 (mode of a scalar is
+// the scalar), so loop_n = (N+1)/2 and the loop runs to floor(loop_n).
+std::vector<std::pair<int, int>> spiral_order(int N) {
+  std::vector<std::pair<int, int>> out;
+  out.reserve((size_t)N * N);
+  const int loops = (N + 1) / 2;
+  for (int k = 1; k <= loops; ++k) {
+    const int cs = N - 2 * (k - 1);
+    const int b = k - 1;
+    if (cs > 1) {
+      for (int i = 1; i <= cs; ++i) out.emplace_back(i - 1 + b, b);
+      for (int j = 2; j <= cs; ++j) out.emplace_back(cs - 1 + b, j - 1 + b);
+      for (int i = cs - 1; i >= 1; --i) out.emplace_back(i - 1 + b, cs - 1 + b);
+      for (int j = cs - 1; j >= 2; --j) out.emplace_back(b, j - 1 + b);
+    } else {
+      out.emplace_back(b, b);
+    }
+  }
+  return out;
+}
+
+// recon:30-38 on per-view sums (NaN -> 0, /sum, -min, /max, <cut -> 0, *0.3).
+std::vector<double> view_weights(const std::vector<double>& sums, double cut) {
+  std::vector<double> w(sums);
+  for (double& x : w)
+    if (std::isnan(x)) x = 0.0;
+  double s = 0.0;
+  for (double x : w) s += x;
+  for (double& x : w) x /= s;
+  double mn = w[0];
+  for (double x : w) mn = std::min(mn, x);
+  for (double& x : w) x -= mn;
+  double mx = w[0];
+  for (double x : w) mx = std::max(mx, x);
+  for (double& x : w) x /= mx;
+  for (double& x : w)
+    if (x < cut) x = 0.0;
+  for (double& x : w) x *= 0.3;
+  return w;
+}
+
+// recon:20-26: NIP_range = ceil(Z/3) rounded up to even; 3*NIP_range filters.
+int nip_range(int Z) {
+  int r = (Z + 2) / 3;
+  return (r % 2 == 1) ? r + 1 : r;
+}
+
+// sample_filter plane -> 0-based filter index (or -1), recon:224-231.
+std::vector<int> filter_plane_index(int Z) {
+  const int nf = 3 * nip_range(Z);
+  const long half_z = matlab_round(Z / 2.0);
+  std::vector<int> idx(Z, -1);
+  for (int z = 1; z <= Z; ++z) {
+    const double lo = half_z - nf / 2.0 + 1.0, hi = half_z + nf / 2.0;
+    if (z >= lo && z <= hi) idx[z - 1] = (int)(z - half_z + nf / 2) - 1;
+  }
+  return idx;
+}
+
+double filter_omega(int Z, int N, int f) {
+  const int nf = 3 * nip_range(Z);
+  const double axis = (f + 1) - nf / 2.0;
+  double om = (N - 7) * std::exp(-std::pow(axis / 100.0 * 2.0 * std::sqrt(std::log(2.0)), 2.0));
+  return om < 0.2 ? 0.2 : om;
+}
+
+// Real-space circular kernel of one axis of the filter bank (recon:54-60):
+// kappa = real(ifft(|fft(g)| / max|fft(g)|)), g(x) = exp(-((x - I/2)/omega)^2),
+// x = 1..I.  Direct DFT in FP64 with exact (k*n mod I) twiddle indexing.
+std::vector<double> filter_kernel_1d(int I, double omega) {
+  std::vector<double> g(I), c(I), s(I), a(I), k(I);
+  const double two_pi = 6.283185307179586476925286766559;
+  for (int m = 0; m < I; ++m) {
+    c[m] = std::cos(two_pi * m / I);
+    s[m] = std::sin(two_pi * m / I);
+  }
+  for (int n = 0; n < I; ++n) {
+    const double x = (n + 1) - I / 2.0;
+    g[n] = std::exp(-(x / omega) * (x / omega));
+  }
+  double mx = 0.0;
+  for (int f = 0; f < I; ++f) {
+    double re = 0.0, im = 0.0;
+    long m = 0;
+    for (int n = 0; n < I; ++n) {
+      re += g[n] * c[m];
+      im -= g[n] * s[m];
+      m += f;
+      if (m >= I) m -= I;
+    }
+    a[f] = std::sqrt(re * re + im * im);
+    mx = std::max(mx, a[f]);
+  }
+  for (int f = 0; f < I; ++f) a[f] /= mx;
+  for (int n = 0; n < I; ++n) {
+    double re = 0.0;
+    long m = 0;
+    for (int f = 0; f < I; ++f) {
+      re += a[f] * c[m];
+      m += n;
+      if (m >= I) m -= I;
+    }
+    k[n] = re / I;
+  }
+  return k;
+}
+
+// Truncate kappa (circular) to the smallest symmetric support whose dropped
+// mass is <= 1e-10 of the total; returns (n0, taps) with taps[j] = kappa[n0+j].
+static void truncate_kernel(const std::vector<double>& k, int* n0, std::vector<float>* taps) {
+  const int I = (int)k.size();
+  double total = 0.0;
+  for (double v : k) total += std::fabs(v);
+  const int half = (I - 1) / 2;
+  int T = half;
+  double kept = std::fabs(k[0]);
+  for (int t = 1; t <= half; ++t) {
+    kept += std::fabs(k[t]) + std::fabs(k[I - t]);
+    if (total - kept <= 1e-10 * total) {
+      T = t;
+      break;
+    }
+  }
+  taps->clear();
+  if (2 * T + 1 >= I) {  // full period
+    *n0 = -((I - 1) / 2);
+    for (int j = 0; j < I; ++j) taps->push_back((float)k[((*n0 + j) % I + I) % I]);
+  } else {
+    *n0 = -T;
+    for (int j = 0; j < 2 * T + 1; ++j) taps->push_back((float)k[((*n0 + j) % I + I) % I]);
+  }
+}
+
+// ================================================================ context
+}  // namespace mlf
+
+using namespace mlf;
+
+struct mlf_context_s {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+};
+
+static mlf_status ctx_activate(mlf_context ctx) {
+  MLF_HIP(hipSetDevice(ctx->device));
+  return MLF_OK;
+}
+
+extern "C" {
+
+const char* mlf_last_error(void) { return mlf::last_error(); }
+
+const char* mlf_status_string(mlf_status s) {
+  switch (s) {
+    case MLF_OK: return "ok";
+    case MLF_ERR_INVALID_ARGUMENT: return "invalid argument";
+    case MLF_ERR_UNSUPPORTED: return "unsupported";
+    case MLF_ERR_DEVICE: return "device error";
+    case MLF_ERR_OUT_OF_MEMORY: return "out of device memory";
+    case MLF_ERR_BUDGET: return "HBM budget exceeded";
+    case MLF_ERR_INTERNAL: return "internal error";
+  }
+  return "unknown status";
+}
+
+int32_t mlf_abi_version(void) { return MLF_ABI_VERSION; }
+
+mlf_status mlf_context_create(int32_t device, mlf_context* out) {
+  clear_error();
+  if (!out) return set_error(MLF_ERR_INVALID_ARGUMENT, "out is NULL");
+  *out = nullptr;
+  int count = 0;
+  hipError_t e = hipGetDeviceCount(&count);
+  if (e != hipSuccess || count == 0)
+    return set_error(MLF_ERR_DEVICE, "no HIP device available (%s)", hipGetErrorString(e));
+  if (device < 0 || device >= count)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "device %d out of range [0, %d)", device, count);
+  auto ctx = std::make_unique<mlf_context_s>();
+  ctx->device = device;
+  MLF_HIP(hipSetDevice(device));
+  MLF_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+  *out = ctx.release();
+  return MLF_OK;
+}
+
+mlf_status mlf_context_destroy(mlf_context ctx) {
+  clear_error();
+  if (!ctx) return MLF_OK;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+  return MLF_OK;
+}
+
+mlf_status mlf_context_synchronize(mlf_context ctx) {
+  clear_error();
+  if (!ctx) return set_error(MLF_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  MLF_CHECK(ctx_activate(ctx));
+  MLF_HIP(hipStreamSynchronize(ctx->stream));
+  return MLF_OK;
+}
+
+mlf_status mlf_context_memory(mlf_context ctx, uint64_t* free_bytes, uint64_t* total_bytes) {
+  clear_error();
+  if (!ctx) return set_error(MLF_ERR_INVALID_ARGUMENT, "ctx is NULL");
+  MLF_CHECK(ctx_activate(ctx));
+  size_t f = 0, t = 0;
+  MLF_HIP(hipMemGetInfo(&f, &t));
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+  return MLF_OK;
+}
+
+}  // extern "C"
+
+// =================================================================== PSF
+struct mlf_psf_s {
+  mlf_context ctx = nullptr;
+  int Z = 0, V = 0;
+  std::vector<int32_t> uv;       // [V][2]
+  std::vector<double> weight;    // [V]
+  std::vector<int32_t> rf, rb;   // [Z]
+  std::vector<int64_t> offf, offb;
+  int64_t per_view_f = 0, per_view_b = 0;
+  DevBuf<float> fwd, bwd;
+  DevBuf<int32_t> d_rf, d_rb;
+  DevBuf<int64_t> d_offf, d_offb;
+};
+
+static int64_t plane_block(int r) { return (int64_t)(2 * r + 1) * (2 * r + 1) * kP; }
+
+// Upload compact tables (host layout == device layout for the FP32 kernels).
+static mlf_status psf_upload(mlf_psf_s* p, const float* fwd, const float* bwd) {
+  p->offf.assign(p->Z, 0);
+  p->offb.assign(p->Z, 0);
+  int64_t of = 0, ob = 0;
+  for (int z = 0; z < p->Z; ++z) {
+    p->offf[z] = of;
+    of += plane_block(p->rf[z]);
+    p->offb[z] = ob;
+    ob += plane_block(p->rb[z]);
+  }
+  p->per_view_f = of;
+  p->per_view_b = ob;
+  MLF_CHECK(p->fwd.alloc((size_t)of * p->V));
+  MLF_CHECK(p->bwd.alloc((size_t)ob * p->V));
+  MLF_HIP(hipMemcpy(p->fwd.ptr, fwd, p->fwd.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->bwd.ptr, bwd, p->bwd.bytes(), hipMemcpyHostToDevice));
+  MLF_CHECK(p->d_rf.alloc(p->Z));
+  MLF_CHECK(p->d_rb.alloc(p->Z));
+  MLF_CHECK(p->d_offf.alloc(p->Z));
+  MLF_CHECK(p->d_offb.alloc(p->Z));
+  MLF_HIP(hipMemcpy(p->d_rf.ptr, p->rf.data(), p->d_rf.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->d_rb.ptr, p->rb.data(), p->d_rb.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->d_offf.ptr, p->offf.data(), p->d_offf.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->d_offb.ptr, p->offb.data(), p->d_offb.bytes(), hipMemcpyHostToDevice));
+  return MLF_OK;
+}
+
+// Exact adjoint of a forward table: bwd[d] = fwd[-d] per plane.
+static void adjoint_tables(const std::vector<int32_t>& rf, int V, const float* fwd,
+                           std::vector<float>* bwd) {
+  int64_t per_view = 0;
+  for (int r : rf) per_view += plane_block(r);
+  bwd->resize((size_t)per_view * V);
+  for (int k = 0; k < V; ++k) {
+    int64_t off = (int64_t)k * per_view;
+    for (size_t z = 0; z < rf.size(); ++z) {
+      const int r = rf[z], K = 2 * r + 1;
+      for (int e2 = 0; e2 < K; ++e2)
+        for (int e1 = 0; e1 < K; ++e1) {
+          const float* src = fwd + off + ((int64_t)(K - 1 - e2) * K + (K - 1 - e1)) * kP;
+          float* dst = bwd->data() + off + ((int64_t)e2 * K + e1) * kP;
+          std::memcpy(dst, src, sizeof(float) * kP);
+        }
+      off += plane_block(r);
+    }
+  }
+}
+
+extern "C" {
+
+mlf_status mlf_psf_create_compact(mlf_context ctx, int64_t N, int64_t Z, int64_t nviews,
+                                  const int32_t* view_uv, const double* view_weight,
+                                  const int32_t* rf, const float* fwd, const int32_t* rb,
+                                  const float* bwd, mlf_psf* out) {
+  clear_error();
+  if (!ctx || !out || !view_uv || !view_weight || !rf || !fwd)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (N != kN) return set_error(MLF_ERR_UNSUPPORTED, "Nnum must be 15 (got %lld)", (long long)N);
+  if (Z < 1 || Z > 4096) return set_error(MLF_ERR_INVALID_ARGUMENT, "bad plane count %lld", (long long)Z);
+  if (nviews < 1 || nviews > kP)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad view count %lld", (long long)nviews);
+  for (int64_t z = 0; z < Z; ++z) {
+    if (rf[z] < 0 || rf[z] > 512 || (rb && (rb[z] < 0 || rb[z] > 512)))
+      return set_error(MLF_ERR_INVALID_ARGUMENT, "bad radius at plane %lld", (long long)z);
+  }
+  for (int64_t k = 0; k < nviews; ++k) {
+    if (view_uv[2 * k] < 0 || view_uv[2 * k] >= kN || view_uv[2 * k + 1] < 0 ||
+        view_uv[2 * k + 1] >= kN)
+      return set_error(MLF_ERR_INVALID_ARGUMENT, "view %lld has (u,v) outside [0,15)", (long long)k);
+    if (!std::isfinite(view_weight[k]) || view_weight[k] == 0.0)
+      return set_error(MLF_ERR_INVALID_ARGUMENT, "view %lld has weight %g", (long long)k,
+                       view_weight[k]);
+  }
+  MLF_CHECK(ctx_activate(ctx));
+  auto p = std::make_unique<mlf_psf_s>();
+  p->ctx = ctx;
+  p->Z = (int)Z;
+  p->V = (int)nviews;
+  p->uv.assign(view_uv, view_uv + 2 * nviews);
+  p->weight.assign(view_weight, view_weight + nviews);
+  p->rf.assign(rf, rf + Z);
+  std::vector<float> adj;
+  if (bwd) {
+    if (!rb) return set_error(MLF_ERR_INVALID_ARGUMENT, "bwd given without rb");
+    p->rb.assign(rb, rb + Z);
+  } else {
+    p->rb = p->rf;
+    adjoint_tables(p->rf, p->V, fwd, &adj);
+    bwd = adj.data();
+  }
+  MLF_CHECK(psf_upload(p.get(), fwd, bwd));
+  *out = p.release();
+  return MLF_OK;
+}
+
+// Dense raw PSFs -> compact tables, following recon:13-40 and
+// gen_phasespace_PSF_theory.m.  With Hcrop the cropped H (index i0 + x) and
+// c0 = 7, the phase-space forward table of view (u,v) is
+//   fwd[d1,d2,s1,s2] = Hn[i0 + u + 15(d1+R) - s1 + c0, i0 + v + 15(d2+R) - s2 + c0, s1, s2, z]
+// (closed form of the de-shift + reshape/permute chain), and the backward
+// table reads the cropped Ht at the view's own (u, v) slot (recon:121):
+//   bwd[d1,d2,s1,s2] = Htn[i0 + 15(d1+R) + c0 + s1 - u, i0 + 15(d2+R) + c0 + s2 - v, u, v, z].
+mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht, int64_t S,
+                                int64_t N, int64_t Z, mlf_psf* out) {
+  clear_error();
+  if (!ctx || !H || !Ht || !out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (N != kN) return set_error(MLF_ERR_UNSUPPORTED, "Nnum must be 15 (got %lld)", (long long)N);
+  if (Z < 1) return set_error(MLF_ERR_INVALID_ARGUMENT, "Z must be >= 1");
+  if (S < N) return set_error(MLF_ERR_INVALID_ARGUMENT, "PSF size %lld smaller than Nnum", (long long)S);
+  const int64_t IMG = S - ((S - N) % (2 * N));  // gen_phasespace:6
+  const int64_t i0 = matlab_round(S / 2.0) - (int64_t)std::trunc(IMG / 2.0) - 1;
+  const int R = (int)((IMG / N - 1) / 2);
+  const size_t SS = (size_t)S * S;
+  auto at = [&](const float* A, int64_t x, int64_t y, int a, int b, int z) -> double {
+    return (double)A[(size_t)x + (size_t)S * y + SS * ((size_t)a + (size_t)N * ((size_t)b + (size_t)N * z))];
+  };
+  // per-depth sums (recon:13-16), FP64
+  std::vector<double> hs(Z, 0.0), hts(Z, 0.0);
+  const size_t slab = SS * N * N;
+  for (int64_t z = 0; z < Z; ++z) {
+    double a = 0.0, b = 0.0;
+    for (size_t o = 0; o < slab; ++o) {
+      a += H[o + slab * z];
+      b += Ht[o + slab * z];
+    }
+    hs[z] = a;
+    hts[z] = b;
+  }
+  const int Kf = 2 * R + 1, Kb = 2 * R + 3;
+  // full forward tables for all 225 views: needed for the weights anyway
+  std::vector<double> wsum(kP, 0.0);
+  auto fwd_val = [&](int u, int v, int z, int d1, int d2, int s1, int s2) -> double {
+    const int64_t x = u + (int64_t)N * (d1 + R) - s1 + kC0;
+    const int64_t y = v + (int64_t)N * (d2 + R) - s2 + kC0;
+    if (x < 0 || x >= IMG || y < 0 || y >= IMG) return 0.0;
+    return at(H, i0 + x, i0 + y, s1, s2, z) / hs[z];
+  };
+  auto bwd_val = [&](int u, int v, int z, int d1, int d2, int s1, int s2) -> double {
+    const int64_t x = (int64_t)N * (d1 + R) + kC0 + s1 - u;
+    const int64_t y = (int64_t)N * (d2 + R) + kC0 + s2 - v;
+    if (x < 0 || x >= IMG || y < 0 || y >= IMG) return 0.0;
+    return at(Ht, i0 + x, i0 + y, u, v, z) / hts[z];
+  };
+  // W(u,v) = sum over H_ps(:,:,u,v,:) (recon:30): H_ps covers exactly the
+  // (d in [-R,R], s) grid of the forward table.
+  for (int u = 0; u < kN; ++u)
+    for (int v = 0; v < kN; ++v) {
+      double s = 0.0;
+      for (int z = 0; z < Z; ++z)
+        for (int d2 = -R; d2 <= R; ++d2)
+          for (int d1 = -R; d1 <= R; ++d1)
+            for (int s2 = 0; s2 < kN; ++s2)
+              for (int s1 = 0; s1 < kN; ++s1) s += fwd_val(u, v, z, d1, d2, s1, s2);
+      wsum[u + kN * v] = s;
+    }
+  // weights indexed [u + 15 v]
+  std::vector<double> w = view_weights(wsum, 0.2);
+  std::vector<int32_t> uv;
+  std::vector<double> wv;
+  for (auto& [u, v] : spiral_order(kN)) {
+    const double x = w[u + kN * v];
+    if (std::isnan(x))
+      return set_error(MLF_ERR_INVALID_ARGUMENT,
+                       "degenerate view weights (NaN): all views have equal PSF sums");
+    if (x == 0.0) continue;
+    uv.push_back(u);
+    uv.push_back(v);
+    wv.push_back(x);
+  }
+  const int V = (int)wv.size();
+  if (V == 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "no active view");
+  // per-plane radii from nonzero extents over active views
+  std::vector<int32_t> rf(Z, 0), rb(Z, 0);
+  for (int z = 0; z < Z; ++z) {
+    for (int k = 0; k < V; ++k) {
+      const int u = uv[2 * k], v = uv[2 * k + 1];
+      for (int d2 = -R; d2 <= R; ++d2)
+        for (int d1 = -R; d1 <= R; ++d1) {
+          const int ad = std::max(std::abs(d1), std::abs(d2));
+          if (ad <= rf[z]) continue;
+          for (int s = 0; s < kP && ad > rf[z]; ++s)
+            if (fwd_val(u, v, z, d1, d2, s % kN, s / kN) != 0.0) rf[z] = ad;
+        }
+      for (int d2 = -R - 1; d2 <= R + 1; ++d2)
+        for (int d1 = -R - 1; d1 <= R + 1; ++d1) {
+          const int ad = std::max(std::abs(d1), std::abs(d2));
+          if (ad <= rb[z]) continue;
+          for (int s = 0; s < kP && ad > rb[z]; ++s)
+            if (bwd_val(u, v, z, d1, d2, s % kN, s / kN) != 0.0) rb[z] = ad;
+        }
+    }
+  }
+  (void)Kf;
+  (void)Kb;
+  int64_t pvf = 0, pvb = 0;
+  for (int z = 0; z < Z; ++z) {
+    pvf += plane_block(rf[z]);
+    pvb += plane_block(rb[z]);
+  }
+  std::vector<float> tf((size_t)pvf * V), tb((size_t)pvb * V);
+  for (int k = 0; k < V; ++k) {
+    const int u = uv[2 * k], v = uv[2 * k + 1];
+    int64_t of = (int64_t)k * pvf, ob = (int64_t)k * pvb;
+    for (int z = 0; z < Z; ++z) {
+      const int r = rf[z], K = 2 * r + 1;
+      for (int d2 = -r; d2 <= r; ++d2)
+        for (int d1 = -r; d1 <= r; ++d1)
+          for (int s2 = 0; s2 < kN; ++s2)
+            for (int s1 = 0; s1 < kN; ++s1)
+              tf[of + ((int64_t)(d2 + r) * K + (d1 + r)) * kP + s2 * kN + s1] =
+                  (float)fwd_val(u, v, z, d1, d2, s1, s2);
+      of += plane_block(r);
+      const int q = rb[z], Q = 2 * q + 1;
+      for (int d2 = -q; d2 <= q; ++d2)
+        for (int d1 = -q; d1 <= q; ++d1)
+          for (int s2 = 0; s2 < kN; ++s2)
+            for (int s1 = 0; s1 < kN; ++s1)
+              tb[ob + ((int64_t)(d2 + q) * Q + (d1 + q)) * kP + s2 * kN + s1] =
+                  (float)bwd_val(u, v, z, d1, d2, s1, s2);
+      ob += plane_block(q);
+    }
+  }
+  return mlf_psf_create_compact(ctx, N, Z, V, uv.data(), wv.data(), rf.data(), tf.data(),
+                                rb.data(), tb.data(), out);
+}
+
+mlf_status mlf_psf_info(mlf_psf psf, int64_t* nviews, int64_t* Z, int32_t* rf, int32_t* rb) {
+  clear_error();
+  if (!psf) return set_error(MLF_ERR_INVALID_ARGUMENT, "psf is NULL");
+  if (nviews) *nviews = psf->V;
+  if (Z) *Z = psf->Z;
+  if (rf) std::copy(psf->rf.begin(), psf->rf.end(), rf);
+  if (rb) std::copy(psf->rb.begin(), psf->rb.end(), rb);
+  return MLF_OK;
+}
+
+mlf_status mlf_psf_views(mlf_psf psf, int32_t* uv, double* weight) {
+  clear_error();
+  if (!psf) return set_error(MLF_ERR_INVALID_ARGUMENT, "psf is NULL");
+  if (uv) std::copy(psf->uv.begin(), psf->uv.end(), uv);
+  if (weight) std::copy(psf->weight.begin(), psf->weight.end(), weight);
+  return MLF_OK;
+}
+
+mlf_status mlf_psf_destroy(mlf_psf psf) {
+  clear_error();
+  if (!psf) return MLF_OK;
+  (void)hipSetDevice(psf->ctx->device);
+  delete psf;
+  return MLF_OK;
+}
+
+}  // extern "C"
+
+// ================================================================= patch
+struct mlf_patch_s {
+  mlf_psf psf = nullptr;
+  int n1 = 0, n2 = 0, Z = 0;
+  mlf_precision prec = MLF_PRECISION_FP32;
+  DevBuf<float> X, LF, part, R, y, tmp;
+  DevBuf<double> pb, pa;
+  // filter plan
+  int f_nplanes = 0, nblk = 0;
+  DevBuf<int32_t> f_plane, f_n0_1, f_len_1, f_n0_2, f_len_2;
+  DevBuf<int64_t> f_off_1, f_off_2;
+  DevBuf<float> f_taps_1, f_taps_2;
+  bool image_set = false, volume_set = false;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  std::mutex mu;
+  ~mlf_patch_s() {
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+  }
+};
+
+static mlf_status build_filter_plan(mlf_patch_s* p) {
+  const int Z = p->Z, I1 = p->n1 * kN, I2 = p->n2 * kN;
+  const std::vector<int> idx = filter_plane_index(Z);
+  std::vector<int32_t> plane, n01, len1, n02, len2;
+  std::vector<int64_t> off1, off2;
+  std::vector<float> t1, t2;
+  std::vector<int> seen;  // filter ids already computed
+  std::vector<std::pair<int, std::vector<float>>> cache1, cache2;
+  std::vector<std::pair<int, int>> n0cache;
+  for (int z = 0; z < Z; ++z) {
+    const int f = idx[z];
+    if (f < 0) continue;
+    const double om = filter_omega(Z, kN, f);
+    int a0, b0;
+    std::vector<float> ta, tbv;
+    truncate_kernel(filter_kernel_1d(I1, om), &a0, &ta);
+    truncate_kernel(filter_kernel_1d(I2, om), &b0, &tbv);
+    plane.push_back(z);
+    n01.push_back(a0);
+    len1.push_back((int32_t)ta.size());
+    off1.push_back((int64_t)t1.size());
+    t1.insert(t1.end(), ta.begin(), ta.end());
+    n02.push_back(b0);
+    len2.push_back((int32_t)tbv.size());
+    off2.push_back((int64_t)t2.size());
+    t2.insert(t2.end(), tbv.begin(), tbv.end());
+  }
+  p->f_nplanes = (int)plane.size();
+  if (p->f_nplanes == 0) return MLF_OK;
+  MLF_CHECK(p->f_plane.alloc(plane.size()));
+  MLF_CHECK(p->f_n0_1.alloc(plane.size()));
+  MLF_CHECK(p->f_len_1.alloc(plane.size()));
+  MLF_CHECK(p->f_off_1.alloc(plane.size()));
+  MLF_CHECK(p->f_n0_2.alloc(plane.size()));
+  MLF_CHECK(p->f_len_2.alloc(plane.size()));
+  MLF_CHECK(p->f_off_2.alloc(plane.size()));
+  MLF_CHECK(p->f_taps_1.alloc(t1.size()));
+  MLF_CHECK(p->f_taps_2.alloc(t2.size()));
+  MLF_HIP(hipMemcpy(p->f_plane.ptr, plane.data(), p->f_plane.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_n0_1.ptr, n01.data(), p->f_n0_1.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_len_1.ptr, len1.data(), p->f_len_1.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_off_1.ptr, off1.data(), p->f_off_1.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_n0_2.ptr, n02.data(), p->f_n0_2.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_len_2.ptr, len2.data(), p->f_len_2.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_off_2.ptr, off2.data(), p->f_off_2.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_taps_1.ptr, t1.data(), p->f_taps_1.bytes(), hipMemcpyHostToDevice));
+  MLF_HIP(hipMemcpy(p->f_taps_2.ptr, t2.data(), p->f_taps_2.bytes(), hipMemcpyHostToDevice));
+  p->nblk = filter_partial_blocks(p->n1, p->n2);
+  MLF_CHECK(p->pb.alloc((size_t)p->nblk * p->f_nplanes));
+  MLF_CHECK(p->pa.alloc((size_t)p->nblk * p->f_nplanes));
+  MLF_CHECK(p->tmp.alloc((size_t)p->f_nplanes * I1 * I2));
+  return MLF_OK;
+}
+
+static FilterPlan filter_plan_of(mlf_patch_s* p) {
+  FilterPlan fp;
+  fp.nplanes = p->f_nplanes;
+  fp.plane = p->f_plane.ptr;
+  fp.n0_1 = p->f_n0_1.ptr;
+  fp.len_1 = p->f_len_1.ptr;
+  fp.off_1 = p->f_off_1.ptr;
+  fp.taps_1 = p->f_taps_1.ptr;
+  fp.n0_2 = p->f_n0_2.ptr;
+  fp.len_2 = p->f_len_2.ptr;
+  fp.off_2 = p->f_off_2.ptr;
+  fp.taps_2 = p->f_taps_2.ptr;
+  return fp;
+}
+
+#define MLF_LAUNCH(call)                                                                      \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return set_error(MLF_ERR_DEVICE, "launch %s failed: %s", #call, hipGetErrorString(e_)); \
+  } while (0)
+
+static mlf_status patch_forward(mlf_patch_s* p, int k) {
+  mlf_psf_s* ps = p->psf;
+  const PlaneTables tf{ps->d_rf.ptr, ps->d_offf.ptr};
+  MLF_LAUNCH(launch_forward_fp32(p->X.ptr, ps->fwd.ptr + (size_t)k * ps->per_view_f, tf, p->n1,
+                                 p->n2, p->Z, p->part.ptr, ps->ctx->stream));
+  return MLF_OK;
+}
+
+static mlf_status patch_view_step(mlf_patch_s* p, int k) {
+  mlf_psf_s* ps = p->psf;
+  hipStream_t st = ps->ctx->stream;
+  MLF_CHECK(patch_forward(p, k));
+  MLF_LAUNCH(launch_ratio(p->part.ptr, p->Z, p->n1, p->n2, p->LF.ptr, ps->uv[2 * k],
+                          ps->uv[2 * k + 1], p->R.ptr, nullptr, st));
+  const PlaneTables tb{ps->d_rb.ptr, ps->d_offb.ptr};
+  MLF_LAUNCH(launch_backward_update_fp32(p->X.ptr, ps->bwd.ptr + (size_t)k * ps->per_view_b, tb,
+                                         p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
+  return MLF_OK;
+}
+
+static mlf_status patch_filter(mlf_patch_s* p) {
+  if (p->f_nplanes == 0) return MLF_OK;
+  hipStream_t st = p->psf->ctx->stream;
+  const FilterPlan fp = filter_plan_of(p);
+  MLF_LAUNCH(launch_filter_pass1(p->X.ptr, p->tmp.ptr, fp, p->n1, p->n2, p->pb.ptr, p->nblk, st));
+  MLF_LAUNCH(launch_filter_pass2(p->tmp.ptr, p->X.ptr, fp, p->n1, p->n2, p->pa.ptr, p->nblk, st));
+  MLF_LAUNCH(launch_filter_scale(p->X.ptr, fp, p->n1, p->n2, p->pb.ptr, p->pa.ptr, p->nblk, st));
+  return MLF_OK;
+}
+
+extern "C" {
+
+mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision prec,
+                            mlf_patch* out) {
+  clear_error();
+  if (!psf || !out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (n1 < 1 || n2 < 1 || n1 > 4096 || n2 > 4096)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad lenslet grid %lld x %lld", (long long)n1,
+                     (long long)n2);
+  if (prec != MLF_PRECISION_FP32)
+    return set_error(MLF_ERR_UNSUPPORTED, "precision %d not built yet", (int)prec);
+  MLF_CHECK(ctx_activate(psf->ctx));
+  auto p = std::make_unique<mlf_patch_s>();
+  p->psf = psf;
+  p->n1 = (int)n1;
+  p->n2 = (int)n2;
+  p->Z = psf->Z;
+  p->prec = prec;
+  const size_t n = (size_t)n1 * n2;
+  MLF_CHECK(p->X.alloc(n * p->Z * kP));
+  MLF_CHECK(p->LF.alloc(n * kP));
+  MLF_CHECK(p->part.alloc(n * p->Z));
+  MLF_CHECK(p->R.alloc(n));
+  MLF_CHECK(p->y.alloc(n));
+  MLF_CHECK(build_filter_plan(p.get()));
+  MLF_HIP(hipEventCreate(&p->ev0));
+  MLF_HIP(hipEventCreate(&p->ev1));
+  *out = p.release();
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_destroy(mlf_patch p) {
+  clear_error();
+  if (!p) return MLF_OK;
+  (void)hipSetDevice(p->psf->ctx->device);
+  (void)hipStreamSynchronize(p->psf->ctx->stream);
+  delete p;
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_set_image(mlf_patch p, const float* lfimg) {
+  clear_error();
+  if (!p || !lfimg) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  MLF_HIP(hipMemcpyAsync(p->LF.ptr, lfimg, p->LF.bytes(), hipMemcpyHostToDevice,
+                         p->psf->ctx->stream));
+  MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+  p->image_set = true;
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_init_volume(mlf_patch p) {
+  clear_error();
+  if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->image_set) return set_error(MLF_ERR_INVALID_ARGUMENT, "image not set");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  // recon:72-73: X = ones ./ numel .* sum(LFIMG); the FP64 image sum is taken
+  // on the host copy of the image.
+  std::vector<float> h(p->LF.count);
+  MLF_HIP(hipMemcpy(h.data(), p->LF.ptr, p->LF.bytes(), hipMemcpyDeviceToHost));
+  double s = 0.0;
+  for (float v : h) s += v;
+  const double numel = (double)p->n1 * kN * p->n2 * kN * p->Z;
+  const float x0 = (float)((1.0 / numel) * s);
+  MLF_LAUNCH(launch_fill(p->X.ptr, p->X.count, x0, p->psf->ctx->stream));
+  MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+  p->volume_set = true;
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_set_volume(mlf_patch p, const float* X) {
+  clear_error();
+  if (!p || !X) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  hipStream_t st = p->psf->ctx->stream;
+  DevBuf<float> tmp;
+  MLF_CHECK(tmp.alloc(p->X.count));
+  MLF_HIP(hipMemcpyAsync(tmp.ptr, X, tmp.bytes(), hipMemcpyHostToDevice, st));
+  MLF_LAUNCH(launch_to_blocked(tmp.ptr, p->X.ptr, p->n1, p->n2, p->Z, st));
+  MLF_HIP(hipStreamSynchronize(st));
+  p->volume_set = true;
+  return MLF_OK;
+}
+
+static mlf_status get_volume_impl(mlf_patch p, void* out, bool f64) {
+  clear_error();
+  if (!p || !out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  hipStream_t st = p->psf->ctx->stream;
+  if (f64) {
+    DevBuf<double> tmp;
+    MLF_CHECK(tmp.alloc(p->X.count));
+    MLF_LAUNCH(launch_from_blocked_f64(p->X.ptr, tmp.ptr, p->n1, p->n2, p->Z, st));
+    MLF_HIP(hipMemcpyAsync(out, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, st));
+    MLF_HIP(hipStreamSynchronize(st));
+  } else {
+    DevBuf<float> tmp;
+    MLF_CHECK(tmp.alloc(p->X.count));
+    MLF_LAUNCH(launch_from_blocked_f32(p->X.ptr, tmp.ptr, p->n1, p->n2, p->Z, st));
+    MLF_HIP(hipMemcpyAsync(out, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, st));
+    MLF_HIP(hipStreamSynchronize(st));
+  }
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_get_volume(mlf_patch p, float* X) { return get_volume_impl(p, X, false); }
+mlf_status mlf_patch_get_volume_f64(mlf_patch p, double* X) { return get_volume_impl(p, X, true); }
+
+mlf_status mlf_patch_run(mlf_patch p, int32_t iterations, int32_t views_limit, int32_t filter) {
+  clear_error();
+  if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->image_set || !p->volume_set)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "image and volume must be set before running");
+  if (iterations < 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "negative iteration count");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  hipStream_t st = p->psf->ctx->stream;
+  const int nv = (views_limit <= 0 || views_limit > p->psf->V) ? p->psf->V : views_limit;
+  MLF_HIP(hipEventRecord(p->ev0, st));
+  for (int it = 0; it < iterations; ++it) {
+    for (int k = 0; k < nv; ++k) MLF_CHECK(patch_view_step(p, k));
+    if (filter) MLF_CHECK(patch_filter(p));
+  }
+  MLF_HIP(hipEventRecord(p->ev1, st));
+  p->timed = true;
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms) {
+  clear_error();
+  if (!p || !ms) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->timed) return set_error(MLF_ERR_INVALID_ARGUMENT, "no run recorded");
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  MLF_HIP(hipEventSynchronize(p->ev1));
+  float f = 0.f;
+  MLF_HIP(hipEventElapsedTime(&f, p->ev0, p->ev1));
+  *ms = f;
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_forward_view(mlf_patch p, int32_t view, float* y) {
+  clear_error();
+  if (!p || !y) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (view < 0 || view >= p->psf->V) return set_error(MLF_ERR_INVALID_ARGUMENT, "view out of range");
+  if (!p->volume_set) return set_error(MLF_ERR_INVALID_ARGUMENT, "volume not set");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  hipStream_t st = p->psf->ctx->stream;
+  MLF_CHECK(patch_forward(p, view));
+  // reuse the ratio kernel's z-reduction with a zero image for y
+  DevBuf<float> zero;
+  MLF_CHECK(zero.alloc(p->LF.count));
+  MLF_HIP(hipMemsetAsync(zero.ptr, 0, zero.bytes(), st));
+  MLF_LAUNCH(launch_ratio(p->part.ptr, p->Z, p->n1, p->n2, zero.ptr, p->psf->uv[2 * view],
+                          p->psf->uv[2 * view + 1], p->R.ptr, p->y.ptr, st));
+  MLF_HIP(hipMemcpyAsync(y, p->y.ptr, p->y.bytes(), hipMemcpyDeviceToHost, st));
+  MLF_HIP(hipStreamSynchronize(st));
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_project_all(mlf_patch p, float* sensor) {
+  clear_error();
+  if (!p || !sensor) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->volume_set) return set_error(MLF_ERR_INVALID_ARGUMENT, "volume not set");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  hipStream_t st = p->psf->ctx->stream;
+  DevBuf<float> img;
+  MLF_CHECK(img.alloc(p->LF.count));
+  MLF_HIP(hipMemsetAsync(img.ptr, 0, img.bytes(), st));
+  for (int k = 0; k < p->psf->V; ++k) {
+    MLF_CHECK(patch_forward(p, k));
+    MLF_LAUNCH(launch_scatter_view(p->part.ptr, p->Z, p->n1, p->n2, p->psf->uv[2 * k],
+                                   p->psf->uv[2 * k + 1], img.ptr, st));
+  }
+  MLF_HIP(hipMemcpyAsync(sensor, img.ptr, img.bytes(), hipMemcpyDeviceToHost, st));
+  MLF_HIP(hipStreamSynchronize(st));
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_sample_filter(mlf_patch p) {
+  clear_error();
+  if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  MLF_CHECK(patch_filter(p));
+  MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+  return MLF_OK;
+}
+
+// ============================================================ drop-ins
+mlf_status mlf_recon_phase_space(const float* lfimg, int64_t I1, int64_t I2, const float* H,
+                                 const float* Ht, int64_t S, int64_t N, int64_t Z,
+                                 int32_t max_iter, int32_t gpu_flag, int32_t device,
+                                 mlf_precision prec, double* X_out, mlf_recon_stats* stats) {
+  clear_error();
+  if (!lfimg || !H || !Ht || !X_out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!gpu_flag)
+    return set_error(MLF_ERR_UNSUPPORTED,
+                     "gpuFlag=false: the reference CPU branch cannot run as written "
+                     "(undefined Xguessv at recon:112); use the GPU path");
+  if (N != kN) return set_error(MLF_ERR_UNSUPPORTED, "Nnum must be 15");
+  if (I1 % N != 0 || I2 % N != 0 || I1 < N || I2 < N)
+    return set_error(MLF_ERR_INVALID_ARGUMENT,
+                     "LFIMG size %lld x %lld is not a positive multiple of Nnum", (long long)I1,
+                     (long long)I2);
+  if (max_iter < 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "maxIter < 0");
+  const auto t0 = std::chrono::steady_clock::now();
+  mlf_context ctx = nullptr;
+  MLF_CHECK(mlf_context_create(device, &ctx));
+  std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> ctx_guard(ctx, mlf_context_destroy);
+  mlf_psf psf = nullptr;
+  MLF_CHECK(mlf_psf_create_dense(ctx, H, Ht, S, N, Z, &psf));
+  std::unique_ptr<mlf_psf_s, mlf_status (*)(mlf_psf)> psf_guard(psf, mlf_psf_destroy);
+  mlf_patch p = nullptr;
+  MLF_CHECK(mlf_patch_create(psf, I1 / N, I2 / N, prec, &p));
+  std::unique_ptr<mlf_patch_s, mlf_status (*)(mlf_patch)> patch_guard(p, mlf_patch_destroy);
+  MLF_CHECK(mlf_patch_set_image(p, lfimg));
+  MLF_CHECK(mlf_patch_init_volume(p));
+  if (stats) {
+    std::memset(stats, 0, sizeof(*stats));
+    stats->views_per_iteration = psf->V;
+  }
+  for (int it = 0; it < max_iter; ++it) {
+    MLF_CHECK(mlf_patch_run(p, 1, 0, 1));
+    double ms = 0.0;
+    MLF_CHECK(mlf_patch_last_run_ms(p, &ms));
+    if (stats && it < 64) stats->iteration_seconds[it] = ms * 1e-3;
+    if (stats) stats->iterations = it + 1;
+  }
+  MLF_CHECK(mlf_patch_get_volume_f64(p, X_out));
+  if (stats)
+    stats->total_seconds =
+        std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  return MLF_OK;
+}
+
+mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap, int64_t width,
+                                          int64_t height, int64_t n, const float* d_kernels,
+                                          int64_t kw, int64_t nphase, float* d_out) {
+  clear_error();
+  if (!ctx || !d_bitmap || !d_kernels || !d_out)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (width < 1 || height < 1 || n < 1)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bitmap dims must be >= 1");
+  if (nphase != kN) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernels must be [kw,kw,15,15]");
+  if (kw < 1 || kw > 2048) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernel width %lld out of [1,2048]", (long long)kw);
+  if (width > (1 << 30) / 1 || height > (1 << 30) || width * height > (int64_t)1 << 40)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bitmap too large");
+  if (n > 65535) return set_error(MLF_ERR_INVALID_ARGUMENT, "batch too large (n > 65535)");
+  MLF_CHECK(ctx_activate(ctx));
+  MLF_LAUNCH(launch_mex_conv(d_bitmap, width, height, n, d_kernels, (int)kw, d_out, ctx->stream));
+  return MLF_OK;
+}
+
+mlf_status mlf_lfm_convolution_vec(const float* bitmap, int64_t width, int64_t height, int64_t n,
+                                   const float* kernels, int64_t kw, int64_t nphase, float* out,
+                                   int32_t device) {
+  clear_error();
+  if (!bitmap || !kernels || !out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (width < 1 || height < 1 || n < 1)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bitmap dims must be >= 1");
+  if (nphase != kN) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernels must be [kw,kw,15,15]");
+  if (kw < 1 || kw > 2048) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernel width out of [1,2048]");
+  mlf_context ctx = nullptr;
+  MLF_CHECK(mlf_context_create(device, &ctx));
+  std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> guard(ctx, mlf_context_destroy);
+  const size_t nb = (size_t)width * height * n;
+  const size_t nk = (size_t)kw * kw * kP;
+  DevBuf<float> db, dk, dout;
+  MLF_CHECK(db.alloc(nb));
+  MLF_CHECK(dk.alloc(nk));
+  MLF_CHECK(dout.alloc(nb));
+  MLF_HIP(hipMemcpyAsync(db.ptr, bitmap, db.bytes(), hipMemcpyHostToDevice, ctx->stream));
+  MLF_HIP(hipMemcpyAsync(dk.ptr, kernels, dk.bytes(), hipMemcpyHostToDevice, ctx->stream));
+  MLF_CHECK(mlf_lfm_convolution_vec_device(ctx, db.ptr, width, height, n, dk.ptr, kw, nphase,
+                                           dout.ptr));
+  MLF_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+  MLF_HIP(hipStreamSynchronize(ctx->stream));
+  return MLF_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,389 @@
+"""ctypes binding of libmesolf (the C ABI in ``include/mesolf.h``).
+
+Arrays follow MATLAB semantics: a numpy array ``A`` with ``A[x, y, ...]``
+is passed in column-major (Fortran) order, so ``A[:, :, z]`` is the MATLAB
+slice ``A(:, :, z)``.
+
+The two drop-in functions keep the reference's argument order and meaning:
+
+* :func:`recon_theory_phase_space_LFM_with_filter` --
+  ``src/recon/recon_theory_phase_space_LFM_with_filter.m:10``
+* :func:`mex_lfm_convolution_vec` -- ``src/cuda/mex_lfm_convolution_vec.cu:276``
+
+There is no CPU fallback: if the shared library is missing every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import threading
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.environ.get("MLF_LIBRARY", os.path.join(HERE, "libmesolf.so"))
+
+MLF_OK = 0
+MLF_ERR_INVALID_ARGUMENT = 1
+MLF_ERR_UNSUPPORTED = 2
+MLF_ERR_DEVICE = 3
+MLF_ERR_OUT_OF_MEMORY = 4
+MLF_ERR_BUDGET = 5
+MLF_ERR_INTERNAL = 6
+
+PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
+
+# error identifier of the reference gateway (mex:296)
+MEX_ERROR_ID = "parallel:gpu:mexGPUExample:InvalidInput"
+MEX_ERROR_MSG = "Invalid input to mex_lfm_convolution_vec()"
+
+
+class MesolfError(RuntimeError):
+    def __init__(self, status: int, message: str):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+        self.message = message
+
+
+class InvalidInput(MesolfError, ValueError):
+    """Raised for MLF_ERR_INVALID_ARGUMENT; carries the reference error id."""
+
+    identifier = MEX_ERROR_ID
+
+
+class Unsupported(MesolfError):
+    pass
+
+
+class BudgetExceeded(MesolfError):
+    pass
+
+
+_lock = threading.Lock()
+_lib = None
+
+_P = C.c_void_p
+_I64 = C.c_int64
+_I32 = C.c_int32
+_F32P = C.POINTER(C.c_float)
+_F64P = C.POINTER(C.c_double)
+_I32P = C.POINTER(C.c_int32)
+_I64P = C.POINTER(C.c_int64)
+_U64P = C.POINTER(C.c_uint64)
+
+_SIGNATURES = {
+    "mlf_last_error": (C.c_char_p, []),
+    "mlf_status_string": (C.c_char_p, [C.c_int]),
+    "mlf_abi_version": (_I32, []),
+    "mlf_context_create": (C.c_int, [_I32, C.POINTER(_P)]),
+    "mlf_context_destroy": (C.c_int, [_P]),
+    "mlf_context_synchronize": (C.c_int, [_P]),
+    "mlf_context_memory": (C.c_int, [_P, _U64P, _U64P]),
+    "mlf_psf_create_dense": (C.c_int, [_P, _F32P, _F32P, _I64, _I64, _I64, C.POINTER(_P)]),
+    "mlf_psf_create_compact": (C.c_int, [_P, _I64, _I64, _I64, _I32P, _F64P, _I32P, _F32P,
+                                         _I32P, _F32P, C.POINTER(_P)]),
+    "mlf_psf_info": (C.c_int, [_P, _I64P, _I64P, _I32P, _I32P]),
+    "mlf_psf_views": (C.c_int, [_P, _I32P, _F64P]),
+    "mlf_psf_destroy": (C.c_int, [_P]),
+    "mlf_patch_create": (C.c_int, [_P, _I64, _I64, C.c_int, C.POINTER(_P)]),
+    "mlf_patch_destroy": (C.c_int, [_P]),
+    "mlf_patch_set_image": (C.c_int, [_P, _F32P]),
+    "mlf_patch_init_volume": (C.c_int, [_P]),
+    "mlf_patch_set_volume": (C.c_int, [_P, _F32P]),
+    "mlf_patch_get_volume": (C.c_int, [_P, _F32P]),
+    "mlf_patch_get_volume_f64": (C.c_int, [_P, _F64P]),
+    "mlf_patch_run": (C.c_int, [_P, _I32, _I32, _I32]),
+    "mlf_patch_forward_view": (C.c_int, [_P, _I32, _F32P]),
+    "mlf_patch_project_all": (C.c_int, [_P, _F32P]),
+    "mlf_patch_sample_filter": (C.c_int, [_P]),
+    "mlf_patch_last_run_ms": (C.c_int, [_P, _F64P]),
+    "mlf_recon_phase_space": (C.c_int, [_F32P, _I64, _I64, _F32P, _F32P, _I64, _I64, _I64, _I32,
+                                        _I32, _I32, C.c_int, _F64P, _P]),
+    "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
+    "mlf_lfm_convolution_vec_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P]),
+}
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def load_library(path: str | None = None):
+    """Load libmesolf.so (raises if it has not been built)."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = path or LIB_PATH
+        if not os.path.exists(p):
+            raise MesolfError(MLF_ERR_INTERNAL,
+                              f"{p} not found: build it with `python -m mesolf_amd.build`")
+        lib = C.CDLL(p)
+        for name, (res, args) in _SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def _check(status: int):
+    if status == MLF_OK:
+        return
+    lib = load_library()
+    msg = (lib.mlf_last_error() or b"").decode(errors="replace")
+    if status == MLF_ERR_INVALID_ARGUMENT:
+        raise InvalidInput(status, msg)
+    if status == MLF_ERR_UNSUPPORTED:
+        raise Unsupported(status, msg)
+    if status == MLF_ERR_BUDGET:
+        raise BudgetExceeded(status, msg)
+    raise MesolfError(status, msg)
+
+
+def _fptr(a: np.ndarray, ctype=C.c_float):
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def _col(a, dtype):
+    """Column-major contiguous copy (no copy if already so)."""
+    return np.asfortranarray(np.asarray(a, dtype=dtype))
+
+
+# --------------------------------------------------------------------- handles
+class Context:
+    def __init__(self, device: int = 0):
+        self._lib = load_library()
+        h = _P()
+        _check(self._lib.mlf_context_create(int(device), C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self):
+        _check(self._lib.mlf_context_synchronize(self.handle))
+
+    def memory(self):
+        f, t = C.c_uint64(), C.c_uint64()
+        _check(self._lib.mlf_context_memory(self.handle, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.mlf_context_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def compact_block_layout(tab: np.ndarray) -> np.ndarray:
+    """[d1+r, d2+r, s1, s2] table -> flat ABI order ((d2*K + d1)*225 + s2*15 + s1)."""
+    return np.ascontiguousarray(np.transpose(tab, (1, 0, 3, 2)), dtype=np.float32).ravel()
+
+
+class PSF:
+    """Device-resident phase-space PSF tables (one handle per PSF)."""
+
+    def __init__(self, ctx: Context, handle):
+        self.ctx = ctx
+        self._lib = ctx._lib
+        self.handle = handle
+        nv, z = C.c_int64(), C.c_int64()
+        _check(self._lib.mlf_psf_info(handle, C.byref(nv), C.byref(z), None, None))
+        self.nviews, self.Z = nv.value, z.value
+
+    @classmethod
+    def from_dense(cls, ctx: Context, H, Ht):
+        """H, Ht [S, S, 15, 15, Z] raw PSFs (recon:13-40 preparation)."""
+        H = _col(H, np.float32)
+        Ht = _col(Ht, np.float32)
+        if H.ndim == 4:
+            H = H[..., None]
+            Ht = Ht[..., None]
+        if H.ndim != 5 or H.shape != Ht.shape or H.shape[0] != H.shape[1]:
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "H and Ht must be [S,S,N,N,Z] and equal")
+        h = _P()
+        _check(ctx._lib.mlf_psf_create_dense(ctx.handle, _fptr(H), _fptr(Ht), H.shape[0],
+                                             H.shape[2], H.shape[4], C.byref(h)))
+        return cls(ctx, h)
+
+    @classmethod
+    def from_compact(cls, ctx: Context, views_uv, weights, rf, fwd_flat, rb=None, bwd_flat=None):
+        """Compact tables in ABI order (see ``compact_block_layout``)."""
+        uv = np.ascontiguousarray(np.asarray(views_uv, dtype=np.int32).reshape(-1, 2))
+        w = np.ascontiguousarray(weights, dtype=np.float64)
+        rf = np.ascontiguousarray(rf, dtype=np.int32)
+        fwd = np.ascontiguousarray(fwd_flat, dtype=np.float32)
+        rbp = None if rb is None else np.ascontiguousarray(rb, dtype=np.int32)
+        bwd = None if bwd_flat is None else np.ascontiguousarray(bwd_flat, dtype=np.float32)
+        h = _P()
+        _check(ctx._lib.mlf_psf_create_compact(
+            ctx.handle, 15, rf.size, uv.shape[0], _fptr(uv, C.c_int32), _fptr(w, C.c_double),
+            _fptr(rf, C.c_int32), _fptr(fwd), None if rbp is None else _fptr(rbp, C.c_int32),
+            None if bwd is None else _fptr(bwd), C.byref(h)))
+        return cls(ctx, h)
+
+    def radii(self):
+        rf = np.zeros(self.Z, np.int32)
+        rb = np.zeros(self.Z, np.int32)
+        _check(self._lib.mlf_psf_info(self.handle, None, None, _fptr(rf, C.c_int32), _fptr(rb, C.c_int32)))
+        return rf, rb
+
+    def views(self):
+        uv = np.zeros((self.nviews, 2), np.int32)
+        w = np.zeros(self.nviews, np.float64)
+        _check(self._lib.mlf_psf_views(self.handle, _fptr(uv, C.c_int32), _fptr(w, C.c_double)))
+        return [tuple(x) for x in uv.tolist()], w
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.mlf_psf_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Patch:
+    """One reconstruction problem (n1 x n2 lenslets, all PSF planes)."""
+
+    def __init__(self, psf: PSF, n1: int, n2: int, precision: str = "fp32"):
+        self.psf = psf
+        self._lib = psf._lib
+        self.n1, self.n2, self.Z = int(n1), int(n2), psf.Z
+        h = _P()
+        _check(self._lib.mlf_patch_create(psf.handle, self.n1, self.n2, PRECISIONS[precision], C.byref(h)))
+        self.handle = h
+
+    @property
+    def shape(self):
+        return (15 * self.n1, 15 * self.n2, self.Z)
+
+    def set_image(self, lf):
+        a = _col(lf, np.float32)
+        if a.shape != self.shape[:2]:
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"image shape {a.shape} != {self.shape[:2]}")
+        _check(self._lib.mlf_patch_set_image(self.handle, _fptr(a)))
+
+    def init_volume(self):
+        _check(self._lib.mlf_patch_init_volume(self.handle))
+
+    def set_volume(self, X):
+        a = _col(X, np.float32)
+        if a.shape != self.shape:
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"volume shape {a.shape} != {self.shape}")
+        _check(self._lib.mlf_patch_set_volume(self.handle, _fptr(a)))
+
+    def get_volume(self, f64: bool = False):
+        if f64:
+            out = np.empty(self.shape, np.float64, order="F")
+            _check(self._lib.mlf_patch_get_volume_f64(self.handle, _fptr(out, C.c_double)))
+        else:
+            out = np.empty(self.shape, np.float32, order="F")
+            _check(self._lib.mlf_patch_get_volume(self.handle, _fptr(out)))
+        return out
+
+    def run(self, iterations: int, views_limit: int = 0, filter: bool = True):
+        _check(self._lib.mlf_patch_run(self.handle, int(iterations), int(views_limit), int(bool(filter))))
+
+    def synchronize(self):
+        self.psf.ctx.synchronize()
+
+    def last_run_ms(self):
+        ms = C.c_double()
+        _check(self._lib.mlf_patch_last_run_ms(self.handle, C.byref(ms)))
+        return ms.value
+
+    def forward_view(self, k: int):
+        out = np.empty((self.n1, self.n2), np.float32, order="F")
+        _check(self._lib.mlf_patch_forward_view(self.handle, int(k), _fptr(out)))
+        return out
+
+    def project_all(self):
+        out = np.empty(self.shape[:2], np.float32, order="F")
+        _check(self._lib.mlf_patch_project_all(self.handle, _fptr(out)))
+        return out
+
+    def sample_filter(self):
+        _check(self._lib.mlf_patch_sample_filter(self.handle))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.mlf_patch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# -------------------------------------------------------------------- drop-ins
+def recon_theory_phase_space_LFM_with_filter(LFIMG, H, Ht, maxIter, gpuFlag=True, gpu_id=1,
+                                             precision="fp32", stats=None):
+    """Drop-in for recon_theory_phase_space_LFM_with_filter.m:10.
+
+    ``gpu_id`` is 1-based like MATLAB's ``gpuDevice(gpu_id)`` (recon:76).
+    Returns Xguess as a float64 [I1, I2, Z] array (recon:147 gathers a double).
+    """
+    lib = load_library()
+    lf = _col(LFIMG, np.float32)
+    Hc = _col(H, np.float32)
+    Htc = _col(Ht, np.float32)
+    if lf.ndim != 2 or Hc.ndim != 5 or Hc.shape != Htc.shape:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "LFIMG must be 2-D and H, Ht [S,S,N,N,Z]")
+    Z = Hc.shape[4]
+    out = np.empty((lf.shape[0], lf.shape[1], Z), np.float64, order="F")
+    st = (C.c_byte * 2048)()
+    _check(lib.mlf_recon_phase_space(_fptr(lf), lf.shape[0], lf.shape[1], _fptr(Hc), _fptr(Htc),
+                                     Hc.shape[0], Hc.shape[2], Z, int(maxIter), int(bool(gpuFlag)),
+                                     int(gpu_id) - 1, PRECISIONS[precision], _fptr(out, C.c_double),
+                                     C.cast(st, C.c_void_p)))
+    if stats is not None:
+        raw = bytes(st)
+        it, vpi = np.frombuffer(raw[:8], np.int32)
+        stats["iterations"] = int(it)
+        stats["views_per_iteration"] = int(vpi)
+        stats["total_seconds"] = float(np.frombuffer(raw[8:16], np.float64)[0])
+        stats["iteration_seconds"] = np.frombuffer(raw[16:16 + 8 * 64], np.float64)[:it].tolist()
+    return out
+
+
+def mex_lfm_convolution_vec(bitmap, kernels, device: int = 0):
+    """Drop-in for the MEX gateway (mex:276-421), host arrays in and out.
+
+    bitmap: single [W, H] or [W, H, n]; kernels: single [kw, kw, 15, 15].
+    Invalid input raises :class:`InvalidInput` whose ``identifier`` is the
+    reference's error id.
+    """
+    lib = load_library()
+    b = np.asarray(bitmap)
+    k = np.asarray(kernels)
+    if b.ndim not in (2, 3) or k.ndim != 4:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, MEX_ERROR_MSG)
+    if b.dtype != np.float32 or k.dtype != np.float32:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, MEX_ERROR_MSG)
+    if k.shape[0] != k.shape[1] or k.shape[2] != 15 or k.shape[3] != 15:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, MEX_ERROR_MSG)
+    b = np.asfortranarray(b)
+    k = np.asfortranarray(k)
+    W, Hh = b.shape[0], b.shape[1]
+    n = b.shape[2] if b.ndim == 3 else 1
+    out = np.empty(b.shape, np.float32, order="F")
+    _check(lib.mlf_lfm_convolution_vec(_fptr(b), W, Hh, n, _fptr(k), k.shape[0], 15, _fptr(out),
+                                       int(device)))
+    return out

@@ -1,0 +1,203 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Tolerances (relative L2, FP32 device arithmetic vs FP64 oracle):
+  forward projections  <= 1e-5
+  volumes              <= 1e-4 (after the stated iterations)
+  tables / view order  bit-exact
+"""
+import numpy as np
+import pytest
+
+import mesolf_amd
+from mesolf_amd import synth, tables
+from oracle import mesolf_oracle as O
+from tests.helpers import oracle_compact_from_abi, rel_l2, synthetic_problem
+
+pytestmark = pytest.mark.gpu
+
+FWD_TOL = 1e-5
+VOL_TOL = 1e-4
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = mesolf_amd.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def small_problem():
+    return synthetic_problem(synth.SMALL, seed=3)
+
+
+def _psf(ctx, tabs):
+    return mesolf_amd.PSF.from_compact(ctx, tabs["uv"], tabs["weights"], tabs["rf"], tabs["fwd"])
+
+
+@pytest.mark.parametrize("wl", [synth.TINY, synth.SMALL], ids=lambda w: w.name)
+def test_forward_projection_parity(ctx, wl):
+    tabs = tables.synthetic_compact(wl)
+    cp = oracle_compact_from_abi(wl, tabs)
+    psf = _psf(ctx, tabs)
+    assert psf.nviews == 177
+    p = mesolf_amd.Patch(psf, wl.n1, wl.n2)
+    rng = np.random.default_rng(11)
+    X = rng.random((wl.I1, wl.I2, wl.Z)).astype(np.float32)
+    p.set_volume(X)
+    Xl = O.to_lenslet(X.astype(np.float64), 15)
+    for k in (0, 1, 57, 176):
+        y = p.forward_view(k)
+        ref = O.forward_compact(Xl, cp.fwd[k], cp.rf)
+        assert rel_l2(y, ref) <= FWD_TOL, (k, rel_l2(y, ref))
+
+
+def test_volume_roundtrip_layout(ctx):
+    wl = synth.TINY
+    psf = _psf(ctx, tables.synthetic_compact(wl))
+    p = mesolf_amd.Patch(psf, wl.n1, wl.n2)
+    X = np.random.default_rng(2).random((wl.I1, wl.I2, wl.Z)).astype(np.float32)
+    p.set_volume(X)
+    assert np.array_equal(p.get_volume(), X)
+    assert np.array_equal(p.get_volume(f64=True), X.astype(np.float64))
+
+
+def test_view_steps_parity(ctx, small_problem):
+    tabs, cp, truth, LF = small_problem
+    wl = synth.SMALL
+    p = mesolf_amd.Patch(_psf(ctx, tabs), wl.n1, wl.n2)
+    p.set_image(LF)
+    p.init_volume()
+    p.run(1, views_limit=5, filter=False)
+    got = p.get_volume(f64=True)
+    ref = O.rl_compact(LF.astype(np.float64), cp, 1, views_limit=5, filter_planes=False)
+    assert rel_l2(got, ref) <= VOL_TOL
+
+
+def test_sample_filter_parity(ctx):
+    wl = synth.SMALL
+    p = mesolf_amd.Patch(_psf(ctx, tables.synthetic_compact(wl)), wl.n1, wl.n2)
+    X = np.random.default_rng(4).random((wl.I1, wl.I2, wl.Z)).astype(np.float32)
+    p.set_volume(X)
+    p.sample_filter()
+    got = p.get_volume(f64=True)
+    bank = O.filter_bank(wl.I1, wl.I2, wl.Z, 15)
+    ref = O.sample_filter_literal(X.astype(np.float64), bank)
+    assert rel_l2(got, ref) <= 1e-6
+    # energy of every plane preserved
+    assert np.allclose(got.sum(axis=(0, 1)), X.astype(np.float64).sum(axis=(0, 1)), rtol=1e-6)
+
+
+def test_rl_parity_small_two_iterations(ctx, small_problem):
+    tabs, cp, truth, LF = small_problem
+    wl = synth.SMALL
+    p = mesolf_amd.Patch(_psf(ctx, tabs), wl.n1, wl.n2)
+    p.set_image(LF)
+    p.init_volume()
+    p.run(2)
+    got = p.get_volume(f64=True)
+    ref = O.rl_compact(LF.astype(np.float64), cp, 2)
+    assert rel_l2(got, ref) <= VOL_TOL
+
+
+def test_determinism_bitwise(ctx, small_problem):
+    tabs, cp, truth, LF = small_problem
+    wl = synth.SMALL
+    psf = _psf(ctx, tabs)
+    outs = []
+    for _ in range(2):
+        p = mesolf_amd.Patch(psf, wl.n1, wl.n2)
+        p.set_image(LF)
+        p.init_volume()
+        p.run(1, views_limit=40)
+        outs.append(p.get_volume())
+    assert np.array_equal(outs[0], outs[1])
+
+
+def test_project_all_parity(ctx):
+    wl = synth.TINY
+    tabs = tables.synthetic_compact(wl)
+    cp = oracle_compact_from_abi(wl, tabs)
+    p = mesolf_amd.Patch(_psf(ctx, tabs), wl.n1, wl.n2)
+    truth = synth.sphere_truth(wl, np.random.default_rng(5)).astype(np.float32)
+    p.set_volume(truth)
+    got = p.project_all()
+    Xl = O.to_lenslet(truth.astype(np.float64), 15)
+    ref = np.zeros((wl.I1, wl.I2))
+    for k, (u, v) in enumerate(cp.views):
+        ref[u::15, v::15] = O.forward_compact(Xl, cp.fwd[k], cp.rf)
+    assert rel_l2(got, ref) <= FWD_TOL
+
+
+def test_dense_psf_preparation(ctx):
+    """PSF.from_dense reproduces the oracle's view order, weights and radii."""
+    wl = synth.TINY
+    Hps = synth.dense_phase_space(wl)
+    H = O.inverse_phasespace(Hps)
+    Ht = synth.adjoint_ht(Hps)
+    psf = mesolf_amd.PSF.from_dense(ctx, H, Ht)
+    cp = O.compact_from_dense(H, Ht)
+    views, w = psf.views()
+    assert views == cp.views                     # bit-exact order and mask
+    assert np.allclose(w, cp.weights, rtol=1e-12)
+    rf, rb = psf.radii()
+    assert list(rf) == cp.rf and list(rb) == cp.rb
+
+
+def test_recon_dropin_dense_matches_oracle(ctx):
+    wl = synth.Workload("dense", 4, 5, (-4, 0, 4), 2, 2)
+    Hps = synth.dense_phase_space(wl)
+    H = O.inverse_phasespace(Hps)
+    Ht = synth.adjoint_ht(Hps)
+    rng = np.random.default_rng(8)
+    LF = rng.poisson(80.0, (wl.I1, wl.I2)).astype(np.float32)
+    st = {}
+    got = mesolf_amd.recon_theory_phase_space_LFM_with_filter(LF, H, Ht, 2, True, 1, stats=st)
+    assert got.dtype == np.float64 and got.shape == (wl.I1, wl.I2, wl.Z)
+    cp = O.compact_from_dense(H, Ht)
+    ref = O.rl_compact(LF.astype(np.float64), cp, 2)
+    assert rel_l2(got, ref) <= VOL_TOL
+    assert st["iterations"] == 2 and st["views_per_iteration"] == 177
+
+
+@pytest.mark.parametrize("W,H,n,kw", [(40, 33, 1, 7), (64, 70, 3, 31), (50, 45, 2, 8),
+                                      (30, 20, 1, 101)])
+def test_mex_operator_parity(W, H, n, kw):
+    rng = np.random.default_rng(W + kw)
+    b = rng.random((W, H, n)).astype(np.float32)
+    K = rng.random((kw, kw, 15, 15)).astype(np.float32)
+    got = mesolf_amd.mex_lfm_convolution_vec(b if n > 1 else b[:, :, 0], K)
+    ref = O.mex_op(b.astype(np.float64), K.astype(np.float64))
+    if n == 1:
+        ref = ref[:, :, 0]
+    assert got.shape == (b.shape if n > 1 else b.shape[:2])
+    assert rel_l2(got, ref) <= 1e-5
+
+
+def test_mex_operator_wide_kernel_is_a_shift():
+    """kw = 401 (> the reference's 341 limit): a single-tap kernel shifts."""
+    kw, h = 401, 200
+    rng = np.random.default_rng(12)
+    b = rng.random((300, 260)).astype(np.float32)
+    K = np.zeros((kw, kw, 15, 15), np.float32)
+    K[h + 3, h - 2] = 1.0
+    got = mesolf_amd.mex_lfm_convolution_vec(b, K)
+    exp = np.zeros_like(b)
+    exp[:-3, 2:] = b[3:, :-2]          # out[x, y] = in[x + 3, y - 2]
+    assert np.array_equal(got, exp)
+
+
+@pytest.mark.slow
+def test_rl_parity_cfg1_eight_iterations(ctx):
+    """cfg1 shapes (33x33 lenslets, 11 planes), 8 RL iterations."""
+    wl = synth.CFG1
+    tabs, cp, truth, LF = synthetic_problem(wl, seed=1)
+    p = mesolf_amd.Patch(_psf(ctx, tabs), wl.n1, wl.n2)
+    p.set_image(LF)
+    p.init_volume()
+    p.run(8)
+    got = p.get_volume(f64=True)
+    ref = O.rl_compact(LF.astype(np.float64), cp, 8)
+    err = rel_l2(got, ref)
+    print(f"cfg1 8-iteration FP32 rel L2 vs FP64 oracle: {err:.3e}")
+    assert err <= VOL_TOL
