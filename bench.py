@@ -1,0 +1,212 @@
+#!/usr/bin/env python3
+"""Throughput benchmark: deconvolved volumes/s of phase-space RL on one GPU.
+
+Workload (default): 101 x 101 lenslets (1515 x 1515 px), 51 planes
+(-100..100 um step 4), 177 active views, 8 RL iterations per volume, synthetic
+PSF / 400-sphere truth / Poisson measurement at 1000 counts peak (seeded).
+One "step" = one full volume: init (recon:72-73) + 8 iterations of 177 view
+steps + sample_filter.  Inputs (PSF tables, image) are resident in HBM
+before the timed region starts.
+
+Prints one JSON line.  ``roofline`` describes the dominant kernel, timed
+live with HIP events on the launch stream; ``cpu_baseline`` is the FP64
+numpy oracle timed on a bounded sample of the same workload on this host.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK = {"fp32": 157.3e12, "bf16": 2.5e15, "fp16": 2.5e15}  # dense MFMA peaks, flop/s
+HBM_PEAK = 8.0e12
+
+
+def algorithmic_flops_per_view(wl) -> float:
+    """2 * n1 * n2 * sum_z K_z^2 with K_z = 30 r_z + 1 px (one projection
+    direction, one view)."""
+    return 2.0 * wl.n1 * wl.n2 * sum((30 * r + 1) ** 2 for r in wl.radii)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(wl, tabs, LF, nviews_sample: int):
+    """FP64 numpy oracle (BLAS threads = host cores) on `nviews_sample` view
+    steps of the workload plus one filtered plane, extrapolated to a volume."""
+    from oracle import mesolf_oracle as O
+    from tests.helpers import oracle_compact_from_abi
+
+    sub = dict(tabs)
+    per_view = sum((2 * r + 1) ** 2 * 225 for r in wl.radii)
+    sub["uv"] = tabs["uv"][:nviews_sample]
+    sub["weights"] = tabs["weights"][:nviews_sample]
+    sub["fwd"] = tabs["fwd"][:per_view * nviews_sample]
+    cp = oracle_compact_from_abi(wl, sub)
+    X = O.initial_volume(LF.astype(np.float64), wl.Z)
+    Xl = O.to_lenslet(X, 15)
+    ones = np.ones((wl.n1, wl.n2))
+    t0 = time.perf_counter()
+    for k in range(nviews_sample):
+        u, v = cp.views[k]
+        y = O.forward_compact(Xl, cp.fwd[k], cp.rf)
+        R = O.ratio_update(LF[u::15, v::15].astype(np.float64), y)
+        B = O.backward_compact(R, cp.bwd[k], cp.rb)
+        B1 = O.backward_compact(ones, cp.bwd[k], cp.rb)
+        Xl = O.multiplicative_update(Xl, B, B1, cp.weights[k])
+    t_view = (time.perf_counter() - t0) / nviews_sample
+    # one plane of sample_filter (FFT form), scaled by the plane count
+    om = O.omega_array(wl.Z, 15)
+    k1 = np.fft.fft(O.filter_kernel_1d(wl.I1, om[0]))
+    k2 = np.fft.fft(O.filter_kernel_1d(wl.I2, om[0]))
+    img = O.from_lenslet(Xl, 15)[:, :, 0]
+    t1 = time.perf_counter()
+    buf = np.real(np.fft.ifft2(np.fft.fft2(img) * (k1[:, None] * k2[None, :])))
+    _ = img.sum() * buf / buf.sum()
+    t_plane = time.perf_counter() - t1
+    t_volume = wl.iters * (len(tabs["uv"]) * t_view + wl.Z * t_plane)
+    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+    return {
+        "value": 1.0 / t_volume,
+        "unit": "volumes/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"FP64 numpy oracle (BLAS, {cores} threads): {nviews_sample} view steps "
+                   f"({t_view:.2f} s each) + 1 filtered plane ({t_plane:.2f} s) of the same "
+                   f"workload, extrapolated x{len(tabs['uv'])} views x{wl.iters} iterations "
+                   f"(+{wl.Z} planes/iteration): {t_volume:.0f} s per volume"),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2"])
+    ap.add_argument("--views-limit", type=int, default=0,
+                    help="profiling only: visit the first N views (result flagged partial)")
+    ap.add_argument("--iters", type=int, default=0, help="profiling only: RL iterations per volume")
+    ap.add_argument("--cpu-views", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
+                          "(sharding across GPUs is described in DESIGN.md, not implemented)"}))
+        sys.exit(2)
+
+    import mesolf_amd
+    from mesolf_amd import synth, tables
+
+    wl = {"cfg1": synth.CFG1, "cfg2": synth.CFG2}[args.workload]
+    iters = args.iters or wl.iters
+    partial = bool(args.views_limit) or iters != wl.iters
+
+    t = time.perf_counter()
+    tabs = tables.synthetic_compact(wl)
+    log(f"PSF tables {tabs['fwd'].nbytes / 1e9:.2f} GB in {time.perf_counter() - t:.1f}s")
+    rng = np.random.default_rng(wl.seed)
+    truth = synth.sphere_truth(wl, rng)
+
+    ctx = mesolf_amd.Context(0)
+    psf = mesolf_amd.PSF.from_compact(ctx, tabs["uv"], tabs["weights"], tabs["rf"], tabs["fwd"])
+    patch = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision="fp32")
+    patch.set_volume(truth)
+    t = time.perf_counter()
+    clean = patch.project_all()
+    log(f"measurement: forward of 177 views in {time.perf_counter() - t:.1f}s")
+    LF = synth.poisson_measurement(clean, rng)
+    del truth, clean
+
+    work = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision=args.precision) \
+        if args.precision != "fp32" else patch
+    work.set_image(LF)
+
+    def volume():
+        work.init_volume()
+        work.run(iters, views_limit=args.views_limit, filter=True)
+
+    for w in range(args.warmup):
+        volume()
+        ctx.synchronize()
+        log(f"warmup {w + 1}/{args.warmup} done")
+
+    work.set_timing(True)
+    work.kernel_times()  # reset
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    for s in range(args.steps):
+        volume()
+        ctx.synchronize()
+        log(f"step {s + 1}/{args.steps} done")
+    elapsed = time.perf_counter() - t0
+    ktimes = work.kernel_times()
+    work.set_timing(False)
+
+    nviews = args.views_limit or psf.nviews
+    vol_per_s = args.steps / elapsed
+    flops_view = algorithmic_flops_per_view(wl)
+    dominant = max(ktimes, key=lambda c: ktimes[c][0])
+    dom_ms, dom_n = ktimes[dominant]
+    avg_s = dom_ms / max(dom_n, 1) * 1e-3
+    if dominant in ("forward", "backward_update"):
+        achieved = flops_view / avg_s
+        roof = {"bound": "mfma", "achieved": achieved / 1e12, "peak": PEAK[args.precision] / 1e12,
+                "unit": "TFLOP/s", "frac": achieved / PEAK[args.precision], "traffic": None,
+                "kernel": dominant, "avg_launch_ms": avg_s * 1e3,
+                "algorithmic_per_launch": f"{flops_view:.4e} flop (2*n1*n2*sum_z(30r_z+1)^2)"}
+    else:
+        nbytes = 4.0 * wl.I1 * wl.I2 * wl.Z * 4
+        achieved = nbytes / avg_s
+        roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dominant,
+                "avg_launch_ms": avg_s * 1e3}
+    per_kernel = {c: {"total_ms": round(ms, 3), "launches": n,
+                      "avg_ms": round(ms / n, 4) if n else None} for c, (ms, n) in ktimes.items()}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(wl, tabs, LF, args.cpu_views)
+
+    out = {
+        "metric": "deconvolved volumes/s, 101x101-lenslet x 51-plane patch, 8 RL iterations",
+        "value": vol_per_s,
+        "unit": "volumes/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": {"fp32": "f32", "bf16": "bf16 (f32 accumulate, f32 update)",
+                  "fp16": "f16 (f32 accumulate, f32 update)"}[args.precision],
+        "data": "synthetic (seeded PSF family, sphere truth, Poisson measurement)",
+        "config": {"workload": wl.name, "lenslets": [wl.n1, wl.n2], "planes": wl.Z,
+                   "views": nviews, "rl_iterations": iters, "precision": args.precision,
+                   "partial_profile_run": partial},
+        "roofline": roof,
+        "kernels": per_kernel,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    patch.close()
+    if work is not patch:
+        work.close()
+    psf.close()
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -126,6 +126,14 @@ mlf_status mlf_patch_project_all(mlf_patch p, float* sensor);
 mlf_status mlf_patch_sample_filter(mlf_patch p);
 /* Time of the last mlf_patch_run on the device, milliseconds (HIP events). */
 mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms);
+/* Per-kernel timing: while enabled, every launch of mlf_patch_run is
+ * bracketed by HIP events on the context stream.  mlf_patch_kernel_times
+ * returns, per kernel class, the summed milliseconds and the launch count
+ * since the previous read, then resets them.  Classes (MLF_KERNEL_CLASSES):
+ * 0 forward projection, 1 ratio, 2 backprojection + update, 3 sample_filter. */
+#define MLF_KERNEL_CLASSES 4
+mlf_status mlf_patch_set_timing(mlf_patch p, int32_t enable);
+mlf_status mlf_patch_kernel_times(mlf_patch p, double* total_ms, int64_t* launches);
 
 /* ------------------------------------------------------------ drop-ins */
 /* recon_theory_phase_space_LFM_with_filter(LFIMG,H,Ht,maxIter,gpuFlag,gpu_id)

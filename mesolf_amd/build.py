@@ -27,7 +27,7 @@ COMMON = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["rl_kernels.hip", "mex_kernels.hip", "runtime.cpp"]
+SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "runtime.cpp"]
 
 
 def _compile(src: str) -> str:

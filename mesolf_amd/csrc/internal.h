@@ -74,6 +74,8 @@ hipError_t launch_to_blocked(const float* src, float* dst, int n1, int n2, int Z
 hipError_t launch_from_blocked_f32(const float* src, float* dst, int n1, int n2, int Z, hipStream_t st);
 hipError_t launch_from_blocked_f64(const float* src, double* dst, int n1, int n2, int Z, hipStream_t st);
 hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t st);
+hipError_t launch_init_volume(const float* LF, size_t nlf, double* sum, float* X, size_t nx,
+                              double inv_numel, hipStream_t st);
 
 // Per-plane geometry of the compact tables of one view.
 struct PlaneTables {

@@ -54,6 +54,31 @@ __global__ void k_fill(float* dst, size_t n, float v) {
     dst[o] = v;
 }
 
+// Single-block FP64 sum of an image (fixed reduction order).
+__global__ void k_image_sum(const float* __restrict__ img, size_t n, double* __restrict__ out) {
+  __shared__ double sh[1024 / 64];
+  double acc = 0.0;
+  for (size_t o = threadIdx.x; o < n; o += blockDim.x) acc += (double)img[o];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+  if (lane == 0) sh[wid] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) s += sh[k];
+    *out = s;
+  }
+}
+
+// X = (1/numel) * sum(LFIMG) everywhere (recon:72-73).
+__global__ void k_fill_from_sum(float* __restrict__ dst, size_t n, const double* __restrict__ sum,
+                                double inv_numel) {
+  const float v = (float)(inv_numel * (*sum));
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n;
+       o += (size_t)gridDim.x * blockDim.x)
+    dst[o] = v;
+}
+
 // One thread per (output lenslet i, plane z):
 //   part[z][i] = sum_{d,phase} X[z][i-d][phase] * W[z][d][phase].
 __global__ void k_forward_fp32(const float* __restrict__ X, const float* __restrict__ W,
@@ -91,10 +116,12 @@ __global__ void k_ratio(const float* __restrict__ part, int Z, int n1, int n2,
   for (int z = 0; z < Z; ++z) y += (double)part[(size_t)z * n + i];
   const float yf = (float)y;
   const int i1 = i % n1, i2 = i / n1;
-  const float lf = LF[(size_t)(u + kN * i1) + (size_t)(kN * n1) * (v + kN * i2)];
-  float q = lf / yf;
-  if (!isfinite(q)) q = 0.f;
-  R[i] = q;
+  if (LF) {
+    const float lf = LF[(size_t)(u + kN * i1) + (size_t)(kN * n1) * (v + kN * i2)];
+    float q = lf / yf;
+    if (!isfinite(q)) q = 0.f;
+    R[i] = q;
+  }
   if (y_out) y_out[i] = yf;
 }
 
@@ -272,6 +299,14 @@ hipError_t launch_from_blocked_f64(const float* src, double* dst, int n1, int n2
   const size_t n = (size_t)n1 * n2 * Z * kP;
   hipLaunchKernelGGL(k_from_blocked<double>, dim3(grid_for(n, 256)), dim3(256), 0, st, src, dst,
                      n1, n2, Z);
+  return hipGetLastError();
+}
+
+hipError_t launch_init_volume(const float* LF, size_t nlf, double* sum, float* X, size_t nx,
+                              double inv_numel, hipStream_t st) {
+  hipLaunchKernelGGL(k_image_sum, dim3(1), dim3(1024), 0, st, LF, nlf, sum);
+  hipLaunchKernelGGL(k_fill_from_sum, dim3(grid_for(nx, 256)), dim3(256), 0, st, X, nx, sum,
+                     inv_numel);
   return hipGetLastError();
 }
 

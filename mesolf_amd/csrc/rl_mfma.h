@@ -1,0 +1,95 @@
+// Geometry descriptors and launchers of the MFMA view-step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mlf {
+
+enum class Precision { kF32 = 0, kBF16 = 1 };
+
+constexpr int kPhiPadF32 = 228;   // 225 phases padded to the f32 MFMA K-step (4)
+constexpr int kPhiPadBF16 = 256;  // 225 phases padded to the bf16 MFMA K-step (32)
+constexpr int kFwdChunk = 64;     // taps per forward B chunk (4 N-tiles)
+constexpr int kFwdMTF32 = 2;      // M-tiles per wave, forward, f32 (block 16 x 8 lenslets)
+constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x 16 lenslets)
+constexpr int kBwdNTiles = 5;     // N-tiles per backward workgroup (80 phases)
+constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases
+constexpr int kBwdTapPad = 32;    // backward taps padded to a multiple of 32
+
+inline int fwd_bm2(Precision p) { return 4 * (p == Precision::kF32 ? kFwdMTF32 : kFwdMTBF16); }
+inline int fwd_phi(Precision p) { return p == Precision::kF32 ? kPhiPadF32 : kPhiPadBF16; }
+inline size_t elem_bytes(Precision p) { return p == Precision::kF32 ? 4 : 2; }
+
+struct FwdGeo {
+  int n1, n2;
+  const int32_t* r;       // [Z] forward radius
+  const int64_t* woff;    // [Z] element offset of the plane block in the packed view
+  const int32_t* nchunk;  // [Z] number of 64-tap chunks
+  const int64_t* poff;    // [Z] offset of the plane's partial windows
+  const int32_t* sched;   // [nblocks][3] = (z, b1, b2)
+};
+
+struct BwdGeo {
+  int n1, n2;
+  const int32_t* r;       // [Z] backward radius
+  const int64_t* boff;    // [Z] element offset of the plane block in the packed view
+  const int32_t* epad;    // [Z] padded tap count (multiple of kBwdTapPad)
+  const int64_t* toffT;   // [Z] offset of the plane's normaliser table in the view
+  const int32_t* C1;      // [Z] normaliser classes along axis 0
+  const int32_t* sched;   // [nblocks][4] = (z, b1, b2, phase group)
+};
+
+struct PackGeo {
+  const int32_t* r;        // [Z]
+  const int32_t* rows;     // [Z] padded row count of the destination plane block
+  const int64_t* src_off;  // [Z] offset in the ABI view block
+  const int64_t* dst_off;  // [Z] offset in the packed view block
+  int64_t src_view, dst_view;
+};
+
+struct NormGeo {
+  int n1, n2;
+  const int32_t* r;
+  const int32_t* C1;
+  const int32_t* C2;
+  const int64_t* src_off;  // [Z] plane offset in the ABI (backward) view block
+  const int64_t* s_off;    // [Z] FP64 scratch offsets
+  const int64_t* t_off;    // [Z] normaliser table offsets within the view
+};
+
+// Normaliser classes along one axis of an n-lenslet grid for a plane of
+// radius r: the backprojection of ones at lenslet a sums taps d in
+// [max(-r, a-n+1), min(r, a)]; for n >= 2r+1 those ranges take 2r+1 values.
+__host__ __device__ inline int norm_classes(int n, int r) { return n < 2 * r + 1 ? n : 2 * r + 1; }
+
+__host__ __device__ inline void norm_range(int c, int n, int r, int* lo, int* hi) {
+  if (n < 2 * r + 1) {
+    *lo = (c - n + 1 > -r) ? c - n + 1 : -r;
+    *hi = c < r ? c : r;
+  } else {
+    *lo = (c <= r) ? -r : c - 2 * r;
+    *hi = (c < r) ? c : r;
+  }
+}
+
+size_t fwd_smem_bytes(Precision p, int rmax, int bm2);
+size_t bwd_smem_bytes(Precision p, int rmax, int epad_max);
+
+hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
+                           size_t smem, float* partial, hipStream_t st);
+hipError_t launch_ratio_gather(const float* partial, FwdGeo g, int Z, int bm2, const float* LF,
+                               int u, int v, float* R, float* y_out, hipStream_t st);
+hipError_t launch_bwd_mfma(Precision p, float* X, const void* Wb, const float* Tn, BwdGeo g,
+                           int nblocks, size_t smem, const float* R, float w, hipStream_t st);
+hipError_t launch_pack_fwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
+                           hipStream_t st);
+hipError_t launch_pack_bwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
+                           hipStream_t st);
+hipError_t launch_norm_table(Precision p, const float* src_view, double* S, float* Tn_view,
+                             NormGeo ng, int Z, hipStream_t st);
+hipError_t mfma_kernel_attributes();
+
+}  // namespace mlf

@@ -8,6 +8,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <mutex>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "internal.h"
+#include "rl_mfma.h"
 
 namespace mlf {
 
@@ -185,7 +187,22 @@ struct mlf_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
   std::mutex mu;
+  int refs = 0;         // live PSFs created on this context
+  bool closed = false;  // destroy requested; freed when refs reaches 0
 };
+
+// Handle lifetimes: a PSF keeps its context alive and a patch keeps its PSF
+// alive, so destroying a parent first only defers its release.
+static std::mutex g_life_mu;
+
+static void ctx_free(mlf_context ctx) {
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) {
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipStreamDestroy(ctx->stream);
+  }
+  delete ctx;
+}
 
 static mlf_status ctx_activate(mlf_context ctx) {
   MLF_HIP(hipSetDevice(ctx->device));
@@ -232,12 +249,10 @@ mlf_status mlf_context_create(int32_t device, mlf_context* out) {
 mlf_status mlf_context_destroy(mlf_context ctx) {
   clear_error();
   if (!ctx) return MLF_OK;
-  (void)hipSetDevice(ctx->device);
-  if (ctx->stream) {
-    (void)hipStreamSynchronize(ctx->stream);
-    (void)hipStreamDestroy(ctx->stream);
-  }
-  delete ctx;
+  std::lock_guard<std::mutex> lk(g_life_mu);
+  if (ctx->closed) return set_error(MLF_ERR_INVALID_ARGUMENT, "context destroyed twice");
+  ctx->closed = true;
+  if (ctx->refs == 0) ctx_free(ctx);
   return MLF_OK;
 }
 
@@ -263,6 +278,29 @@ mlf_status mlf_context_memory(mlf_context ctx, uint64_t* free_bytes, uint64_t* t
 }  // extern "C"
 
 // =================================================================== PSF
+// PSF tables re-laid out for the MFMA kernels of one precision.
+struct PrecPack {
+  bool built = false;
+  DevBuf<uint8_t> fwd, bwd;           // [V][z][e_pad64][PHI], [V][z][240][e_pad32]
+  int64_t pvf = 0, pvb = 0;           // elements per view
+  std::vector<int64_t> woff, boff;    // per-plane element offsets inside a view
+  std::vector<int32_t> nchunk, epad;
+  DevBuf<int64_t> d_woff, d_boff;
+  DevBuf<int32_t> d_nchunk, d_epad;
+};
+
+// Backprojection-of-ones table for one grid-size class (FP64 sums of the taps
+// as the kernels see them, rounded once to FP32).
+struct NormTable {
+  int key1 = -1, key2 = -1;  // -1: grid at least 2*rb_max+1 lenslets on that axis
+  DevBuf<float> T;
+  int64_t pvt = 0;
+  std::vector<int64_t> toff;
+  std::vector<int32_t> C1, C2;
+  DevBuf<int64_t> d_toff;
+  DevBuf<int32_t> d_C1;
+};
+
 struct mlf_psf_s {
   mlf_context ctx = nullptr;
   int Z = 0, V = 0;
@@ -271,12 +309,135 @@ struct mlf_psf_s {
   std::vector<int32_t> rf, rb;   // [Z]
   std::vector<int64_t> offf, offb;
   int64_t per_view_f = 0, per_view_b = 0;
-  DevBuf<float> fwd, bwd;
+  DevBuf<float> fwd, bwd;        // ABI layout, FP32
   DevBuf<int32_t> d_rf, d_rb;
   DevBuf<int64_t> d_offf, d_offb;
+  PrecPack pack[2];
+  std::vector<std::unique_ptr<NormTable>> norms[2];
+  std::mutex mu;
+  int refs = 0;         // live patches
+  bool closed = false;
 };
 
+// Called with g_life_mu held.
+static void psf_free(mlf_psf psf) {
+  mlf_context ctx = psf->ctx;
+  (void)hipSetDevice(ctx->device);
+  delete psf;
+  if (--ctx->refs == 0 && ctx->closed) ctx_free(ctx);
+}
+
 static int64_t plane_block(int r) { return (int64_t)(2 * r + 1) * (2 * r + 1) * kP; }
+
+template <typename T>
+static mlf_status upload_vec(DevBuf<T>& d, const std::vector<T>& h) {
+  MLF_CHECK(d.alloc(h.size()));
+  if (!h.empty()) MLF_HIP(hipMemcpy(d.ptr, h.data(), d.bytes(), hipMemcpyHostToDevice));
+  return MLF_OK;
+}
+
+// Build the packed forward/backward tables of one precision (once per PSF).
+static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
+  PrecPack& pk = ps->pack[(int)prec];
+  if (pk.built) return MLF_OK;
+  const int Z = ps->Z, V = ps->V;
+  const int phi = fwd_phi(prec);
+  pk.woff.assign(Z, 0);
+  pk.boff.assign(Z, 0);
+  pk.nchunk.assign(Z, 0);
+  pk.epad.assign(Z, 0);
+  std::vector<int32_t> rows_f(Z), rows_b(Z);
+  int64_t of = 0, ob = 0;
+  for (int z = 0; z < Z; ++z) {
+    const int nt_f = (2 * ps->rf[z] + 1) * (2 * ps->rf[z] + 1);
+    const int nt_b = (2 * ps->rb[z] + 1) * (2 * ps->rb[z] + 1);
+    pk.nchunk[z] = (nt_f + kFwdChunk - 1) / kFwdChunk;
+    rows_f[z] = pk.nchunk[z] * kFwdChunk;
+    pk.epad[z] = (nt_b + kBwdTapPad - 1) / kBwdTapPad * kBwdTapPad;
+    rows_b[z] = pk.epad[z];
+    pk.woff[z] = of;
+    of += (int64_t)rows_f[z] * phi;
+    pk.boff[z] = ob;
+    ob += (int64_t)kPhiPadBwd * pk.epad[z];
+  }
+  pk.pvf = of;
+  pk.pvb = ob;
+  const size_t es = elem_bytes(prec);
+  MLF_CHECK(pk.fwd.alloc((size_t)of * V * es));
+  MLF_CHECK(pk.bwd.alloc((size_t)ob * V * es));
+  MLF_CHECK(upload_vec(pk.d_woff, pk.woff));
+  MLF_CHECK(upload_vec(pk.d_boff, pk.boff));
+  MLF_CHECK(upload_vec(pk.d_nchunk, pk.nchunk));
+  MLF_CHECK(upload_vec(pk.d_epad, pk.epad));
+  DevBuf<int32_t> d_rows_f, d_rows_b;
+  MLF_CHECK(upload_vec(d_rows_f, rows_f));
+  MLF_CHECK(upload_vec(d_rows_b, rows_b));
+  hipStream_t st = ps->ctx->stream;
+  PackGeo gf{ps->d_rf.ptr, d_rows_f.ptr, ps->d_offf.ptr, pk.d_woff.ptr, ps->per_view_f, pk.pvf};
+  hipError_t e = launch_pack_fwd(prec, ps->fwd.ptr, pk.fwd.ptr, gf, Z, V, st);
+  if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "pack_fwd: %s", hipGetErrorString(e));
+  PackGeo gb{ps->d_rb.ptr, d_rows_b.ptr, ps->d_offb.ptr, pk.d_boff.ptr, ps->per_view_b, pk.pvb};
+  e = launch_pack_bwd(prec, ps->bwd.ptr, pk.bwd.ptr, gb, Z, V, st);
+  if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "pack_bwd: %s", hipGetErrorString(e));
+  MLF_HIP(hipStreamSynchronize(st));
+  pk.built = true;
+  return MLF_OK;
+}
+
+// Normaliser table for a grid of n1 x n2 lenslets (shared by every grid that
+// is at least 2*rb_max+1 lenslets along an axis).
+static mlf_status psf_ensure_norm(mlf_psf_s* ps, Precision prec, int n1, int n2, NormTable** out) {
+  int rbmax = 0;
+  for (int r : ps->rb) rbmax = std::max(rbmax, r);
+  const int key1 = n1 >= 2 * rbmax + 1 ? -1 : n1;
+  const int key2 = n2 >= 2 * rbmax + 1 ? -1 : n2;
+  for (auto& t : ps->norms[(int)prec])
+    if (t->key1 == key1 && t->key2 == key2) {
+      *out = t.get();
+      return MLF_OK;
+    }
+  auto nt = std::make_unique<NormTable>();
+  nt->key1 = key1;
+  nt->key2 = key2;
+  const int Z = ps->Z;
+  // representative grid sizes: any size >= 2*rb_max+1 gives the same classes
+  const int g1 = key1 < 0 ? 2 * rbmax + 1 : n1, g2 = key2 < 0 ? 2 * rbmax + 1 : n2;
+  std::vector<int64_t> soff(Z), toff(Z);
+  int64_t so = 0, to = 0;
+  nt->C1.resize(Z);
+  nt->C2.resize(Z);
+  for (int z = 0; z < Z; ++z) {
+    const int r = ps->rb[z];
+    nt->C1[z] = norm_classes(g1, r);
+    nt->C2[z] = norm_classes(g2, r);
+    soff[z] = so;
+    so += (int64_t)(2 * r + 1) * nt->C1[z] * kP;
+    toff[z] = to;
+    to += (int64_t)nt->C2[z] * nt->C1[z] * kP;
+  }
+  nt->toff = toff;
+  nt->pvt = to;
+  MLF_CHECK(nt->T.alloc((size_t)to * ps->V));
+  MLF_CHECK(upload_vec(nt->d_toff, toff));
+  MLF_CHECK(upload_vec(nt->d_C1, nt->C1));
+  DevBuf<int32_t> d_C2;
+  DevBuf<int64_t> d_soff;
+  MLF_CHECK(upload_vec(d_C2, nt->C2));
+  MLF_CHECK(upload_vec(d_soff, soff));
+  DevBuf<double> S;
+  MLF_CHECK(S.alloc((size_t)so));
+  NormGeo ng{g1, g2, ps->d_rb.ptr, nt->d_C1.ptr, d_C2.ptr, ps->d_offb.ptr, d_soff.ptr, nt->d_toff.ptr};
+  hipStream_t st = ps->ctx->stream;
+  for (int v = 0; v < ps->V; ++v) {
+    hipError_t e = launch_norm_table(prec, ps->bwd.ptr + (size_t)v * ps->per_view_b, S.ptr,
+                                     nt->T.ptr + (size_t)v * to, ng, Z, st);
+    if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "norm table: %s", hipGetErrorString(e));
+  }
+  MLF_HIP(hipStreamSynchronize(st));
+  *out = nt.get();
+  ps->norms[(int)prec].push_back(std::move(nt));
+  return MLF_OK;
+}
 
 // Upload compact tables (host layout == device layout for the FP32 kernels).
 static mlf_status psf_upload(mlf_psf_s* p, const float* fwd, const float* bwd) {
@@ -356,6 +517,20 @@ mlf_status mlf_psf_create_compact(mlf_context ctx, int64_t N, int64_t Z, int64_t
   MLF_CHECK(ctx_activate(ctx));
   auto p = std::make_unique<mlf_psf_s>();
   p->ctx = ctx;
+  {
+    std::lock_guard<std::mutex> lk(g_life_mu);
+    if (ctx->closed) return set_error(MLF_ERR_INVALID_ARGUMENT, "context already destroyed");
+    ++ctx->refs;
+  }
+  struct RefUndo {  // drop the reference if creation fails below
+    mlf_psf_s* p;
+    ~RefUndo() {
+      if (p) {
+        std::lock_guard<std::mutex> lk(g_life_mu);
+        --p->ctx->refs;
+      }
+    }
+  } undo{p.get()};
   p->Z = (int)Z;
   p->V = (int)nviews;
   p->uv.assign(view_uv, view_uv + 2 * nviews);
@@ -371,6 +546,7 @@ mlf_status mlf_psf_create_compact(mlf_context ctx, int64_t N, int64_t Z, int64_t
     bwd = adj.data();
   }
   MLF_CHECK(psf_upload(p.get(), fwd, bwd));
+  undo.p = nullptr;
   *out = p.release();
   return MLF_OK;
 }
@@ -528,8 +704,10 @@ mlf_status mlf_psf_views(mlf_psf psf, int32_t* uv, double* weight) {
 mlf_status mlf_psf_destroy(mlf_psf psf) {
   clear_error();
   if (!psf) return MLF_OK;
-  (void)hipSetDevice(psf->ctx->device);
-  delete psf;
+  std::lock_guard<std::mutex> lk(g_life_mu);
+  if (psf->closed) return set_error(MLF_ERR_INVALID_ARGUMENT, "psf destroyed twice");
+  psf->closed = true;
+  if (psf->refs == 0) psf_free(psf);
   return MLF_OK;
 }
 
@@ -541,21 +719,60 @@ struct mlf_patch_s {
   int n1 = 0, n2 = 0, Z = 0;
   mlf_precision prec = MLF_PRECISION_FP32;
   DevBuf<float> X, LF, part, R, y, tmp;
-  DevBuf<double> pb, pa;
+  DevBuf<double> pb, pa, dsum;
   // filter plan
   int f_nplanes = 0, nblk = 0;
   DevBuf<int32_t> f_plane, f_n0_1, f_len_1, f_n0_2, f_len_2;
   DevBuf<int64_t> f_off_1, f_off_2;
   DevBuf<float> f_taps_1, f_taps_2;
   bool image_set = false, volume_set = false;
+  // MFMA path
+  bool reference = false;              // FP32 reference kernels instead of MFMA
+  Precision kp = Precision::kF32;
+  int bm2 = 8;
+  DevBuf<float> partial;               // forward window partials
+  DevBuf<int64_t> d_poff;
+  DevBuf<int32_t> d_fsched, d_bsched;
+  int nblk_f = 0, nblk_b = 0;
+  size_t smem_f = 0, smem_b = 0;
+  NormTable* norm = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  // per-kernel timing
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<int> ev_class;  // class of the pair (2k, 2k+1)
+  size_t ev_used = 0;
   std::mutex mu;
   ~mlf_patch_s() {
     if (ev0) (void)hipEventDestroy(ev0);
     if (ev1) (void)hipEventDestroy(ev1);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
   }
 };
+
+// Bracket a launch with a timing event pair (no-op unless timing is on).
+static mlf_status tim_begin(mlf_patch_s* p, int cls) {
+  if (!p->timing) return MLF_OK;
+  if (p->ev_used + 2 > p->ev_pool.size()) {
+    for (int k = 0; k < 2; ++k) {
+      hipEvent_t e;
+      MLF_HIP(hipEventCreate(&e));
+      p->ev_pool.push_back(e);
+    }
+    p->ev_class.push_back(0);
+  }
+  p->ev_class[p->ev_used / 2] = cls;
+  MLF_HIP(hipEventRecord(p->ev_pool[p->ev_used], p->psf->ctx->stream));
+  return MLF_OK;
+}
+
+static mlf_status tim_end(mlf_patch_s* p) {
+  if (!p->timing) return MLF_OK;
+  MLF_HIP(hipEventRecord(p->ev_pool[p->ev_used + 1], p->psf->ctx->stream));
+  p->ev_used += 2;
+  return MLF_OK;
+}
 
 static mlf_status build_filter_plan(mlf_patch_s* p) {
   const int Z = p->Z, I1 = p->n1 * kN, I2 = p->n2 * kN;
@@ -633,23 +850,133 @@ static FilterPlan filter_plan_of(mlf_patch_s* p) {
       return set_error(MLF_ERR_DEVICE, "launch %s failed: %s", #call, hipGetErrorString(e_)); \
   } while (0)
 
-static mlf_status patch_forward(mlf_patch_s* p, int k) {
+static FwdGeo fwd_geo(mlf_patch_s* p) {
   mlf_psf_s* ps = p->psf;
-  const PlaneTables tf{ps->d_rf.ptr, ps->d_offf.ptr};
-  MLF_LAUNCH(launch_forward_fp32(p->X.ptr, ps->fwd.ptr + (size_t)k * ps->per_view_f, tf, p->n1,
-                                 p->n2, p->Z, p->part.ptr, ps->ctx->stream));
+  const PrecPack& pk = ps->pack[(int)p->kp];
+  return FwdGeo{p->n1, p->n2, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, p->d_poff.ptr,
+                p->d_fsched.ptr};
+}
+
+static BwdGeo bwd_geo(mlf_patch_s* p) {
+  mlf_psf_s* ps = p->psf;
+  const PrecPack& pk = ps->pack[(int)p->kp];
+  return BwdGeo{p->n1, p->n2, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, p->norm->d_toff.ptr,
+                p->norm->d_C1.ptr, p->d_bsched.ptr};
+}
+
+// Forward projection of view k, then the ratio (LF != null) and/or y.
+static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, float* y_out) {
+  mlf_psf_s* ps = p->psf;
+  hipStream_t st = ps->ctx->stream;
+  const int u = ps->uv[2 * k], v = ps->uv[2 * k + 1];
+  if (p->reference) {
+    const PlaneTables tf{ps->d_rf.ptr, ps->d_offf.ptr};
+    MLF_CHECK(tim_begin(p, 0));
+    MLF_LAUNCH(launch_forward_fp32(p->X.ptr, ps->fwd.ptr + (size_t)k * ps->per_view_f, tf, p->n1,
+                                   p->n2, p->Z, p->part.ptr, st));
+    MLF_CHECK(tim_end(p));
+    MLF_CHECK(tim_begin(p, 1));
+    MLF_LAUNCH(launch_ratio(p->part.ptr, p->Z, p->n1, p->n2, LF, u, v, p->R.ptr, y_out, st));
+    MLF_CHECK(tim_end(p));
+    return MLF_OK;
+  }
+  const PrecPack& pk = ps->pack[(int)p->kp];
+  const FwdGeo g = fwd_geo(p);
+  MLF_CHECK(tim_begin(p, 0));
+  MLF_LAUNCH(launch_fwd_mfma(p->kp, p->X.ptr, pk.fwd.ptr + (size_t)k * pk.pvf * elem_bytes(p->kp), g,
+                             p->nblk_f, p->smem_f, p->partial.ptr, st));
+  MLF_CHECK(tim_end(p));
+  MLF_CHECK(tim_begin(p, 1));
+  MLF_LAUNCH(launch_ratio_gather(p->partial.ptr, g, p->Z, p->bm2, LF, u, v, p->R.ptr, y_out, st));
+  MLF_CHECK(tim_end(p));
   return MLF_OK;
+}
+
+static mlf_status patch_forward(mlf_patch_s* p, int k) {
+  return patch_forward_ratio(p, k, nullptr, p->y.ptr);
 }
 
 static mlf_status patch_view_step(mlf_patch_s* p, int k) {
   mlf_psf_s* ps = p->psf;
   hipStream_t st = ps->ctx->stream;
-  MLF_CHECK(patch_forward(p, k));
-  MLF_LAUNCH(launch_ratio(p->part.ptr, p->Z, p->n1, p->n2, p->LF.ptr, ps->uv[2 * k],
-                          ps->uv[2 * k + 1], p->R.ptr, nullptr, st));
-  const PlaneTables tb{ps->d_rb.ptr, ps->d_offb.ptr};
-  MLF_LAUNCH(launch_backward_update_fp32(p->X.ptr, ps->bwd.ptr + (size_t)k * ps->per_view_b, tb,
-                                         p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
+  MLF_CHECK(patch_forward_ratio(p, k, p->LF.ptr, nullptr));
+  MLF_CHECK(tim_begin(p, 2));
+  if (p->reference) {
+    const PlaneTables tb{ps->d_rb.ptr, ps->d_offb.ptr};
+    MLF_LAUNCH(launch_backward_update_fp32(p->X.ptr, ps->bwd.ptr + (size_t)k * ps->per_view_b, tb,
+                                           p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
+  } else {
+    const PrecPack& pk = ps->pack[(int)p->kp];
+    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->X.ptr, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
+                               p->norm->T.ptr + (size_t)k * p->norm->pvt, bwd_geo(p), p->nblk_b,
+                               p->smem_b, p->R.ptr, (float)ps->weight[k], st));
+  }
+  MLF_CHECK(tim_end(p));
+  return MLF_OK;
+}
+
+// Schedules (largest planes first), partial buffers, packed tables.
+static mlf_status patch_setup_mfma(mlf_patch_s* p) {
+  mlf_psf_s* ps = p->psf;
+  std::lock_guard<std::mutex> lk(ps->mu);
+  static std::once_flag attr_once;
+  static hipError_t attr_err = hipSuccess;
+  std::call_once(attr_once, [] { attr_err = mfma_kernel_attributes(); });
+  if (attr_err != hipSuccess)
+    return set_error(MLF_ERR_DEVICE, "kernel attributes: %s", hipGetErrorString(attr_err));
+  MLF_CHECK(psf_ensure_pack(ps, p->kp));
+  MLF_CHECK(psf_ensure_norm(ps, p->kp, p->n1, p->n2, &p->norm));
+  const int Z = p->Z;
+  p->bm2 = fwd_bm2(p->kp);
+  const int nb1 = (p->n1 + 15) / 16, nb2 = (p->n2 + p->bm2 - 1) / p->bm2;
+  std::vector<int> order(Z);
+  for (int z = 0; z < Z; ++z) order[z] = z;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ps->rf[a] > ps->rf[b]; });
+  std::vector<int64_t> poff(Z);
+  int64_t po = 0;
+  int rfmax = 0, rbmax = 0, epmax = 0;
+  for (int z = 0; z < Z; ++z) {
+    const int r = ps->rf[z];
+    rfmax = std::max(rfmax, r);
+    rbmax = std::max(rbmax, ps->rb[z]);
+    epmax = std::max(epmax, ps->pack[(int)p->kp].epad[z]);
+    poff[z] = po;
+    po += (int64_t)nb1 * nb2 * (16 + 2 * r) * (p->bm2 + 2 * r);
+  }
+  std::vector<int32_t> fs;
+  for (int z : order)
+    for (int b2 = 0; b2 < nb2; ++b2)
+      for (int b1 = 0; b1 < nb1; ++b1) {
+        fs.push_back(z);
+        fs.push_back(b1);
+        fs.push_back(b2);
+      }
+  p->nblk_f = (int)(fs.size() / 3);
+  std::vector<int> border(Z);
+  for (int z = 0; z < Z; ++z) border[z] = z;
+  std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return ps->rb[a] > ps->rb[b]; });
+  const int mb1 = (p->n1 + 15) / 16, mb2 = (p->n2 + 15) / 16;
+  const int ngrp = kPhiPadBwd / (16 * kBwdNTiles);
+  std::vector<int32_t> bs;
+  for (int z : border)
+    for (int b2 = 0; b2 < mb2; ++b2)
+      for (int b1 = 0; b1 < mb1; ++b1)
+        for (int gq = 0; gq < ngrp; ++gq) {
+          bs.push_back(z);
+          bs.push_back(b1);
+          bs.push_back(b2);
+          bs.push_back(gq);
+        }
+  p->nblk_b = (int)(bs.size() / 4);
+  MLF_CHECK(upload_vec(p->d_poff, poff));
+  MLF_CHECK(upload_vec(p->d_fsched, fs));
+  MLF_CHECK(upload_vec(p->d_bsched, bs));
+  MLF_CHECK(p->partial.alloc((size_t)po));
+  p->smem_f = fwd_smem_bytes(p->kp, rfmax, p->bm2);
+  p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax);
+  if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
+    return set_error(MLF_ERR_UNSUPPORTED,
+                     "PSF radius too large for the LDS tiles (rf_max=%d, rb_max=%d)", rfmax, rbmax);
   return MLF_OK;
 }
 
@@ -657,9 +984,11 @@ static mlf_status patch_filter(mlf_patch_s* p) {
   if (p->f_nplanes == 0) return MLF_OK;
   hipStream_t st = p->psf->ctx->stream;
   const FilterPlan fp = filter_plan_of(p);
+  MLF_CHECK(tim_begin(p, 3));
   MLF_LAUNCH(launch_filter_pass1(p->X.ptr, p->tmp.ptr, fp, p->n1, p->n2, p->pb.ptr, p->nblk, st));
   MLF_LAUNCH(launch_filter_pass2(p->tmp.ptr, p->X.ptr, fp, p->n1, p->n2, p->pa.ptr, p->nblk, st));
   MLF_LAUNCH(launch_filter_scale(p->X.ptr, fp, p->n1, p->n2, p->pb.ptr, p->pa.ptr, p->nblk, st));
+  MLF_CHECK(tim_end(p));
   return MLF_OK;
 }
 
@@ -673,11 +1002,25 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   if (n1 < 1 || n2 < 1 || n1 > 4096 || n2 > 4096)
     return set_error(MLF_ERR_INVALID_ARGUMENT, "bad lenslet grid %lld x %lld", (long long)n1,
                      (long long)n2);
-  if (prec != MLF_PRECISION_FP32)
-    return set_error(MLF_ERR_UNSUPPORTED, "precision %d not built yet", (int)prec);
+  if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16)
+    return set_error(MLF_ERR_UNSUPPORTED, "precision %d not supported", (int)prec);
   MLF_CHECK(ctx_activate(psf->ctx));
   auto p = std::make_unique<mlf_patch_s>();
   p->psf = psf;
+  {
+    std::lock_guard<std::mutex> lk(g_life_mu);
+    if (psf->closed) return set_error(MLF_ERR_INVALID_ARGUMENT, "psf already destroyed");
+    ++psf->refs;
+  }
+  struct RefUndo {
+    mlf_patch_s* p;
+    ~RefUndo() {
+      if (p) {
+        std::lock_guard<std::mutex> lk(g_life_mu);
+        --p->psf->refs;
+      }
+    }
+  } undo{p.get()};
   p->n1 = (int)n1;
   p->n2 = (int)n2;
   p->Z = psf->Z;
@@ -688,9 +1031,15 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   MLF_CHECK(p->part.alloc(n * p->Z));
   MLF_CHECK(p->R.alloc(n));
   MLF_CHECK(p->y.alloc(n));
+  MLF_CHECK(p->dsum.alloc(1));
   MLF_CHECK(build_filter_plan(p.get()));
+  const char* ref = std::getenv("MLF_REFERENCE_KERNELS");
+  p->reference = ref && ref[0] == '1' && prec == MLF_PRECISION_FP32;
+  p->kp = prec == MLF_PRECISION_BF16 ? Precision::kBF16 : Precision::kF32;
+  if (!p->reference) MLF_CHECK(patch_setup_mfma(p.get()));
   MLF_HIP(hipEventCreate(&p->ev0));
   MLF_HIP(hipEventCreate(&p->ev1));
+  undo.p = nullptr;
   *out = p.release();
   return MLF_OK;
 }
@@ -698,9 +1047,12 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
 mlf_status mlf_patch_destroy(mlf_patch p) {
   clear_error();
   if (!p) return MLF_OK;
-  (void)hipSetDevice(p->psf->ctx->device);
-  (void)hipStreamSynchronize(p->psf->ctx->stream);
+  std::lock_guard<std::mutex> lk(g_life_mu);
+  mlf_psf psf = p->psf;
+  (void)hipSetDevice(psf->ctx->device);
+  (void)hipStreamSynchronize(psf->ctx->stream);
   delete p;
+  if (--psf->refs == 0 && psf->closed) psf_free(psf);
   return MLF_OK;
 }
 
@@ -722,16 +1074,11 @@ mlf_status mlf_patch_init_volume(mlf_patch p) {
   if (!p->image_set) return set_error(MLF_ERR_INVALID_ARGUMENT, "image not set");
   std::lock_guard<std::mutex> lk(p->mu);
   MLF_CHECK(ctx_activate(p->psf->ctx));
-  // recon:72-73: X = ones ./ numel .* sum(LFIMG); the FP64 image sum is taken
-  // on the host copy of the image.
-  std::vector<float> h(p->LF.count);
-  MLF_HIP(hipMemcpy(h.data(), p->LF.ptr, p->LF.bytes(), hipMemcpyDeviceToHost));
-  double s = 0.0;
-  for (float v : h) s += v;
+  // recon:72-73: X = ones ./ numel .* sum(LFIMG), FP64 image sum on the device.
+  // Asynchronous on the context stream.
   const double numel = (double)p->n1 * kN * p->n2 * kN * p->Z;
-  const float x0 = (float)((1.0 / numel) * s);
-  MLF_LAUNCH(launch_fill(p->X.ptr, p->X.count, x0, p->psf->ctx->stream));
-  MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+  MLF_LAUNCH(launch_init_volume(p->LF.ptr, p->LF.count, p->dsum.ptr, p->X.ptr, p->X.count,
+                                1.0 / numel, p->psf->ctx->stream));
   p->volume_set = true;
   return MLF_OK;
 }
@@ -808,6 +1155,36 @@ mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms) {
   return MLF_OK;
 }
 
+mlf_status mlf_patch_set_timing(mlf_patch p, int32_t enable) {
+  clear_error();
+  if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  p->timing = enable != 0;
+  p->ev_used = 0;
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_kernel_times(mlf_patch p, double* total_ms, int64_t* launches) {
+  clear_error();
+  if (!p || !total_ms || !launches) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  for (int c = 0; c < MLF_KERNEL_CLASSES; ++c) {
+    total_ms[c] = 0.0;
+    launches[c] = 0;
+  }
+  if (p->ev_used) MLF_HIP(hipEventSynchronize(p->ev_pool[p->ev_used - 1]));
+  for (size_t k = 0; k + 1 < p->ev_used; k += 2) {
+    float ms = 0.f;
+    MLF_HIP(hipEventElapsedTime(&ms, p->ev_pool[k], p->ev_pool[k + 1]));
+    const int c = p->ev_class[k / 2];
+    total_ms[c] += ms;
+    launches[c] += 1;
+  }
+  p->ev_used = 0;
+  return MLF_OK;
+}
+
 mlf_status mlf_patch_forward_view(mlf_patch p, int32_t view, float* y) {
   clear_error();
   if (!p || !y) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -817,12 +1194,6 @@ mlf_status mlf_patch_forward_view(mlf_patch p, int32_t view, float* y) {
   MLF_CHECK(ctx_activate(p->psf->ctx));
   hipStream_t st = p->psf->ctx->stream;
   MLF_CHECK(patch_forward(p, view));
-  // reuse the ratio kernel's z-reduction with a zero image for y
-  DevBuf<float> zero;
-  MLF_CHECK(zero.alloc(p->LF.count));
-  MLF_HIP(hipMemsetAsync(zero.ptr, 0, zero.bytes(), st));
-  MLF_LAUNCH(launch_ratio(p->part.ptr, p->Z, p->n1, p->n2, zero.ptr, p->psf->uv[2 * view],
-                          p->psf->uv[2 * view + 1], p->R.ptr, p->y.ptr, st));
   MLF_HIP(hipMemcpyAsync(y, p->y.ptr, p->y.bytes(), hipMemcpyDeviceToHost, st));
   MLF_HIP(hipStreamSynchronize(st));
   return MLF_OK;
@@ -840,7 +1211,7 @@ mlf_status mlf_patch_project_all(mlf_patch p, float* sensor) {
   MLF_HIP(hipMemsetAsync(img.ptr, 0, img.bytes(), st));
   for (int k = 0; k < p->psf->V; ++k) {
     MLF_CHECK(patch_forward(p, k));
-    MLF_LAUNCH(launch_scatter_view(p->part.ptr, p->Z, p->n1, p->n2, p->psf->uv[2 * k],
+    MLF_LAUNCH(launch_scatter_view(p->y.ptr, 1, p->n1, p->n2, p->psf->uv[2 * k],
                                    p->psf->uv[2 * k + 1], img.ptr, st));
   }
   MLF_HIP(hipMemcpyAsync(sensor, img.ptr, img.bytes(), hipMemcpyDeviceToHost, st));

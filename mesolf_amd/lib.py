@@ -97,6 +97,8 @@ _SIGNATURES = {
     "mlf_patch_project_all": (C.c_int, [_P, _F32P]),
     "mlf_patch_sample_filter": (C.c_int, [_P]),
     "mlf_patch_last_run_ms": (C.c_int, [_P, _F64P]),
+    "mlf_patch_set_timing": (C.c_int, [_P, _I32]),
+    "mlf_patch_kernel_times": (C.c_int, [_P, _F64P, _I64P]),
     "mlf_recon_phase_space": (C.c_int, [_F32P, _I64, _I64, _F32P, _F32P, _I64, _I64, _I64, _I32,
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
@@ -306,6 +308,18 @@ class Patch:
         ms = C.c_double()
         _check(self._lib.mlf_patch_last_run_ms(self.handle, C.byref(ms)))
         return ms.value
+
+    KERNEL_CLASSES = ("forward", "ratio", "backward_update", "sample_filter")
+
+    def set_timing(self, enable: bool = True):
+        _check(self._lib.mlf_patch_set_timing(self.handle, int(bool(enable))))
+
+    def kernel_times(self):
+        """{class: (total_ms, launches)} since the previous read."""
+        ms = np.zeros(4, np.float64)
+        n = np.zeros(4, np.int64)
+        _check(self._lib.mlf_patch_kernel_times(self.handle, _fptr(ms, C.c_double), _fptr(n, C.c_int64)))
+        return {c: (float(ms[i]), int(n[i])) for i, c in enumerate(self.KERNEL_CLASSES)}
 
     def forward_view(self, k: int):
         out = np.empty((self.n1, self.n2), np.float32, order="F")

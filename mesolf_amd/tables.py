@@ -48,18 +48,23 @@ def synthetic_compact(wl: synth.Workload, views=None):
     views = synth.aperture_views() if views is None else views
     Z = wl.Z
     radii = wl.radii
-    # per-depth normalisation (recon:13-16): every view kernel sums to 1, so
-    # the depth slab sums to the number of views
-    blocks = {}
+    # The synthetic kernels are separable: block[d1,d2,s1,s2] = tx[d1,s1] * ty[d2,s2].
+    # Pass 1: per-depth normalisation (recon:13-16) and per-view sums (recon:30)
+    # from the factor sums; pass 2 writes the normalised FP32 blocks.
+    fac = {}
     depth_sum = np.zeros(Z)
     for (u, v) in views:
         for zi, z in enumerate(wl.z_um):
-            t = synth.compact_fwd_table(u, v, z)
-            blocks[(u, v, zi)] = t
-            depth_sum[zi] += t.sum()
+            r = radii[zi]
+            gx, gy = synth.view_profiles(u, v, z)
+            tx = synth._phase_table(gx, r)
+            ty = synth._phase_table(gy, r)
+            fac[(u, v, zi)] = (tx, ty)
+            depth_sum[zi] += tx.sum() * ty.sum()
     sums = np.zeros((15, 15))
     for (u, v) in views:
-        sums[u, v] = sum(blocks[(u, v, zi)].sum() / depth_sum[zi] for zi in range(Z))
+        sums[u, v] = sum(fac[(u, v, zi)][0].sum() * fac[(u, v, zi)][1].sum() / depth_sum[zi]
+                         for zi in range(Z))
     w = view_weights(sums)
     order = [(u, v) for (u, v) in spiral_order(15) if w[u, v] != 0]
     per_view = sum((2 * r + 1) ** 2 * 225 for r in radii)
@@ -67,9 +72,11 @@ def synthetic_compact(wl: synth.Workload, views=None):
     o = 0
     for (u, v) in order:
         for zi in range(Z):
-            t = blocks[(u, v, zi)] / depth_sum[zi]
-            flat = np.transpose(t, (1, 0, 3, 2)).ravel()
-            fwd[o:o + flat.size] = flat
-            o += flat.size
+            tx, ty = fac[(u, v, zi)]
+            # ABI order [d2, d1, s2, s1]
+            blk = np.einsum("bt,as->bats", ty / depth_sum[zi], tx)
+            n = blk.size
+            fwd[o:o + n] = blk.ravel()
+            o += n
     return dict(uv=np.array(order, np.int32), weights=np.array([w[u, v] for u, v in order]),
                 rf=np.array(radii, np.int32), fwd=fwd)
