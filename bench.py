@@ -91,13 +91,15 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp32", choices=["fp32", "bf16", "fp16"])
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"])
     ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2"])
     ap.add_argument("--views-limit", type=int, default=0,
                     help="profiling only: visit the first N views (result flagged partial)")
     ap.add_argument("--iters", type=int, default=0, help="profiling only: RL iterations per volume")
     ap.add_argument("--cpu-views", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the FP32-path volume used for the speed ratio and the error")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -171,8 +173,36 @@ def main():
         roof = {"bound": "hbm", "achieved": achieved / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": achieved / HBM_PEAK, "traffic": None, "kernel": dominant,
                 "avg_launch_ms": avg_s * 1e3}
+    tfile = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2_101x101x51.json")
+    if args.workload == "cfg2" and os.path.exists(tfile):
+        tr = json.load(open(tfile))["bytes_per_launch"].get(args.precision, {}).get(dominant)
+        if tr:
+            roof["traffic"] = tr["read"] + tr["write"]
+            roof["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE, profiles/"
     per_kernel = {c: {"total_ms": round(ms, 3), "launches": n,
                       "avg_ms": round(ms / n, 4) if n else None} for c, (ms, n) in ktimes.items()}
+
+    compare = None
+    if not args.no_compare and args.precision != "fp32" and not partial:
+        Xb = work.get_volume()
+        patch.set_image(LF)
+        patch.init_volume()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        patch.run(iters, filter=True)
+        ctx.synchronize()
+        t_fp32 = time.perf_counter() - t1
+        Xf = patch.get_volume()
+        err = float(np.linalg.norm((Xb.astype(np.float64) - Xf).ravel()) /
+                    np.linalg.norm(Xf.astype(np.float64).ravel()))
+        compare = {"fp32_path_volumes_per_s": 1.0 / t_fp32,
+                   f"{args.precision}_over_fp32": vol_per_s * t_fp32,
+                   "rel_l2_vs_fp32_path": err,
+                   "note": ("error of this path against the plain FP32 GPU path on the same "
+                            "inputs; the FP32 path is within 1e-6 of the FP64 oracle at the "
+                            "33x33-lenslet, 11-plane shape (tests)")}
+        log(f"fp32 path {1.0 / t_fp32:.4f} vol/s, rel L2 {err:.3e}")
+        del Xb, Xf
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -198,6 +228,7 @@ def main():
                    "partial_profile_run": partial},
         "roofline": roof,
         "kernels": per_kernel,
+        "fp32_comparison": compare,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
