@@ -18,6 +18,8 @@ constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x
 constexpr int kBwdNTiles = 5;     // N-tiles per backward workgroup (80 phases)
 constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases
 constexpr int kBwdTapPad = 32;    // backward taps padded to a multiple of 32
+constexpr int kTpzMTF32 = 2;      // Toeplitz forward: input columns per wave, f32
+constexpr int kTpzMTBF16 = 4;     // Toeplitz forward: input columns per wave, bf16
 
 inline int fwd_bm2(Precision p) { return 4 * (p == Precision::kF32 ? kFwdMTF32 : kFwdMTBF16); }
 inline int fwd_phi(Precision p) { return p == Precision::kF32 ? kPhiPadF32 : kPhiPadBF16; }
@@ -48,6 +50,7 @@ struct PackGeo {
   const int64_t* src_off;  // [Z] offset in the ABI view block
   const int64_t* dst_off;  // [Z] offset in the packed view block
   int64_t src_view, dst_view;
+  int row8;                // backward pack: tap rows padded to a multiple of 8 (BF16)
 };
 
 struct NormGeo {
@@ -76,6 +79,14 @@ __host__ __device__ inline void norm_range(int c, int n, int r, int* lo, int* hi
 }
 
 size_t fwd_smem_bytes(Precision p, int rmax, int bm2);
+size_t tpz_smem_bytes(Precision p, int rmax);
+int tpz_max_radius();
+hipError_t launch_fwd_tpz(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
+                          size_t smem, float* partial, hipStream_t st);
+hipError_t launch_ratio_gather_tpz(const float* partial, FwdGeo g, int Z, int cw, const float* LF,
+                                   int u, int v, float* R, float* y_out, hipStream_t st);
+hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
+                               hipStream_t st);
 size_t bwd_smem_bytes(Precision p, int rmax, int epad_max);
 
 hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,

@@ -56,8 +56,18 @@ struct Mma<__bf16> {
 __device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
 // ------------------------------------------------------------------ forward
+template <typename T>
+struct FwdCfg {
+  static constexpr int NBUF = 2;  // FP32: double-buffered chunks, 1 workgroup per CU
+};
+template <>
+struct FwdCfg<__bf16> {
+  static constexpr int NBUF = 2;  // BF16 (fallback kernel for r > 23)
+};
+
 template <typename T, int MT>
-__global__ __launch_bounds__(256) void k_fwd_mfma(const float* __restrict__ X,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdCfg<T>::NBUF == 1 ? 2 : 1)))
+void k_fwd_mfma(const float* __restrict__ X,
                                                   const T* __restrict__ W, FwdGeo g,
                                                   float* __restrict__ partial) {
   using M = Mma<T>;
@@ -65,9 +75,10 @@ __global__ __launch_bounds__(256) void k_fwd_mfma(const float* __restrict__ X,
   constexpr int NKS = PHI / KSTEP;
   constexpr int BM2 = 4 * MT;
   constexpr int CH = kFwdChunk;  // taps per chunk (4 N-tiles of 16)
+  constexpr int NBUF = FwdCfg<T>::NBUF;
   extern __shared__ __align__(16) char smem[];
-  T* Bs = reinterpret_cast<T*>(smem);                                  // [2][CH][LDB]
-  float* ywin = reinterpret_cast<float*>(smem + sizeof(T) * 2 * CH * LDB);
+  T* Bs = reinterpret_cast<T*>(smem);                                  // [NBUF][CH][LDB]
+  float* ywin = reinterpret_cast<float*>(smem + sizeof(T) * NBUF * CH * LDB);
 
   const int* s = g.sched + 3 * blockIdx.x;
   const int z = s[0], b1 = s[1], b2 = s[2];
@@ -92,24 +103,16 @@ __global__ __launch_bounds__(256) void k_fwd_mfma(const float* __restrict__ X,
       const int a2 = BM2 * b2 + MT * w + mt;
       const bool ok = a1 < g.n1 && a2 < g.n2;
       const float* xp = X + (((size_t)z * g.n2 + (ok ? a2 : 0)) * g.n1 + (ok ? a1 : 0)) * kP;
+      // branch-free: load from a clamped in-row phase, select zero afterwards
 #pragma unroll
       for (int ks = 0; ks < NKS; ++ks) {
-        if constexpr (KPL == 1) {
-          const int ph = ks * KSTEP + lg;
-          a[mt][ks] = (ok && ph < kP) ? xp[ph] : 0.f;
-        } else {
-          // 8 consecutive phases; a lenslet row is 900 B (4 B aligned only)
-          const int ph0 = ks * KSTEP + lg * KPL;
-          float v[KPL];
-          if (ok && ph0 + KPL <= kP) {
 #pragma unroll
-            for (int j = 0; j < KPL; ++j) v[j] = xp[ph0 + j];
-          } else {
-#pragma unroll
-            for (int j = 0; j < KPL; ++j) v[j] = (ok && ph0 + j < kP) ? xp[ph0 + j] : 0.f;
-          }
-#pragma unroll
-          for (int j = 0; j < KPL; ++j) a[mt][ks][j] = (__bf16)v[j];
+        for (int j = 0; j < KPL; ++j) {
+          const int ph = ks * KSTEP + lg * KPL + j;
+          const float v = xp[ph < kP ? ph : kP - 1];
+          const float val = (ok && ph < kP) ? v : 0.f;
+          if constexpr (KPL == 1) a[mt][ks] = val;
+          else a[mt][ks][j] = (__bf16)val;
         }
       }
     }
@@ -141,12 +144,20 @@ __global__ __launch_bounds__(256) void k_fwd_mfma(const float* __restrict__ X,
     }
   };
 
-  issue_chunk(0, 0);
-  __builtin_amdgcn_s_waitcnt(0);
+  if (NBUF == 2) {
+    issue_chunk(0, 0);
+    __builtin_amdgcn_s_waitcnt(0);
+  }
   for (int c = 0; c < nch; ++c) {
-    __syncthreads();  // chunk c landed; everyone done reading the other buffer
-    if (c + 1 < nch) issue_chunk(c + 1, (c + 1) & 1);
-    const T* Bc = Bs + (size_t)(c & 1) * CH * LDB;
+    __syncthreads();  // previous chunk landed / everyone done reading the buffer
+    if (NBUF == 1) {
+      issue_chunk(c, 0);
+      __builtin_amdgcn_s_waitcnt(0);
+      __syncthreads();
+    } else if (c + 1 < nch) {
+      issue_chunk(c + 1, (c + 1) & 1);
+    }
+    const T* Bc = Bs + (size_t)(NBUF == 2 ? (c & 1) : 0) * CH * LDB;
     floatx4 acc[MT][4];
 #pragma unroll
     for (int mt = 0; mt < MT; ++mt)
@@ -183,7 +194,7 @@ __global__ __launch_bounds__(256) void k_fwd_mfma(const float* __restrict__ X,
         for (int j = 0; j < 4; ++j) atomicAdd(row + 4 * lg + j, acc[mt][nt][j]);
       }
     }
-    __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c+1 done
+    if (NBUF == 2) __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c+1 done
   }
   __syncthreads();
   const int nb1 = ceil_div(g.n1, 16);
@@ -197,6 +208,254 @@ __global__ __launch_bounds__(256) void k_fwd_mfma(const float* __restrict__ X,
       if (yy >= 0 && yy < WW) acc += ywin[(q * WW + yy) * W1 + x];
     }
     out[o] = acc;
+  }
+}
+
+// ------------------------------------------------------- forward (Toeplitz)
+// Per plane z and view:  P[(i1, c2), e2] = sum_{e1, phase} X[i1 - d1][c2][phase] * W[e1][e2][phase]
+// (d1 = e1 - r), then y[i1][c2 + d2] += P.  M = 16 output rows i1 of one
+// input column c2 (one M-tile per c2), N = e2 (16 * ceil(K/16)), K = (e1, phase).
+// The A operand is a register window over i1: stepping e1 shifts it by one
+// lane (DPP row_shr:1 within each 16-lane row) and lane 0 takes the next
+// lenslet from an LDS ring filled by direct-to-LDS loads.  The B operand (one
+// e1 column of taps) is copied global -> LDS while the previous column's
+// MFMAs run.  The sum over e1 happens inside the MFMA accumulators, so only
+// the final P tile is scattered (a few LDS atomics per wave and plane).
+constexpr int kTpzNT = 3;      // N-tiles (taps along axis 1): K <= 48, i.e. r <= 23
+constexpr int kRingRow = 256;  // floats per lenslet row of the injection ring (phases >= 225 stay 0)
+
+template <typename T, int MT>
+__global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
+                                                 const T* __restrict__ W, FwdGeo g,
+                                                 float* __restrict__ partial) {
+  using M = Mma<T>;
+  constexpr int KSTEP = M::KSTEP, KPL = M::KPL, PHI = M::PHI;
+  constexpr int NKS = PHI / KSTEP;
+  constexpr int CW = 4 * MT;  // input columns c2 per workgroup
+  constexpr int VEC = 16 / sizeof(T);
+  constexpr int RV = PHI / VEC;
+  constexpr bool SWZ = (RV == 32);
+  extern __shared__ __align__(16) char smem[];
+  T* Wb = reinterpret_cast<T*>(smem);  // [2][16*kTpzNT][PHI]
+  float* ring = reinterpret_cast<float*>(smem + sizeof(T) * 2 * 16 * kTpzNT * PHI);  // [2][CW][kRingRow]
+  float* ywin = ring + 2 * CW * kRingRow;  // [4 waves][16][MT + 2r]
+
+  const int* s = g.sched + 3 * blockIdx.x;
+  const int z = s[0], b1 = s[1], b2 = s[2];
+  const int r = g.r[z];
+  const int K = 2 * r + 1;
+  const int NT = (K + 15) / 16;
+  const int nrow = 16 * NT;
+  const int WW = MT + 2 * r;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const int c2_0 = CW * b2;
+  const float* Xz = X + (size_t)z * g.n2 * g.n1 * kP;
+  float* yw = ywin + w * 16 * WW;
+  for (int o = tid; o < 4 * 16 * WW; o += 256) ywin[o] = 0.f;
+  for (int o = tid; o < 2 * CW * kRingRow; o += 256) ring[o] = 0.f;  // pads stay zero
+
+  // A operand at e1 = 0 (d1 = -r): lane row lr holds lenslet i1 = 16*b1 + lr + r
+  typename M::frag a[MT][NKS];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt) {
+    const int c2 = c2_0 + MT * w + mt;
+    const int i1 = 16 * b1 + lr + r;
+    const bool ok = c2 < g.n2 && i1 < g.n1;
+    const float* xp = Xz + ((size_t)(ok ? c2 : 0) * g.n1 + (ok ? i1 : 0)) * kP;
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+#pragma unroll
+      for (int j = 0; j < KPL; ++j) {
+        const int ph = ks * KSTEP + lg * KPL + j;
+        const float v = xp[ph < kP ? ph : kP - 1];
+        const float val = (ok && ph < kP) ? v : 0.f;
+        if constexpr (KPL == 1) a[mt][ks] = val;
+        else a[mt][ks][j] = (__bf16)val;
+      }
+    }
+  }
+
+  const T* Wz = W + g.woff[z];  // [K][nrow][PHI]
+  auto issue_w = [&](int t, int buf) {
+    const T* src = Wz + (size_t)t * nrow * PHI;
+    char* dst = reinterpret_cast<char*>(Wb) + (size_t)buf * 16 * kTpzNT * PHI * sizeof(T);
+    const int nslot = nrow * RV;
+    for (int base = w * 64; base < nslot; base += 256) {
+      const int slot = base + lane;
+      if (slot < nslot) {
+        const int row = slot / RV, pos = slot - row * RV;
+        const int col = SWZ ? (pos ^ (row & 31)) : pos;
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
+            (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
+      }
+    }
+  };
+  // injection row needed when entering step t (t >= 1): i1 = 16*b1 + r - t
+  auto issue_inj = [&](int t, int slot_buf) {
+    const int i1 = 16 * b1 + r - t;
+    if (i1 < 0 || i1 >= g.n1) return;
+    float* dst = ring + slot_buf * CW * kRingRow;
+    // rows of 225 phases in 4 passes of 64 dwords (the DMA writes 64
+    // consecutive LDS dwords per wave instruction)
+    constexpr int NPASS = CW * 4;
+    for (int q = w; q < NPASS; q += 4) {
+      const int c = q >> 2, ph = (q & 3) * 64 + lane;
+      const int c2 = c2_0 + c;
+      if (ph < kP && c2 < g.n2) {
+        const float* src = Xz + ((size_t)c2 * g.n1 + i1) * kP + ph;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                         (__attribute__((address_space(3))) void*)(dst + c * kRingRow + (q & 3) * 64),
+                                         4, 0, 0);
+      }
+    }
+  };
+
+  floatx4 acc[MT][kTpzNT];
+#pragma unroll
+  for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+    for (int nt = 0; nt < kTpzNT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+  issue_w(0, 0);
+  __builtin_amdgcn_s_waitcnt(0);
+  __syncthreads();
+  for (int t = 0; t < K; ++t) {
+    if (t + 1 < K) {
+      issue_w(t + 1, (t + 1) & 1);
+      issue_inj(t + 1, (t + 1) & 1);
+    }
+    const T* Wc = Wb + (size_t)(t & 1) * 16 * kTpzNT * PHI;
+#pragma unroll
+    for (int ks = 0; ks < NKS; ++ks) {
+#pragma unroll
+      for (int nt = 0; nt < kTpzNT; ++nt) {
+        if (nt >= NT) break;
+        const int row = nt * 16 + lr;
+        typename M::frag bf;
+        if constexpr (SWZ) {
+          const int col = (ks * KSTEP + lg * KPL) / VEC;
+          bf = M::load(Wc + row * PHI + ((col ^ (row & 31)) * VEC));
+        } else {
+          bf = M::load(Wc + row * PHI + ks * KSTEP + lg * KPL);
+        }
+#pragma unroll
+        for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
+      }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();  // column t+1 and injection row landed; column t no longer read
+    if (t + 1 < K) {
+      const int i1n = 16 * b1 + r - (t + 1);
+      const float* rg = ring + ((t + 1) & 1) * CW * kRingRow;
+      const bool rowok = (i1n >= 0) && (i1n < g.n1);
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const int c = MT * w + mt;
+        const bool ok = rowok && (c2_0 + c < g.n2);
+        // every lane reads its row group's slot (broadcast); only lane 0 of
+        // each 16-lane row keeps it (the DPP 'old' operand)
+        const float* rp = rg + c * kRingRow + lg * KPL;
+#pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+          if constexpr (KPL == 1) {
+            const float v = rp[ks * KSTEP];
+            const float inj = ok ? v : 0.f;
+            a[mt][ks] = __builtin_bit_cast(
+                float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, inj),
+                                                   __builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF,
+                                                   false));
+          } else {
+            bf16x8 inj;
+#pragma unroll
+            for (int j = 0; j < KPL; ++j) {
+              const float v = rp[ks * KSTEP + j];
+              inj[j] = (__bf16)(ok ? v : 0.f);
+            }
+            typedef int intx4 __attribute__((ext_vector_type(4)));
+            const intx4 iv = __builtin_bit_cast(intx4, inj);
+            intx4 cv = __builtin_bit_cast(intx4, a[mt][ks]);
+#pragma unroll
+            for (int q = 0; q < 4; ++q) cv[q] = __builtin_amdgcn_update_dpp(iv[q], cv[q], 0x111, 0xF, 0xF, false);
+            a[mt][ks] = __builtin_bit_cast(bf16x8, cv);
+          }
+        }
+      }
+    }
+  }
+
+  // P tile -> this wave's window (rows i1 - 16*b1, columns c2 + d2 - (c2_0 + MT*w - r))
+#pragma unroll
+  for (int nt = 0; nt < kTpzNT; ++nt) {
+    const int e2 = nt * 16 + lr;
+    if (nt >= NT || e2 >= K) continue;
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) atomicAdd(yw + (4 * lg + j) * WW + mt + e2, acc[mt][nt][j]);
+  }
+  __syncthreads();
+  const int Wc2 = CW + 2 * r;
+  const int nb1 = ceil_div(g.n1, 16);
+  float* out = partial + g.poff[z] + (size_t)(b2 * nb1 + b1) * 16 * Wc2;
+  for (int o = tid; o < 16 * Wc2; o += 256) {
+    const int i = o / Wc2, c = o - i * Wc2;
+    float sum = 0.f;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int cc = c - MT * q;
+      if (cc >= 0 && cc < WW) sum += ywin[(q * 16 + i) * WW + cc];
+    }
+    out[o] = sum;
+  }
+}
+
+// Gather for the Toeplitz partials: y[i] = sum_z sum_{b2} window(b1 = i1/16, b2)
+// in a fixed order (FP64), then the ratio as k_ratio_gather.
+__global__ void k_ratio_gather_tpz(const float* __restrict__ partial, FwdGeo g, int Z, int CW,
+                                   const float* __restrict__ LF, int u, int v,
+                                   float* __restrict__ R, float* __restrict__ y_out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= g.n1 * g.n2) return;
+  const int i1 = i % g.n1, i2 = i / g.n1;
+  const int nb1 = ceil_div(g.n1, 16), nb2 = ceil_div(g.n2, CW);
+  const int b1 = i1 / 16;
+  double y = 0.0;
+  for (int z = 0; z < Z; ++z) {
+    const int r = g.r[z];
+    const int Wc2 = CW + 2 * r;
+    const int b2h = min(nb2 - 1, (i2 + r) / CW);
+    const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
+    const float* pz = partial + g.poff[z];
+    for (int b2 = b2l; b2 <= b2h; ++b2)
+      y += (double)pz[(size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r)];
+  }
+  const float yf = (float)y;
+  if (LF) {
+    const float lf = LF[(size_t)(u + kN * i1) + (size_t)(kN * g.n1) * (v + kN * i2)];
+    float q = lf / yf;
+    if (!isfinite(q)) q = 0.f;
+    R[i] = q;
+  }
+  if (y_out) y_out[i] = yf;
+}
+
+// ABI block [e2][e1][phase] -> Toeplitz pack [e1][e2 padded to 16*ceil(K/16)][PHI].
+template <typename T>
+__global__ void k_pack_fwd_tpz(const float* __restrict__ src, T* __restrict__ dst, PackGeo pg) {
+  const int z = blockIdx.y, v = blockIdx.z;
+  const int K = 2 * pg.r[z] + 1;
+  const int nrow = 16 * ((K + 15) / 16);
+  const int PHI = Mma<T>::PHI;
+  const float* s = src + (size_t)v * pg.src_view + pg.src_off[z];
+  T* d = dst + (size_t)v * pg.dst_view + pg.dst_off[z];
+  const size_t total = (size_t)K * nrow * PHI;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int ph = (int)(o % PHI);
+    const int e2 = (int)((o / PHI) % nrow);
+    const int e1 = (int)(o / ((size_t)PHI * nrow));
+    d[o] = (T)((e2 < K && ph < kP) ? s[((size_t)e2 * K + e1) * kP + ph] : 0.f);
   }
 }
 
@@ -253,93 +512,144 @@ __global__ __launch_bounds__(256) void k_bwd_mfma(float* __restrict__ X, const T
   const int K = 2 * r + 1, ntaps = K * K;
   const int epad = g.epad[z];
   const int Wd = 16 + 2 * r;
-  T* Rw = reinterpret_cast<T*>(smem);                               // [Wd][Wd]
-  int* toff = reinterpret_cast<int*>(smem + sizeof(T) * ((Wd * Wd + 7) & ~7));  // [epad]
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
-
-  // ratio window (zero outside the grid) and tap offsets
-  for (int o = tid; o < Wd * Wd; o += 256) {
-    const int x = o % Wd, y = o / Wd;
-    const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
-    const float val = (a1 >= 0 && a1 < g.n1 && a2 >= 0 && a2 < g.n2) ? R[a1 + g.n1 * a2] : 0.f;
-    Rw[o] = (T)val;
-  }
-  for (int e = tid; e < epad; e += 256) {
-    int off = 0;  // padded taps have zero weights; any in-window read is fine
-    if (e < ntaps) {
-      const int e2 = e / K, e1 = e - e2 * K;
-      off = -(e1 - r) - (e2 - r) * Wd;
-    }
-    toff[e] = off;
-  }
-  __syncthreads();
-
-  int base[MT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
-
+  const T* Bz = Wb + g.boff[z] + (size_t)(grp * 16 * NT) * epad;
+  const int nks = epad / KSTEP;
   floatx4 acc[MT][NT];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt)
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  const T* Bz = Wb + g.boff[z] + (size_t)(grp * 16 * NT) * epad;
-  const int nks = epad / KSTEP;
-  typename M::frag bn[NT];
-#pragma unroll
-  for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
-  for (int ks = 0; ks < nks; ++ks) {
-    typename M::frag bf[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
-    if (ks + 1 < nks) {
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt)
-        bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+  if constexpr (KPL == 1) {
+    // FP32: scalar im2col reads, taps e1-fastest
+    float* Rw = reinterpret_cast<float*>(smem);                                 // [Wd][Wd]
+    int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // [epad]
+    for (int o = tid; o < Wd * Wd; o += 256) {
+      const int x = o % Wd, y = o / Wd;
+      const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
+      Rw[o] = (a1 >= 0 && a1 < g.n1 && a2 >= 0 && a2 < g.n2) ? R[a1 + g.n1 * a2] : 0.f;
     }
-    int o[KPL];
-#pragma unroll
-    for (int j = 0; j < KPL; ++j) o[j] = toff[ks * KSTEP + lg * KPL + j];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      typename M::frag af;
-      if constexpr (KPL == 1) {
-        af = Rw[base[mt] + o[0]];
-      } else {
-#pragma unroll
-        for (int j = 0; j < KPL; ++j) af[j] = Rw[base[mt] + o[j]];
+    for (int e = tid; e < epad; e += 256) {
+      int off = 0;  // padded taps have zero weights; any in-window read is fine
+      if (e < ntaps) {
+        const int e2 = e / K, e1 = e - e2 * K;
+        off = -(e1 - r) - (e2 - r) * Wd;
       }
+      toff[e] = off;
+    }
+    __syncthreads();
+    int base[MT];
 #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+    for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
+    typename M::frag bn[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
+    for (int ks = 0; ks < nks; ++ks) {
+      typename M::frag bf[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
+      if (ks + 1 < nks) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+      }
+      const int o = toff[ks * KSTEP + lg];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const typename M::frag af = Rw[base[mt] + o];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+      }
+    }
+  } else {
+    // BF16: taps ordered (e2, e1) with e1 padded to a multiple of 8, so a
+    // lane's 8 taps of a K-step are 8 consecutive e1 in one row.  Cell (x, y)
+    // of the pre-shifted window holds R(x-j, y) for j = 0..7, so each A
+    // fragment is a single 16-byte LDS read.
+    const int Kp8 = (K + 7) & ~7;
+    float* Rs = reinterpret_cast<float*>(smem);                                  // [Wd][Wd]
+    bf16x8* Rv = reinterpret_cast<bf16x8*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // [Wd][Wd]
+    int* toff8 = reinterpret_cast<int*>(reinterpret_cast<char*>(Rv) + sizeof(bf16x8) * Wd * Wd);
+    for (int o = tid; o < Wd * Wd; o += 256) {
+      const int x = o % Wd, y = o / Wd;
+      const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
+      Rs[o] = (a1 >= 0 && a1 < g.n1 && a2 >= 0 && a2 < g.n2) ? R[a1 + g.n1 * a2] : 0.f;
+    }
+    for (int q = tid; q < epad / 8; q += 256) {
+      const int t0 = 8 * q;
+      const int e2 = t0 / Kp8, e10 = t0 - e2 * Kp8;
+      toff8[q] = (e2 < K) ? (r - e10) - (e2 - r) * Wd : 0;
+    }
+    __syncthreads();
+    for (int o = tid; o < Wd * Wd; o += 256) {
+      const int x = o % Wd;
+      bf16x8 v;
+#pragma unroll
+      for (int j = 0; j < 8; ++j) v[j] = (__bf16)(x - j >= 0 ? Rs[o - j] : 0.f);
+      Rv[o] = v;
+    }
+    __syncthreads();
+    int base[MT];
+#pragma unroll
+    for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
+    typename M::frag bn[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
+    for (int ks = 0; ks < nks; ++ks) {
+      typename M::frag bf[NT];
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
+      if (ks + 1 < nks) {
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+      }
+      const int o = toff8[ks * 4 + lg];
+#pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const typename M::frag af = Rv[base[mt] + o];
+#pragma unroll
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+      }
     }
   }
 
-  // fused multiplicative update (recon:130-137)
+  // fused multiplicative update (recon:130-137).  Loads are unconditional
+  // from clamped indices (no divergent branch per element, so the compiler
+  // can batch them); only the stores are predicated.
   const int C1 = g.C1[z];
   const float* Tz = Tn + g.toffT[z];
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
     const int a2 = 16 * b2 + 4 * w + mt;
-    if (a2 >= g.n2) continue;
-    const int c2 = norm_class(a2, g.n2, r);
+    const bool ok2 = a2 < g.n2;
+    const int a2c = ok2 ? a2 : g.n2 - 1;
+    const int c2 = norm_class(a2c, g.n2, r);
 #pragma unroll
     for (int nt = 0; nt < NT; ++nt) {
       const int ph = grp * 16 * NT + nt * 16 + lr;
-      if (ph >= kP) continue;
+      const bool okp = ph < kP;
+      const int phc = okp ? ph : kP - 1;
+      float xv[4], b1v[4];
+      size_t idx[4];
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
         const int a1 = 16 * b1 + 4 * lg + j;
-        if (a1 >= g.n1) continue;
-        const int c1 = norm_class(a1, g.n1, r);
-        const float B1 = Tz[((size_t)c2 * C1 + c1) * kP + ph];
-        const size_t idx = (((size_t)z * g.n2 + a2) * g.n1 + a1) * kP + ph;
-        const float x = X[idx];
-        float C = x * acc[mt][nt][j] / B1;
+        const int a1c = a1 < g.n1 ? a1 : g.n1 - 1;
+        const int c1 = norm_class(a1c, g.n1, r);
+        b1v[j] = Tz[((size_t)c2 * C1 + c1) * kP + phc];
+        idx[j] = (((size_t)z * g.n2 + a2c) * g.n1 + a1c) * kP + phc;
+        xv[j] = X[idx[j]];
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int a1 = 16 * b1 + 4 * lg + j;
+        float C = xv[j] * acc[mt][nt][j] / b1v[j];
         if (isnan(C) || isinf(C) || C < 0.f) C = 0.f;
-        float xn = C * wgt + (1.f - wgt) * x;
+        float xn = C * wgt + (1.f - wgt) * xv[j];
         if (xn < 0.f) xn = 0.f;
-        X[idx] = xn;
+        if (ok2 && okp && a1 < g.n1) X[idx[j]] = xn;
       }
     }
   }
@@ -376,8 +686,18 @@ __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, P
   T* d = dst + (size_t)v * pg.dst_view + pg.dst_off[z];
   for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < (size_t)kPhiPadBwd * epad;
        o += (size_t)gridDim.x * blockDim.x) {
-    const int ph = (int)(o / epad), e = (int)(o % epad);
-    d[o] = (T)((e < ntaps && ph < kP) ? s[(size_t)e * kP + ph] : 0.f);
+    const int ph = (int)(o / epad), t = (int)(o % epad);
+    int e2, e1;
+    if (pg.row8) {  // (e2, e1) with e1 padded to a multiple of 8
+      const int Kp8 = (K + 7) & ~7;
+      e2 = t / Kp8;
+      e1 = t - e2 * Kp8;
+    } else {        // e1-fastest, dense
+      e2 = t / K;
+      e1 = t - e2 * K;
+    }
+    const bool ok = ph < kP && e2 < K && e1 < K;
+    d[o] = (T)(ok ? s[((size_t)e2 * K + e1) * kP + ph] : 0.f);
   }
 }
 
@@ -428,15 +748,16 @@ __global__ void k_norm_pass2(const double* __restrict__ S, float* __restrict__ T
 
 // ---------------------------------------------------------------- launchers
 size_t fwd_smem_bytes(Precision p, int rmax, int bm2) {
-  const size_t b = (p == Precision::kF32) ? sizeof(float) * 2 * kFwdChunk * Mma<float>::LDB
-                                          : sizeof(__bf16) * 2 * kFwdChunk * Mma<__bf16>::LDB;
+  const size_t b = (p == Precision::kF32)
+                       ? sizeof(float) * FwdCfg<float>::NBUF * kFwdChunk * Mma<float>::LDB
+                       : sizeof(__bf16) * FwdCfg<__bf16>::NBUF * kFwdChunk * Mma<__bf16>::LDB;
   return b + sizeof(float) * 4 * (16 + 2 * rmax) * (bm2 / 4 + 2 * rmax);
 }
 
 size_t bwd_smem_bytes(Precision p, int rmax, int epad_max) {
   const int Wd = 16 + 2 * rmax;
-  const size_t es = (p == Precision::kF32) ? sizeof(float) : sizeof(__bf16);
-  return es * ((Wd * Wd + 7) & ~7) + sizeof(int) * epad_max;
+  if (p == Precision::kF32) return sizeof(float) * ((Wd * Wd + 7) & ~7) + sizeof(int) * epad_max;
+  return sizeof(float) * ((Wd * Wd + 3) & ~3) + 16 * (size_t)Wd * Wd + sizeof(int) * (epad_max / 8);
 }
 
 hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
@@ -448,6 +769,44 @@ hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g,
     hipLaunchKernelGGL((k_fwd_mfma<__bf16, kFwdMTBF16>), dim3(nblocks), dim3(256), smem, st, X,
                        (const __bf16*)W, g, partial);
   }
+  return hipGetLastError();
+}
+
+size_t tpz_smem_bytes(Precision p, int rmax) {
+  const int MT = p == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
+  const size_t wb = (p == Precision::kF32) ? sizeof(float) * 2 * 16 * kTpzNT * Mma<float>::PHI
+                                           : sizeof(__bf16) * 2 * 16 * kTpzNT * Mma<__bf16>::PHI;
+  return wb + sizeof(float) * 2 * 4 * MT * kRingRow + sizeof(float) * 4 * 16 * (MT + 2 * rmax);
+}
+
+int tpz_max_radius() { return (16 * kTpzNT - 1) / 2; }
+
+hipError_t launch_fwd_tpz(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
+                          size_t smem, float* partial, hipStream_t st) {
+  if (p == Precision::kF32) {
+    hipLaunchKernelGGL((k_fwd_tpz<float, kTpzMTF32>), dim3(nblocks), dim3(256), smem, st, X,
+                       (const float*)W, g, partial);
+  } else {
+    hipLaunchKernelGGL((k_fwd_tpz<__bf16, kTpzMTBF16>), dim3(nblocks), dim3(256), smem, st, X,
+                       (const __bf16*)W, g, partial);
+  }
+  return hipGetLastError();
+}
+
+hipError_t launch_ratio_gather_tpz(const float* partial, FwdGeo g, int Z, int cw, const float* LF,
+                                   int u, int v, float* R, float* y_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ratio_gather_tpz, dim3((g.n1 * g.n2 + 255) / 256), dim3(256), 0, st,
+                     partial, g, Z, cw, LF, u, v, R, y_out);
+  return hipGetLastError();
+}
+
+hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
+                               hipStream_t st) {
+  dim3 grid(64, Z, V);
+  if (p == Precision::kF32)
+    hipLaunchKernelGGL(k_pack_fwd_tpz<float>, grid, dim3(256), 0, st, src, (float*)dst, pg);
+  else
+    hipLaunchKernelGGL(k_pack_fwd_tpz<__bf16>, grid, dim3(256), 0, st, src, (__bf16*)dst, pg);
   return hipGetLastError();
 }
 
@@ -511,6 +870,12 @@ hipError_t mfma_kernel_attributes() {
       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_mfma<__bf16, kFwdMTBF16>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_tpz<float, kTpzMTF32>),
+                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (e != hipSuccess) return e;
+  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, kTpzMTBF16>),
                           hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (e != hipSuccess) return e;
   e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bwd_mfma<float>),

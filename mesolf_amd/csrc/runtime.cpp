@@ -281,6 +281,7 @@ mlf_status mlf_context_memory(mlf_context ctx, uint64_t* free_bytes, uint64_t* t
 // PSF tables re-laid out for the MFMA kernels of one precision.
 struct PrecPack {
   bool built = false;
+  bool tpz = false;                   // forward tables in the Toeplitz layout [z][e1][e2pad][PHI]
   DevBuf<uint8_t> fwd, bwd;           // [V][z][e_pad64][PHI], [V][z][240][e_pad32]
   int64_t pvf = 0, pvb = 0;           // elements per view
   std::vector<int64_t> woff, boff;    // per-plane element offsets inside a view
@@ -347,13 +348,19 @@ static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
   pk.nchunk.assign(Z, 0);
   pk.epad.assign(Z, 0);
   std::vector<int32_t> rows_f(Z), rows_b(Z);
+  int rfmax = 0;
+  for (int r : ps->rf) rfmax = std::max(rfmax, r);
+  pk.tpz = rfmax <= tpz_max_radius();
   int64_t of = 0, ob = 0;
   for (int z = 0; z < Z; ++z) {
     const int nt_f = (2 * ps->rf[z] + 1) * (2 * ps->rf[z] + 1);
     const int nt_b = (2 * ps->rb[z] + 1) * (2 * ps->rb[z] + 1);
     pk.nchunk[z] = (nt_f + kFwdChunk - 1) / kFwdChunk;
-    rows_f[z] = pk.nchunk[z] * kFwdChunk;
-    pk.epad[z] = (nt_b + kBwdTapPad - 1) / kBwdTapPad * kBwdTapPad;
+    const int Kf = 2 * ps->rf[z] + 1;
+    rows_f[z] = pk.tpz ? Kf * 16 * ((Kf + 15) / 16) : pk.nchunk[z] * kFwdChunk;
+    const int Kb = 2 * ps->rb[z] + 1;
+    const int nt_b_pad = prec == Precision::kBF16 ? Kb * ((Kb + 7) & ~7) : nt_b;
+    pk.epad[z] = (nt_b_pad + kBwdTapPad - 1) / kBwdTapPad * kBwdTapPad;
     rows_b[z] = pk.epad[z];
     pk.woff[z] = of;
     of += (int64_t)rows_f[z] * phi;
@@ -373,10 +380,12 @@ static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
   MLF_CHECK(upload_vec(d_rows_f, rows_f));
   MLF_CHECK(upload_vec(d_rows_b, rows_b));
   hipStream_t st = ps->ctx->stream;
-  PackGeo gf{ps->d_rf.ptr, d_rows_f.ptr, ps->d_offf.ptr, pk.d_woff.ptr, ps->per_view_f, pk.pvf};
-  hipError_t e = launch_pack_fwd(prec, ps->fwd.ptr, pk.fwd.ptr, gf, Z, V, st);
+  PackGeo gf{ps->d_rf.ptr, d_rows_f.ptr, ps->d_offf.ptr, pk.d_woff.ptr, ps->per_view_f, pk.pvf, 0};
+  hipError_t e = pk.tpz ? launch_pack_fwd_tpz(prec, ps->fwd.ptr, pk.fwd.ptr, gf, Z, V, st)
+                        : launch_pack_fwd(prec, ps->fwd.ptr, pk.fwd.ptr, gf, Z, V, st);
   if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "pack_fwd: %s", hipGetErrorString(e));
-  PackGeo gb{ps->d_rb.ptr, d_rows_b.ptr, ps->d_offb.ptr, pk.d_boff.ptr, ps->per_view_b, pk.pvb};
+  PackGeo gb{ps->d_rb.ptr, d_rows_b.ptr, ps->d_offb.ptr, pk.d_boff.ptr, ps->per_view_b, pk.pvb,
+             prec == Precision::kBF16 ? 1 : 0};
   e = launch_pack_bwd(prec, ps->bwd.ptr, pk.bwd.ptr, gb, Z, V, st);
   if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "pack_bwd: %s", hipGetErrorString(e));
   MLF_HIP(hipStreamSynchronize(st));
@@ -730,6 +739,8 @@ struct mlf_patch_s {
   bool reference = false;              // FP32 reference kernels instead of MFMA
   Precision kp = Precision::kF32;
   int bm2 = 8;
+  bool tpz = false;                    // Toeplitz forward kernel
+  int cw = 16;                         // Toeplitz: input columns per workgroup
   DevBuf<float> partial;               // forward window partials
   DevBuf<int64_t> d_poff;
   DevBuf<int32_t> d_fsched, d_bsched;
@@ -882,12 +893,20 @@ static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, fl
   }
   const PrecPack& pk = ps->pack[(int)p->kp];
   const FwdGeo g = fwd_geo(p);
+  const void* Wk = pk.fwd.ptr + (size_t)k * pk.pvf * elem_bytes(p->kp);
   MLF_CHECK(tim_begin(p, 0));
-  MLF_LAUNCH(launch_fwd_mfma(p->kp, p->X.ptr, pk.fwd.ptr + (size_t)k * pk.pvf * elem_bytes(p->kp), g,
-                             p->nblk_f, p->smem_f, p->partial.ptr, st));
+  if (p->tpz) {
+    MLF_LAUNCH(launch_fwd_tpz(p->kp, p->X.ptr, Wk, g, p->nblk_f, p->smem_f, p->partial.ptr, st));
+  } else {
+    MLF_LAUNCH(launch_fwd_mfma(p->kp, p->X.ptr, Wk, g, p->nblk_f, p->smem_f, p->partial.ptr, st));
+  }
   MLF_CHECK(tim_end(p));
   MLF_CHECK(tim_begin(p, 1));
-  MLF_LAUNCH(launch_ratio_gather(p->partial.ptr, g, p->Z, p->bm2, LF, u, v, p->R.ptr, y_out, st));
+  if (p->tpz) {
+    MLF_LAUNCH(launch_ratio_gather_tpz(p->partial.ptr, g, p->Z, p->cw, LF, u, v, p->R.ptr, y_out, st));
+  } else {
+    MLF_LAUNCH(launch_ratio_gather(p->partial.ptr, g, p->Z, p->bm2, LF, u, v, p->R.ptr, y_out, st));
+  }
   MLF_CHECK(tim_end(p));
   return MLF_OK;
 }
@@ -927,8 +946,11 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
   MLF_CHECK(psf_ensure_pack(ps, p->kp));
   MLF_CHECK(psf_ensure_norm(ps, p->kp, p->n1, p->n2, &p->norm));
   const int Z = p->Z;
+  p->tpz = ps->pack[(int)p->kp].tpz;
   p->bm2 = fwd_bm2(p->kp);
-  const int nb1 = (p->n1 + 15) / 16, nb2 = (p->n2 + p->bm2 - 1) / p->bm2;
+  p->cw = 4 * (p->kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16);
+  const int colw = p->tpz ? p->cw : p->bm2;  // lenslets per block along axis 1
+  const int nb1 = (p->n1 + 15) / 16, nb2 = (p->n2 + colw - 1) / colw;
   std::vector<int> order(Z);
   for (int z = 0; z < Z; ++z) order[z] = z;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ps->rf[a] > ps->rf[b]; });
@@ -941,7 +963,8 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
     rbmax = std::max(rbmax, ps->rb[z]);
     epmax = std::max(epmax, ps->pack[(int)p->kp].epad[z]);
     poff[z] = po;
-    po += (int64_t)nb1 * nb2 * (16 + 2 * r) * (p->bm2 + 2 * r);
+    po += p->tpz ? (int64_t)nb1 * nb2 * 16 * (p->cw + 2 * r)
+                 : (int64_t)nb1 * nb2 * (16 + 2 * r) * (p->bm2 + 2 * r);
   }
   std::vector<int32_t> fs;
   for (int z : order)
@@ -972,7 +995,7 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
   MLF_CHECK(upload_vec(p->d_fsched, fs));
   MLF_CHECK(upload_vec(p->d_bsched, bs));
   MLF_CHECK(p->partial.alloc((size_t)po));
-  p->smem_f = fwd_smem_bytes(p->kp, rfmax, p->bm2);
+  p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
   p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax);
   if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
     return set_error(MLF_ERR_UNSUPPORTED,
