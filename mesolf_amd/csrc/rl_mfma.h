@@ -20,6 +20,7 @@ constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases
 constexpr int kBwdTapPad = 32;    // backward taps padded to a multiple of 32
 constexpr int kTpzMTF32 = 2;      // Toeplitz forward: input columns per wave, f32
 constexpr int kTpzMTBF16 = 4;     // Toeplitz forward: input columns per wave, bf16
+constexpr int kShadowRow = 256;   // BF16 volume shadow: phases per lenslet (zero beyond 225)
 
 inline int fwd_bm2(Precision p) { return 4 * (p == Precision::kF32 ? kFwdMTF32 : kFwdMTBF16); }
 inline int fwd_phi(Precision p) { return p == Precision::kF32 ? kPhiPadF32 : kPhiPadBF16; }
@@ -81,8 +82,9 @@ __host__ __device__ inline void norm_range(int c, int n, int r, int* lo, int* hi
 size_t fwd_smem_bytes(Precision p, int rmax, int bm2);
 size_t tpz_smem_bytes(Precision p, int rmax);
 int tpz_max_radius();
-hipError_t launch_fwd_tpz(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
-                          size_t smem, float* partial, hipStream_t st);
+hipError_t launch_fwd_tpz(Precision p, const float* X, const void* Xb, const void* W, FwdGeo g,
+                          int nblocks, size_t smem, float* partial, hipStream_t st);
+hipError_t launch_make_shadow(const float* X, void* Xb, size_t nlens, hipStream_t st);
 hipError_t launch_ratio_gather_tpz(const float* partial, FwdGeo g, int Z, int cw, const float* LF,
                                    int u, int v, float* R, float* y_out, hipStream_t st);
 hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
@@ -93,8 +95,9 @@ hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g,
                            size_t smem, float* partial, hipStream_t st);
 hipError_t launch_ratio_gather(const float* partial, FwdGeo g, int Z, int bm2, const float* LF,
                                int u, int v, float* R, float* y_out, hipStream_t st);
-hipError_t launch_bwd_mfma(Precision p, float* X, const void* Wb, const float* Tn, BwdGeo g,
-                           int nblocks, size_t smem, const float* R, float w, hipStream_t st);
+hipError_t launch_bwd_mfma(Precision p, float* X, void* Xb, const void* Wb, const float* Tn,
+                           BwdGeo g, int nblocks, size_t smem, const float* R, float w,
+                           hipStream_t st);
 hipError_t launch_pack_fwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                            hipStream_t st);
 hipError_t launch_pack_bwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,

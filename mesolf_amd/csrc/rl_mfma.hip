@@ -54,6 +54,26 @@ struct Mma<__bf16> {
 };
 
 __device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
+// s_waitcnt vmcnt(<= n) without waiting on LDS/export counters (gfx9 encoding:
+// vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
+template <int N>
+__device__ __forceinline__ void wait_vm() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+__device__ __forceinline__ void wait_vm_le(int n) {
+  switch (n) {
+#define MLF_W(k) case k: wait_vm<k>(); break;
+    MLF_W(0) MLF_W(1) MLF_W(2) MLF_W(3) MLF_W(4) MLF_W(5) MLF_W(6) MLF_W(7) MLF_W(8) MLF_W(9)
+    MLF_W(10) MLF_W(11) MLF_W(12) MLF_W(13) MLF_W(14) MLF_W(15) MLF_W(16) MLF_W(17) MLF_W(18)
+    MLF_W(19) MLF_W(20) MLF_W(21) MLF_W(22) MLF_W(23) MLF_W(24) MLF_W(25) MLF_W(26) MLF_W(27)
+    MLF_W(28) MLF_W(29) MLF_W(30) MLF_W(31) MLF_W(32) MLF_W(33) MLF_W(34) MLF_W(35) MLF_W(36)
+    MLF_W(37) MLF_W(38) MLF_W(39) MLF_W(40)
+#undef MLF_W
+    default: wait_vm<0>(); break;
+  }
+}
+
 
 // ------------------------------------------------------------------ forward
 template <typename T>
@@ -222,10 +242,19 @@ void k_fwd_mfma(const float* __restrict__ X,
 // MFMAs run.  The sum over e1 happens inside the MFMA accumulators, so only
 // the final P tile is scattered (a few LDS atomics per wave and plane).
 constexpr int kTpzNT = 3;      // N-tiles (taps along axis 1): K <= 48, i.e. r <= 23
-constexpr int kRingRow = 256;  // floats per lenslet row of the injection ring (phases >= 225 stay 0)
+template <typename T>
+struct TpzBufs {
+  static constexpr int value = 2;  // FP32: W buffers are 45 KB, prefetch one step ahead
+};
+template <>
+struct TpzBufs<__bf16> {
+  static constexpr int value = 3;  // BF16: prefetch two steps ahead
+};
+constexpr int kRingRow = 256;  // elements per lenslet row of the injection ring (phases >= 225 stay 0)
 
 template <typename T, int MT>
 __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
+                                                 const __bf16* __restrict__ Xb,
                                                  const T* __restrict__ W, FwdGeo g,
                                                  float* __restrict__ partial) {
   using M = Mma<T>;
@@ -235,10 +264,13 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
   constexpr int VEC = 16 / sizeof(T);
   constexpr int RV = PHI / VEC;
   constexpr bool SWZ = (RV == 32);
+  constexpr int NB = TpzBufs<T>::value;                        // buffers; DMA runs NB-1 steps ahead
+  constexpr int WSLOT = ((16 * kTpzNT * RV + 255) / 256) * 256;  // 16-byte slots per W buffer
   extern __shared__ __align__(16) char smem[];
-  T* Wb = reinterpret_cast<T*>(smem);  // [2][16*kTpzNT][PHI]
-  float* ring = reinterpret_cast<float*>(smem + sizeof(T) * 2 * 16 * kTpzNT * PHI);  // [2][CW][kRingRow]
-  float* ywin = ring + 2 * CW * kRingRow;  // [4 waves][16][MT + 2r]
+  T* Wb = reinterpret_cast<T*>(smem);                                        // [NB][WSLOT*VEC]
+  // injection ring: FP32 volume rows (FP32) or BF16 shadow rows (BF16), kRingRow elements each
+  T* ring = reinterpret_cast<T*>(smem + (size_t)NB * WSLOT * 16);            // [NB][CW][kRingRow]
+  float* ywin = reinterpret_cast<float*>(ring + NB * CW * kRingRow);         // [4 waves][16][MT + 2r]
 
   const int* s = g.sched + 3 * blockIdx.x;
   const int z = s[0], b1 = s[1], b2 = s[2];
@@ -252,7 +284,7 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
   const float* Xz = X + (size_t)z * g.n2 * g.n1 * kP;
   float* yw = ywin + w * 16 * WW;
   for (int o = tid; o < 4 * 16 * WW; o += 256) ywin[o] = 0.f;
-  for (int o = tid; o < 2 * CW * kRingRow; o += 256) ring[o] = 0.f;  // pads stay zero
+  for (int o = tid; o < NB * CW * kRingRow; o += 256) ring[o] = (T)0.f;  // pads stay zero
 
   // A operand at e1 = 0 (d1 = -r): lane row lr holds lenslet i1 = 16*b1 + lr + r
   typename M::frag a[MT][NKS];
@@ -261,55 +293,66 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
     const int c2 = c2_0 + MT * w + mt;
     const int i1 = 16 * b1 + lr + r;
     const bool ok = c2 < g.n2 && i1 < g.n1;
-    const float* xp = Xz + ((size_t)(ok ? c2 : 0) * g.n1 + (ok ? i1 : 0)) * kP;
+    const size_t lens = ((size_t)z * g.n2 + (ok ? c2 : 0)) * g.n1 + (ok ? i1 : 0);
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
-#pragma unroll
-      for (int j = 0; j < KPL; ++j) {
-        const int ph = ks * KSTEP + lg * KPL + j;
-        const float v = xp[ph < kP ? ph : kP - 1];
-        const float val = (ok && ph < kP) ? v : 0.f;
-        if constexpr (KPL == 1) a[mt][ks] = val;
-        else a[mt][ks][j] = (__bf16)val;
+      if constexpr (KPL == 1) {
+        const int ph = ks * KSTEP + lg;
+        const float v = X[lens * kP + (ph < kP ? ph : kP - 1)];
+        a[mt][ks] = (ok && ph < kP) ? v : 0.f;
+      } else {
+        // BF16 shadow rows are 256 phases (zero beyond 225), 16-byte aligned
+        const bf16x8 v = *reinterpret_cast<const bf16x8*>(Xb + lens * kShadowRow + ks * KSTEP + lg * KPL);
+        a[mt][ks] = ok ? v : bf16x8{};
       }
     }
   }
+  __syncthreads();  // ring zeroed before any DMA lands
 
   const T* Wz = W + g.woff[z];  // [K][nrow][PHI]
+  // Every wave issues the same number of DMA instructions per step (W: WSLOT
+  // slots / 256 threads; ring: CW*RPASS passes / 4 waves), so a step can wait
+  // for an earlier batch with a fixed vmcnt.
+  const int nslot = nrow * RV;
   auto issue_w = [&](int t, int buf) {
     const T* src = Wz + (size_t)t * nrow * PHI;
-    char* dst = reinterpret_cast<char*>(Wb) + (size_t)buf * 16 * kTpzNT * PHI * sizeof(T);
-    const int nslot = nrow * RV;
-    for (int base = w * 64; base < nslot; base += 256) {
-      const int slot = base + lane;
-      if (slot < nslot) {
-        const int row = slot / RV, pos = slot - row * RV;
-        const int col = SWZ ? (pos ^ (row & 31)) : pos;
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
-            (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
-      }
+    char* dst = reinterpret_cast<char*>(Wb) + (size_t)buf * WSLOT * 16;
+    for (int base = w * 64; base < WSLOT; base += 256) {
+      const int slot = min(base + lane, nslot - 1);  // tail slots re-read the last vector
+      const int row = slot / RV, pos = slot - row * RV;
+      const int col = SWZ ? (pos ^ (row & 31)) : pos;
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
+          (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
     }
   };
   // injection row needed when entering step t (t >= 1): i1 = 16*b1 + r - t
+  // (rows outside the grid are loaded clamped and zeroed at the shift)
+  // FP32: 225 floats per lenslet in 4 passes of 64 dwords; BF16: the 512-byte
+  // shadow row in 2 passes of 64 dwords.
+  constexpr int RPASS = (KPL == 1) ? 4 : 2;
   auto issue_inj = [&](int t, int slot_buf) {
-    const int i1 = 16 * b1 + r - t;
-    if (i1 < 0 || i1 >= g.n1) return;
-    float* dst = ring + slot_buf * CW * kRingRow;
-    // rows of 225 phases in 4 passes of 64 dwords (the DMA writes 64
-    // consecutive LDS dwords per wave instruction)
-    constexpr int NPASS = CW * 4;
-    for (int q = w; q < NPASS; q += 4) {
-      const int c = q >> 2, ph = (q & 3) * 64 + lane;
-      const int c2 = c2_0 + c;
-      if (ph < kP && c2 < g.n2) {
-        const float* src = Xz + ((size_t)c2 * g.n1 + i1) * kP + ph;
-        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                         (__attribute__((address_space(3))) void*)(dst + c * kRingRow + (q & 3) * 64),
-                                         4, 0, 0);
+    const int i1 = min(max(16 * b1 + r - t, 0), g.n1 - 1);
+    T* dst = ring + slot_buf * CW * kRingRow;
+    for (int q = w; q < CW * RPASS; q += 4) {
+      const int c = q / RPASS, part = q - c * RPASS;
+      const int c2 = min(c2_0 + c, g.n2 - 1);
+      const size_t lens = ((size_t)z * g.n2 + c2) * g.n1 + i1;
+      const int d = part * 64 + lane;  // dword within the row
+      const void* src;
+      if constexpr (KPL == 1) {
+        if (d >= kP) continue;
+        src = X + lens * kP + d;
+      } else {
+        src = reinterpret_cast<const char*>(Xb + lens * kShadowRow) + 4 * d;
       }
+      __builtin_amdgcn_global_load_lds(
+          (const __attribute__((address_space(1))) void*)src,
+          (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(dst + c * kRingRow) + 4 * part * 64),
+          4, 0, 0);
     }
   };
+  const int n_batch = WSLOT / 256 + CW * RPASS / 4;  // DMA instructions per wave, step t >= 1
 
   floatx4 acc[MT][kTpzNT];
 #pragma unroll
@@ -317,37 +360,59 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
 #pragma unroll
     for (int nt = 0; nt < kTpzNT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
 
+  // batch t = W column t (+ injection row t for t >= 1); batches with index
+  // >= 1 are n_batch instructions per wave
   issue_w(0, 0);
-  __builtin_amdgcn_s_waitcnt(0);
+  for (int t = 1; t < NB - 1 && t < K; ++t) {
+    issue_w(t, t % NB);
+    issue_inj(t, t % NB);
+  }
+  wait_vm_le(max(0, min(K, NB - 1) - 1) * n_batch);  // batch 0 landed
   __syncthreads();
   for (int t = 0; t < K; ++t) {
-    if (t + 1 < K) {
-      issue_w(t + 1, (t + 1) & 1);
-      issue_inj(t + 1, (t + 1) & 1);
+    if (t + NB - 1 < K) {
+      issue_w(t + NB - 1, (t + NB - 1) % NB);
+      issue_inj(t + NB - 1, (t + NB - 1) % NB);
     }
-    const T* Wc = Wb + (size_t)(t & 1) * 16 * kTpzNT * PHI;
+    const T* Wc = Wb + (size_t)(t % NB) * WSLOT * VEC;
+    auto load_b = [&](int ks, int nt) {
+      const int row = nt * 16 + lr;
+      if constexpr (SWZ) {
+        const int col = (ks * KSTEP + lg * KPL) / VEC;
+        return M::load(Wc + row * PHI + ((col ^ (row & 31)) * VEC));
+      } else {
+        return M::load(Wc + row * PHI + ks * KSTEP + lg * KPL);
+      }
+    };
+    // B fragments of the next K-step are read while this step's MFMAs run
+    typename M::frag bcur[kTpzNT], bnxt[kTpzNT];
+#pragma unroll
+    for (int nt = 0; nt < kTpzNT; ++nt)
+      if (nt < NT) bcur[nt] = load_b(0, nt);
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
+      if (ks + 1 < NKS) {
+#pragma unroll
+        for (int nt = 0; nt < kTpzNT; ++nt)
+          if (nt < NT) bnxt[nt] = load_b(ks + 1, nt);
+      }
 #pragma unroll
       for (int nt = 0; nt < kTpzNT; ++nt) {
         if (nt >= NT) break;
-        const int row = nt * 16 + lr;
-        typename M::frag bf;
-        if constexpr (SWZ) {
-          const int col = (ks * KSTEP + lg * KPL) / VEC;
-          bf = M::load(Wc + row * PHI + ((col ^ (row & 31)) * VEC));
-        } else {
-          bf = M::load(Wc + row * PHI + ks * KSTEP + lg * KPL);
-        }
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
+        for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bcur[nt], acc[mt][nt]);
+      }
+      if (ks + 1 < NKS) {
+#pragma unroll
+        for (int nt = 0; nt < kTpzNT; ++nt) bcur[nt] = bnxt[nt];
       }
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();  // column t+1 and injection row landed; column t no longer read
+    // batch t+1 landed; later batches may still be in flight
+    wait_vm_le(max(0, min(K, t + NB) - t - 2) * n_batch);
+    __syncthreads();
     if (t + 1 < K) {
       const int i1n = 16 * b1 + r - (t + 1);
-      const float* rg = ring + ((t + 1) & 1) * CW * kRingRow;
+      const T* rg = ring + ((t + 1) % NB) * CW * kRingRow;
       const bool rowok = (i1n >= 0) && (i1n < g.n1);
 #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
@@ -355,7 +420,7 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
         const bool ok = rowok && (c2_0 + c < g.n2);
         // every lane reads its row group's slot (broadcast); only lane 0 of
         // each 16-lane row keeps it (the DPP 'old' operand)
-        const float* rp = rg + c * kRingRow + lg * KPL;
+        const T* rp = rg + c * kRingRow + lg * KPL;
 #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
           if constexpr (KPL == 1) {
@@ -366,12 +431,8 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
                                                    __builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF,
                                                    false));
           } else {
-            bf16x8 inj;
-#pragma unroll
-            for (int j = 0; j < KPL; ++j) {
-              const float v = rp[ks * KSTEP + j];
-              inj[j] = (__bf16)(ok ? v : 0.f);
-            }
+            const bf16x8 v = *reinterpret_cast<const bf16x8*>(rp + ks * KSTEP);
+            const bf16x8 inj = ok ? v : bf16x8{};
             typedef int intx4 __attribute__((ext_vector_type(4)));
             const intx4 iv = __builtin_bit_cast(intx4, inj);
             intx4 cv = __builtin_bit_cast(intx4, a[mt][ks]);
@@ -499,7 +560,8 @@ __device__ __forceinline__ int norm_class(int a, int n, int r) {
 }
 
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_mfma(float* __restrict__ X, const T* __restrict__ Wb,
+__global__ __launch_bounds__(256) void k_bwd_mfma(float* __restrict__ X, __bf16* __restrict__ Xsh,
+                                                  const T* __restrict__ Wb,
                                                   const float* __restrict__ Tn, BwdGeo g,
                                                   const float* __restrict__ R, float wgt) {
   using M = Mma<T>;
@@ -639,8 +701,8 @@ __global__ __launch_bounds__(256) void k_bwd_mfma(float* __restrict__ X, const T
         const int a1c = a1 < g.n1 ? a1 : g.n1 - 1;
         const int c1 = norm_class(a1c, g.n1, r);
         b1v[j] = Tz[((size_t)c2 * C1 + c1) * kP + phc];
-        idx[j] = (((size_t)z * g.n2 + a2c) * g.n1 + a1c) * kP + phc;
-        xv[j] = X[idx[j]];
+        idx[j] = (((size_t)z * g.n2 + a2c) * g.n1 + a1c);  // lenslet; element = lenslet*kP + phc
+        xv[j] = X[idx[j] * kP + phc];
       }
 #pragma unroll
       for (int j = 0; j < 4; ++j) {
@@ -649,7 +711,10 @@ __global__ __launch_bounds__(256) void k_bwd_mfma(float* __restrict__ X, const T
         if (isnan(C) || isinf(C) || C < 0.f) C = 0.f;
         float xn = C * wgt + (1.f - wgt) * xv[j];
         if (xn < 0.f) xn = 0.f;
-        if (ok2 && okp && a1 < g.n1) X[idx[j]] = xn;
+        if (ok2 && okp && a1 < g.n1) {
+          X[idx[j] * kP + phc] = xn;
+          if (Xsh) Xsh[idx[j] * kShadowRow + phc] = (__bf16)xn;
+        }
       }
     }
   }
@@ -773,23 +838,42 @@ hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g,
 }
 
 size_t tpz_smem_bytes(Precision p, int rmax) {
-  const int MT = p == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
-  const size_t wb = (p == Precision::kF32) ? sizeof(float) * 2 * 16 * kTpzNT * Mma<float>::PHI
-                                           : sizeof(__bf16) * 2 * 16 * kTpzNT * Mma<__bf16>::PHI;
-  return wb + sizeof(float) * 2 * 4 * MT * kRingRow + sizeof(float) * 4 * 16 * (MT + 2 * rmax);
+  const bool f32 = p == Precision::kF32;
+  const int MT = f32 ? kTpzMTF32 : kTpzMTBF16;
+  const int RV = f32 ? Mma<float>::PHI / 4 : Mma<__bf16>::PHI / 8;
+  const int NB = f32 ? TpzBufs<float>::value : TpzBufs<__bf16>::value;
+  const size_t wslot = ((16 * kTpzNT * RV + 255) / 256) * 256;
+  return (size_t)NB * wslot * 16 + elem_bytes(p) * NB * 4 * MT * kRingRow +
+         sizeof(float) * 4 * 16 * (MT + 2 * rmax);
 }
 
 int tpz_max_radius() { return (16 * kTpzNT - 1) / 2; }
 
-hipError_t launch_fwd_tpz(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
-                          size_t smem, float* partial, hipStream_t st) {
+hipError_t launch_fwd_tpz(Precision p, const float* X, const void* Xb, const void* W, FwdGeo g,
+                          int nblocks, size_t smem, float* partial, hipStream_t st) {
   if (p == Precision::kF32) {
     hipLaunchKernelGGL((k_fwd_tpz<float, kTpzMTF32>), dim3(nblocks), dim3(256), smem, st, X,
-                       (const float*)W, g, partial);
+                       (const __bf16*)Xb, (const float*)W, g, partial);
   } else {
     hipLaunchKernelGGL((k_fwd_tpz<__bf16, kTpzMTBF16>), dim3(nblocks), dim3(256), smem, st, X,
-                       (const __bf16*)W, g, partial);
+                       (const __bf16*)Xb, (const __bf16*)W, g, partial);
   }
+  return hipGetLastError();
+}
+
+__global__ void k_make_shadow(const float* __restrict__ X, __bf16* __restrict__ Xb, size_t nlens) {
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < nlens * kShadowRow;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const size_t l = o / kShadowRow;
+    const int ph = (int)(o - l * kShadowRow);
+    Xb[o] = (__bf16)(ph < kP ? X[l * kP + ph] : 0.f);
+  }
+}
+
+hipError_t launch_make_shadow(const float* X, void* Xb, size_t nlens, hipStream_t st) {
+  size_t g = (nlens * kShadowRow + 255) / 256;
+  if (g > 65536) g = 65536;
+  hipLaunchKernelGGL(k_make_shadow, dim3((unsigned)g), dim3(256), 0, st, X, (__bf16*)Xb, nlens);
   return hipGetLastError();
 }
 
@@ -817,13 +901,14 @@ hipError_t launch_ratio_gather(const float* partial, FwdGeo g, int Z, int bm2, c
   return hipGetLastError();
 }
 
-hipError_t launch_bwd_mfma(Precision p, float* X, const void* Wb, const float* Tn, BwdGeo g,
-                           int nblocks, size_t smem, const float* R, float w, hipStream_t st) {
+hipError_t launch_bwd_mfma(Precision p, float* X, void* Xb, const void* Wb, const float* Tn,
+                           BwdGeo g, int nblocks, size_t smem, const float* R, float w,
+                           hipStream_t st) {
   if (p == Precision::kF32) {
-    hipLaunchKernelGGL(k_bwd_mfma<float>, dim3(nblocks), dim3(256), smem, st, X,
+    hipLaunchKernelGGL(k_bwd_mfma<float>, dim3(nblocks), dim3(256), smem, st, X, (__bf16*)Xb,
                        (const float*)Wb, Tn, g, R, w);
   } else {
-    hipLaunchKernelGGL(k_bwd_mfma<__bf16>, dim3(nblocks), dim3(256), smem, st, X,
+    hipLaunchKernelGGL(k_bwd_mfma<__bf16>, dim3(nblocks), dim3(256), smem, st, X, (__bf16*)Xb,
                        (const __bf16*)Wb, Tn, g, R, w);
   }
   return hipGetLastError();

@@ -742,6 +742,7 @@ struct mlf_patch_s {
   bool tpz = false;                    // Toeplitz forward kernel
   int cw = 16;                         // Toeplitz: input columns per workgroup
   DevBuf<float> partial;               // forward window partials
+  DevBuf<uint16_t> Xb;                 // BF16 shadow of X, [lenslet][kShadowRow] (BF16 Toeplitz)
   DevBuf<int64_t> d_poff;
   DevBuf<int32_t> d_fsched, d_bsched;
   int nblk_f = 0, nblk_b = 0;
@@ -896,7 +897,8 @@ static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, fl
   const void* Wk = pk.fwd.ptr + (size_t)k * pk.pvf * elem_bytes(p->kp);
   MLF_CHECK(tim_begin(p, 0));
   if (p->tpz) {
-    MLF_LAUNCH(launch_fwd_tpz(p->kp, p->X.ptr, Wk, g, p->nblk_f, p->smem_f, p->partial.ptr, st));
+    MLF_LAUNCH(launch_fwd_tpz(p->kp, p->X.ptr, p->Xb.ptr, Wk, g, p->nblk_f, p->smem_f,
+                              p->partial.ptr, st));
   } else {
     MLF_LAUNCH(launch_fwd_mfma(p->kp, p->X.ptr, Wk, g, p->nblk_f, p->smem_f, p->partial.ptr, st));
   }
@@ -926,7 +928,8 @@ static mlf_status patch_view_step(mlf_patch_s* p, int k) {
                                            p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
   } else {
     const PrecPack& pk = ps->pack[(int)p->kp];
-    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->X.ptr, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
+    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->X.ptr, p->Xb.ptr,
+                               pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
                                p->norm->T.ptr + (size_t)k * p->norm->pvt, bwd_geo(p), p->nblk_b,
                                p->smem_b, p->R.ptr, (float)ps->weight[k], st));
   }
@@ -995,11 +998,21 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
   MLF_CHECK(upload_vec(p->d_fsched, fs));
   MLF_CHECK(upload_vec(p->d_bsched, bs));
   MLF_CHECK(p->partial.alloc((size_t)po));
+  if (p->tpz && p->kp == Precision::kBF16)
+    MLF_CHECK(p->Xb.alloc((size_t)p->n1 * p->n2 * Z * kShadowRow));
   p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
   p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax);
   if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
     return set_error(MLF_ERR_UNSUPPORTED,
                      "PSF radius too large for the LDS tiles (rf_max=%d, rb_max=%d)", rfmax, rbmax);
+  return MLF_OK;
+}
+
+// Re-derive the BF16 shadow after X changed outside the backprojection.
+static mlf_status patch_refresh_shadow(mlf_patch_s* p) {
+  if (!p->Xb.ptr) return MLF_OK;
+  MLF_LAUNCH(launch_make_shadow(p->X.ptr, p->Xb.ptr, (size_t)p->n1 * p->n2 * p->Z,
+                                p->psf->ctx->stream));
   return MLF_OK;
 }
 
@@ -1011,6 +1024,7 @@ static mlf_status patch_filter(mlf_patch_s* p) {
   MLF_LAUNCH(launch_filter_pass1(p->X.ptr, p->tmp.ptr, fp, p->n1, p->n2, p->pb.ptr, p->nblk, st));
   MLF_LAUNCH(launch_filter_pass2(p->tmp.ptr, p->X.ptr, fp, p->n1, p->n2, p->pa.ptr, p->nblk, st));
   MLF_LAUNCH(launch_filter_scale(p->X.ptr, fp, p->n1, p->n2, p->pb.ptr, p->pa.ptr, p->nblk, st));
+  MLF_CHECK(patch_refresh_shadow(p));
   MLF_CHECK(tim_end(p));
   return MLF_OK;
 }
@@ -1102,6 +1116,7 @@ mlf_status mlf_patch_init_volume(mlf_patch p) {
   const double numel = (double)p->n1 * kN * p->n2 * kN * p->Z;
   MLF_LAUNCH(launch_init_volume(p->LF.ptr, p->LF.count, p->dsum.ptr, p->X.ptr, p->X.count,
                                 1.0 / numel, p->psf->ctx->stream));
+  MLF_CHECK(patch_refresh_shadow(p));
   p->volume_set = true;
   return MLF_OK;
 }
@@ -1116,6 +1131,7 @@ mlf_status mlf_patch_set_volume(mlf_patch p, const float* X) {
   MLF_CHECK(tmp.alloc(p->X.count));
   MLF_HIP(hipMemcpyAsync(tmp.ptr, X, tmp.bytes(), hipMemcpyHostToDevice, st));
   MLF_LAUNCH(launch_to_blocked(tmp.ptr, p->X.ptr, p->n1, p->n2, p->Z, st));
+  MLF_CHECK(patch_refresh_shadow(p));
   MLF_HIP(hipStreamSynchronize(st));
   p->volume_set = true;
   return MLF_OK;
