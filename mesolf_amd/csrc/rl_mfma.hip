@@ -242,13 +242,12 @@ void k_fwd_mfma(const float* __restrict__ X,
 // MFMAs run.  The sum over e1 happens inside the MFMA accumulators, so only
 // the final P tile is scattered (a few LDS atomics per wave and plane).
 constexpr int kTpzNT = 3;      // N-tiles (taps along axis 1): K <= 48, i.e. r <= 23
+// W/ring buffers: the DMA runs NB-1 steps ahead.  Two buffers keep the BF16
+// kernel at 75 KB of LDS (two workgroups per CU, which measured 1.9x faster
+// than three buffers at one workgroup per CU); FP32 W buffers are 45 KB each.
 template <typename T>
 struct TpzBufs {
-  static constexpr int value = 2;  // FP32: W buffers are 45 KB, prefetch one step ahead
-};
-template <>
-struct TpzBufs<__bf16> {
-  static constexpr int value = 3;  // BF16: prefetch two steps ahead
+  static constexpr int value = 2;
 };
 constexpr int kRingRow = 256;  // elements per lenslet row of the injection ring (phases >= 225 stay 0)
 
@@ -384,27 +383,14 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
         return M::load(Wc + row * PHI + ks * KSTEP + lg * KPL);
       }
     };
-    // B fragments of the next K-step are read while this step's MFMAs run
-    typename M::frag bcur[kTpzNT], bnxt[kTpzNT];
-#pragma unroll
-    for (int nt = 0; nt < kTpzNT; ++nt)
-      if (nt < NT) bcur[nt] = load_b(0, nt);
 #pragma unroll
     for (int ks = 0; ks < NKS; ++ks) {
-      if (ks + 1 < NKS) {
-#pragma unroll
-        for (int nt = 0; nt < kTpzNT; ++nt)
-          if (nt < NT) bnxt[nt] = load_b(ks + 1, nt);
-      }
 #pragma unroll
       for (int nt = 0; nt < kTpzNT; ++nt) {
         if (nt >= NT) break;
+        const typename M::frag bf = load_b(ks, nt);
 #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bcur[nt], acc[mt][nt]);
-      }
-      if (ks + 1 < NKS) {
-#pragma unroll
-        for (int nt = 0; nt < kTpzNT; ++nt) bcur[nt] = bnxt[nt];
+        for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
       }
     }
     // batch t+1 landed; later batches may still be in flight
@@ -559,8 +545,12 @@ __device__ __forceinline__ int norm_class(int a, int n, int r) {
   return a < r ? a : (a > n - 1 - r ? 2 * r - (n - 1 - a) : r);
 }
 
+// Dword 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
+// no swizzle, no index stride).
+constexpr int kBufferDword3 = 0x00020000;
+
 template <typename T>
-__global__ __launch_bounds__(256) void k_bwd_mfma(float* __restrict__ X, __bf16* __restrict__ Xsh,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_bwd_mfma(float* __restrict__ X, __bf16* __restrict__ Xsh,
                                                   const T* __restrict__ Wb,
                                                   const float* __restrict__ Tn, BwdGeo g,
                                                   const float* __restrict__ R, float wgt) {
@@ -678,42 +668,78 @@ __global__ __launch_bounds__(256) void k_bwd_mfma(float* __restrict__ X, __bf16*
   }
 
   // fused multiplicative update (recon:130-137).  Loads are unconditional
-  // from clamped indices (no divergent branch per element, so the compiler
-  // can batch them); only the stores are predicated.
-  const int C1 = g.C1[z];
-  const float* Tz = Tn + g.toffT[z];
+  // from clamped indices and issued one M-tile ahead (40 in flight per lane),
+  // so HBM latency is paid a few times per workgroup rather than once per
+  // N-tile; only the stores are predicated.  Buffer instructions take 32-bit
+  // offsets from per-plane resources (one VGPR per address; a plane is below
+  // 2^31 bytes, checked at patch creation) and drop any access outside the
+  // plane.
+  const int C1 = g.C1[z], C2 = norm_classes(g.n2, r);
+  const size_t nlp = (size_t)g.n2 * g.n1;  // lenslets per plane
+  const __amdgpu_buffer_rsrc_t xr =
+      __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kP, 0, (int)(nlp * kP * 4), kBufferDword3);
+  const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
+      const_cast<float*>(Tn + g.toffT[z]), 0, C2 * C1 * kP * 4, kBufferDword3);
+  const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc(
+      Xsh ? Xsh + z * nlp * kShadowRow : nullptr, 0, Xsh ? (int)(nlp * kShadowRow * 2) : 0,
+      kBufferDword3);
+  int phc[NT];
+  bool okp[NT];
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int ph = grp * 16 * NT + nt * 16 + lr;
+    okp[nt] = ph < kP;
+    phc[nt] = okp[nt] ? ph : kP - 1;
+  }
+  int a1c[4], c1v[4];
+  bool ok1[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const int a1 = 16 * b1 + 4 * lg + j;
+    ok1[j] = a1 < g.n1;
+    a1c[j] = ok1[j] ? a1 : g.n1 - 1;
+    c1v[j] = norm_class(a1c[j], g.n1, r);
+  }
+  float xv[2][NT][4], bv[2][NT][4];
+  auto load_tile = [&](int mt, int buf) {
+    const int a2 = 16 * b2 + 4 * w + mt;
+    const int a2c = a2 < g.n2 ? a2 : g.n2 - 1;
+    const int c2 = norm_class(a2c, g.n2, r);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int xo = (a2c * g.n1 + a1c[j]) * kP;
+      const int to = (c2 * C1 + c1v[j]) * kP;
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        xv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, 4 * (xo + phc[nt]), 0, 0));
+        bv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tr, 4 * (to + phc[nt]), 0, 0));
+      }
+    }
+  };
+  load_tile(0, 0);
 #pragma unroll
   for (int mt = 0; mt < MT; ++mt) {
+    const int buf = mt & 1;
+    if (mt + 1 < MT) load_tile(mt + 1, buf ^ 1);
     const int a2 = 16 * b2 + 4 * w + mt;
     const bool ok2 = a2 < g.n2;
     const int a2c = ok2 ? a2 : g.n2 - 1;
-    const int c2 = norm_class(a2c, g.n2, r);
 #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int ph = grp * 16 * NT + nt * 16 + lr;
-      const bool okp = ph < kP;
-      const int phc = okp ? ph : kP - 1;
-      float xv[4], b1v[4];
-      size_t idx[4];
+    for (int j = 0; j < 4; ++j) {
+      const int l = a2c * g.n1 + a1c[j];
 #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int a1 = 16 * b1 + 4 * lg + j;
-        const int a1c = a1 < g.n1 ? a1 : g.n1 - 1;
-        const int c1 = norm_class(a1c, g.n1, r);
-        b1v[j] = Tz[((size_t)c2 * C1 + c1) * kP + phc];
-        idx[j] = (((size_t)z * g.n2 + a2c) * g.n1 + a1c);  // lenslet; element = lenslet*kP + phc
-        xv[j] = X[idx[j] * kP + phc];
-      }
-#pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int a1 = 16 * b1 + 4 * lg + j;
-        float C = xv[j] * acc[mt][nt][j] / b1v[j];
-        if (isnan(C) || isinf(C) || C < 0.f) C = 0.f;
-        float xn = C * wgt + (1.f - wgt) * xv[j];
-        if (xn < 0.f) xn = 0.f;
-        if (ok2 && okp && a1 < g.n1) {
-          X[idx[j] * kP + phc] = xn;
-          if (Xsh) Xsh[idx[j] * kShadowRow + phc] = (__bf16)xn;
+      for (int nt = 0; nt < NT; ++nt) {
+        const float x = xv[buf][nt][j];
+        float C = x * acc[mt][nt][j] / bv[buf][nt][j];
+        // NaN, +-Inf and negatives -> 0, as a select (no branches)
+        C = (C >= 0.f && C <= 3.402823466e38f) ? C : 0.f;
+        float xn = C * wgt + (1.f - wgt) * x;
+        xn = xn < 0.f ? 0.f : xn;
+        if (ok2 && okp[nt] && ok1[j]) {
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn), xr, 4 * (l * kP + phc[nt]), 0, 0);
+          if (Xsh)
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)xn), sr,
+                                                  2 * (l * kShadowRow + phc[nt]), 0, 0);
         }
       }
     }
