@@ -1041,6 +1041,10 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
                      (long long)n2);
   if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16)
     return set_error(MLF_ERR_UNSUPPORTED, "precision %d not supported", (int)prec);
+  // kernels address one plane of the volume with 32-bit byte offsets
+  if ((double)n1 * n2 * kP * sizeof(float) >= 2147483648.0)
+    return set_error(MLF_ERR_UNSUPPORTED, "lenslet grid %lld x %lld: a plane exceeds 2 GiB",
+                     (long long)n1, (long long)n2);
   MLF_CHECK(ctx_activate(psf->ctx));
   auto p = std::make_unique<mlf_patch_s>();
   p->psf = psf;
