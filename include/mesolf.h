@@ -135,6 +135,74 @@ mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms);
 mlf_status mlf_patch_set_timing(mlf_patch p, int32_t enable);
 mlf_status mlf_patch_kernel_times(mlf_patch p, double* total_ms, int64_t* launches);
 
+/* ----------------------------------------------------------------- batches */
+/* No counterpart in the reference, which reconstructs one patch per MATLAB
+ * worker (LFM_reconstruction_module.m:26-37 inside mesolf.m:55-61's parfor).
+ * A batch runs RL iterations (recon:98-146) for up to `capacity` patches of
+ * one PSF and precision in shared launches: each view-step kernel covers the
+ * work entries of every attached patch, whatever its lenslet grid, and one
+ * iteration's 3 x nviews launches are replayed from one captured hipGraph.
+ * A patch's result does not depend on the batch it ran in (the same work
+ * entries produce the same bits).  Patches keep their own image and volume;
+ * attach them after set_image and init_volume / set_volume.  A destroyed
+ * patch is detached first.  Calls on one batch are serialized; running a
+ * batch and one of its patches from two threads at once is not supported. */
+typedef struct mlf_batch_s* mlf_batch;
+mlf_status mlf_batch_create(mlf_psf psf, mlf_precision prec, int32_t capacity, mlf_batch* out);
+mlf_status mlf_batch_destroy(mlf_batch b);
+mlf_status mlf_batch_attach(mlf_batch b, mlf_patch p);
+mlf_status mlf_batch_detach(mlf_batch b, mlf_patch p);
+mlf_status mlf_batch_size(mlf_batch b, int32_t* attached);
+/* enable != 0 (default): replay a captured graph per iteration. */
+mlf_status mlf_batch_set_graph(mlf_batch b, int32_t enable);
+/* `iterations` RL iterations of every attached patch, asynchronously on the
+ * context stream; filter != 0 applies sample_filter after each (recon:142). */
+mlf_status mlf_batch_run(mlf_batch b, int32_t iterations, int32_t filter);
+mlf_status mlf_batch_last_run_ms(mlf_batch b, double* ms);
+
+/* ----------------------------------------------------------------- serving */
+/* No counterpart in the reference.  A request loop for frame x patch
+ * deconvolutions: requests are queued as they arrive, admitted into batch
+ * slots at RL-iteration boundaries while other requests are mid-flight
+ * (continuous batching), and run `iterations` RL iterations each.  A patch id
+ * keeps its volume resident between requests, so the next frame of that
+ * patch starts warm from the previous result; its first request (or the first
+ * after an eviction) starts from recon:72-73.  Resident per-patch state is
+ * capped by `hbm_budget_bytes`: a request whose patch is not resident is
+ * admitted only when its state fits, evicting idle patches (least recently
+ * used) if needed.  Frames of one patch run in submission order. */
+typedef struct mlf_server_s* mlf_server;
+typedef struct {
+  int32_t max_slots;          /* patches in flight at once (batch capacity) */
+  int32_t iterations;         /* RL iterations per request */
+  int32_t filter;             /* sample_filter after each iteration */
+  int32_t use_graph;          /* replay captured graphs */
+  uint64_t hbm_budget_bytes;  /* cap on resident per-patch device memory */
+} mlf_server_config;
+typedef struct {
+  int64_t submitted, completed, pending, in_flight;
+  int64_t resident_patches, evictions, cold_starts, iterations_run;
+  uint64_t resident_bytes, peak_resident_bytes;
+  double p50_ms, p99_ms, mean_ms, max_ms; /* submit -> volume ready, completed requests */
+  double volumes_per_s;                   /* completed / (last completion - first submit) */
+  double mean_slots;                      /* attached patches per iteration, averaged */
+} mlf_server_stats;
+mlf_status mlf_server_create(mlf_psf psf, mlf_precision prec, const mlf_server_config* cfg,
+                             mlf_server* out);
+/* Queue one frame of patch `patch_id` (grid n1 x n2, fixed per id): lfimg
+ * [15*n1, 15*n2] single is copied before return.  volume_out, if not NULL,
+ * receives the volume [15*n1, 15*n2, Z] single when the request completes and
+ * must stay valid until then.  Returns a ticket for mlf_server_wait. */
+mlf_status mlf_server_submit(mlf_server s, int32_t patch_id, int64_t n1, int64_t n2,
+                             const float* lfimg, float* volume_out, uint64_t* ticket);
+/* Block until the request completes; latency_ms (may be NULL) = submit to
+ * completion. */
+mlf_status mlf_server_wait(mlf_server s, uint64_t ticket, double* latency_ms);
+/* Block until every submitted request has completed. */
+mlf_status mlf_server_drain(mlf_server s);
+mlf_status mlf_server_get_stats(mlf_server s, mlf_server_stats* st);
+mlf_status mlf_server_destroy(mlf_server s);
+
 /* ------------------------------------------------------------ drop-ins */
 /* recon_theory_phase_space_LFM_with_filter(LFIMG,H,Ht,maxIter,gpuFlag,gpu_id)
  * (recon:10).  LFIMG [I1, I2] single; H, Ht [S,S,N,N,Z] single; X_out

@@ -5,13 +5,13 @@ built in-tree by ``python -m mesolf_amd.build``.  This package is the thin
 host-side mirror of the reference's MATLAB-facing interface.
 """
 from .lib import (  # noqa: F401
-    BudgetExceeded, Context, InvalidInput, MesolfError, Patch, PSF, Unsupported,
+    Batch, BudgetExceeded, Context, InvalidInput, MesolfError, Patch, PSF, Server, Unsupported,
     compact_block_layout, exported_symbols, load_library, mex_lfm_convolution_vec,
     recon_theory_phase_space_LFM_with_filter,
 )
 
 __all__ = [
-    "Context", "PSF", "Patch", "MesolfError", "InvalidInput", "Unsupported", "BudgetExceeded",
+    "Context", "PSF", "Patch", "Batch", "Server", "MesolfError", "InvalidInput", "Unsupported", "BudgetExceeded",
     "load_library", "exported_symbols", "compact_block_layout",
     "recon_theory_phase_space_LFM_with_filter", "mex_lfm_convolution_vec",
 ]

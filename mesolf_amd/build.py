@@ -27,7 +27,8 @@ COMMON = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "runtime.cpp"]
+SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "runtime.cpp", "batch.cpp",
+           "server.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -1,0 +1,341 @@
+// Batches: RL iterations of several patches of one PSF in shared launches.
+//
+// Each view-step kernel walks a list of work entries (slot, plane, block...)
+// with a grid-stride loop, so one launch serves every attached patch and a
+// captured graph of fixed grid size stays valid while patches come and go:
+// only the entry lists and slot descriptors in device memory change.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <mutex>
+#include <vector>
+
+#include "internal.h"
+#include "rl_mfma.h"
+#include "state.h"
+
+using namespace mlf;
+
+struct mlf_batch_s {
+  mlf_psf psf = nullptr;
+  Precision kp = Precision::kF32;
+  int capacity = 0;
+  std::vector<mlf_patch_s*> slots;
+  // device tables (rebuilt when the composition changes)
+  DevBuf<SlotDesc> d_slots;
+  DevBuf<int32_t> d_fsched, d_bsched, d_rsched, d_counts;
+  int n_f = 0, n_b = 0, n_r = 0;
+  bool dirty = true;
+  // launch shape (fixed for the life of the batch)
+  bool tpz = false;
+  int cw = 16, bm2 = 8;
+  int grid_f = 0, grid_b = 0, grid_r = 0;
+  size_t smem_f = 0, smem_b = 0;
+  // one RL iteration's view loop as a graph
+  bool use_graph = true;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  hipStream_t cap = nullptr;  // private stream for capture (other threads may use the context stream)
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+  std::mutex mu;
+  ~mlf_batch_s() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    if (ev0) (void)hipEventDestroy(ev0);
+    if (ev1) (void)hipEventDestroy(ev1);
+    if (cap) (void)hipStreamDestroy(cap);
+  }
+};
+
+namespace {
+
+void drop_graph(mlf_batch_s* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  b->exec = nullptr;
+  b->graph = nullptr;
+}
+
+// Grow a device table; a new allocation invalidates the captured graph.
+mlf_status upload_table(mlf_batch_s* b, DevBuf<int32_t>& d, const std::vector<int32_t>& h) {
+  if (h.size() > d.count) {
+    drop_graph(b);
+    MLF_CHECK(d.alloc(std::max(h.size(), 2 * d.count)));
+  }
+  if (!h.empty()) MLF_HIP(hipMemcpy(d.ptr, h.data(), h.size() * sizeof(int32_t), hipMemcpyHostToDevice));
+  return MLF_OK;
+}
+
+// Entry lists of all attached patches, merged largest radius first so the
+// grid-stride loop hands every workgroup a similar mix of costs.
+mlf_status rebuild(mlf_batch_s* b) {
+  mlf_psf_s* ps = b->psf;
+  hipStream_t st = ps->ctx->stream;
+  MLF_HIP(hipStreamSynchronize(st));  // kernels in flight may read the old tables
+  std::vector<SlotDesc> sd;
+  struct Ent {
+    int r;
+    int32_t v[5];
+  };
+  std::vector<Ent> fe, be;
+  std::vector<int32_t> rs;
+  for (size_t i = 0; i < b->slots.size(); ++i) {
+    mlf_patch_s* p = b->slots[i];
+    sd.push_back(p->slot);
+    for (size_t e = 0; e + 3 <= p->fent.size(); e += 3) {
+      const int z = p->fent[e];
+      fe.push_back(Ent{ps->rf[z], {(int32_t)i, z, p->fent[e + 1], p->fent[e + 2], 0}});
+    }
+    for (size_t e = 0; e + 4 <= p->bent.size(); e += 4) {
+      const int z = p->bent[e];
+      be.push_back(Ent{ps->rb[z], {(int32_t)i, z, p->bent[e + 1], p->bent[e + 2], p->bent[e + 3]}});
+    }
+    for (int j = 0; j < p->nblk_r; ++j) {
+      rs.push_back((int32_t)i);
+      rs.push_back(256 * j);
+    }
+  }
+  auto by_r = [](const Ent& a, const Ent& c) { return a.r > c.r; };
+  std::stable_sort(fe.begin(), fe.end(), by_r);
+  std::stable_sort(be.begin(), be.end(), by_r);
+  std::vector<int32_t> fs, bs;
+  for (const Ent& e : fe) fs.insert(fs.end(), e.v, e.v + 4);
+  for (const Ent& e : be) bs.insert(bs.end(), e.v, e.v + 5);
+  b->n_f = (int)fe.size();
+  b->n_b = (int)be.size();
+  b->n_r = (int)(rs.size() / 2);
+  if (!sd.empty())
+    MLF_HIP(hipMemcpy(b->d_slots.ptr, sd.data(), sd.size() * sizeof(SlotDesc), hipMemcpyHostToDevice));
+  MLF_CHECK(upload_table(b, b->d_fsched, fs));
+  MLF_CHECK(upload_table(b, b->d_bsched, bs));
+  MLF_CHECK(upload_table(b, b->d_rsched, rs));
+  const std::vector<int32_t> counts{b->n_f, b->n_b, b->n_r};
+  MLF_HIP(hipMemcpy(b->d_counts.ptr, counts.data(), 3 * sizeof(int32_t), hipMemcpyHostToDevice));
+  b->dirty = false;
+  return MLF_OK;
+}
+
+// One RL iteration's view loop (recon:100-138) for every attached patch.
+mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st) {
+  mlf_psf_s* ps = b->psf;
+  const PrecPack& pk = ps->pack[(int)b->kp];
+  const size_t eb = elem_bytes(b->kp);
+  const FwdGeo g{b->d_slots.ptr, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, b->d_fsched.ptr,
+                 b->d_counts.ptr + 0};
+  const RatioGeo rg{b->d_rsched.ptr, b->d_counts.ptr + 2};
+  for (int k = 0; k < ps->V; ++k) {
+    const int u = ps->uv[2 * k], v = ps->uv[2 * k + 1];
+    const void* Wk = pk.fwd.ptr + (size_t)k * pk.pvf * eb;
+    if (b->tpz) {
+      MLF_LAUNCH(launch_fwd_tpz(b->kp, Wk, g, b->grid_f, b->smem_f, st));
+      MLF_LAUNCH(launch_ratio_gather_tpz(g, rg, b->grid_r, ps->Z, b->cw, u, v, 1, nullptr, st));
+    } else {
+      MLF_LAUNCH(launch_fwd_mfma(b->kp, Wk, g, b->grid_f, b->smem_f, st));
+      MLF_LAUNCH(launch_ratio_gather(g, rg, b->grid_r, ps->Z, b->bm2, u, v, 1, nullptr, st));
+    }
+    const BwdGeo bg{b->d_slots.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, b->d_bsched.ptr,
+                    b->d_counts.ptr + 1, k};
+    MLF_LAUNCH(launch_bwd_mfma(b->kp, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg, b->grid_b,
+                               b->smem_b, (float)ps->weight[k], st));
+  }
+  return MLF_OK;
+}
+
+mlf_status capture(mlf_batch_s* b) {
+  drop_graph(b);
+  hipStream_t st = b->cap;
+  MLF_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+  const mlf_status s = enqueue_views(b, st);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(st, &g);
+  if (s != MLF_OK) {
+    if (g) (void)hipGraphDestroy(g);
+    return s;
+  }
+  MLF_HIP(e);
+  b->graph = g;
+  MLF_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return MLF_OK;
+}
+
+mlf_status detach_locked(mlf_batch_s* b, mlf_patch_s* p) {
+  auto it = std::find(b->slots.begin(), b->slots.end(), p);
+  if (it == b->slots.end()) return set_error(MLF_ERR_INVALID_ARGUMENT, "patch is not attached");
+  // work in flight reads the slot table
+  MLF_HIP(hipStreamSynchronize(b->psf->ctx->stream));
+  b->slots.erase(it);
+  p->batch = nullptr;
+  b->dirty = true;
+  return MLF_OK;
+}
+
+}  // namespace
+
+namespace mlf {
+// Called by mlf_patch_destroy.
+mlf_status batch_forget_patch(mlf_patch_s* p) {
+  mlf_batch_s* b = p->batch;
+  if (!b) return MLF_OK;
+  std::lock_guard<std::mutex> lk(b->mu);
+  return detach_locked(b, p);
+}
+}  // namespace mlf
+
+extern "C" {
+
+mlf_status mlf_batch_create(mlf_psf psf, mlf_precision prec, int32_t capacity, mlf_batch* out) {
+  clear_error();
+  if (!psf || !out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  *out = nullptr;
+  if (capacity < 1 || capacity > 4096)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "batch capacity %d out of range", (int)capacity);
+  if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16)
+    return set_error(MLF_ERR_UNSUPPORTED, "precision %d not supported", (int)prec);
+  MLF_CHECK(ctx_activate(psf->ctx));
+  {
+    std::lock_guard<std::mutex> lk(g_life_mu);
+    if (psf->closed) return set_error(MLF_ERR_INVALID_ARGUMENT, "psf already destroyed");
+    ++psf->refs;
+  }
+  auto b = std::make_unique<mlf_batch_s>();
+  b->psf = psf;
+  struct Undo {
+    mlf_batch_s* b;
+    ~Undo() {
+      if (b) {
+        std::lock_guard<std::mutex> lk(g_life_mu);
+        --b->psf->refs;
+      }
+    }
+  } undo{b.get()};
+  b->kp = prec == MLF_PRECISION_BF16 ? Precision::kBF16 : Precision::kF32;
+  b->capacity = capacity;
+  MLF_CHECK(b->d_slots.alloc((size_t)capacity));
+  MLF_CHECK(b->d_counts.alloc(3));
+  const std::vector<int32_t> zero{0, 0, 0};
+  MLF_HIP(hipMemcpy(b->d_counts.ptr, zero.data(), 3 * sizeof(int32_t), hipMemcpyHostToDevice));
+  int cus = 0;
+  MLF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, psf->ctx->device));
+  // two workgroups per CU for the MFMA kernels (their LDS and register
+  // budgets allow two), more for the light ratio kernel
+  b->grid_f = 2 * cus;
+  b->grid_b = 2 * cus;
+  b->grid_r = 4 * cus;
+  MLF_HIP(hipEventCreate(&b->ev0));
+  MLF_HIP(hipEventCreate(&b->ev1));
+  MLF_HIP(hipStreamCreateWithFlags(&b->cap, hipStreamNonBlocking));
+  undo.b = nullptr;
+  *out = b.release();
+  return MLF_OK;
+}
+
+mlf_status mlf_batch_destroy(mlf_batch b) {
+  clear_error();
+  if (!b) return MLF_OK;
+  std::lock_guard<std::mutex> life(g_life_mu);
+  mlf_psf psf = b->psf;
+  (void)hipSetDevice(psf->ctx->device);
+  (void)hipStreamSynchronize(psf->ctx->stream);
+  {
+    std::lock_guard<std::mutex> lk(b->mu);
+    for (mlf_patch_s* p : b->slots) p->batch = nullptr;
+    b->slots.clear();
+  }
+  delete b;
+  if (--psf->refs == 0 && psf->closed) psf_free(psf);
+  return MLF_OK;
+}
+
+mlf_status mlf_batch_attach(mlf_batch b, mlf_patch p) {
+  clear_error();
+  if (!b || !p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (p->psf != b->psf) return set_error(MLF_ERR_INVALID_ARGUMENT, "patch uses another PSF");
+  if (p->reference)
+    return set_error(MLF_ERR_UNSUPPORTED, "reference-kernel patches cannot be batched");
+  if (p->kp != b->kp) return set_error(MLF_ERR_INVALID_ARGUMENT, "patch precision differs from the batch");
+  if (p->batch) return set_error(MLF_ERR_INVALID_ARGUMENT, "patch already attached to a batch");
+  if ((int)b->slots.size() >= b->capacity)
+    return set_error(MLF_ERR_BUDGET, "batch is full (%d patches)", b->capacity);
+  MLF_CHECK(ctx_activate(b->psf->ctx));
+  if (b->slots.empty() && !b->exec) {
+    b->tpz = p->tpz;
+    b->cw = p->cw;
+    b->bm2 = p->bm2;
+    b->smem_f = p->smem_f;
+    b->smem_b = p->smem_b;
+  }
+  b->slots.push_back(p);
+  p->batch = b;
+  b->dirty = true;
+  return MLF_OK;
+}
+
+mlf_status mlf_batch_detach(mlf_batch b, mlf_patch p) {
+  clear_error();
+  if (!b || !p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  MLF_CHECK(ctx_activate(b->psf->ctx));
+  return detach_locked(b, p);
+}
+
+mlf_status mlf_batch_size(mlf_batch b, int32_t* attached) {
+  clear_error();
+  if (!b || !attached) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  *attached = (int32_t)b->slots.size();
+  return MLF_OK;
+}
+
+mlf_status mlf_batch_set_graph(mlf_batch b, int32_t enable) {
+  clear_error();
+  if (!b) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->use_graph = enable != 0;
+  return MLF_OK;
+}
+
+mlf_status mlf_batch_run(mlf_batch b, int32_t iterations, int32_t filter) {
+  clear_error();
+  if (!b) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (iterations < 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "negative iteration count");
+  std::lock_guard<std::mutex> lk(b->mu);
+  MLF_CHECK(ctx_activate(b->psf->ctx));
+  for (mlf_patch_s* p : b->slots)
+    if (!p->image_set || !p->volume_set)
+      return set_error(MLF_ERR_INVALID_ARGUMENT, "image and volume must be set on every attached patch");
+  hipStream_t st = b->psf->ctx->stream;
+  if (b->dirty) MLF_CHECK(rebuild(b));
+  MLF_HIP(hipEventRecord(b->ev0, st));
+  if (!b->slots.empty()) {
+    for (int it = 0; it < iterations; ++it) {
+      if (b->use_graph) {
+        if (!b->exec) MLF_CHECK(capture(b));
+        MLF_HIP(hipGraphLaunch(b->exec, st));
+      } else {
+        MLF_CHECK(enqueue_views(b, st));
+      }
+      if (filter)
+        for (mlf_patch_s* p : b->slots) MLF_CHECK(patch_filter(p));
+    }
+  }
+  MLF_HIP(hipEventRecord(b->ev1, st));
+  b->timed = true;
+  return MLF_OK;
+}
+
+mlf_status mlf_batch_last_run_ms(mlf_batch b, double* ms) {
+  clear_error();
+  if (!b || !ms) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!b->timed) return set_error(MLF_ERR_INVALID_ARGUMENT, "no run recorded");
+  MLF_CHECK(ctx_activate(b->psf->ctx));
+  MLF_HIP(hipEventSynchronize(b->ev1));
+  float f = 0.f;
+  MLF_HIP(hipEventElapsedTime(&f, b->ev0, b->ev1));
+  *ms = f;
+  return MLF_OK;
+}
+
+}  // extern "C"

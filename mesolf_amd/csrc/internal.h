@@ -37,6 +37,14 @@ mlf_status clear_error();
     if (s_ != MLF_OK) return s_;        \
   } while (0)
 
+#define MLF_LAUNCH(call)                                                                      \
+  do {                                                                                        \
+    hipError_t e_ = (call);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return ::mlf::set_error(MLF_ERR_DEVICE, "launch %s failed: %s", #call,                  \
+                              hipGetErrorString(e_));                                         \
+  } while (0)
+
 // --------------------------------------------------------- device buffer
 template <typename T>
 struct DevBuf {
@@ -74,6 +82,9 @@ hipError_t launch_to_blocked(const float* src, float* dst, int n1, int n2, int Z
 hipError_t launch_from_blocked_f32(const float* src, float* dst, int n1, int n2, int Z, hipStream_t st);
 hipError_t launch_from_blocked_f64(const float* src, double* dst, int n1, int n2, int Z, hipStream_t st);
 hipError_t launch_fill(float* dst, size_t n, float value, hipStream_t st);
+// parts[2k], parts[2k+1] = FP64 sums of (a-b)^2 and b^2 over block k's share.
+hipError_t launch_diff_norms(const float* a, const float* b, size_t n, double* parts, int nblk,
+                             hipStream_t st);
 hipError_t launch_init_volume(const float* LF, size_t nlf, double* sum, float* X, size_t nx,
                               double inv_numel, hipStream_t st);
 

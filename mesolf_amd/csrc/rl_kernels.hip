@@ -70,6 +70,39 @@ __global__ void k_image_sum(const float* __restrict__ img, size_t n, double* __r
   }
 }
 
+// Per-block FP64 partials of sum((a-b)^2) and sum(b^2) over a grid-stride
+// range (fixed order within the block); the host adds the blocks in order.
+__global__ void k_diff_norms(const float* __restrict__ a, const float* __restrict__ b, size_t n,
+                             double* __restrict__ parts) {
+  __shared__ double sh[2][256 / 64];
+  double dd = 0.0, bb = 0.0;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const double x = a[o], y = b[o];
+    dd += (x - y) * (x - y);
+    bb += y * y;
+  }
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  for (int off = 32; off > 0; off >>= 1) {
+    dd += __shfl_down(dd, off, 64);
+    bb += __shfl_down(bb, off, 64);
+  }
+  if (lane == 0) {
+    sh[0][wid] = dd;
+    sh[1][wid] = bb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s0 = 0.0, s1 = 0.0;
+    for (int k = 0; k < (int)(blockDim.x >> 6); ++k) {
+      s0 += sh[0][k];
+      s1 += sh[1][k];
+    }
+    parts[2 * blockIdx.x] = s0;
+    parts[2 * blockIdx.x + 1] = s1;
+  }
+}
+
 // X = (1/numel) * sum(LFIMG) everywhere (recon:72-73).
 __global__ void k_fill_from_sum(float* __restrict__ dst, size_t n, const double* __restrict__ sum,
                                 double inv_numel) {
@@ -307,6 +340,12 @@ hipError_t launch_init_volume(const float* LF, size_t nlf, double* sum, float* X
   hipLaunchKernelGGL(k_image_sum, dim3(1), dim3(1024), 0, st, LF, nlf, sum);
   hipLaunchKernelGGL(k_fill_from_sum, dim3(grid_for(nx, 256)), dim3(256), 0, st, X, nx, sum,
                      inv_numel);
+  return hipGetLastError();
+}
+
+hipError_t launch_diff_norms(const float* a, const float* b, size_t n, double* parts, int nblk,
+                             hipStream_t st) {
+  hipLaunchKernelGGL(k_diff_norms, dim3(nblk), dim3(256), 0, st, a, b, n, parts);
   return hipGetLastError();
 }
 

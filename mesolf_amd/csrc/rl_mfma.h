@@ -26,23 +26,48 @@ inline int fwd_bm2(Precision p) { return 4 * (p == Precision::kF32 ? kFwdMTF32 :
 inline int fwd_phi(Precision p) { return p == Precision::kF32 ? kPhiPadF32 : kPhiPadBF16; }
 inline size_t elem_bytes(Precision p) { return p == Precision::kF32 ? 4 : 2; }
 
-struct FwdGeo {
+// One patch (reconstruction problem) as the view-step kernels see it.  A
+// launch covers the work entries of one or more slots; every entry names its
+// slot, so patches of different lenslet grids share one launch.
+struct SlotDesc {
+  float* X;               // FP32 volume [z][n2][n1][225]
+  __bf16* Xb;             // BF16 shadow [z][n2][n1][256], or null
+  const float* LF;        // measured image, column-major (15 n1) x (15 n2)
+  float* R;               // ratio image [n2][n1]
+  float* partial;         // forward partial windows
+  const int64_t* poff;    // [Z] offset of each plane's partial windows
+  const float* Tn;        // normaliser tables of view 0 for this grid's class key
+  int64_t pvt;            // normaliser elements per view
+  const int64_t* toffT;   // [Z] plane offset inside one view's normaliser table
+  const int32_t* C1;      // [Z] normaliser classes along axis 0
   int n1, n2;
+};
+
+// Work entries are processed by a grid-stride loop over *count entries, so a
+// launch (or a captured graph node) of fixed grid size serves any number of
+// entries written to device memory before it runs.
+struct FwdGeo {
+  const SlotDesc* slots;
   const int32_t* r;       // [Z] forward radius
   const int64_t* woff;    // [Z] element offset of the plane block in the packed view
-  const int32_t* nchunk;  // [Z] number of 64-tap chunks
-  const int64_t* poff;    // [Z] offset of the plane's partial windows
-  const int32_t* sched;   // [nblocks][3] = (z, b1, b2)
+  const int32_t* nchunk;  // [Z] number of 64-tap chunks (Q-scatter kernel)
+  const int32_t* sched;   // [entries][4] = (slot, z, b1, b2)
+  const int32_t* count;   // number of entries
+};
+
+struct RatioGeo {
+  const int32_t* sched;   // [entries][2] = (slot, first lenslet); 256 lenslets per entry
+  const int32_t* count;
 };
 
 struct BwdGeo {
-  int n1, n2;
+  const SlotDesc* slots;
   const int32_t* r;       // [Z] backward radius
   const int64_t* boff;    // [Z] element offset of the plane block in the packed view
   const int32_t* epad;    // [Z] padded tap count (multiple of kBwdTapPad)
-  const int64_t* toffT;   // [Z] offset of the plane's normaliser table in the view
-  const int32_t* C1;      // [Z] normaliser classes along axis 0
-  const int32_t* sched;   // [nblocks][4] = (z, b1, b2, phase group)
+  const int32_t* sched;   // [entries][5] = (slot, z, b1, b2, phase group)
+  const int32_t* count;
+  int view;               // selects the normaliser table of the view
 };
 
 struct PackGeo {
@@ -82,21 +107,22 @@ __host__ __device__ inline void norm_range(int c, int n, int r, int* lo, int* hi
 size_t fwd_smem_bytes(Precision p, int rmax, int bm2);
 size_t tpz_smem_bytes(Precision p, int rmax);
 int tpz_max_radius();
-hipError_t launch_fwd_tpz(Precision p, const float* X, const void* Xb, const void* W, FwdGeo g,
-                          int nblocks, size_t smem, float* partial, hipStream_t st);
+hipError_t launch_fwd_tpz(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+                          hipStream_t st);
 hipError_t launch_make_shadow(const float* X, void* Xb, size_t nlens, hipStream_t st);
-hipError_t launch_ratio_gather_tpz(const float* partial, FwdGeo g, int Z, int cw, const float* LF,
-                                   int u, int v, float* R, float* y_out, hipStream_t st);
+// y = sum of the forward partials; with use_lf, R = LF(u::15, v::15) ./ y
+// (non-finite -> 0) into each slot's R; y_out (single-slot launches) gets y.
+hipError_t launch_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int grid, int Z, int cw, int u, int v,
+                                   int use_lf, float* y_out, hipStream_t st);
 hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                                hipStream_t st);
 size_t bwd_smem_bytes(Precision p, int rmax, int epad_max);
 
-hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
-                           size_t smem, float* partial, hipStream_t st);
-hipError_t launch_ratio_gather(const float* partial, FwdGeo g, int Z, int bm2, const float* LF,
-                               int u, int v, float* R, float* y_out, hipStream_t st);
-hipError_t launch_bwd_mfma(Precision p, float* X, void* Xb, const void* Wb, const float* Tn,
-                           BwdGeo g, int nblocks, size_t smem, const float* R, float w,
+hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+                           hipStream_t st);
+hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, int u, int v,
+                               int use_lf, float* y_out, hipStream_t st);
+hipError_t launch_bwd_mfma(Precision p, const void* Wb, BwdGeo g, int grid, size_t smem, float w,
                            hipStream_t st);
 hipError_t launch_pack_fwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                            hipStream_t st);

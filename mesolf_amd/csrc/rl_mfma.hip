@@ -54,6 +54,40 @@ struct Mma<__bf16> {
 };
 
 __device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
+
+// Work entries and slot descriptors are read from memory the kernel also
+// writes through other pointers, so the compiler cannot keep them in scalar
+// registers by itself; they are wave-uniform by construction.  Values that
+// feed buffer resources must be scalar, or the compiler emits a waterfall
+// loop per buffer instruction.
+__device__ __forceinline__ int uni(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ __forceinline__ int64_t uni64(int64_t v) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)v >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+template <typename P>
+__device__ __forceinline__ P* uni_ptr(P* p) {
+  return reinterpret_cast<P*>(uni64(reinterpret_cast<int64_t>(p)));
+}
+// Slot descriptor with every field wave-uniform.
+__device__ __forceinline__ SlotDesc load_slot(const SlotDesc* slots, int i) {
+  const SlotDesc& d = slots[i];
+  SlotDesc o;
+  o.X = uni_ptr(d.X);
+  o.Xb = uni_ptr(d.Xb);
+  o.LF = uni_ptr(d.LF);
+  o.R = uni_ptr(d.R);
+  o.partial = uni_ptr(d.partial);
+  o.poff = uni_ptr(d.poff);
+  o.Tn = uni_ptr(d.Tn);
+  o.pvt = uni64(d.pvt);
+  o.toffT = uni_ptr(d.toffT);
+  o.C1 = uni_ptr(d.C1);
+  o.n1 = uni(d.n1);
+  o.n2 = uni(d.n2);
+  return o;
+}
 // s_waitcnt vmcnt(<= n) without waiting on LDS/export counters (gfx9 encoding:
 // vmcnt bits [3:0] and [15:14], expcnt [6:4], lgkmcnt [11:8]).
 template <int N>
@@ -87,9 +121,7 @@ struct FwdCfg<__bf16> {
 
 template <typename T, int MT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdCfg<T>::NBUF == 1 ? 2 : 1)))
-void k_fwd_mfma(const float* __restrict__ X,
-                                                  const T* __restrict__ W, FwdGeo g,
-                                                  float* __restrict__ partial) {
+void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
   using M = Mma<T>;
   constexpr int KSTEP = M::KSTEP, KPL = M::KPL, PHI = M::PHI, LDB = M::LDB;
   constexpr int NKS = PHI / KSTEP;
@@ -100,134 +132,141 @@ void k_fwd_mfma(const float* __restrict__ X,
   T* Bs = reinterpret_cast<T*>(smem);                                  // [NBUF][CH][LDB]
   float* ywin = reinterpret_cast<float*>(smem + sizeof(T) * NBUF * CH * LDB);
 
-  const int* s = g.sched + 3 * blockIdx.x;
-  const int z = s[0], b1 = s[1], b2 = s[2];
-  const int r = g.r[z];
-  const int K = 2 * r + 1, ntaps = K * K;
-  const int nch = g.nchunk[z];
-  const int W1 = 16 + 2 * r, W2 = BM2 + 2 * r;
-  const int WW = MT + 2 * r;  // rows of one wave's window
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
-  // one window per wave: atomics of a single wave land in program/lane order,
-  // and the four windows are summed in a fixed order -> deterministic partials
-  float* yw = ywin + w * W1 * WW;
+  const int nent = uni(*g.count);
+  for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
+    const int* s = g.sched + 4 * ent;
+    const SlotDesc sd = load_slot(g.slots, uni(s[0]));
+    const float* __restrict__ X = sd.X;
+    const int n1 = sd.n1, n2 = sd.n2;
+    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]);
+    const int r = uni(g.r[z]);
+    const int K = 2 * r + 1, ntaps = K * K;
+    const int nch = uni(g.nchunk[z]);
+    const int W1 = 16 + 2 * r, W2 = BM2 + 2 * r;
+    const int WW = MT + 2 * r;  // rows of one wave's window
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+    // one window per wave: atomics of a single wave land in program/lane order,
+    // and the four windows are summed in a fixed order -> deterministic partials
+    float* yw = ywin + w * W1 * WW;
 
-  for (int o = tid; o < 4 * W1 * WW; o += 256) ywin[o] = 0.f;
+    for (int o = tid; o < 4 * W1 * WW; o += 256) ywin[o] = 0.f;
 
-  // A operand: this lane's lenslet rows of the volume block, all phases.
-  typename M::frag a[MT][NKS];
-  {
-    const int a1 = 16 * b1 + lr;
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const int a2 = BM2 * b2 + MT * w + mt;
-      const bool ok = a1 < g.n1 && a2 < g.n2;
-      const float* xp = X + (((size_t)z * g.n2 + (ok ? a2 : 0)) * g.n1 + (ok ? a1 : 0)) * kP;
-      // branch-free: load from a clamped in-row phase, select zero afterwards
-#pragma unroll
-      for (int ks = 0; ks < NKS; ++ks) {
-#pragma unroll
-        for (int j = 0; j < KPL; ++j) {
-          const int ph = ks * KSTEP + lg * KPL + j;
-          const float v = xp[ph < kP ? ph : kP - 1];
-          const float val = (ok && ph < kP) ? v : 0.f;
-          if constexpr (KPL == 1) a[mt][ks] = val;
-          else a[mt][ks][j] = (__bf16)val;
-        }
-      }
-    }
-  }
-
-  const T* Wz = W + g.woff[z];
-  // Chunk c (CH rows of PHI elements) is copied global -> LDS with
-  // global_load_lds_dwordx4 (no staging registers), into the buffer the
-  // waves are not reading, while the MFMAs of chunk c-1 run.  Each wave pass
-  // fills 64 consecutive 16-byte LDS slots; for BF16 the slot of logical
-  // vector (row, col) is row*RV + (col ^ (row & 31)) so that the sixteen rows
-  // read by one ds_read_b128 land in distinct banks.
-  constexpr int VEC = 16 / sizeof(T);            // elements per 16 B
-  constexpr int RV = PHI / VEC;                  // vectors per row
-  constexpr bool SWZ = (RV == 32);
-  constexpr int NSLOT = CH * RV;
-  auto issue_chunk = [&](int c, int buf) {
-    const T* src = Wz + (size_t)c * CH * PHI;
-    char* dst = reinterpret_cast<char*>(Bs) + (size_t)buf * CH * LDB * sizeof(T);
-    for (int base = w * 64; base < NSLOT; base += 256) {
-      const int slot = base + lane;
-      if (slot < NSLOT) {
-        const int row = slot / RV, pos = slot - row * RV;
-        const int col = SWZ ? (pos ^ (row & 31)) : pos;
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
-            (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
-      }
-    }
-  };
-
-  if (NBUF == 2) {
-    issue_chunk(0, 0);
-    __builtin_amdgcn_s_waitcnt(0);
-  }
-  for (int c = 0; c < nch; ++c) {
-    __syncthreads();  // previous chunk landed / everyone done reading the buffer
-    if (NBUF == 1) {
-      issue_chunk(c, 0);
-      __builtin_amdgcn_s_waitcnt(0);
-      __syncthreads();
-    } else if (c + 1 < nch) {
-      issue_chunk(c + 1, (c + 1) & 1);
-    }
-    const T* Bc = Bs + (size_t)(NBUF == 2 ? (c & 1) : 0) * CH * LDB;
-    floatx4 acc[MT][4];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
-      typename M::frag bf[4];
-#pragma unroll
-      for (int nt = 0; nt < 4; ++nt) {
-        const int row = nt * 16 + lr;
-        if constexpr (SWZ) {
-          const int col = (ks * KSTEP + lg * KPL) / VEC;
-          bf[nt] = M::load(Bc + row * LDB + ((col ^ (row & 31)) * VEC));
-        } else {
-          bf[nt] = M::load(Bc + row * LDB + ks * KSTEP + lg * KPL);
-        }
-      }
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = M::mma(a[mt][ks], bf[nt], acc[mt][nt]);
-    }
-    // diagonal scatter into the LDS window: y[a + d] += Q[a, e(d)]
-#pragma unroll
-    for (int nt = 0; nt < 4; ++nt) {
-      const int e = c * CH + nt * 16 + lr;
-      if (e >= ntaps) continue;
-      const int e1 = e / K, e2 = e - e1 * K;  // packed rows are e2-fastest
-#pragma unroll
+    // A operand: this lane's lenslet rows of the volume block, all phases.
+    typename M::frag a[MT][NKS];
+    {
+      const int a1 = 16 * b1 + lr;
+  #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        float* row = yw + (mt + e2) * W1 + e1;
-#pragma unroll
-        for (int j = 0; j < 4; ++j) atomicAdd(row + 4 * lg + j, acc[mt][nt][j]);
+        const int a2 = BM2 * b2 + MT * w + mt;
+        const bool ok = a1 < n1 && a2 < n2;
+        const float* xp = X + (((size_t)z * n2 + (ok ? a2 : 0)) * n1 + (ok ? a1 : 0)) * kP;
+        // branch-free: load from a clamped in-row phase, select zero afterwards
+  #pragma unroll
+        for (int ks = 0; ks < NKS; ++ks) {
+  #pragma unroll
+          for (int j = 0; j < KPL; ++j) {
+            const int ph = ks * KSTEP + lg * KPL + j;
+            const float v = xp[ph < kP ? ph : kP - 1];
+            const float val = (ok && ph < kP) ? v : 0.f;
+            if constexpr (KPL == 1) a[mt][ks] = val;
+            else a[mt][ks][j] = (__bf16)val;
+          }
+        }
       }
     }
-    if (NBUF == 2) __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c+1 done
-  }
-  __syncthreads();
-  const int nb1 = ceil_div(g.n1, 16);
-  float* out = partial + g.poff[z] + (size_t)(b2 * nb1 + b1) * W1 * W2;
-  for (int o = tid; o < W1 * W2; o += 256) {
-    const int y = o / W1, x = o - y * W1;
-    float acc = 0.f;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int yy = y - MT * q;
-      if (yy >= 0 && yy < WW) acc += ywin[(q * WW + yy) * W1 + x];
+
+    const T* Wz = W + uni64(g.woff[z]);
+    // Chunk c (CH rows of PHI elements) is copied global -> LDS with
+    // global_load_lds_dwordx4 (no staging registers), into the buffer the
+    // waves are not reading, while the MFMAs of chunk c-1 run.  Each wave pass
+    // fills 64 consecutive 16-byte LDS slots; for BF16 the slot of logical
+    // vector (row, col) is row*RV + (col ^ (row & 31)) so that the sixteen rows
+    // read by one ds_read_b128 land in distinct banks.
+    constexpr int VEC = 16 / sizeof(T);            // elements per 16 B
+    constexpr int RV = PHI / VEC;                  // vectors per row
+    constexpr bool SWZ = (RV == 32);
+    constexpr int NSLOT = CH * RV;
+    auto issue_chunk = [&](int c, int buf) {
+      const T* src = Wz + (size_t)c * CH * PHI;
+      char* dst = reinterpret_cast<char*>(Bs) + (size_t)buf * CH * LDB * sizeof(T);
+      for (int base = w * 64; base < NSLOT; base += 256) {
+        const int slot = base + lane;
+        if (slot < NSLOT) {
+          const int row = slot / RV, pos = slot - row * RV;
+          const int col = SWZ ? (pos ^ (row & 31)) : pos;
+          __builtin_amdgcn_global_load_lds(
+              (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
+              (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
+        }
+      }
+    };
+
+    if (NBUF == 2) {
+      issue_chunk(0, 0);
+      __builtin_amdgcn_s_waitcnt(0);
     }
-    out[o] = acc;
+    for (int c = 0; c < nch; ++c) {
+      __syncthreads();  // previous chunk landed / everyone done reading the buffer
+      if (NBUF == 1) {
+        issue_chunk(c, 0);
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+      } else if (c + 1 < nch) {
+        issue_chunk(c + 1, (c + 1) & 1);
+      }
+      const T* Bc = Bs + (size_t)(NBUF == 2 ? (c & 1) : 0) * CH * LDB;
+      floatx4 acc[MT][4];
+  #pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+  #pragma unroll
+        for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+  #pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        typename M::frag bf[4];
+  #pragma unroll
+        for (int nt = 0; nt < 4; ++nt) {
+          const int row = nt * 16 + lr;
+          if constexpr (SWZ) {
+            const int col = (ks * KSTEP + lg * KPL) / VEC;
+            bf[nt] = M::load(Bc + row * LDB + ((col ^ (row & 31)) * VEC));
+          } else {
+            bf[nt] = M::load(Bc + row * LDB + ks * KSTEP + lg * KPL);
+          }
+        }
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+  #pragma unroll
+          for (int nt = 0; nt < 4; ++nt) acc[mt][nt] = M::mma(a[mt][ks], bf[nt], acc[mt][nt]);
+      }
+      // diagonal scatter into the LDS window: y[a + d] += Q[a, e(d)]
+  #pragma unroll
+      for (int nt = 0; nt < 4; ++nt) {
+        const int e = c * CH + nt * 16 + lr;
+        if (e >= ntaps) continue;
+        const int e1 = e / K, e2 = e - e1 * K;  // packed rows are e2-fastest
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          float* row = yw + (mt + e2) * W1 + e1;
+  #pragma unroll
+          for (int j = 0; j < 4; ++j) atomicAdd(row + 4 * lg + j, acc[mt][nt][j]);
+        }
+      }
+      if (NBUF == 2) __builtin_amdgcn_s_waitcnt(0);  // this wave's copies of chunk c+1 done
+    }
+    __syncthreads();
+    const int nb1 = ceil_div(n1, 16);
+    float* out = sd.partial + uni64(sd.poff[z]) + (size_t)(b2 * nb1 + b1) * W1 * W2;
+    for (int o = tid; o < W1 * W2; o += 256) {
+      const int y = o / W1, x = o - y * W1;
+      float acc = 0.f;
+  #pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int yy = y - MT * q;
+        if (yy >= 0 && yy < WW) acc += ywin[(q * WW + yy) * W1 + x];
+      }
+      out[o] = acc;
+    }
+    __syncthreads();  // LDS is reused by the next entry
   }
 }
 
@@ -252,10 +291,8 @@ struct TpzBufs {
 constexpr int kRingRow = 256;  // elements per lenslet row of the injection ring (phases >= 225 stay 0)
 
 template <typename T, int MT>
-__global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
-                                                 const __bf16* __restrict__ Xb,
-                                                 const T* __restrict__ W, FwdGeo g,
-                                                 float* __restrict__ partial) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
   using M = Mma<T>;
   constexpr int KSTEP = M::KSTEP, KPL = M::KPL, PHI = M::PHI;
   constexpr int NKS = PHI / KSTEP;
@@ -271,220 +308,239 @@ __global__ __launch_bounds__(256) void k_fwd_tpz(const float* __restrict__ X,
   T* ring = reinterpret_cast<T*>(smem + (size_t)NB * WSLOT * 16);            // [NB][CW][kRingRow]
   float* ywin = reinterpret_cast<float*>(ring + NB * CW * kRingRow);         // [4 waves][16][MT + 2r]
 
-  const int* s = g.sched + 3 * blockIdx.x;
-  const int z = s[0], b1 = s[1], b2 = s[2];
-  const int r = g.r[z];
-  const int K = 2 * r + 1;
-  const int NT = (K + 15) / 16;
-  const int nrow = 16 * NT;
-  const int WW = MT + 2 * r;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
-  const int c2_0 = CW * b2;
-  const float* Xz = X + (size_t)z * g.n2 * g.n1 * kP;
-  float* yw = ywin + w * 16 * WW;
-  for (int o = tid; o < 4 * 16 * WW; o += 256) ywin[o] = 0.f;
-  for (int o = tid; o < NB * CW * kRingRow; o += 256) ring[o] = (T)0.f;  // pads stay zero
+  const int nent = uni(*g.count);
+  for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
+    const int* s = g.sched + 4 * ent;
+    const SlotDesc sd = load_slot(g.slots, uni(s[0]));
+    const float* __restrict__ X = sd.X;
+    const __bf16* __restrict__ Xb = sd.Xb;
+    const int n1 = sd.n1, n2 = sd.n2;
+    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]);
+    const int r = uni(g.r[z]);
+    const int K = 2 * r + 1;
+    const int NT = (K + 15) / 16;
+    const int nrow = 16 * NT;
+    const int WW = MT + 2 * r;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+    const int c2_0 = CW * b2;
+    const float* Xz = X + (size_t)z * n2 * n1 * kP;
+    float* yw = ywin + w * 16 * WW;
+    for (int o = tid; o < 4 * 16 * WW; o += 256) ywin[o] = 0.f;
+    for (int o = tid; o < NB * CW * kRingRow; o += 256) ring[o] = (T)0.f;  // pads stay zero
 
-  // A operand at e1 = 0 (d1 = -r): lane row lr holds lenslet i1 = 16*b1 + lr + r
-  typename M::frag a[MT][NKS];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    const int c2 = c2_0 + MT * w + mt;
-    const int i1 = 16 * b1 + lr + r;
-    const bool ok = c2 < g.n2 && i1 < g.n1;
-    const size_t lens = ((size_t)z * g.n2 + (ok ? c2 : 0)) * g.n1 + (ok ? i1 : 0);
-#pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
-      if constexpr (KPL == 1) {
-        const int ph = ks * KSTEP + lg;
-        const float v = X[lens * kP + (ph < kP ? ph : kP - 1)];
-        a[mt][ks] = (ok && ph < kP) ? v : 0.f;
-      } else {
-        // BF16 shadow rows are 256 phases (zero beyond 225), 16-byte aligned
-        const bf16x8 v = *reinterpret_cast<const bf16x8*>(Xb + lens * kShadowRow + ks * KSTEP + lg * KPL);
-        a[mt][ks] = ok ? v : bf16x8{};
+    // A operand at e1 = 0 (d1 = -r): lane row lr holds lenslet i1 = 16*b1 + lr + r
+    typename M::frag a[MT][NKS];
+  #pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int c2 = c2_0 + MT * w + mt;
+      const int i1 = 16 * b1 + lr + r;
+      const bool ok = c2 < n2 && i1 < n1;
+      const size_t lens = ((size_t)z * n2 + (ok ? c2 : 0)) * n1 + (ok ? i1 : 0);
+  #pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+        if constexpr (KPL == 1) {
+          const int ph = ks * KSTEP + lg;
+          const float v = X[lens * kP + (ph < kP ? ph : kP - 1)];
+          a[mt][ks] = (ok && ph < kP) ? v : 0.f;
+        } else {
+          // BF16 shadow rows are 256 phases (zero beyond 225), 16-byte aligned
+          const bf16x8 v = *reinterpret_cast<const bf16x8*>(Xb + lens * kShadowRow + ks * KSTEP + lg * KPL);
+          a[mt][ks] = ok ? v : bf16x8{};
+        }
       }
     }
-  }
-  __syncthreads();  // ring zeroed before any DMA lands
+    __syncthreads();  // ring zeroed before any DMA lands
 
-  const T* Wz = W + g.woff[z];  // [K][nrow][PHI]
-  // Every wave issues the same number of DMA instructions per step (W: WSLOT
-  // slots / 256 threads; ring: CW*RPASS passes / 4 waves), so a step can wait
-  // for an earlier batch with a fixed vmcnt.
-  const int nslot = nrow * RV;
-  auto issue_w = [&](int t, int buf) {
-    const T* src = Wz + (size_t)t * nrow * PHI;
-    char* dst = reinterpret_cast<char*>(Wb) + (size_t)buf * WSLOT * 16;
-    for (int base = w * 64; base < WSLOT; base += 256) {
-      const int slot = min(base + lane, nslot - 1);  // tail slots re-read the last vector
-      const int row = slot / RV, pos = slot - row * RV;
-      const int col = SWZ ? (pos ^ (row & 31)) : pos;
-      __builtin_amdgcn_global_load_lds(
-          (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
-          (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
-    }
-  };
-  // injection row needed when entering step t (t >= 1): i1 = 16*b1 + r - t
-  // (rows outside the grid are loaded clamped and zeroed at the shift)
-  // FP32: 225 floats per lenslet in 4 passes of 64 dwords; BF16: the 512-byte
-  // shadow row in 2 passes of 64 dwords.
-  constexpr int RPASS = (KPL == 1) ? 4 : 2;
-  auto issue_inj = [&](int t, int slot_buf) {
-    const int i1 = min(max(16 * b1 + r - t, 0), g.n1 - 1);
-    T* dst = ring + slot_buf * CW * kRingRow;
-    for (int q = w; q < CW * RPASS; q += 4) {
-      const int c = q / RPASS, part = q - c * RPASS;
-      const int c2 = min(c2_0 + c, g.n2 - 1);
-      const size_t lens = ((size_t)z * g.n2 + c2) * g.n1 + i1;
-      const int d = part * 64 + lane;  // dword within the row
-      const void* src;
-      if constexpr (KPL == 1) {
-        if (d >= kP) continue;
-        src = X + lens * kP + d;
-      } else {
-        src = reinterpret_cast<const char*>(Xb + lens * kShadowRow) + 4 * d;
-      }
-      __builtin_amdgcn_global_load_lds(
-          (const __attribute__((address_space(1))) void*)src,
-          (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(dst + c * kRingRow) + 4 * part * 64),
-          4, 0, 0);
-    }
-  };
-  const int n_batch = WSLOT / 256 + CW * RPASS / 4;  // DMA instructions per wave, step t >= 1
-
-  floatx4 acc[MT][kTpzNT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < kTpzNT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-
-  // batch t = W column t (+ injection row t for t >= 1); batches with index
-  // >= 1 are n_batch instructions per wave
-  issue_w(0, 0);
-  for (int t = 1; t < NB - 1 && t < K; ++t) {
-    issue_w(t, t % NB);
-    issue_inj(t, t % NB);
-  }
-  wait_vm_le(max(0, min(K, NB - 1) - 1) * n_batch);  // batch 0 landed
-  __syncthreads();
-  for (int t = 0; t < K; ++t) {
-    if (t + NB - 1 < K) {
-      issue_w(t + NB - 1, (t + NB - 1) % NB);
-      issue_inj(t + NB - 1, (t + NB - 1) % NB);
-    }
-    const T* Wc = Wb + (size_t)(t % NB) * WSLOT * VEC;
-    auto load_b = [&](int ks, int nt) {
-      const int row = nt * 16 + lr;
-      if constexpr (SWZ) {
-        const int col = (ks * KSTEP + lg * KPL) / VEC;
-        return M::load(Wc + row * PHI + ((col ^ (row & 31)) * VEC));
-      } else {
-        return M::load(Wc + row * PHI + ks * KSTEP + lg * KPL);
+    const T* Wz = W + uni64(g.woff[z]);  // [K][nrow][PHI]
+    // Every wave issues the same number of DMA instructions per step (W: WSLOT
+    // slots / 256 threads; ring: CW*RPASS passes / 4 waves), so a step can wait
+    // for an earlier batch with a fixed vmcnt.
+    const int nslot = nrow * RV;
+    auto issue_w = [&](int t, int buf) {
+      const T* src = Wz + (size_t)t * nrow * PHI;
+      char* dst = reinterpret_cast<char*>(Wb) + (size_t)buf * WSLOT * 16;
+      for (int base = w * 64; base < WSLOT; base += 256) {
+        const int slot = min(base + lane, nslot - 1);  // tail slots re-read the last vector
+        const int row = slot / RV, pos = slot - row * RV;
+        const int col = SWZ ? (pos ^ (row & 31)) : pos;
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
+            (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
       }
     };
-#pragma unroll
-    for (int ks = 0; ks < NKS; ++ks) {
-#pragma unroll
-      for (int nt = 0; nt < kTpzNT; ++nt) {
-        if (nt >= NT) break;
-        const typename M::frag bf = load_b(ks, nt);
-#pragma unroll
-        for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
+    // injection row needed when entering step t (t >= 1): i1 = 16*b1 + r - t
+    // (rows outside the grid are loaded clamped and zeroed at the shift)
+    // FP32: 225 floats per lenslet in 4 passes of 64 dwords; BF16: the 512-byte
+    // shadow row in 2 passes of 64 dwords.
+    constexpr int RPASS = (KPL == 1) ? 4 : 2;
+    auto issue_inj = [&](int t, int slot_buf) {
+      const int i1 = min(max(16 * b1 + r - t, 0), n1 - 1);
+      T* dst = ring + slot_buf * CW * kRingRow;
+      for (int q = w; q < CW * RPASS; q += 4) {
+        const int c = q / RPASS, part = q - c * RPASS;
+        const int c2 = min(c2_0 + c, n2 - 1);
+        const size_t lens = ((size_t)z * n2 + c2) * n1 + i1;
+        const int d = part * 64 + lane;  // dword within the row
+        const void* src;
+        if constexpr (KPL == 1) {
+          if (d >= kP) continue;
+          src = X + lens * kP + d;
+        } else {
+          src = reinterpret_cast<const char*>(Xb + lens * kShadowRow) + 4 * d;
+        }
+        __builtin_amdgcn_global_load_lds(
+            (const __attribute__((address_space(1))) void*)src,
+            (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(dst + c * kRingRow) + 4 * part * 64),
+            4, 0, 0);
       }
+    };
+    const int n_batch = WSLOT / 256 + CW * RPASS / 4;  // DMA instructions per wave, step t >= 1
+
+    floatx4 acc[MT][kTpzNT];
+  #pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+  #pragma unroll
+      for (int nt = 0; nt < kTpzNT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // batch t = W column t (+ injection row t for t >= 1); batches with index
+    // >= 1 are n_batch instructions per wave
+    issue_w(0, 0);
+    for (int t = 1; t < NB - 1 && t < K; ++t) {
+      issue_w(t, t % NB);
+      issue_inj(t, t % NB);
     }
-    // batch t+1 landed; later batches may still be in flight
-    wait_vm_le(max(0, min(K, t + NB) - t - 2) * n_batch);
+    wait_vm_le(max(0, min(K, NB - 1) - 1) * n_batch);  // batch 0 landed
     __syncthreads();
-    if (t + 1 < K) {
-      const int i1n = 16 * b1 + r - (t + 1);
-      const T* rg = ring + ((t + 1) % NB) * CW * kRingRow;
-      const bool rowok = (i1n >= 0) && (i1n < g.n1);
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const int c = MT * w + mt;
-        const bool ok = rowok && (c2_0 + c < g.n2);
-        // every lane reads its row group's slot (broadcast); only lane 0 of
-        // each 16-lane row keeps it (the DPP 'old' operand)
-        const T* rp = rg + c * kRingRow + lg * KPL;
-#pragma unroll
-        for (int ks = 0; ks < NKS; ++ks) {
-          if constexpr (KPL == 1) {
-            const float v = rp[ks * KSTEP];
-            const float inj = ok ? v : 0.f;
-            a[mt][ks] = __builtin_bit_cast(
-                float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, inj),
-                                                   __builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF,
-                                                   false));
-          } else {
-            const bf16x8 v = *reinterpret_cast<const bf16x8*>(rp + ks * KSTEP);
-            const bf16x8 inj = ok ? v : bf16x8{};
-            typedef int intx4 __attribute__((ext_vector_type(4)));
-            const intx4 iv = __builtin_bit_cast(intx4, inj);
-            intx4 cv = __builtin_bit_cast(intx4, a[mt][ks]);
-#pragma unroll
-            for (int q = 0; q < 4; ++q) cv[q] = __builtin_amdgcn_update_dpp(iv[q], cv[q], 0x111, 0xF, 0xF, false);
-            a[mt][ks] = __builtin_bit_cast(bf16x8, cv);
+    for (int t = 0; t < K; ++t) {
+      if (t + NB - 1 < K) {
+        issue_w(t + NB - 1, (t + NB - 1) % NB);
+        issue_inj(t + NB - 1, (t + NB - 1) % NB);
+      }
+      const T* Wc = Wb + (size_t)(t % NB) * WSLOT * VEC;
+      auto load_b = [&](int ks, int nt) {
+        const int row = nt * 16 + lr;
+        if constexpr (SWZ) {
+          const int col = (ks * KSTEP + lg * KPL) / VEC;
+          return M::load(Wc + row * PHI + ((col ^ (row & 31)) * VEC));
+        } else {
+          return M::load(Wc + row * PHI + ks * KSTEP + lg * KPL);
+        }
+      };
+  #pragma unroll
+      for (int ks = 0; ks < NKS; ++ks) {
+  #pragma unroll
+        for (int nt = 0; nt < kTpzNT; ++nt) {
+          if (nt >= NT) break;
+          const typename M::frag bf = load_b(ks, nt);
+  #pragma unroll
+          for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
+        }
+      }
+      // batch t+1 landed; later batches may still be in flight
+      wait_vm_le(max(0, min(K, t + NB) - t - 2) * n_batch);
+      __syncthreads();
+      if (t + 1 < K) {
+        const int i1n = 16 * b1 + r - (t + 1);
+        const T* rg = ring + ((t + 1) % NB) * CW * kRingRow;
+        const bool rowok = (i1n >= 0) && (i1n < n1);
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const int c = MT * w + mt;
+          const bool ok = rowok && (c2_0 + c < n2);
+          // every lane reads its row group's slot (broadcast); only lane 0 of
+          // each 16-lane row keeps it (the DPP 'old' operand)
+          const T* rp = rg + c * kRingRow + lg * KPL;
+  #pragma unroll
+          for (int ks = 0; ks < NKS; ++ks) {
+            if constexpr (KPL == 1) {
+              const float v = rp[ks * KSTEP];
+              const float inj = ok ? v : 0.f;
+              a[mt][ks] = __builtin_bit_cast(
+                  float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, inj),
+                                                     __builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF,
+                                                     false));
+            } else {
+              const bf16x8 v = *reinterpret_cast<const bf16x8*>(rp + ks * KSTEP);
+              const bf16x8 inj = ok ? v : bf16x8{};
+              typedef int intx4 __attribute__((ext_vector_type(4)));
+              const intx4 iv = __builtin_bit_cast(intx4, inj);
+              intx4 cv = __builtin_bit_cast(intx4, a[mt][ks]);
+  #pragma unroll
+              for (int q = 0; q < 4; ++q) cv[q] = __builtin_amdgcn_update_dpp(iv[q], cv[q], 0x111, 0xF, 0xF, false);
+              a[mt][ks] = __builtin_bit_cast(bf16x8, cv);
+            }
           }
         }
       }
     }
-  }
 
-  // P tile -> this wave's window (rows i1 - 16*b1, columns c2 + d2 - (c2_0 + MT*w - r))
-#pragma unroll
-  for (int nt = 0; nt < kTpzNT; ++nt) {
-    const int e2 = nt * 16 + lr;
-    if (nt >= NT || e2 >= K) continue;
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-      for (int j = 0; j < 4; ++j) atomicAdd(yw + (4 * lg + j) * WW + mt + e2, acc[mt][nt][j]);
-  }
-  __syncthreads();
-  const int Wc2 = CW + 2 * r;
-  const int nb1 = ceil_div(g.n1, 16);
-  float* out = partial + g.poff[z] + (size_t)(b2 * nb1 + b1) * 16 * Wc2;
-  for (int o = tid; o < 16 * Wc2; o += 256) {
-    const int i = o / Wc2, c = o - i * Wc2;
-    float sum = 0.f;
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int cc = c - MT * q;
-      if (cc >= 0 && cc < WW) sum += ywin[(q * 16 + i) * WW + cc];
+    // P tile -> this wave's window (rows i1 - 16*b1, columns c2 + d2 - (c2_0 + MT*w - r))
+  #pragma unroll
+    for (int nt = 0; nt < kTpzNT; ++nt) {
+      const int e2 = nt * 16 + lr;
+      if (nt >= NT || e2 >= K) continue;
+  #pragma unroll
+      for (int mt = 0; mt < MT; ++mt)
+  #pragma unroll
+        for (int j = 0; j < 4; ++j) atomicAdd(yw + (4 * lg + j) * WW + mt + e2, acc[mt][nt][j]);
     }
-    out[o] = sum;
+    __syncthreads();
+    const int Wc2 = CW + 2 * r;
+    const int nb1 = ceil_div(n1, 16);
+    float* out = sd.partial + uni64(sd.poff[z]) + (size_t)(b2 * nb1 + b1) * 16 * Wc2;
+    for (int o = tid; o < 16 * Wc2; o += 256) {
+      const int i = o / Wc2, c = o - i * Wc2;
+      float sum = 0.f;
+  #pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int cc = c - MT * q;
+        if (cc >= 0 && cc < WW) sum += ywin[(q * 16 + i) * WW + cc];
+      }
+      out[o] = sum;
+    }
+    __syncthreads();  // LDS is reused by the next entry
   }
 }
 
-// Gather for the Toeplitz partials: y[i] = sum_z sum_{b2} window(b1 = i1/16, b2)
-// in a fixed order (FP64), then the ratio as k_ratio_gather.
-__global__ void k_ratio_gather_tpz(const float* __restrict__ partial, FwdGeo g, int Z, int CW,
-                                   const float* __restrict__ LF, int u, int v,
-                                   float* __restrict__ R, float* __restrict__ y_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n1 * g.n2) return;
-  const int i1 = i % g.n1, i2 = i / g.n1;
-  const int nb1 = ceil_div(g.n1, 16), nb2 = ceil_div(g.n2, CW);
-  const int b1 = i1 / 16;
-  double y = 0.0;
-  for (int z = 0; z < Z; ++z) {
-    const int r = g.r[z];
-    const int Wc2 = CW + 2 * r;
-    const int b2h = min(nb2 - 1, (i2 + r) / CW);
-    const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
-    const float* pz = partial + g.poff[z];
-    for (int b2 = b2l; b2 <= b2h; ++b2)
-      y += (double)pz[(size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r)];
-  }
+// Ratio of one lenslet (recon:116-117): R = LF(u::15, v::15) ./ y with
+// non-finite -> 0; y is the forward projection summed in FP64.
+__device__ __forceinline__ void ratio_out(const SlotDesc& sd, int i, int i1, int i2, double y,
+                                          int u, int v, int use_lf, float* y_out) {
   const float yf = (float)y;
-  if (LF) {
-    const float lf = LF[(size_t)(u + kN * i1) + (size_t)(kN * g.n1) * (v + kN * i2)];
+  if (use_lf) {
+    const float lf = sd.LF[(size_t)(u + kN * i1) + (size_t)(kN * sd.n1) * (v + kN * i2)];
     float q = lf / yf;
     if (!isfinite(q)) q = 0.f;
-    R[i] = q;
+    sd.R[i] = q;
   }
   if (y_out) y_out[i] = yf;
+}
+
+// Gather for the Toeplitz partials: y[i] = sum_z sum_{b2} window(b1 = i1/16, b2)
+// in a fixed order (FP64), then the ratio.
+__global__ void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int Z, int CW, int u, int v, int use_lf,
+                                   float* __restrict__ y_out) {
+  const int nent = uni(*rg.count);
+  for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
+    const SlotDesc sd = load_slot(g.slots, uni(rg.sched[2 * ent]));
+    const int n1 = sd.n1, n2 = sd.n2;
+    const int i = uni(rg.sched[2 * ent + 1]) + threadIdx.x;
+    if (i >= n1 * n2) continue;
+    const int i1 = i % n1, i2 = i / n1;
+    const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, CW);
+    const int b1 = i1 / 16;
+    double y = 0.0;
+    for (int z = 0; z < Z; ++z) {
+      const int r = uni(g.r[z]);
+      const int Wc2 = CW + 2 * r;
+      const int b2h = min(nb2 - 1, (i2 + r) / CW);
+      const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
+      const float* pz = sd.partial + uni64(sd.poff[z]);
+      for (int b2 = b2l; b2 <= b2h; ++b2)
+        y += (double)pz[(size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r)];
+    }
+    ratio_out(sd, i, i1, i2, y, u, v, use_lf, y_out);
+  }
 }
 
 // ABI block [e2][e1][phase] -> Toeplitz pack [e1][e2 padded to 16*ceil(K/16)][PHI].
@@ -507,36 +563,33 @@ __global__ void k_pack_fwd_tpz(const float* __restrict__ src, T* __restrict__ ds
 }
 
 // y[i] = sum over planes and covering blocks of the window partials (fixed
-// order, FP64), then R = LF(u::15, v::15) ./ y with non-finite -> 0.
-__global__ void k_ratio_gather(const float* __restrict__ partial, FwdGeo g, int Z, int BM2,
-                               const float* __restrict__ LF, int u, int v, float* __restrict__ R,
+// order, FP64), then the ratio (Q-scatter forward).
+__global__ void k_ratio_gather(FwdGeo g, RatioGeo rg, int Z, int BM2, int u, int v, int use_lf,
                                float* __restrict__ y_out) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= g.n1 * g.n2) return;
-  const int i1 = i % g.n1, i2 = i / g.n1;
-  const int nb1 = ceil_div(g.n1, 16), nb2 = ceil_div(g.n2, BM2);
-  double y = 0.0;
-  for (int z = 0; z < Z; ++z) {
-    const int r = g.r[z];
-    const int W1 = 16 + 2 * r, W2 = BM2 + 2 * r;
-    const int b1h = min(nb1 - 1, (i1 + r) / 16);
-    const int b1l = (i1 - r - 15 <= 0) ? 0 : (i1 - r - 15 + 15) / 16;
-    const int b2h = min(nb2 - 1, (i2 + r) / BM2);
-    const int b2l = (i2 - r - BM2 + 1 <= 0) ? 0 : (i2 - r - BM2 + 1 + BM2 - 1) / BM2;
-    const float* pz = partial + g.poff[z];
-    for (int b2 = b2l; b2 <= b2h; ++b2)
-      for (int b1 = b1l; b1 <= b1h; ++b1)
-        y += (double)pz[(size_t)(b2 * nb1 + b1) * W1 * W2 + (i2 - BM2 * b2 + r) * W1 +
-                        (i1 - 16 * b1 + r)];
+  const int nent = uni(*rg.count);
+  for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
+    const SlotDesc sd = load_slot(g.slots, uni(rg.sched[2 * ent]));
+    const int n1 = sd.n1, n2 = sd.n2;
+    const int i = uni(rg.sched[2 * ent + 1]) + threadIdx.x;
+    if (i >= n1 * n2) continue;
+    const int i1 = i % n1, i2 = i / n1;
+    const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, BM2);
+    double y = 0.0;
+    for (int z = 0; z < Z; ++z) {
+      const int r = uni(g.r[z]);
+      const int W1 = 16 + 2 * r, W2 = BM2 + 2 * r;
+      const int b1h = min(nb1 - 1, (i1 + r) / 16);
+      const int b1l = (i1 - r - 15 <= 0) ? 0 : (i1 - r - 15 + 15) / 16;
+      const int b2h = min(nb2 - 1, (i2 + r) / BM2);
+      const int b2l = (i2 - r - BM2 + 1 <= 0) ? 0 : (i2 - r - BM2 + 1 + BM2 - 1) / BM2;
+      const float* pz = sd.partial + uni64(sd.poff[z]);
+      for (int b2 = b2l; b2 <= b2h; ++b2)
+        for (int b1 = b1l; b1 <= b1h; ++b1)
+          y += (double)pz[(size_t)(b2 * nb1 + b1) * W1 * W2 + (i2 - BM2 * b2 + r) * W1 +
+                          (i1 - 16 * b1 + r)];
+    }
+    ratio_out(sd, i, i1, i2, y, u, v, use_lf, y_out);
   }
-  const float yf = (float)y;
-  if (LF) {
-    const float lf = LF[(size_t)(u + kN * i1) + (size_t)(kN * g.n1) * (v + kN * i2)];
-    float q = lf / yf;
-    if (!isfinite(q)) q = 0.f;
-    R[i] = q;
-  }
-  if (y_out) y_out[i] = yf;
 }
 
 // --------------------------------------------------------------- backward
@@ -550,199 +603,207 @@ __device__ __forceinline__ int norm_class(int a, int n, int r) {
 constexpr int kBufferDword3 = 0x00020000;
 
 template <typename T>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_bwd_mfma(float* __restrict__ X, __bf16* __restrict__ Xsh,
-                                                  const T* __restrict__ Wb,
-                                                  const float* __restrict__ Tn, BwdGeo g,
-                                                  const float* __restrict__ R, float wgt) {
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   using M = Mma<T>;
   constexpr int KSTEP = M::KSTEP, KPL = M::KPL;
   constexpr int MT = 4, NT = kBwdNTiles;
   extern __shared__ __align__(16) char smem[];
-  const int* s = g.sched + 4 * blockIdx.x;
-  const int z = s[0], b1 = s[1], b2 = s[2], grp = s[3];
-  const int r = g.r[z];
-  const int K = 2 * r + 1, ntaps = K * K;
-  const int epad = g.epad[z];
-  const int Wd = 16 + 2 * r;
-  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
-  const T* Bz = Wb + g.boff[z] + (size_t)(grp * 16 * NT) * epad;
-  const int nks = epad / KSTEP;
-  floatx4 acc[MT][NT];
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt)
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+  const int nent = uni(*g.count);
+  for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
+    const int* s = g.sched + 5 * ent;
+    const SlotDesc sd = load_slot(g.slots, uni(s[0]));
+    float* __restrict__ X = sd.X;
+    __bf16* __restrict__ Xsh = sd.Xb;
+    const float* __restrict__ R = sd.R;
+    const int n1 = sd.n1, n2 = sd.n2;
+    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]), grp = uni(s[4]);
+    const int r = uni(g.r[z]);
+    const int K = 2 * r + 1, ntaps = K * K;
+    const int epad = uni(g.epad[z]);
+    const int Wd = 16 + 2 * r;
+    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+    const T* Bz = Wb + uni64(g.boff[z]) + (size_t)(grp * 16 * NT) * epad;
+    const int nks = epad / KSTEP;
+    floatx4 acc[MT][NT];
+  #pragma unroll
+    for (int mt = 0; mt < MT; ++mt)
+  #pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
 
-  if constexpr (KPL == 1) {
-    // FP32: scalar im2col reads, taps e1-fastest
-    float* Rw = reinterpret_cast<float*>(smem);                                 // [Wd][Wd]
-    int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // [epad]
-    for (int o = tid; o < Wd * Wd; o += 256) {
-      const int x = o % Wd, y = o / Wd;
-      const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
-      Rw[o] = (a1 >= 0 && a1 < g.n1 && a2 >= 0 && a2 < g.n2) ? R[a1 + g.n1 * a2] : 0.f;
-    }
-    for (int e = tid; e < epad; e += 256) {
-      int off = 0;  // padded taps have zero weights; any in-window read is fine
-      if (e < ntaps) {
-        const int e2 = e / K, e1 = e - e2 * K;
-        off = -(e1 - r) - (e2 - r) * Wd;
+    if constexpr (KPL == 1) {
+      // FP32: scalar im2col reads, taps e1-fastest
+      float* Rw = reinterpret_cast<float*>(smem);                                 // [Wd][Wd]
+      int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // [epad]
+      for (int o = tid; o < Wd * Wd; o += 256) {
+        const int x = o % Wd, y = o / Wd;
+        const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
+        Rw[o] = (a1 >= 0 && a1 < n1 && a2 >= 0 && a2 < n2) ? R[a1 + n1 * a2] : 0.f;
       }
-      toff[e] = off;
-    }
-    __syncthreads();
-    int base[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
-    typename M::frag bn[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
-    for (int ks = 0; ks < nks; ++ks) {
-      typename M::frag bf[NT];
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
-      if (ks + 1 < nks) {
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+      for (int e = tid; e < epad; e += 256) {
+        int off = 0;  // padded taps have zero weights; any in-window read is fine
+        if (e < ntaps) {
+          const int e2 = e / K, e1 = e - e2 * K;
+          off = -(e1 - r) - (e2 - r) * Wd;
+        }
+        toff[e] = off;
       }
-      const int o = toff[ks * KSTEP + lg];
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const typename M::frag af = Rw[base[mt] + o];
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+      __syncthreads();
+      int base[MT];
+  #pragma unroll
+      for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
+      typename M::frag bn[NT];
+  #pragma unroll
+      for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
+      for (int ks = 0; ks < nks; ++ks) {
+        typename M::frag bf[NT];
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
+        if (ks + 1 < nks) {
+  #pragma unroll
+          for (int nt = 0; nt < NT; ++nt)
+            bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+        }
+        const int o = toff[ks * KSTEP + lg];
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const typename M::frag af = Rw[base[mt] + o];
+  #pragma unroll
+          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+        }
       }
-    }
-  } else {
-    // BF16: taps ordered (e2, e1) with e1 padded to a multiple of 8, so a
-    // lane's 8 taps of a K-step are 8 consecutive e1 in one row.  Cell (x, y)
-    // of the pre-shifted window holds R(x-j, y) for j = 0..7, so each A
-    // fragment is a single 16-byte LDS read.
-    const int Kp8 = (K + 7) & ~7;
-    float* Rs = reinterpret_cast<float*>(smem);                                  // [Wd][Wd]
-    bf16x8* Rv = reinterpret_cast<bf16x8*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // [Wd][Wd]
-    int* toff8 = reinterpret_cast<int*>(reinterpret_cast<char*>(Rv) + sizeof(bf16x8) * Wd * Wd);
-    for (int o = tid; o < Wd * Wd; o += 256) {
-      const int x = o % Wd, y = o / Wd;
-      const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
-      Rs[o] = (a1 >= 0 && a1 < g.n1 && a2 >= 0 && a2 < g.n2) ? R[a1 + g.n1 * a2] : 0.f;
-    }
-    for (int q = tid; q < epad / 8; q += 256) {
-      const int t0 = 8 * q;
-      const int e2 = t0 / Kp8, e10 = t0 - e2 * Kp8;
-      toff8[q] = (e2 < K) ? (r - e10) - (e2 - r) * Wd : 0;
-    }
-    __syncthreads();
-    for (int o = tid; o < Wd * Wd; o += 256) {
-      const int x = o % Wd;
-      bf16x8 v;
-#pragma unroll
-      for (int j = 0; j < 8; ++j) v[j] = (__bf16)(x - j >= 0 ? Rs[o - j] : 0.f);
-      Rv[o] = v;
-    }
-    __syncthreads();
-    int base[MT];
-#pragma unroll
-    for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
-    typename M::frag bn[NT];
-#pragma unroll
-    for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
-    for (int ks = 0; ks < nks; ++ks) {
-      typename M::frag bf[NT];
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
-      if (ks + 1 < nks) {
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+    } else {
+      // BF16: taps ordered (e2, e1) with e1 padded to a multiple of 8, so a
+      // lane's 8 taps of a K-step are 8 consecutive e1 in one row.  Cell (x, y)
+      // of the pre-shifted window holds R(x-j, y) for j = 0..7, so each A
+      // fragment is a single 16-byte LDS read.
+      const int Kp8 = (K + 7) & ~7;
+      float* Rs = reinterpret_cast<float*>(smem);                                  // [Wd][Wd]
+      bf16x8* Rv = reinterpret_cast<bf16x8*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // [Wd][Wd]
+      int* toff8 = reinterpret_cast<int*>(reinterpret_cast<char*>(Rv) + sizeof(bf16x8) * Wd * Wd);
+      for (int o = tid; o < Wd * Wd; o += 256) {
+        const int x = o % Wd, y = o / Wd;
+        const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
+        Rs[o] = (a1 >= 0 && a1 < n1 && a2 >= 0 && a2 < n2) ? R[a1 + n1 * a2] : 0.f;
       }
-      const int o = toff8[ks * 4 + lg];
-#pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        const typename M::frag af = Rv[base[mt] + o];
-#pragma unroll
-        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+      for (int q = tid; q < epad / 8; q += 256) {
+        const int t0 = 8 * q;
+        const int e2 = t0 / Kp8, e10 = t0 - e2 * Kp8;
+        toff8[q] = (e2 < K) ? (r - e10) - (e2 - r) * Wd : 0;
       }
-    }
-  }
-
-  // fused multiplicative update (recon:130-137).  Loads are unconditional
-  // from clamped indices and issued one M-tile ahead (40 in flight per lane),
-  // so HBM latency is paid a few times per workgroup rather than once per
-  // N-tile; only the stores are predicated.  Buffer instructions take 32-bit
-  // offsets from per-plane resources (one VGPR per address; a plane is below
-  // 2^31 bytes, checked at patch creation) and drop any access outside the
-  // plane.
-  const int C1 = g.C1[z], C2 = norm_classes(g.n2, r);
-  const size_t nlp = (size_t)g.n2 * g.n1;  // lenslets per plane
-  const __amdgpu_buffer_rsrc_t xr =
-      __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kP, 0, (int)(nlp * kP * 4), kBufferDword3);
-  const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
-      const_cast<float*>(Tn + g.toffT[z]), 0, C2 * C1 * kP * 4, kBufferDword3);
-  const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc(
-      Xsh ? Xsh + z * nlp * kShadowRow : nullptr, 0, Xsh ? (int)(nlp * kShadowRow * 2) : 0,
-      kBufferDword3);
-  int phc[NT];
-  bool okp[NT];
-#pragma unroll
-  for (int nt = 0; nt < NT; ++nt) {
-    const int ph = grp * 16 * NT + nt * 16 + lr;
-    okp[nt] = ph < kP;
-    phc[nt] = okp[nt] ? ph : kP - 1;
-  }
-  int a1c[4], c1v[4];
-  bool ok1[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    const int a1 = 16 * b1 + 4 * lg + j;
-    ok1[j] = a1 < g.n1;
-    a1c[j] = ok1[j] ? a1 : g.n1 - 1;
-    c1v[j] = norm_class(a1c[j], g.n1, r);
-  }
-  float xv[2][NT][4], bv[2][NT][4];
-  auto load_tile = [&](int mt, int buf) {
-    const int a2 = 16 * b2 + 4 * w + mt;
-    const int a2c = a2 < g.n2 ? a2 : g.n2 - 1;
-    const int c2 = norm_class(a2c, g.n2, r);
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int xo = (a2c * g.n1 + a1c[j]) * kP;
-      const int to = (c2 * C1 + c1v[j]) * kP;
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        xv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, 4 * (xo + phc[nt]), 0, 0));
-        bv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tr, 4 * (to + phc[nt]), 0, 0));
+      __syncthreads();
+      for (int o = tid; o < Wd * Wd; o += 256) {
+        const int x = o % Wd;
+        bf16x8 v;
+  #pragma unroll
+        for (int j = 0; j < 8; ++j) v[j] = (__bf16)(x - j >= 0 ? Rs[o - j] : 0.f);
+        Rv[o] = v;
       }
-    }
-  };
-  load_tile(0, 0);
-#pragma unroll
-  for (int mt = 0; mt < MT; ++mt) {
-    const int buf = mt & 1;
-    if (mt + 1 < MT) load_tile(mt + 1, buf ^ 1);
-    const int a2 = 16 * b2 + 4 * w + mt;
-    const bool ok2 = a2 < g.n2;
-    const int a2c = ok2 ? a2 : g.n2 - 1;
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int l = a2c * g.n1 + a1c[j];
-#pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        const float x = xv[buf][nt][j];
-        float C = x * acc[mt][nt][j] / bv[buf][nt][j];
-        // NaN, +-Inf and negatives -> 0, as a select (no branches)
-        C = (C >= 0.f && C <= 3.402823466e38f) ? C : 0.f;
-        float xn = C * wgt + (1.f - wgt) * x;
-        xn = xn < 0.f ? 0.f : xn;
-        if (ok2 && okp[nt] && ok1[j]) {
-          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn), xr, 4 * (l * kP + phc[nt]), 0, 0);
-          if (Xsh)
-            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)xn), sr,
-                                                  2 * (l * kShadowRow + phc[nt]), 0, 0);
+      __syncthreads();
+      int base[MT];
+  #pragma unroll
+      for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
+      typename M::frag bn[NT];
+  #pragma unroll
+      for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
+      for (int ks = 0; ks < nks; ++ks) {
+        typename M::frag bf[NT];
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
+        if (ks + 1 < nks) {
+  #pragma unroll
+          for (int nt = 0; nt < NT; ++nt)
+            bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+        }
+        const int o = toff8[ks * 4 + lg];
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          const typename M::frag af = Rv[base[mt] + o];
+  #pragma unroll
+          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
         }
       }
     }
+
+    // fused multiplicative update (recon:130-137).  Loads are unconditional
+    // from clamped indices and issued one M-tile ahead (40 in flight per lane),
+    // so HBM latency is paid a few times per workgroup rather than once per
+    // N-tile; only the stores are predicated.  Buffer instructions take 32-bit
+    // offsets from per-plane resources (one VGPR per address; a plane is below
+    // 2^31 bytes, checked at patch creation) and drop any access outside the
+    // plane.
+    const int C1 = uni(sd.C1[z]), C2 = norm_classes(n2, r);
+    const size_t nlp = (size_t)n2 * n1;  // lenslets per plane
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kP, 0, (int)(nlp * kP * 4), kBufferDword3);
+    const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(sd.toffT[z])), 0, C2 * C1 * kP * 4,
+        kBufferDword3);
+    const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc(
+        Xsh ? Xsh + z * nlp * kShadowRow : nullptr, 0, Xsh ? (int)(nlp * kShadowRow * 2) : 0,
+        kBufferDword3);
+    int phc[NT];
+    bool okp[NT];
+  #pragma unroll
+    for (int nt = 0; nt < NT; ++nt) {
+      const int ph = grp * 16 * NT + nt * 16 + lr;
+      okp[nt] = ph < kP;
+      phc[nt] = okp[nt] ? ph : kP - 1;
+    }
+    int a1c[4], c1v[4];
+    bool ok1[4];
+  #pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int a1 = 16 * b1 + 4 * lg + j;
+      ok1[j] = a1 < n1;
+      a1c[j] = ok1[j] ? a1 : n1 - 1;
+      c1v[j] = norm_class(a1c[j], n1, r);
+    }
+    float xv[2][NT][4], bv[2][NT][4];
+    auto load_tile = [&](int mt, int buf) {
+      const int a2 = 16 * b2 + 4 * w + mt;
+      const int a2c = a2 < n2 ? a2 : n2 - 1;
+      const int c2 = norm_class(a2c, n2, r);
+  #pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int xo = (a2c * n1 + a1c[j]) * kP;
+        const int to = (c2 * C1 + c1v[j]) * kP;
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          xv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, 4 * (xo + phc[nt]), 0, 0));
+          bv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tr, 4 * (to + phc[nt]), 0, 0));
+        }
+      }
+    };
+    load_tile(0, 0);
+  #pragma unroll
+    for (int mt = 0; mt < MT; ++mt) {
+      const int buf = mt & 1;
+      if (mt + 1 < MT) load_tile(mt + 1, buf ^ 1);
+      const int a2 = 16 * b2 + 4 * w + mt;
+      const bool ok2 = a2 < n2;
+      const int a2c = ok2 ? a2 : n2 - 1;
+  #pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int l = a2c * n1 + a1c[j];
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          const float x = xv[buf][nt][j];
+          float C = x * acc[mt][nt][j] / bv[buf][nt][j];
+          // NaN, +-Inf and negatives -> 0, as a select (no branches)
+          C = (C >= 0.f && C <= 3.402823466e38f) ? C : 0.f;
+          float xn = C * wgt + (1.f - wgt) * x;
+          xn = xn < 0.f ? 0.f : xn;
+          if (ok2 && okp[nt] && ok1[j]) {
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn), xr, 4 * (l * kP + phc[nt]), 0, 0);
+            if (Xsh)
+              __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)xn), sr,
+                                                    2 * (l * kShadowRow + phc[nt]), 0, 0);
+          }
+        }
+      }
+    }
+    __syncthreads();  // LDS is reused by the next entry
   }
 }
 
@@ -851,14 +912,14 @@ size_t bwd_smem_bytes(Precision p, int rmax, int epad_max) {
   return sizeof(float) * ((Wd * Wd + 3) & ~3) + 16 * (size_t)Wd * Wd + sizeof(int) * (epad_max / 8);
 }
 
-hipError_t launch_fwd_mfma(Precision p, const float* X, const void* W, FwdGeo g, int nblocks,
-                           size_t smem, float* partial, hipStream_t st) {
+hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+                           hipStream_t st) {
   if (p == Precision::kF32) {
-    hipLaunchKernelGGL((k_fwd_mfma<float, kFwdMTF32>), dim3(nblocks), dim3(256), smem, st, X,
-                       (const float*)W, g, partial);
+    hipLaunchKernelGGL((k_fwd_mfma<float, kFwdMTF32>), dim3(grid), dim3(256), smem, st,
+                       (const float*)W, g);
   } else {
-    hipLaunchKernelGGL((k_fwd_mfma<__bf16, kFwdMTBF16>), dim3(nblocks), dim3(256), smem, st, X,
-                       (const __bf16*)W, g, partial);
+    hipLaunchKernelGGL((k_fwd_mfma<__bf16, kFwdMTBF16>), dim3(grid), dim3(256), smem, st,
+                       (const __bf16*)W, g);
   }
   return hipGetLastError();
 }
@@ -875,14 +936,14 @@ size_t tpz_smem_bytes(Precision p, int rmax) {
 
 int tpz_max_radius() { return (16 * kTpzNT - 1) / 2; }
 
-hipError_t launch_fwd_tpz(Precision p, const float* X, const void* Xb, const void* W, FwdGeo g,
-                          int nblocks, size_t smem, float* partial, hipStream_t st) {
+hipError_t launch_fwd_tpz(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+                          hipStream_t st) {
   if (p == Precision::kF32) {
-    hipLaunchKernelGGL((k_fwd_tpz<float, kTpzMTF32>), dim3(nblocks), dim3(256), smem, st, X,
-                       (const __bf16*)Xb, (const float*)W, g, partial);
+    hipLaunchKernelGGL((k_fwd_tpz<float, kTpzMTF32>), dim3(grid), dim3(256), smem, st,
+                       (const float*)W, g);
   } else {
-    hipLaunchKernelGGL((k_fwd_tpz<__bf16, kTpzMTBF16>), dim3(nblocks), dim3(256), smem, st, X,
-                       (const __bf16*)Xb, (const __bf16*)W, g, partial);
+    hipLaunchKernelGGL((k_fwd_tpz<__bf16, kTpzMTBF16>), dim3(grid), dim3(256), smem, st,
+                       (const __bf16*)W, g);
   }
   return hipGetLastError();
 }
@@ -903,10 +964,10 @@ hipError_t launch_make_shadow(const float* X, void* Xb, size_t nlens, hipStream_
   return hipGetLastError();
 }
 
-hipError_t launch_ratio_gather_tpz(const float* partial, FwdGeo g, int Z, int cw, const float* LF,
-                                   int u, int v, float* R, float* y_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_ratio_gather_tpz, dim3((g.n1 * g.n2 + 255) / 256), dim3(256), 0, st,
-                     partial, g, Z, cw, LF, u, v, R, y_out);
+hipError_t launch_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int grid, int Z, int cw, int u, int v,
+                                   int use_lf, float* y_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ratio_gather_tpz, dim3(grid), dim3(256), 0, st, g, rg, Z, cw, u, v, use_lf,
+                     y_out);
   return hipGetLastError();
 }
 
@@ -920,22 +981,20 @@ hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo
   return hipGetLastError();
 }
 
-hipError_t launch_ratio_gather(const float* partial, FwdGeo g, int Z, int bm2, const float* LF,
-                               int u, int v, float* R, float* y_out, hipStream_t st) {
-  hipLaunchKernelGGL(k_ratio_gather, dim3((g.n1 * g.n2 + 255) / 256), dim3(256), 0, st, partial, g,
-                     Z, bm2, LF, u, v, R, y_out);
+hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, int u, int v,
+                               int use_lf, float* y_out, hipStream_t st) {
+  hipLaunchKernelGGL(k_ratio_gather, dim3(grid), dim3(256), 0, st, g, rg, Z, bm2, u, v, use_lf,
+                     y_out);
   return hipGetLastError();
 }
 
-hipError_t launch_bwd_mfma(Precision p, float* X, void* Xb, const void* Wb, const float* Tn,
-                           BwdGeo g, int nblocks, size_t smem, const float* R, float w,
+hipError_t launch_bwd_mfma(Precision p, const void* Wb, BwdGeo g, int grid, size_t smem, float w,
                            hipStream_t st) {
   if (p == Precision::kF32) {
-    hipLaunchKernelGGL(k_bwd_mfma<float>, dim3(nblocks), dim3(256), smem, st, X, (__bf16*)Xb,
-                       (const float*)Wb, Tn, g, R, w);
+    hipLaunchKernelGGL(k_bwd_mfma<float>, dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
   } else {
-    hipLaunchKernelGGL(k_bwd_mfma<__bf16>, dim3(nblocks), dim3(256), smem, st, X, (__bf16*)Xb,
-                       (const __bf16*)Wb, Tn, g, R, w);
+    hipLaunchKernelGGL(k_bwd_mfma<__bf16>, dim3(grid), dim3(256), smem, st, (const __bf16*)Wb, g,
+                       w);
   }
   return hipGetLastError();
 }

@@ -17,6 +17,7 @@
 
 #include "internal.h"
 #include "rl_mfma.h"
+#include "state.h"
 
 namespace mlf {
 
@@ -183,19 +184,11 @@ static void truncate_kernel(const std::vector<double>& k, int* n0, std::vector<f
 
 using namespace mlf;
 
-struct mlf_context_s {
-  int device = 0;
-  hipStream_t stream = nullptr;
-  std::mutex mu;
-  int refs = 0;         // live PSFs created on this context
-  bool closed = false;  // destroy requested; freed when refs reaches 0
-};
+// Handle lifetimes: a PSF keeps its context alive and a patch (or batch)
+// keeps its PSF alive, so destroying a parent first only defers its release.
+std::mutex mlf::g_life_mu;
 
-// Handle lifetimes: a PSF keeps its context alive and a patch keeps its PSF
-// alive, so destroying a parent first only defers its release.
-static std::mutex g_life_mu;
-
-static void ctx_free(mlf_context ctx) {
+void mlf::ctx_free(mlf_context ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) {
     (void)hipStreamSynchronize(ctx->stream);
@@ -204,7 +197,7 @@ static void ctx_free(mlf_context ctx) {
   delete ctx;
 }
 
-static mlf_status ctx_activate(mlf_context ctx) {
+mlf_status mlf::ctx_activate(mlf_context ctx) {
   MLF_HIP(hipSetDevice(ctx->device));
   return MLF_OK;
 }
@@ -278,50 +271,9 @@ mlf_status mlf_context_memory(mlf_context ctx, uint64_t* free_bytes, uint64_t* t
 }  // extern "C"
 
 // =================================================================== PSF
-// PSF tables re-laid out for the MFMA kernels of one precision.
-struct PrecPack {
-  bool built = false;
-  bool tpz = false;                   // forward tables in the Toeplitz layout [z][e1][e2pad][PHI]
-  DevBuf<uint8_t> fwd, bwd;           // [V][z][e_pad64][PHI], [V][z][240][e_pad32]
-  int64_t pvf = 0, pvb = 0;           // elements per view
-  std::vector<int64_t> woff, boff;    // per-plane element offsets inside a view
-  std::vector<int32_t> nchunk, epad;
-  DevBuf<int64_t> d_woff, d_boff;
-  DevBuf<int32_t> d_nchunk, d_epad;
-};
-
-// Backprojection-of-ones table for one grid-size class (FP64 sums of the taps
-// as the kernels see them, rounded once to FP32).
-struct NormTable {
-  int key1 = -1, key2 = -1;  // -1: grid at least 2*rb_max+1 lenslets on that axis
-  DevBuf<float> T;
-  int64_t pvt = 0;
-  std::vector<int64_t> toff;
-  std::vector<int32_t> C1, C2;
-  DevBuf<int64_t> d_toff;
-  DevBuf<int32_t> d_C1;
-};
-
-struct mlf_psf_s {
-  mlf_context ctx = nullptr;
-  int Z = 0, V = 0;
-  std::vector<int32_t> uv;       // [V][2]
-  std::vector<double> weight;    // [V]
-  std::vector<int32_t> rf, rb;   // [Z]
-  std::vector<int64_t> offf, offb;
-  int64_t per_view_f = 0, per_view_b = 0;
-  DevBuf<float> fwd, bwd;        // ABI layout, FP32
-  DevBuf<int32_t> d_rf, d_rb;
-  DevBuf<int64_t> d_offf, d_offb;
-  PrecPack pack[2];
-  std::vector<std::unique_ptr<NormTable>> norms[2];
-  std::mutex mu;
-  int refs = 0;         // live patches
-  bool closed = false;
-};
 
 // Called with g_life_mu held.
-static void psf_free(mlf_psf psf) {
+void mlf::psf_free(mlf_psf psf) {
   mlf_context ctx = psf->ctx;
   (void)hipSetDevice(ctx->device);
   delete psf;
@@ -330,12 +282,6 @@ static void psf_free(mlf_psf psf) {
 
 static int64_t plane_block(int r) { return (int64_t)(2 * r + 1) * (2 * r + 1) * kP; }
 
-template <typename T>
-static mlf_status upload_vec(DevBuf<T>& d, const std::vector<T>& h) {
-  MLF_CHECK(d.alloc(h.size()));
-  if (!h.empty()) MLF_HIP(hipMemcpy(d.ptr, h.data(), d.bytes(), hipMemcpyHostToDevice));
-  return MLF_OK;
-}
 
 // Build the packed forward/backward tables of one precision (once per PSF).
 static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
@@ -723,45 +669,6 @@ mlf_status mlf_psf_destroy(mlf_psf psf) {
 }  // extern "C"
 
 // ================================================================= patch
-struct mlf_patch_s {
-  mlf_psf psf = nullptr;
-  int n1 = 0, n2 = 0, Z = 0;
-  mlf_precision prec = MLF_PRECISION_FP32;
-  DevBuf<float> X, LF, part, R, y, tmp;
-  DevBuf<double> pb, pa, dsum;
-  // filter plan
-  int f_nplanes = 0, nblk = 0;
-  DevBuf<int32_t> f_plane, f_n0_1, f_len_1, f_n0_2, f_len_2;
-  DevBuf<int64_t> f_off_1, f_off_2;
-  DevBuf<float> f_taps_1, f_taps_2;
-  bool image_set = false, volume_set = false;
-  // MFMA path
-  bool reference = false;              // FP32 reference kernels instead of MFMA
-  Precision kp = Precision::kF32;
-  int bm2 = 8;
-  bool tpz = false;                    // Toeplitz forward kernel
-  int cw = 16;                         // Toeplitz: input columns per workgroup
-  DevBuf<float> partial;               // forward window partials
-  DevBuf<uint16_t> Xb;                 // BF16 shadow of X, [lenslet][kShadowRow] (BF16 Toeplitz)
-  DevBuf<int64_t> d_poff;
-  DevBuf<int32_t> d_fsched, d_bsched;
-  int nblk_f = 0, nblk_b = 0;
-  size_t smem_f = 0, smem_b = 0;
-  NormTable* norm = nullptr;
-  hipEvent_t ev0 = nullptr, ev1 = nullptr;
-  bool timed = false;
-  // per-kernel timing
-  bool timing = false;
-  std::vector<hipEvent_t> ev_pool;
-  std::vector<int> ev_class;  // class of the pair (2k, 2k+1)
-  size_t ev_used = 0;
-  std::mutex mu;
-  ~mlf_patch_s() {
-    if (ev0) (void)hipEventDestroy(ev0);
-    if (ev1) (void)hipEventDestroy(ev1);
-    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
-  }
-};
 
 // Bracket a launch with a timing event pair (no-op unless timing is on).
 static mlf_status tim_begin(mlf_patch_s* p, int cls) {
@@ -855,25 +762,19 @@ static FilterPlan filter_plan_of(mlf_patch_s* p) {
   return fp;
 }
 
-#define MLF_LAUNCH(call)                                                                      \
-  do {                                                                                        \
-    hipError_t e_ = (call);                                                                   \
-    if (e_ != hipSuccess)                                                                     \
-      return set_error(MLF_ERR_DEVICE, "launch %s failed: %s", #call, hipGetErrorString(e_)); \
-  } while (0)
 
 static FwdGeo fwd_geo(mlf_patch_s* p) {
   mlf_psf_s* ps = p->psf;
   const PrecPack& pk = ps->pack[(int)p->kp];
-  return FwdGeo{p->n1, p->n2, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, p->d_poff.ptr,
-                p->d_fsched.ptr};
+  return FwdGeo{p->d_slot.ptr, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, p->d_fsched.ptr,
+                p->d_counts.ptr + 0};
 }
 
-static BwdGeo bwd_geo(mlf_patch_s* p) {
+static BwdGeo bwd_geo(mlf_patch_s* p, int k) {
   mlf_psf_s* ps = p->psf;
   const PrecPack& pk = ps->pack[(int)p->kp];
-  return BwdGeo{p->n1, p->n2, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, p->norm->d_toff.ptr,
-                p->norm->d_C1.ptr, p->d_bsched.ptr};
+  return BwdGeo{p->d_slot.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, p->d_bsched.ptr,
+                p->d_counts.ptr + 1, k};
 }
 
 // Forward projection of view k, then the ratio (LF != null) and/or y.
@@ -895,19 +796,20 @@ static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, fl
   const PrecPack& pk = ps->pack[(int)p->kp];
   const FwdGeo g = fwd_geo(p);
   const void* Wk = pk.fwd.ptr + (size_t)k * pk.pvf * elem_bytes(p->kp);
+  const RatioGeo rg{p->d_rsched.ptr, p->d_counts.ptr + 2};
+  const int use_lf = LF != nullptr;
   MLF_CHECK(tim_begin(p, 0));
   if (p->tpz) {
-    MLF_LAUNCH(launch_fwd_tpz(p->kp, p->X.ptr, p->Xb.ptr, Wk, g, p->nblk_f, p->smem_f,
-                              p->partial.ptr, st));
+    MLF_LAUNCH(launch_fwd_tpz(p->kp, Wk, g, p->nblk_f, p->smem_f, st));
   } else {
-    MLF_LAUNCH(launch_fwd_mfma(p->kp, p->X.ptr, Wk, g, p->nblk_f, p->smem_f, p->partial.ptr, st));
+    MLF_LAUNCH(launch_fwd_mfma(p->kp, Wk, g, p->nblk_f, p->smem_f, st));
   }
   MLF_CHECK(tim_end(p));
   MLF_CHECK(tim_begin(p, 1));
   if (p->tpz) {
-    MLF_LAUNCH(launch_ratio_gather_tpz(p->partial.ptr, g, p->Z, p->cw, LF, u, v, p->R.ptr, y_out, st));
+    MLF_LAUNCH(launch_ratio_gather_tpz(g, rg, p->nblk_r, p->Z, p->cw, u, v, use_lf, y_out, st));
   } else {
-    MLF_LAUNCH(launch_ratio_gather(p->partial.ptr, g, p->Z, p->bm2, LF, u, v, p->R.ptr, y_out, st));
+    MLF_LAUNCH(launch_ratio_gather(g, rg, p->nblk_r, p->Z, p->bm2, u, v, use_lf, y_out, st));
   }
   MLF_CHECK(tim_end(p));
   return MLF_OK;
@@ -928,13 +830,34 @@ static mlf_status patch_view_step(mlf_patch_s* p, int k) {
                                            p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
   } else {
     const PrecPack& pk = ps->pack[(int)p->kp];
-    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->X.ptr, p->Xb.ptr,
-                               pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
-                               p->norm->T.ptr + (size_t)k * p->norm->pvt, bwd_geo(p), p->nblk_b,
-                               p->smem_b, p->R.ptr, (float)ps->weight[k], st));
+    MLF_LAUNCH(launch_bwd_mfma(p->kp, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
+                               bwd_geo(p, k), p->nblk_b, p->smem_b, (float)ps->weight[k], st));
   }
   MLF_CHECK(tim_end(p));
   return MLF_OK;
+}
+
+// Forward partial-window elements of each plane for an n1 x n2 grid.
+static int64_t partial_elems(const mlf_psf_s* ps, bool tpz, int cw, int bm2, int n1, int n2,
+                             std::vector<int64_t>* poff) {
+  const int colw = tpz ? cw : bm2;  // lenslets per block along axis 1
+  const int nb1 = (n1 + 15) / 16, nb2 = (n2 + colw - 1) / colw;
+  int64_t po = 0;
+  for (int z = 0; z < ps->Z; ++z) {
+    const int r = ps->rf[z];
+    if (poff) (*poff)[z] = po;
+    po += tpz ? (int64_t)nb1 * nb2 * 16 * (cw + 2 * r) : (int64_t)nb1 * nb2 * (16 + 2 * r) * (bm2 + 2 * r);
+  }
+  return po;
+}
+
+// Prefix each `width`-int entry of `src` with a slot index.
+static void append_entries(const std::vector<int32_t>& src, int width, int slot,
+                           std::vector<int32_t>* dst) {
+  for (size_t i = 0; i + width <= src.size(); i += width) {
+    dst->push_back(slot);
+    dst->insert(dst->end(), src.begin() + i, src.begin() + i + width);
+  }
 }
 
 // Schedules (largest planes first), partial buffers, packed tables.
@@ -958,48 +881,68 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
   for (int z = 0; z < Z; ++z) order[z] = z;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return ps->rf[a] > ps->rf[b]; });
   std::vector<int64_t> poff(Z);
-  int64_t po = 0;
+  const int64_t po = partial_elems(ps, p->tpz, p->cw, p->bm2, p->n1, p->n2, &poff);
   int rfmax = 0, rbmax = 0, epmax = 0;
   for (int z = 0; z < Z; ++z) {
-    const int r = ps->rf[z];
-    rfmax = std::max(rfmax, r);
+    rfmax = std::max(rfmax, ps->rf[z]);
     rbmax = std::max(rbmax, ps->rb[z]);
     epmax = std::max(epmax, ps->pack[(int)p->kp].epad[z]);
-    poff[z] = po;
-    po += p->tpz ? (int64_t)nb1 * nb2 * 16 * (p->cw + 2 * r)
-                 : (int64_t)nb1 * nb2 * (16 + 2 * r) * (p->bm2 + 2 * r);
   }
-  std::vector<int32_t> fs;
+  p->fent.clear();
   for (int z : order)
     for (int b2 = 0; b2 < nb2; ++b2)
       for (int b1 = 0; b1 < nb1; ++b1) {
-        fs.push_back(z);
-        fs.push_back(b1);
-        fs.push_back(b2);
+        p->fent.push_back(z);
+        p->fent.push_back(b1);
+        p->fent.push_back(b2);
       }
-  p->nblk_f = (int)(fs.size() / 3);
+  p->nblk_f = (int)(p->fent.size() / 3);
   std::vector<int> border(Z);
   for (int z = 0; z < Z; ++z) border[z] = z;
   std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return ps->rb[a] > ps->rb[b]; });
   const int mb1 = (p->n1 + 15) / 16, mb2 = (p->n2 + 15) / 16;
   const int ngrp = kPhiPadBwd / (16 * kBwdNTiles);
-  std::vector<int32_t> bs;
+  p->bent.clear();
   for (int z : border)
     for (int b2 = 0; b2 < mb2; ++b2)
       for (int b1 = 0; b1 < mb1; ++b1)
         for (int gq = 0; gq < ngrp; ++gq) {
-          bs.push_back(z);
-          bs.push_back(b1);
-          bs.push_back(b2);
-          bs.push_back(gq);
+          p->bent.push_back(z);
+          p->bent.push_back(b1);
+          p->bent.push_back(b2);
+          p->bent.push_back(gq);
         }
-  p->nblk_b = (int)(bs.size() / 4);
+  p->nblk_b = (int)(p->bent.size() / 4);
+  p->nblk_r = (p->n1 * p->n2 + 255) / 256;
+  std::vector<int32_t> fs, bs, rs;
+  append_entries(p->fent, 3, 0, &fs);
+  append_entries(p->bent, 4, 0, &bs);
+  for (int i = 0; i < p->nblk_r; ++i) {
+    rs.push_back(0);
+    rs.push_back(256 * i);
+  }
   MLF_CHECK(upload_vec(p->d_poff, poff));
   MLF_CHECK(upload_vec(p->d_fsched, fs));
   MLF_CHECK(upload_vec(p->d_bsched, bs));
+  MLF_CHECK(upload_vec(p->d_rsched, rs));
+  MLF_CHECK(upload_vec(p->d_counts, std::vector<int32_t>{p->nblk_f, p->nblk_b, p->nblk_r}));
   MLF_CHECK(p->partial.alloc((size_t)po));
   if (p->tpz && p->kp == Precision::kBF16)
     MLF_CHECK(p->Xb.alloc((size_t)p->n1 * p->n2 * Z * kShadowRow));
+  SlotDesc& sd = p->slot;
+  sd.X = p->X.ptr;
+  sd.Xb = reinterpret_cast<__bf16*>(p->Xb.ptr);
+  sd.LF = p->LF.ptr;
+  sd.R = p->R.ptr;
+  sd.partial = p->partial.ptr;
+  sd.poff = p->d_poff.ptr;
+  sd.Tn = p->norm->T.ptr;
+  sd.pvt = p->norm->pvt;
+  sd.toffT = p->norm->d_toff.ptr;
+  sd.C1 = p->norm->d_C1.ptr;
+  sd.n1 = p->n1;
+  sd.n2 = p->n2;
+  MLF_CHECK(upload_vec(p->d_slot, std::vector<SlotDesc>{sd}));
   p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
   p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax);
   if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
@@ -1009,14 +952,14 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
 }
 
 // Re-derive the BF16 shadow after X changed outside the backprojection.
-static mlf_status patch_refresh_shadow(mlf_patch_s* p) {
+mlf_status mlf::patch_refresh_shadow(mlf_patch_s* p) {
   if (!p->Xb.ptr) return MLF_OK;
   MLF_LAUNCH(launch_make_shadow(p->X.ptr, p->Xb.ptr, (size_t)p->n1 * p->n2 * p->Z,
                                 p->psf->ctx->stream));
   return MLF_OK;
 }
 
-static mlf_status patch_filter(mlf_patch_s* p) {
+mlf_status mlf::patch_filter(mlf_patch_s* p) {
   if (p->f_nplanes == 0) return MLF_OK;
   hipStream_t st = p->psf->ctx->stream;
   const FilterPlan fp = filter_plan_of(p);
@@ -1027,6 +970,29 @@ static mlf_status patch_filter(mlf_patch_s* p) {
   MLF_CHECK(patch_refresh_shadow(p));
   MLF_CHECK(tim_end(p));
   return MLF_OK;
+}
+
+size_t mlf::patch_device_bytes(const mlf_patch_s* p) {
+  size_t b = p->X.bytes() + p->LF.bytes() + p->part.bytes() + p->R.bytes() + p->y.bytes() +
+             p->tmp.bytes() + p->pb.bytes() + p->pa.bytes() + p->partial.bytes() + p->Xb.bytes();
+  b += p->d_fsched.bytes() + p->d_bsched.bytes() + p->d_rsched.bytes() + p->f_taps_1.bytes() +
+       p->f_taps_2.bytes();
+  return b;
+}
+
+size_t mlf::patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int n2) {
+  const Precision kp = prec == MLF_PRECISION_BF16 ? Precision::kBF16 : Precision::kF32;
+  int rfmax = 0;
+  for (int z = 0; z < ps->Z; ++z) rfmax = std::max(rfmax, ps->rf[z]);
+  const bool tpz = rfmax <= tpz_max_radius();
+  const int cw = 4 * (kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16);
+  const size_t n = (size_t)n1 * n2, Z = (size_t)ps->Z;
+  size_t b = 4 * n * Z * kP * 2;  // volume + sample_filter temporary
+  b += 4 * n * kP + 4 * n * Z + 8 * n;
+  b += 4 * (size_t)partial_elems(ps, tpz, cw, fwd_bm2(kp), n1, n2, nullptr);
+  if (kp == Precision::kBF16 && tpz) b += 2 * n * Z * kShadowRow;
+  // entry lists and filter taps: < 1 MB
+  return b + (1u << 20);
 }
 
 extern "C" {
@@ -1091,6 +1057,7 @@ mlf_status mlf_patch_destroy(mlf_patch p) {
   std::lock_guard<std::mutex> lk(g_life_mu);
   mlf_psf psf = p->psf;
   (void)hipSetDevice(psf->ctx->device);
+  (void)batch_forget_patch(p);
   (void)hipStreamSynchronize(psf->ctx->stream);
   delete p;
   if (--psf->refs == 0 && psf->closed) psf_free(psf);
@@ -1305,12 +1272,37 @@ mlf_status mlf_recon_phase_space(const float* lfimg, int64_t I1, int64_t I2, con
     std::memset(stats, 0, sizeof(*stats));
     stats->views_per_iteration = psf->V;
   }
+  // relative change per iteration, FP64 on the device (stats only)
+  constexpr int kNormBlocks = 1024;
+  DevBuf<float> prev;
+  DevBuf<double> parts;
+  if (stats) {
+    MLF_CHECK(prev.alloc(p->X.count));
+    MLF_CHECK(parts.alloc(2 * kNormBlocks));
+  }
+  std::vector<double> hparts(2 * kNormBlocks);
+  hipStream_t st = psf->ctx->stream;
   for (int it = 0; it < max_iter; ++it) {
+    if (stats)
+      MLF_HIP(hipMemcpyAsync(prev.ptr, p->X.ptr, p->X.bytes(), hipMemcpyDeviceToDevice, st));
     MLF_CHECK(mlf_patch_run(p, 1, 0, 1));
     double ms = 0.0;
     MLF_CHECK(mlf_patch_last_run_ms(p, &ms));
-    if (stats && it < 64) stats->iteration_seconds[it] = ms * 1e-3;
-    if (stats) stats->iterations = it + 1;
+    if (stats) {
+      MLF_LAUNCH(launch_diff_norms(p->X.ptr, prev.ptr, p->X.count, parts.ptr, kNormBlocks, st));
+      MLF_HIP(hipMemcpyAsync(hparts.data(), parts.ptr, parts.bytes(), hipMemcpyDeviceToHost, st));
+      MLF_HIP(hipStreamSynchronize(st));
+      double dd = 0.0, bb = 0.0;
+      for (int k = 0; k < kNormBlocks; ++k) {
+        dd += hparts[2 * k];
+        bb += hparts[2 * k + 1];
+      }
+      if (it < 64) {
+        stats->iteration_seconds[it] = ms * 1e-3;
+        stats->relative_change[it] = bb > 0.0 ? std::sqrt(dd / bb) : 0.0;
+      }
+      stats->iterations = it + 1;
+    }
   }
   MLF_CHECK(mlf_patch_get_volume_f64(p, X_out));
   if (stats)

@@ -99,6 +99,20 @@ _SIGNATURES = {
     "mlf_patch_last_run_ms": (C.c_int, [_P, _F64P]),
     "mlf_patch_set_timing": (C.c_int, [_P, _I32]),
     "mlf_patch_kernel_times": (C.c_int, [_P, _F64P, _I64P]),
+    "mlf_batch_create": (C.c_int, [_P, C.c_int, _I32, C.POINTER(_P)]),
+    "mlf_batch_destroy": (C.c_int, [_P]),
+    "mlf_batch_attach": (C.c_int, [_P, _P]),
+    "mlf_batch_detach": (C.c_int, [_P, _P]),
+    "mlf_batch_size": (C.c_int, [_P, _I32P]),
+    "mlf_batch_set_graph": (C.c_int, [_P, _I32]),
+    "mlf_batch_run": (C.c_int, [_P, _I32, _I32]),
+    "mlf_batch_last_run_ms": (C.c_int, [_P, _F64P]),
+    "mlf_server_create": (C.c_int, [_P, C.c_int, _P, C.POINTER(_P)]),
+    "mlf_server_submit": (C.c_int, [_P, _I32, _I64, _I64, _F32P, _F32P, _U64P]),
+    "mlf_server_wait": (C.c_int, [_P, C.c_uint64, _F64P]),
+    "mlf_server_drain": (C.c_int, [_P]),
+    "mlf_server_get_stats": (C.c_int, [_P, _P]),
+    "mlf_server_destroy": (C.c_int, [_P]),
     "mlf_recon_phase_space": (C.c_int, [_F32P, _I64, _I64, _F32P, _F32P, _I64, _I64, _I64, _I32,
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
@@ -347,6 +361,127 @@ class Patch:
 
 
 # -------------------------------------------------------------------- drop-ins
+class Batch:
+    """Several patches of one PSF run in shared launches (``mlf_batch_*``).
+
+    A patch's result does not depend on the batch it ran in."""
+
+    def __init__(self, psf: PSF, capacity: int, precision: str = "bf16", graph: bool = True):
+        self.psf = psf
+        self._lib = psf._lib
+        h = _P()
+        _check(self._lib.mlf_batch_create(psf.handle, PRECISIONS[precision], int(capacity), C.byref(h)))
+        self.handle = h
+        self.set_graph(graph)
+
+    def set_graph(self, enable: bool):
+        _check(self._lib.mlf_batch_set_graph(self.handle, int(bool(enable))))
+
+    def attach(self, patch: Patch):
+        _check(self._lib.mlf_batch_attach(self.handle, patch.handle))
+
+    def detach(self, patch: Patch):
+        _check(self._lib.mlf_batch_detach(self.handle, patch.handle))
+
+    def __len__(self):
+        n = C.c_int32()
+        _check(self._lib.mlf_batch_size(self.handle, C.byref(n)))
+        return n.value
+
+    def run(self, iterations: int, filter: bool = True):
+        _check(self._lib.mlf_batch_run(self.handle, int(iterations), int(bool(filter))))
+
+    def last_run_ms(self):
+        ms = C.c_double()
+        _check(self._lib.mlf_batch_last_run_ms(self.handle, C.byref(ms)))
+        return ms.value
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.mlf_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("max_slots", C.c_int32), ("iterations", C.c_int32), ("filter", C.c_int32),
+                ("use_graph", C.c_int32), ("hbm_budget_bytes", C.c_uint64)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("pending", C.c_int64),
+                ("in_flight", C.c_int64), ("resident_patches", C.c_int64),
+                ("evictions", C.c_int64), ("cold_starts", C.c_int64),
+                ("iterations_run", C.c_int64), ("resident_bytes", C.c_uint64),
+                ("peak_resident_bytes", C.c_uint64), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double),
+                ("volumes_per_s", C.c_double), ("mean_slots", C.c_double)]
+
+
+class Server:
+    """Continuous-batching request loop (``mlf_server_*``).
+
+    ``submit`` queues one frame of a patch and returns a ticket; results are
+    written into the caller's array (if given) when the request completes."""
+
+    def __init__(self, psf: PSF, precision: str = "bf16", max_slots: int = 8, iterations: int = 3,
+                 hbm_budget_bytes: int = 64 << 30, filter: bool = True, graph: bool = True):
+        self.psf = psf
+        self._lib = psf._lib
+        cfg = _ServerConfig(int(max_slots), int(iterations), int(bool(filter)), int(bool(graph)),
+                            int(hbm_budget_bytes))
+        h = _P()
+        _check(self._lib.mlf_server_create(psf.handle, PRECISIONS[precision], C.byref(cfg), C.byref(h)))
+        self.handle = h
+        self._outs = {}  # ticket -> output array kept alive until completion
+
+    def submit(self, patch_id: int, lfimg, want_volume: bool = False):
+        a = _col(lfimg, np.float32)
+        if a.ndim != 2 or a.shape[0] % 15 or a.shape[1] % 15:
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "image must be [15*n1, 15*n2]")
+        n1, n2 = a.shape[0] // 15, a.shape[1] // 15
+        out = np.empty((a.shape[0], a.shape[1], self.psf.Z), np.float32, order="F") if want_volume else None
+        t = C.c_uint64()
+        _check(self._lib.mlf_server_submit(self.handle, int(patch_id), n1, n2, _fptr(a),
+                                           None if out is None else _fptr(out), C.byref(t)))
+        self._outs[t.value] = out
+        return t.value
+
+    def wait(self, ticket: int):
+        """(latency_ms, volume or None)."""
+        ms = C.c_double()
+        try:
+            _check(self._lib.mlf_server_wait(self.handle, C.c_uint64(ticket), C.byref(ms)))
+        finally:
+            out = self._outs.pop(ticket, None)
+        return ms.value, out
+
+    def drain(self):
+        _check(self._lib.mlf_server_drain(self.handle))
+
+    def stats(self):
+        st = _ServerStats()
+        _check(self._lib.mlf_server_get_stats(self.handle, C.byref(st)))
+        return {name: getattr(st, name) for name, _ in _ServerStats._fields_}
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.mlf_server_destroy(self.handle)
+            self.handle = None
+            self._outs.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def recon_theory_phase_space_LFM_with_filter(LFIMG, H, Ht, maxIter, gpuFlag=True, gpu_id=1,
                                              precision="fp32", stats=None):
     """Drop-in for recon_theory_phase_space_LFM_with_filter.m:10.
@@ -374,6 +509,7 @@ def recon_theory_phase_space_LFM_with_filter(LFIMG, H, Ht, maxIter, gpuFlag=True
         stats["views_per_iteration"] = int(vpi)
         stats["total_seconds"] = float(np.frombuffer(raw[8:16], np.float64)[0])
         stats["iteration_seconds"] = np.frombuffer(raw[16:16 + 8 * 64], np.float64)[:it].tolist()
+        stats["relative_change"] = np.frombuffer(raw[16 + 8 * 64:16 + 16 * 64], np.float64)[:it].tolist()
     return out
 
 

@@ -1,0 +1,211 @@
+"""Batches and the serving loop on the GPU.
+
+A patch's work entries are the same whether it runs alone or in a batch, so
+the batched results must equal the single-patch results bit for bit; one
+batched patch is also checked against the FP64 oracle.  Graph replays are
+compared with eager launches iteration by iteration, with host work and
+synchronisation between replays.
+"""
+import numpy as np
+import pytest
+
+import mesolf_amd
+from mesolf_amd import synth, tables
+from oracle import mesolf_oracle as O
+from tests.helpers import oracle_compact_from_abi, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+WL = synth.SMALL                       # 5 planes, radii 2..3
+GRIDS = [(12, 12), (9, 14), (20, 7)]   # uneven patches sharing one PSF
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = mesolf_amd.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def tabs():
+    return tables.synthetic_compact(WL)
+
+
+@pytest.fixture(scope="module")
+def psf(ctx, tabs):
+    p = mesolf_amd.PSF.from_compact(ctx, tabs["uv"], tabs["weights"], tabs["rf"], tabs["fwd"])
+    yield p
+    p.close()
+
+
+def _image(n1, n2, seed):
+    rng = np.random.default_rng(seed)
+    return rng.poisson(40.0, (15 * n1, 15 * n2)).astype(np.float32) + 1.0
+
+
+def _alone(psf, grid, img, iters, prec, warm=None):
+    p = mesolf_amd.Patch(psf, *grid, precision=prec)
+    p.set_image(img)
+    if warm is None:
+        p.init_volume()
+    else:
+        p.set_volume(warm)
+    p.run(iters, filter=True)
+    out = p.get_volume()
+    p.close()
+    return out
+
+
+@pytest.mark.parametrize("graph", [True, False], ids=["graph", "eager"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+def test_batch_equals_alone_bitwise(psf, prec, graph):
+    imgs = [_image(*g, seed=10 + i) for i, g in enumerate(GRIDS)]
+    ref = [_alone(psf, g, im, 2, prec) for g, im in zip(GRIDS, imgs)]
+    b = mesolf_amd.Batch(psf, capacity=4, precision=prec, graph=graph)
+    patches = []
+    for g, im in zip(GRIDS, imgs):
+        p = mesolf_amd.Patch(psf, *g, precision=prec)
+        p.set_image(im)
+        p.init_volume()
+        b.attach(p)
+        patches.append(p)
+    assert len(b) == 3
+    b.run(2, filter=True)
+    for p, r in zip(patches, ref):
+        got = p.get_volume()
+        assert np.array_equal(got, r), (prec, graph, p.n1, p.n2, float(np.abs(got - r).max()))
+    b.close()
+    for p in patches:
+        p.close()
+
+
+def test_batched_patch_matches_oracle(psf, tabs):
+    cp = oracle_compact_from_abi(WL, tabs)
+    grid = GRIDS[2]
+    img = _image(*grid, seed=5)
+    b = mesolf_amd.Batch(psf, capacity=3, precision="fp32")
+    ps = []
+    for i, g in enumerate(GRIDS):
+        p = mesolf_amd.Patch(psf, *g, precision="fp32")
+        p.set_image(img if g == grid else _image(*g, seed=50 + i))
+        p.init_volume()
+        b.attach(p)
+        ps.append(p)
+    b.run(2, filter=True)
+    got = ps[2].get_volume()
+    ref = O.rl_compact(img.astype(np.float64), cp, 2)
+    err = rel_l2(got, ref)
+    print(f"batched 20x7 patch, 2 iterations: rel L2 vs FP64 oracle = {err:.3e}")
+    assert err <= 1e-4
+    b.close()
+    for p in ps:
+        p.close()
+
+
+def test_graph_replays_match_eager_each_iteration(psf):
+    imgs = [_image(*g, seed=30 + i) for i, g in enumerate(GRIDS)]
+    sets = []
+    for graph in (True, False):
+        b = mesolf_amd.Batch(psf, capacity=3, precision="bf16", graph=graph)
+        ps = []
+        for g, im in zip(GRIDS, imgs):
+            p = mesolf_amd.Patch(psf, *g, precision="bf16")
+            p.set_image(im)
+            p.init_volume()
+            b.attach(p)
+            ps.append(p)
+        sets.append((b, ps))
+    for it in range(4):
+        vols = []
+        for b, ps in sets:
+            b.run(1, filter=True)
+            psf.ctx.synchronize()
+            vols.append([p.get_volume() for p in ps])  # eager host work between replays
+        for a, e in zip(*vols):
+            assert np.array_equal(a, e), f"replay {it} differs from eager launches"
+    for b, ps in sets:
+        b.close()
+        for p in ps:
+            p.close()
+
+
+def test_attach_detach_midway(psf):
+    """Patches join and leave between iterations; each result equals its own
+    single-patch history."""
+    prec = "bf16"
+    imgs = [_image(*g, seed=70 + i) for i, g in enumerate(GRIDS)]
+    # alone: patch 0 runs 3 iterations, patch 1 runs 1, patch 2 runs 2
+    ref = [_alone(psf, GRIDS[0], imgs[0], 3, prec), _alone(psf, GRIDS[1], imgs[1], 1, prec),
+           _alone(psf, GRIDS[2], imgs[2], 2, prec)]
+    b = mesolf_amd.Batch(psf, capacity=2, precision=prec)
+    ps = []
+    for g, im in zip(GRIDS, imgs):
+        p = mesolf_amd.Patch(psf, *g, precision=prec)
+        p.set_image(im)
+        p.init_volume()
+        ps.append(p)
+    b.attach(ps[0])
+    b.attach(ps[1])
+    with pytest.raises(mesolf_amd.BudgetExceeded):
+        b.attach(ps[2])  # capacity 2
+    b.run(1)
+    b.detach(ps[1])
+    b.attach(ps[2])
+    b.run(2)
+    for p, r in zip(ps, ref):
+        assert np.array_equal(p.get_volume(), r)
+    ps[0].close()  # destroying an attached patch detaches it
+    assert len(b) == 1
+    b.close()
+    ps[1].close()
+    ps[2].close()
+
+
+def test_server_results_equal_alone_and_warm_start(psf):
+    """Two frames for each of three patches, submitted together: each result
+    equals the single-patch computation (cold start for the first frame,
+    warm start from that result for the second)."""
+    prec = "bf16"
+    frames = {i: [_image(*g, seed=100 + 10 * i + f) for f in range(2)] for i, g in enumerate(GRIDS)}
+    ref = {}
+    for i, g in enumerate(GRIDS):
+        v1 = _alone(psf, g, frames[i][0], 3, prec)
+        v2 = _alone(psf, g, frames[i][1], 3, prec, warm=v1)
+        ref[i] = (v1, v2)
+    srv = mesolf_amd.Server(psf, precision=prec, max_slots=2, iterations=3)
+    tickets = []
+    for f in range(2):
+        for i in range(len(GRIDS)):
+            tickets.append((i, f, srv.submit(i, frames[i][f], want_volume=True)))
+    for i, f, t in tickets:
+        lat, vol = srv.wait(t)
+        assert lat > 0
+        assert np.array_equal(vol, ref[i][f]), (i, f)
+    st = srv.stats()
+    assert st["completed"] == 6 and st["cold_starts"] == 3 and st["evictions"] == 0
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    assert 1.0 <= st["mean_slots"] <= 2.0
+    srv.close()
+
+
+def test_server_budget_evicts_idle_patches(psf):
+    """A budget that holds one patch: every new patch evicts the idle one, so
+    every request starts cold and still matches the cold single-patch run."""
+    prec = "fp32"
+    g = (40, 40)
+    imgs = [_image(*g, seed=200 + k) for k in range(3)]
+    probe = mesolf_amd.Server(psf, precision=prec, max_slots=1, iterations=1)
+    probe.wait(probe.submit(0, imgs[0]))
+    one = probe.stats()["resident_bytes"]  # device bytes of one 40x40 patch
+    probe.close()
+    budget = int(1.6 * one)
+    srv = mesolf_amd.Server(psf, precision=prec, max_slots=2, iterations=2, hbm_budget_bytes=budget)
+    ts = [srv.submit(k, imgs[k], want_volume=True) for k in range(3)]
+    for k, t in enumerate(ts):
+        _, vol = srv.wait(t)
+        assert np.array_equal(vol, _alone(psf, g, imgs[k], 2, prec))
+    st = srv.stats()
+    assert st["cold_starts"] == 3 and st["evictions"] == 2 and st["resident_patches"] == 1
+    assert st["peak_resident_bytes"] <= budget
+    srv.close()
