@@ -86,13 +86,218 @@ def cpu_baseline(wl, tabs, LF, nviews_sample: int):
     }
 
 
+def _movie_patches(psf, grids, frames, rng, groups=4):
+    """Movie on the largest grid, crop offsets per patch."""
+    from mesolf_amd import serving
+
+    n1m = max(g[0] for g in grids)
+    n2m = max(g[1] for g in grids)
+    t = time.perf_counter()
+    movie = serving.make_patch(psf, n1m, n2m, frames, rng, groups=groups)
+    offs = [(int(rng.integers(0, n1m - g[0] + 1)), int(rng.integers(0, n2m - g[1] + 1))) for g in grids]
+    log(f"movie {n1m}x{n2m}, {groups} activity groups, {frames} frames in {time.perf_counter() - t:.1f}s")
+    return movie, offs
+
+
+def _pct(xs, q):
+    xs = sorted(xs)
+    return xs[min(len(xs) - 1, max(0, int(np.ceil(q * len(xs))) - 1))] if xs else None
+
+
+def bench_batched(args):
+    """12 uneven patches x consecutive frames, 3 warm-start RL iterations per
+    frame, all patches of a frame in one batch (shared launches, one graph
+    replay per iteration)."""
+    import mesolf_amd
+    from mesolf_amd import serving, synth, tables
+
+    wl = synth.CFG2
+    rng = np.random.default_rng(4)
+    grids = serving.patch_grids(rng, args.patches, 61, 121)
+    tabs = tables.synthetic_compact(wl)
+    ctx = mesolf_amd.Context(0)
+    psf = mesolf_amd.PSF.from_compact(ctx, tabs["uv"], tabs["weights"], tabs["rf"], tabs["fwd"])
+    del tabs
+    movie, offs = _movie_patches(psf, grids, args.frames, rng)
+    nrng = np.random.default_rng(5)
+    timed = min(args.frames, args.timed_frames)
+    frames = [[serving.crop(movie.frame(f, nrng), g[0], g[1], *o) for g, o in zip(grids, offs)]
+              for f in range(timed + 1)]
+    log(f"{len(grids)} patches {grids}; {timed + 1} frames prepared")
+
+    b = mesolf_amd.Batch(psf, capacity=len(grids), precision=args.precision, graph=True)
+    ps = [mesolf_amd.Patch(psf, *g, precision=args.precision) for g in grids]
+    # frame 0: cold start (recon:72-73), kept for the FP32 comparison
+    for p, im in zip(ps, frames[0]):
+        p.set_image(im)
+        p.init_volume()
+        b.attach(p)
+    t0 = time.perf_counter()
+    b.run(args.rl_iters)
+    ctx.synchronize()
+    t_first = time.perf_counter() - t0
+    first = [p.get_volume() for p in ps]
+    lat = []
+    t0 = time.perf_counter()
+    for f in range(1, timed + 1):
+        tf = time.perf_counter()
+        for p, im in zip(ps, frames[f]):
+            p.set_image(im)  # warm start: the volume of the previous frame stays
+        b.run(args.rl_iters)
+        ctx.synchronize()
+        lat.append((time.perf_counter() - tf) * 1e3)
+        log(f"frame {f}/{timed}: {lat[-1]:.0f} ms")
+    elapsed = time.perf_counter() - t0
+    value = len(grids) * timed / elapsed
+    b.close()
+    for p in ps:
+        p.close()
+
+    # plain FP32 per-call path on frame 0 (one patch at a time), and the error
+    errs, t32 = [], 0.0
+    for g, im, ref in zip(grids, frames[0], first):
+        p = mesolf_amd.Patch(psf, *g, precision="fp32")
+        p.set_image(im)
+        p.init_volume()
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        p.run(args.rl_iters)
+        ctx.synchronize()
+        t32 += time.perf_counter() - t1
+        x = p.get_volume()
+        errs.append(float(np.linalg.norm((ref.astype(np.float64) - x).ravel()) /
+                          np.linalg.norm(x.astype(np.float64).ravel())))
+        p.close()
+    fp32_rate = len(grids) / t32
+    out = {
+        "metric": "deconvolved patch-frames/s, 12 uneven patches (61-121 lenslets/side) x 51 planes, "
+                  "3 warm-start RL iterations per frame, batched",
+        "value": value, "unit": "patch-frames/s", "n_gpus": 1, "steps": timed, "warmup": 1,
+        "ms_per_step": elapsed / timed * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic movie (calcium-like traces per sphere group, Poisson noise), "
+                "patches are crops of one 121x121 movie",
+        "config": {"workload": "cfg4_movie_12_patches", "patches": grids, "planes": 51,
+                   "frames_timed": timed, "rl_iterations_per_frame": args.rl_iters,
+                   "graph": True, "precision": args.precision},
+        "frame_latency_ms": {"p50": _pct(lat, 0.5), "p99": _pct(lat, 0.99), "max": max(lat)},
+        "first_frame_cold_ms": t_first * 1e3,
+        "fp32_per_call": {"patch_frames_per_s": fp32_rate, "speedup": value / fp32_rate,
+                          "rel_l2_frame0_vs_fp32": {"max": max(errs), "mean": float(np.mean(errs))}},
+    }
+    print(json.dumps(out), flush=True)
+    psf.close()
+    ctx.close()
+
+
+def bench_serving(args):
+    """Poisson arrivals of frame x patch requests over 36 patches (81-121
+    lenslets/side) into the continuous-batching server with an HBM budget."""
+    import mesolf_amd
+    from mesolf_amd import serving, synth, tables
+
+    wl = synth.CFG2
+    rng = np.random.default_rng(6)
+    grids = serving.patch_grids(rng, args.patches, 81, 121)
+    tabs = tables.synthetic_compact(wl)
+    ctx = mesolf_amd.Context(0)
+    psf = mesolf_amd.PSF.from_compact(ctx, tabs["uv"], tabs["weights"], tabs["rf"], tabs["fwd"])
+    del tabs
+    nfr = 4
+    movie, offs = _movie_patches(psf, grids, nfr, rng)
+    nrng = np.random.default_rng(7)
+    imgs = [[serving.crop(movie.frame(f, nrng), g[0], g[1], *o) for f in range(nfr)]
+            for g, o in zip(grids, offs)]
+    budget = int(args.hbm_budget_gb * (1 << 30))
+    srv = mesolf_amd.Server(psf, precision=args.precision, max_slots=args.slots,
+                            iterations=args.rl_iters, hbm_budget_bytes=budget)
+    # capacity: every patch queued at once, cold round then warm round
+    t0 = time.perf_counter()
+    ts = [srv.submit(i, imgs[i][0], want_volume=(i < 2)) for i in range(len(grids))]
+    served0 = {}
+    for i, t in enumerate(ts):
+        _, vol = srv.wait(t)
+        if vol is not None:
+            served0[i] = vol
+    t_cold = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for t in [srv.submit(i, imgs[i][1]) for i in range(len(grids))]:
+        srv.wait(t)
+    t_warm = time.perf_counter() - t0
+    capacity = len(grids) / t_warm
+    log(f"capacity: cold round {t_cold:.1f}s, warm round {t_warm:.1f}s -> {capacity:.2f} req/s")
+    # open-loop Poisson arrivals at load x capacity
+    lam = args.load * capacity
+    arr = np.cumsum(np.random.default_rng(8).exponential(1.0 / lam, int(lam * args.duration * 2) + 8))
+    arr = arr[arr < args.duration]
+    counters = [2] * len(grids)
+    pick = np.random.default_rng(9).integers(0, len(grids), len(arr))
+    tickets = []
+    t0 = time.perf_counter()
+    for a, i in zip(arr, pick):
+        dt = a - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        tickets.append(srv.submit(int(i), imgs[i][counters[i] % nfr]))
+        counters[i] += 1
+    lat = [srv.wait(t)[0] for t in tickets]
+    span = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    # error of two served (cold, first-frame) results against the FP32 per-call path
+    errs = []
+    for i, vol in served0.items():
+        p = mesolf_amd.Patch(psf, *grids[i], precision="fp32")
+        p.set_image(imgs[i][0])
+        p.init_volume()
+        p.run(args.rl_iters)
+        x = p.get_volume()
+        p.close()
+        errs.append(float(np.linalg.norm((vol.astype(np.float64) - x).ravel()) /
+                          np.linalg.norm(x.astype(np.float64).ravel())))
+    nominal = len(grids) * serving.FPS
+    out = {
+        "metric": "p99 latency of frame x patch deconvolution requests (Poisson arrivals), "
+                  "36 patches (81-121 lenslets/side) x 51 planes, 3 warm-start RL iterations",
+        "value": _pct(lat, 0.99), "unit": "ms", "n_gpus": 1, "steps": len(lat), "warmup": 2 * len(grids),
+        "ms_per_step": span / max(1, len(lat)) * 1e3, "higher_is_better": False, "scaling": "weak",
+        "vs_baseline": None, "dtype": args.precision,
+        "data": "synthetic movie crops (calcium-like traces, Poisson noise)",
+        "config": {"workload": "cfg5_serving_36_patches", "patches": grids, "planes": 51,
+                   "slots": args.slots, "hbm_budget_gb": args.hbm_budget_gb,
+                   "rl_iterations": args.rl_iters, "offered_load": args.load,
+                   "arrival_rate_per_s": lam, "duration_s": args.duration},
+        "latency_ms": {"p50": _pct(lat, 0.5), "p99": _pct(lat, 0.99), "mean": float(np.mean(lat)),
+                       "max": max(lat)},
+        "sustained_volumes_per_s": len(lat) / span,
+        "capacity_volumes_per_s": capacity,
+        "nominal_arrivals_per_s": nominal,
+        "nominal_over_capacity": nominal / capacity,
+        "server": {k: st[k] for k in ("completed", "evictions", "cold_starts", "iterations_run",
+                                      "peak_resident_bytes", "mean_slots")},
+        "rel_l2_vs_fp32_per_call": errs,
+    }
+    print(json.dumps(out), flush=True)
+    psf.close()
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"])
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2"])
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5"],
+                    help="cfg4: batched movie of 12 uneven patches; cfg5: serving loop")
+    ap.add_argument("--patches", type=int, default=0, help="cfg4/cfg5: patch count (12 / 36)")
+    ap.add_argument("--frames", type=int, default=64, help="cfg4: movie length")
+    ap.add_argument("--timed-frames", type=int, default=6, help="cfg4: warm frames timed")
+    ap.add_argument("--rl-iters", type=int, default=3, help="cfg4/cfg5: RL iterations per frame")
+    ap.add_argument("--slots", type=int, default=8, help="cfg5: patches in flight")
+    ap.add_argument("--hbm-budget-gb", type=float, default=48.0, help="cfg5: resident patch memory")
+    ap.add_argument("--load", type=float, default=0.8, help="cfg5: offered load / capacity")
+    ap.add_argument("--duration", type=float, default=40.0, help="cfg5: arrival window, s")
     ap.add_argument("--views-limit", type=int, default=0,
                     help="profiling only: visit the first N views (result flagged partial)")
     ap.add_argument("--iters", type=int, default=0, help="profiling only: RL iterations per volume")
@@ -106,6 +311,13 @@ def main():
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
                           "(sharding across GPUs is described in DESIGN.md, not implemented)"}))
         sys.exit(2)
+
+    if args.workload == "cfg4":
+        args.patches = args.patches or 12
+        return bench_batched(args)
+    if args.workload == "cfg5":
+        args.patches = args.patches or 36
+        return bench_serving(args)
 
     import mesolf_amd
     from mesolf_amd import synth, tables

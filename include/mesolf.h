@@ -201,6 +201,8 @@ mlf_status mlf_server_wait(mlf_server s, uint64_t ticket, double* latency_ms);
 /* Block until every submitted request has completed. */
 mlf_status mlf_server_drain(mlf_server s);
 mlf_status mlf_server_get_stats(mlf_server s, mlf_server_stats* st);
+/* Requests still queued are cancelled (their wait returns an error); those
+ * in flight complete first. */
 mlf_status mlf_server_destroy(mlf_server s);
 
 /* ------------------------------------------------------------ drop-ins */

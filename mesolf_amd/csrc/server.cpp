@@ -204,7 +204,16 @@ void worker_main(mlf_server_s* s) {
     {
       std::unique_lock<std::mutex> lk(s->mu);
       s->cv_work.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->active.empty(); });
-      if (s->stop && s->pending.empty() && s->active.empty()) return;
+      if (s->stop && !s->pending.empty()) {
+        // shutting down: queued requests are cancelled, those in flight finish
+        std::deque<std::shared_ptr<Request>> dropped;
+        dropped.swap(s->pending);
+        lk.unlock();
+        for (const auto& r : dropped) finish(s, r, MLF_ERR_INVALID_ARGUMENT, "cancelled: server destroyed");
+        lk.lock();
+      }
+      if (s->stop && s->active.empty()) return;
+      if (s->pending.empty() && s->active.empty()) continue;
       choose(s, &admit, &evict);
       if (admit.empty() && s->active.empty()) {
         // nothing fits even with every idle patch evicted: fail the oldest
