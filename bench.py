@@ -132,6 +132,8 @@ def bench_batched(args):
         p.set_image(im)
         p.init_volume()
         b.attach(p)
+    tuned = None if args.no_autotune else b.autotune(views=8)
+    log(f"batch tile shape {tuned}")
     t0 = time.perf_counter()
     b.run(args.rl_iters)
     ctx.synchronize()
@@ -179,7 +181,7 @@ def bench_batched(args):
                 "patches are crops of one 121x121 movie",
         "config": {"workload": "cfg4_movie_12_patches", "patches": grids, "planes": 51,
                    "frames_timed": timed, "rl_iterations_per_frame": args.rl_iters,
-                   "graph": True, "precision": args.precision},
+                   "graph": True, "precision": args.precision, "autotuned": tuned},
         "frame_latency_ms": {"p50": _pct(lat, 0.5), "p99": _pct(lat, 0.99), "max": max(lat)},
         "first_frame_cold_ms": t_first * 1e3,
         "fp32_per_call": {"patch_frames_per_s": fp32_rate, "speedup": value / fp32_rate,
@@ -303,6 +305,7 @@ def main():
     ap.add_argument("--iters", type=int, default=0, help="profiling only: RL iterations per volume")
     ap.add_argument("--cpu-views", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the FP32-path volume used for the speed ratio and the error")
     args = ap.parse_args()
@@ -345,6 +348,11 @@ def main():
     work = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision=args.precision) \
         if args.precision != "fp32" else patch
     work.set_image(LF)
+    tuned = None
+    if not args.no_autotune and args.precision != "fp32":
+        work.init_volume()
+        tuned = work.autotune(views=8)
+        log(f"autotuned tile shape {tuned}")
 
     def volume():
         work.init_volume()
@@ -437,6 +445,7 @@ def main():
         "data": "synthetic (seeded PSF family, sphere truth, Poisson measurement)",
         "config": {"workload": wl.name, "lenslets": [wl.n1, wl.n2], "planes": wl.Z,
                    "views": nviews, "rl_iterations": iters, "precision": args.precision,
+                   "tile_shape": list(work.tile_shape()), "autotuned": tuned,
                    "partial_profile_run": partial},
         "roofline": roof,
         "kernels": per_kernel,

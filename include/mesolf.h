@@ -143,8 +143,10 @@ mlf_status mlf_patch_kernel_times(mlf_patch p, double* total_ms, int64_t* launch
  * work entries of every attached patch, whatever its lenslet grid, and one
  * iteration's 3 x nviews launches are replayed from one captured hipGraph.
  * A patch's result does not depend on the batch it ran in (the same work
- * entries produce the same bits).  Patches keep their own image and volume;
- * attach them after set_image and init_volume / set_volume.  A destroyed
+ * entries produce the same bits for the same tile shape; the first attached
+ * patch sets the batch's shape and later ones are switched to it).  Patches
+ * keep their own image and volume; attach them after set_image and
+ * init_volume / set_volume.  A destroyed
  * patch is detached first.  Calls on one batch are serialized; running a
  * batch and one of its patches from two threads at once is not supported. */
 typedef struct mlf_batch_s* mlf_batch;
@@ -159,6 +161,29 @@ mlf_status mlf_batch_set_graph(mlf_batch b, int32_t enable);
  * context stream; filter != 0 applies sample_filter after each (recon:142). */
 mlf_status mlf_batch_run(mlf_batch b, int32_t iterations, int32_t filter);
 mlf_status mlf_batch_last_run_ms(mlf_batch b, double* ms);
+
+/* -------------------------------------------------------------- autotuning */
+/* No counterpart in the reference.  The view-step kernels are built in
+ * several tile shapes: the forward projection's input columns per wave
+ * (BF16: 4 or 2, FP32: 2 or 1; a workgroup covers 16 x 4*columns lenslets)
+ * and the backprojection's 16-phase tiles per workgroup (5 or 3).
+ * Autotuning times every shape on `views` view steps of the patch (or of all
+ * patches of the batch, in its shared launches), restores the volumes, and
+ * keeps the fastest.  Results are cached per precision, lenslet grid(s) and
+ * PSF radii for the process and, if the environment variable MLF_TUNE_CACHE
+ * names a file, across processes: patches created later with a cached key
+ * start with the tuned shape.  Image and volume must be set. */
+typedef struct {
+  int32_t fwd_columns_per_wave;
+  int32_t bwd_phase_tiles;
+  double ms_per_view;  /* measured view step (forward + ratio + backprojection) */
+} mlf_tile_shape;
+mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best);
+mlf_status mlf_patch_tile_shape(mlf_patch p, mlf_tile_shape* current);
+/* Select a shape explicitly (one of the built ones; ms_per_view is ignored). */
+mlf_status mlf_patch_set_tile_shape(mlf_patch p, const mlf_tile_shape* shape);
+/* Tunes the shape shared by every attached patch (key: all their grids). */
+mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best);
 
 /* ----------------------------------------------------------------- serving */
 /* No counterpart in the reference.  A request loop for frame x patch

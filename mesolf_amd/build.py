@@ -28,7 +28,7 @@ COMMON = [
 ]
 
 SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "runtime.cpp", "batch.cpp",
-           "server.cpp"]
+           "server.cpp", "tune.cpp"]
 
 
 def _compile(src: str) -> str:

@@ -8,7 +8,9 @@
 
 #include <algorithm>
 #include <cstring>
+#include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "internal.h"
@@ -30,6 +32,7 @@ struct mlf_batch_s {
   // launch shape (fixed for the life of the batch)
   bool tpz = false;
   int cw = 16, bm2 = 8;
+  int fwd_mt = 0, bwd_nt = 0;  // tile shape shared by every attached patch
   int grid_f = 0, grid_b = 0, grid_r = 0;
   size_t smem_f = 0, smem_b = 0;
   // one RL iteration's view loop as a graph
@@ -117,19 +120,21 @@ mlf_status rebuild(mlf_batch_s* b) {
   return MLF_OK;
 }
 
-// One RL iteration's view loop (recon:100-138) for every attached patch.
-mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st) {
+// One RL iteration's view loop (recon:100-138) for every attached patch
+// (the first `nviews` views; all when nviews <= 0).
+mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st, int nviews = 0) {
   mlf_psf_s* ps = b->psf;
   const PrecPack& pk = ps->pack[(int)b->kp];
   const size_t eb = elem_bytes(b->kp);
   const FwdGeo g{b->d_slots.ptr, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, b->d_fsched.ptr,
                  b->d_counts.ptr + 0};
   const RatioGeo rg{b->d_rsched.ptr, b->d_counts.ptr + 2};
-  for (int k = 0; k < ps->V; ++k) {
+  const int nv = (nviews <= 0 || nviews > ps->V) ? ps->V : nviews;
+  for (int k = 0; k < nv; ++k) {
     const int u = ps->uv[2 * k], v = ps->uv[2 * k + 1];
     const void* Wk = pk.fwd.ptr + (size_t)k * pk.pvf * eb;
     if (b->tpz) {
-      MLF_LAUNCH(launch_fwd_tpz(b->kp, Wk, g, b->grid_f, b->smem_f, st));
+      MLF_LAUNCH(launch_fwd_tpz(b->kp, b->fwd_mt, Wk, g, b->grid_f, b->smem_f, st));
       MLF_LAUNCH(launch_ratio_gather_tpz(g, rg, b->grid_r, ps->Z, b->cw, u, v, 1, nullptr, st));
     } else {
       MLF_LAUNCH(launch_fwd_mfma(b->kp, Wk, g, b->grid_f, b->smem_f, st));
@@ -137,8 +142,8 @@ mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st) {
     }
     const BwdGeo bg{b->d_slots.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, b->d_bsched.ptr,
                     b->d_counts.ptr + 1, k};
-    MLF_LAUNCH(launch_bwd_mfma(b->kp, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg, b->grid_b,
-                               b->smem_b, (float)ps->weight[k], st));
+    MLF_LAUNCH(launch_bwd_mfma(b->kp, b->bwd_nt, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg,
+                               b->grid_b, b->smem_b, (float)ps->weight[k], st));
   }
   return MLF_OK;
 }
@@ -260,12 +265,22 @@ mlf_status mlf_batch_attach(mlf_batch b, mlf_patch p) {
   if ((int)b->slots.size() >= b->capacity)
     return set_error(MLF_ERR_BUDGET, "batch is full (%d patches)", b->capacity);
   MLF_CHECK(ctx_activate(b->psf->ctx));
-  if (b->slots.empty() && !b->exec) {
+  if (b->slots.empty()) {
+    if (b->fwd_mt != p->fwd_mt || b->bwd_nt != p->bwd_nt) drop_graph(b);
     b->tpz = p->tpz;
     b->cw = p->cw;
     b->bm2 = p->bm2;
+    b->fwd_mt = p->fwd_mt;
+    b->bwd_nt = p->bwd_nt;
     b->smem_f = p->smem_f;
     b->smem_b = p->smem_b;
+  } else if (p->fwd_mt != b->fwd_mt || p->bwd_nt != b->bwd_nt) {
+    // one launch serves every patch: the patch takes the batch's tile shape
+    MLF_HIP(hipStreamSynchronize(b->psf->ctx->stream));
+    p->fwd_mt = b->fwd_mt;
+    p->bwd_nt = b->bwd_nt;
+    MLF_CHECK(patch_setup_tiles(p));
+    if (p->volume_set) MLF_CHECK(patch_refresh_shadow(p));
   }
   b->slots.push_back(p);
   p->batch = b;
@@ -323,6 +338,88 @@ mlf_status mlf_batch_run(mlf_batch b, int32_t iterations, int32_t filter) {
   }
   MLF_HIP(hipEventRecord(b->ev1, st));
   b->timed = true;
+  return MLF_OK;
+}
+
+mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) {
+  clear_error();
+  if (!b) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(b->mu);
+  if (b->slots.empty()) return set_error(MLF_ERR_INVALID_ARGUMENT, "no patch attached");
+  for (mlf_patch_s* p : b->slots)
+    if (!p->image_set || !p->volume_set)
+      return set_error(MLF_ERR_INVALID_ARGUMENT, "image and volume must be set on every attached patch");
+  MLF_CHECK(ctx_activate(b->psf->ctx));
+  hipStream_t st = b->psf->ctx->stream;
+  const int nv = std::max(1, std::min<int>(views <= 0 ? 8 : views, b->psf->V));
+  std::vector<std::unique_ptr<DevBuf<float>>> save;
+  for (mlf_patch_s* p : b->slots) {
+    save.emplace_back(new DevBuf<float>());
+    MLF_CHECK(save.back()->alloc(p->X.count));
+    MLF_HIP(hipMemcpyAsync(save.back()->ptr, p->X.ptr, p->X.bytes(), hipMemcpyDeviceToDevice, st));
+  }
+  auto apply = [&](int mt, int nt) -> mlf_status {
+    MLF_HIP(hipStreamSynchronize(st));
+    for (mlf_patch_s* p : b->slots) {
+      p->fwd_mt = mt;
+      p->bwd_nt = nt;
+      MLF_CHECK(patch_setup_tiles(p));
+      MLF_CHECK(patch_refresh_shadow(p));
+    }
+    mlf_patch_s* p0 = b->slots[0];
+    b->fwd_mt = mt;
+    b->bwd_nt = nt;
+    b->cw = p0->cw;
+    b->smem_f = p0->smem_f;
+    b->smem_b = p0->smem_b;
+    drop_graph(b);
+    b->dirty = true;
+    return rebuild(b);
+  };
+  auto restore = [&]() -> mlf_status {
+    for (size_t i = 0; i < b->slots.size(); ++i)
+      MLF_HIP(hipMemcpyAsync(b->slots[i]->X.ptr, save[i]->ptr, save[i]->bytes(),
+                             hipMemcpyDeviceToDevice, st));
+    return MLF_OK;
+  };
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  MLF_HIP(hipEventCreate(&e0));
+  MLF_HIP(hipEventCreate(&e1));
+  double best_ms = 1e300;
+  int bmt = b->fwd_mt, bnt = b->bwd_nt;
+  mlf_status s = MLF_OK;
+  for (const auto& c : tile_candidates(b->kp)) {
+    if (apply(c.first, c.second) != MLF_OK) continue;  // shape does not fit this PSF
+    s = enqueue_views(b, st, 1);  // warm-up
+    if (s == MLF_OK && hipEventRecord(e0, st) != hipSuccess) s = set_error(MLF_ERR_DEVICE, "event");
+    if (s == MLF_OK) s = enqueue_views(b, st, nv);
+    if (s == MLF_OK && hipEventRecord(e1, st) != hipSuccess) s = set_error(MLF_ERR_DEVICE, "event");
+    float ms = 0.f;
+    if (s == MLF_OK && (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess))
+      s = set_error(MLF_ERR_DEVICE, "event timing failed");
+    if (s == MLF_OK) s = restore();
+    if (s != MLF_OK) break;
+    if (ms / nv < best_ms) {
+      best_ms = ms / nv;
+      bmt = c.first;
+      bnt = c.second;
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  MLF_CHECK(s);
+  MLF_CHECK(apply(bmt, bnt));
+  std::vector<std::pair<int, int>> grids;
+  for (mlf_patch_s* p : b->slots) grids.emplace_back(p->n1, p->n2);
+  std::sort(grids.begin(), grids.end());
+  std::string key = tune_key(b->slots[0], "batch");
+  for (const auto& g : grids) key += "|" + std::to_string(g.first) + "x" + std::to_string(g.second);
+  tune_cache_put(key, bmt, bnt, best_ms);
+  if (best) {
+    best->fwd_columns_per_wave = bmt;
+    best->bwd_phase_tiles = bnt;
+    best->ms_per_view = best_ms;
+  }
   return MLF_OK;
 }
 

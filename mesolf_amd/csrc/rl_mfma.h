@@ -15,11 +15,11 @@ constexpr int kPhiPadBF16 = 256;  // 225 phases padded to the bf16 MFMA K-step (
 constexpr int kFwdChunk = 64;     // taps per forward B chunk (4 N-tiles)
 constexpr int kFwdMTF32 = 2;      // M-tiles per wave, forward, f32 (block 16 x 8 lenslets)
 constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x 16 lenslets)
-constexpr int kBwdNTiles = 5;     // N-tiles per backward workgroup (80 phases)
-constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases
+constexpr int kBwdNTiles = 5;     // default N-tiles per backward workgroup (80 phases; 3 also built)
+constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases (or 5 of 48)
 constexpr int kBwdTapPad = 32;    // backward taps padded to a multiple of 32
-constexpr int kTpzMTF32 = 2;      // Toeplitz forward: input columns per wave, f32
-constexpr int kTpzMTBF16 = 4;     // Toeplitz forward: input columns per wave, bf16
+constexpr int kTpzMTF32 = 2;      // Toeplitz forward: default input columns per wave, f32 (1 also built)
+constexpr int kTpzMTBF16 = 4;     // Toeplitz forward: default input columns per wave, bf16 (2 also built)
 constexpr int kShadowRow = 256;   // BF16 volume shadow: phases per lenslet (zero beyond 225)
 
 inline int fwd_bm2(Precision p) { return 4 * (p == Precision::kF32 ? kFwdMTF32 : kFwdMTBF16); }
@@ -105,9 +105,10 @@ __host__ __device__ inline void norm_range(int c, int n, int r, int* lo, int* hi
 }
 
 size_t fwd_smem_bytes(Precision p, int rmax, int bm2);
-size_t tpz_smem_bytes(Precision p, int rmax);
+size_t tpz_smem_bytes(Precision p, int rmax, int mt);
 int tpz_max_radius();
-hipError_t launch_fwd_tpz(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+// mt: input columns per wave (f32: 1 or 2; bf16: 2 or 4)
+hipError_t launch_fwd_tpz(Precision p, int mt, const void* W, FwdGeo g, int grid, size_t smem,
                           hipStream_t st);
 hipError_t launch_make_shadow(const float* X, void* Xb, size_t nlens, hipStream_t st);
 // y = sum of the forward partials; with use_lf, R = LF(u::15, v::15) ./ y
@@ -122,8 +123,9 @@ hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_
                            hipStream_t st);
 hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, int u, int v,
                                int use_lf, float* y_out, hipStream_t st);
-hipError_t launch_bwd_mfma(Precision p, const void* Wb, BwdGeo g, int grid, size_t smem, float w,
-                           hipStream_t st);
+// nt: 16-phase N-tiles per workgroup (3 or 5)
+hipError_t launch_bwd_mfma(Precision p, int nt, const void* Wb, BwdGeo g, int grid, size_t smem,
+                           float w, hipStream_t st);
 hipError_t launch_pack_fwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                            hipStream_t st);
 hipError_t launch_pack_bwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,

@@ -602,12 +602,12 @@ __device__ __forceinline__ int norm_class(int a, int n, int r) {
 // no swizzle, no index stride).
 constexpr int kBufferDword3 = 0x00020000;
 
-template <typename T>
+template <typename T, int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   using M = Mma<T>;
   constexpr int KSTEP = M::KSTEP, KPL = M::KPL;
-  constexpr int MT = 4, NT = kBwdNTiles;
+  constexpr int MT = 4;  // NT: N-tiles (16 phases each) per workgroup
   extern __shared__ __align__(16) char smem[];
   const int nent = uni(*g.count);
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
@@ -924,9 +924,8 @@ hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_
   return hipGetLastError();
 }
 
-size_t tpz_smem_bytes(Precision p, int rmax) {
+size_t tpz_smem_bytes(Precision p, int rmax, int MT) {
   const bool f32 = p == Precision::kF32;
-  const int MT = f32 ? kTpzMTF32 : kTpzMTBF16;
   const int RV = f32 ? Mma<float>::PHI / 4 : Mma<__bf16>::PHI / 8;
   const int NB = f32 ? TpzBufs<float>::value : TpzBufs<__bf16>::value;
   const size_t wslot = ((16 * kTpzNT * RV + 255) / 256) * 256;
@@ -936,14 +935,18 @@ size_t tpz_smem_bytes(Precision p, int rmax) {
 
 int tpz_max_radius() { return (16 * kTpzNT - 1) / 2; }
 
-hipError_t launch_fwd_tpz(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+hipError_t launch_fwd_tpz(Precision p, int mt, const void* W, FwdGeo g, int grid, size_t smem,
                           hipStream_t st) {
   if (p == Precision::kF32) {
-    hipLaunchKernelGGL((k_fwd_tpz<float, kTpzMTF32>), dim3(grid), dim3(256), smem, st,
-                       (const float*)W, g);
+    if (mt == 1)
+      hipLaunchKernelGGL((k_fwd_tpz<float, 1>), dim3(grid), dim3(256), smem, st, (const float*)W, g);
+    else
+      hipLaunchKernelGGL((k_fwd_tpz<float, 2>), dim3(grid), dim3(256), smem, st, (const float*)W, g);
   } else {
-    hipLaunchKernelGGL((k_fwd_tpz<__bf16, kTpzMTBF16>), dim3(grid), dim3(256), smem, st,
-                       (const __bf16*)W, g);
+    if (mt == 2)
+      hipLaunchKernelGGL((k_fwd_tpz<__bf16, 2>), dim3(grid), dim3(256), smem, st, (const __bf16*)W, g);
+    else
+      hipLaunchKernelGGL((k_fwd_tpz<__bf16, 4>), dim3(grid), dim3(256), smem, st, (const __bf16*)W, g);
   }
   return hipGetLastError();
 }
@@ -988,13 +991,18 @@ hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, 
   return hipGetLastError();
 }
 
-hipError_t launch_bwd_mfma(Precision p, const void* Wb, BwdGeo g, int grid, size_t smem, float w,
-                           hipStream_t st) {
+hipError_t launch_bwd_mfma(Precision p, int nt, const void* Wb, BwdGeo g, int grid, size_t smem,
+                           float w, hipStream_t st) {
   if (p == Precision::kF32) {
-    hipLaunchKernelGGL(k_bwd_mfma<float>, dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
+    if (nt == 3)
+      hipLaunchKernelGGL((k_bwd_mfma<float, 3>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
+    else
+      hipLaunchKernelGGL((k_bwd_mfma<float, 5>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
   } else {
-    hipLaunchKernelGGL(k_bwd_mfma<__bf16>, dim3(grid), dim3(256), smem, st, (const __bf16*)Wb, g,
-                       w);
+    if (nt == 3)
+      hipLaunchKernelGGL((k_bwd_mfma<__bf16, 3>), dim3(grid), dim3(256), smem, st, (const __bf16*)Wb, g, w);
+    else
+      hipLaunchKernelGGL((k_bwd_mfma<__bf16, 5>), dim3(grid), dim3(256), smem, st, (const __bf16*)Wb, g, w);
   }
   return hipGetLastError();
 }
@@ -1035,23 +1043,22 @@ hipError_t launch_norm_table(Precision p, const float* src_view, double* S, floa
 namespace mlf {
 // Allow the forward kernels more than 64 KB of dynamic LDS (gfx950 has 160 KB).
 hipError_t mfma_kernel_attributes() {
-  hipError_t e = hipFuncSetAttribute(
+  const void* kernels[] = {
       reinterpret_cast<const void*>(&k_fwd_mfma<float, kFwdMTF32>),
-      hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_mfma<__bf16, kFwdMTBF16>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_tpz<float, kTpzMTF32>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, kTpzMTBF16>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bwd_mfma<float>),
-                          hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-  if (e != hipSuccess) return e;
-  return hipFuncSetAttribute(reinterpret_cast<const void*>(&k_bwd_mfma<__bf16>),
-                             hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+      reinterpret_cast<const void*>(&k_fwd_mfma<__bf16, kFwdMTBF16>),
+      reinterpret_cast<const void*>(&k_fwd_tpz<float, 1>),
+      reinterpret_cast<const void*>(&k_fwd_tpz<float, 2>),
+      reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 2>),
+      reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 4>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<float, 3>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<float, 5>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5>),
+  };
+  for (const void* k : kernels) {
+    const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 }  // namespace mlf

@@ -800,7 +800,7 @@ static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, fl
   const int use_lf = LF != nullptr;
   MLF_CHECK(tim_begin(p, 0));
   if (p->tpz) {
-    MLF_LAUNCH(launch_fwd_tpz(p->kp, Wk, g, p->nblk_f, p->smem_f, st));
+    MLF_LAUNCH(launch_fwd_tpz(p->kp, p->fwd_mt, Wk, g, p->nblk_f, p->smem_f, st));
   } else {
     MLF_LAUNCH(launch_fwd_mfma(p->kp, Wk, g, p->nblk_f, p->smem_f, st));
   }
@@ -830,7 +830,7 @@ static mlf_status patch_view_step(mlf_patch_s* p, int k) {
                                            p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
   } else {
     const PrecPack& pk = ps->pack[(int)p->kp];
-    MLF_LAUNCH(launch_bwd_mfma(p->kp, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
+    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->bwd_nt, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
                                bwd_geo(p, k), p->nblk_b, p->smem_b, (float)ps->weight[k], st));
   }
   MLF_CHECK(tim_end(p));
@@ -861,7 +861,7 @@ static void append_entries(const std::vector<int32_t>& src, int width, int slot,
 }
 
 // Schedules (largest planes first), partial buffers, packed tables.
-static mlf_status patch_setup_mfma(mlf_patch_s* p) {
+mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   mlf_psf_s* ps = p->psf;
   std::lock_guard<std::mutex> lk(ps->mu);
   static std::once_flag attr_once;
@@ -874,7 +874,7 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
   const int Z = p->Z;
   p->tpz = ps->pack[(int)p->kp].tpz;
   p->bm2 = fwd_bm2(p->kp);
-  p->cw = 4 * (p->kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16);
+  p->cw = 4 * p->fwd_mt;
   const int colw = p->tpz ? p->cw : p->bm2;  // lenslets per block along axis 1
   const int nb1 = (p->n1 + 15) / 16, nb2 = (p->n2 + colw - 1) / colw;
   std::vector<int> order(Z);
@@ -901,7 +901,7 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
   for (int z = 0; z < Z; ++z) border[z] = z;
   std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return ps->rb[a] > ps->rb[b]; });
   const int mb1 = (p->n1 + 15) / 16, mb2 = (p->n2 + 15) / 16;
-  const int ngrp = kPhiPadBwd / (16 * kBwdNTiles);
+  const int ngrp = kPhiPadBwd / (16 * p->bwd_nt);
   p->bent.clear();
   for (int z : border)
     for (int b2 = 0; b2 < mb2; ++b2)
@@ -943,7 +943,7 @@ static mlf_status patch_setup_mfma(mlf_patch_s* p) {
   sd.n1 = p->n1;
   sd.n2 = p->n2;
   MLF_CHECK(upload_vec(p->d_slot, std::vector<SlotDesc>{sd}));
-  p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
+  p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax, p->fwd_mt) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
   p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax);
   if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
     return set_error(MLF_ERR_UNSUPPORTED,
@@ -1043,7 +1043,16 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   const char* ref = std::getenv("MLF_REFERENCE_KERNELS");
   p->reference = ref && ref[0] == '1' && prec == MLF_PRECISION_FP32;
   p->kp = prec == MLF_PRECISION_BF16 ? Precision::kBF16 : Precision::kF32;
-  if (!p->reference) MLF_CHECK(patch_setup_mfma(p.get()));
+  if (!p->reference) {
+    p->fwd_mt = p->kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
+    p->bwd_nt = kBwdNTiles;
+    int mt = 0, nt = 0;
+    if (tune_cache_get(tune_key(p.get(), "patch"), &mt, &nt)) {
+      p->fwd_mt = mt;
+      p->bwd_nt = nt;
+    }
+    MLF_CHECK(patch_setup_tiles(p.get()));
+  }
   MLF_HIP(hipEventCreate(&p->ev0));
   MLF_HIP(hipEventCreate(&p->ev1));
   undo.p = nullptr;
@@ -1236,6 +1245,101 @@ mlf_status mlf_patch_sample_filter(mlf_patch p) {
   MLF_CHECK(ctx_activate(p->psf->ctx));
   MLF_CHECK(patch_filter(p));
   MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_tile_shape(mlf_patch p, mlf_tile_shape* cur) {
+  clear_error();
+  if (!p || !cur) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  cur->fwd_columns_per_wave = p->fwd_mt;
+  cur->bwd_phase_tiles = p->bwd_nt;
+  cur->ms_per_view = 0.0;
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_set_tile_shape(mlf_patch p, const mlf_tile_shape* shape) {
+  clear_error();
+  if (!p || !shape) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (p->reference) return set_error(MLF_ERR_UNSUPPORTED, "reference kernels have one shape");
+  if (p->batch) return set_error(MLF_ERR_INVALID_ARGUMENT, "detach the patch from its batch first");
+  bool ok = false;
+  for (const auto& c : tile_candidates(p->kp))
+    ok = ok || (c.first == shape->fwd_columns_per_wave && c.second == shape->bwd_phase_tiles);
+  if (!ok)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "tile shape (%d, %d) is not built for this precision",
+                     (int)shape->fwd_columns_per_wave, (int)shape->bwd_phase_tiles);
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+  const int mt = p->fwd_mt, nt = p->bwd_nt;
+  p->fwd_mt = shape->fwd_columns_per_wave;
+  p->bwd_nt = shape->bwd_phase_tiles;
+  if (patch_setup_tiles(p) != MLF_OK) {
+    const std::string e = last_error();
+    p->fwd_mt = mt;
+    p->bwd_nt = nt;
+    MLF_CHECK(patch_setup_tiles(p));
+    return set_error(MLF_ERR_UNSUPPORTED, "%s", e.c_str());
+  }
+  if (p->volume_set) MLF_CHECK(patch_refresh_shadow(p));
+  return MLF_OK;
+}
+
+mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) {
+  clear_error();
+  if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (p->reference) return set_error(MLF_ERR_UNSUPPORTED, "reference kernels have one shape");
+  if (p->batch) return set_error(MLF_ERR_INVALID_ARGUMENT, "detach the patch from its batch first");
+  if (!p->image_set || !p->volume_set)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "image and volume must be set before tuning");
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  hipStream_t st = p->psf->ctx->stream;
+  const int nv = std::max(1, std::min<int>(views <= 0 ? 8 : views, p->psf->V));
+  DevBuf<float> save;
+  MLF_CHECK(save.alloc(p->X.count));
+  MLF_HIP(hipMemcpyAsync(save.ptr, p->X.ptr, p->X.bytes(), hipMemcpyDeviceToDevice, st));
+  struct Ev {
+    hipEvent_t e = nullptr;
+    ~Ev() {
+      if (e) (void)hipEventDestroy(e);
+    }
+  } e0, e1;
+  MLF_HIP(hipEventCreate(&e0.e));
+  MLF_HIP(hipEventCreate(&e1.e));
+  double best_ms = 1e300;
+  int bmt = p->fwd_mt, bnt = p->bwd_nt;
+  for (const auto& c : tile_candidates(p->kp)) {
+    MLF_HIP(hipStreamSynchronize(st));
+    p->fwd_mt = c.first;
+    p->bwd_nt = c.second;
+    if (patch_setup_tiles(p) != MLF_OK) continue;  // shape does not fit this PSF
+    MLF_CHECK(patch_refresh_shadow(p));
+    MLF_CHECK(patch_view_step(p, 0));  // warm-up
+    MLF_HIP(hipEventRecord(e0.e, st));
+    for (int k = 0; k < nv; ++k) MLF_CHECK(patch_view_step(p, k));
+    MLF_HIP(hipEventRecord(e1.e, st));
+    MLF_HIP(hipEventSynchronize(e1.e));
+    float ms = 0.f;
+    MLF_HIP(hipEventElapsedTime(&ms, e0.e, e1.e));
+    MLF_HIP(hipMemcpyAsync(p->X.ptr, save.ptr, p->X.bytes(), hipMemcpyDeviceToDevice, st));
+    if (ms / nv < best_ms) {
+      best_ms = ms / nv;
+      bmt = c.first;
+      bnt = c.second;
+    }
+  }
+  MLF_HIP(hipStreamSynchronize(st));
+  p->fwd_mt = bmt;
+  p->bwd_nt = bnt;
+  MLF_CHECK(patch_setup_tiles(p));
+  MLF_CHECK(patch_refresh_shadow(p));
+  tune_cache_put(tune_key(p, "patch"), bmt, bnt, best_ms);
+  if (best) {
+    best->fwd_columns_per_wave = bmt;
+    best->bwd_phase_tiles = bnt;
+    best->ms_per_view = best_ms;
+  }
   return MLF_OK;
 }
 

@@ -7,6 +7,8 @@
 
 #include <memory>
 #include <mutex>
+#include <string>
+#include <utility>
 #include <vector>
 
 #include "internal.h"
@@ -98,7 +100,8 @@ struct mlf_patch_s {
   mlf::Precision kp = mlf::Precision::kF32;
   int bm2 = 8;
   bool tpz = false;                    // Toeplitz forward kernel
-  int cw = 16;                         // Toeplitz: input columns per workgroup
+  int cw = 16;                         // Toeplitz: input columns per workgroup (4 * fwd_mt)
+  int fwd_mt = 0, bwd_nt = 0;          // tile shape: forward columns per wave, backward N-tiles
   mlf::DevBuf<float> partial;               // forward window partials
   mlf::DevBuf<uint16_t> Xb;                 // BF16 shadow of X, [lenslet][kShadowRow] (BF16 Toeplitz)
   mlf::DevBuf<int64_t> d_poff;
@@ -136,4 +139,14 @@ mlf_status patch_refresh_shadow(mlf_patch_s* p);
 size_t patch_device_bytes(const mlf_patch_s* p);
 size_t patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int n2);
 mlf_status batch_forget_patch(mlf_patch_s* p);  // detach before destruction
+// (Re)build a patch's work entries, partial buffers and slot descriptor for
+// its tile shape (fwd_mt, bwd_nt); the caller synchronises the stream first.
+mlf_status patch_setup_tiles(mlf_patch_s* p);
+// Candidate tile shapes of a precision, default first.
+std::vector<std::pair<int, int>> tile_candidates(Precision kp);
+// Tile shape cache (per process, optionally persisted in the file named by
+// MLF_TUNE_CACHE): key = precision, grid(s), PSF radii.
+bool tune_cache_get(const std::string& key, int* mt, int* nt);
+void tune_cache_put(const std::string& key, int mt, int nt, double ms);
+std::string tune_key(const mlf_patch_s* p, const char* scope);
 }  // namespace mlf

@@ -107,6 +107,10 @@ _SIGNATURES = {
     "mlf_batch_set_graph": (C.c_int, [_P, _I32]),
     "mlf_batch_run": (C.c_int, [_P, _I32, _I32]),
     "mlf_batch_last_run_ms": (C.c_int, [_P, _F64P]),
+    "mlf_patch_autotune": (C.c_int, [_P, _I32, _P]),
+    "mlf_patch_tile_shape": (C.c_int, [_P, _P]),
+    "mlf_patch_set_tile_shape": (C.c_int, [_P, _P]),
+    "mlf_batch_autotune": (C.c_int, [_P, _I32, _P]),
     "mlf_server_create": (C.c_int, [_P, C.c_int, _P, C.POINTER(_P)]),
     "mlf_server_submit": (C.c_int, [_P, _I32, _I64, _I64, _F32P, _F32P, _U64P]),
     "mlf_server_wait": (C.c_int, [_P, C.c_uint64, _F64P]),
@@ -335,6 +339,21 @@ class Patch:
         _check(self._lib.mlf_patch_kernel_times(self.handle, _fptr(ms, C.c_double), _fptr(n, C.c_int64)))
         return {c: (float(ms[i]), int(n[i])) for i, c in enumerate(self.KERNEL_CLASSES)}
 
+    def autotune(self, views: int = 8):
+        """Time every tile shape on `views` view steps; keeps the fastest."""
+        t = _TileShape()
+        _check(self._lib.mlf_patch_autotune(self.handle, int(views), C.byref(t)))
+        return _shape_dict(t)
+
+    def tile_shape(self):
+        t = _TileShape()
+        _check(self._lib.mlf_patch_tile_shape(self.handle, C.byref(t)))
+        return (t.fwd_columns_per_wave, t.bwd_phase_tiles)
+
+    def set_tile_shape(self, fwd_columns_per_wave: int, bwd_phase_tiles: int):
+        t = _TileShape(int(fwd_columns_per_wave), int(bwd_phase_tiles), 0.0)
+        _check(self._lib.mlf_patch_set_tile_shape(self.handle, C.byref(t)))
+
     def forward_view(self, k: int):
         out = np.empty((self.n1, self.n2), np.float32, order="F")
         _check(self._lib.mlf_patch_forward_view(self.handle, int(k), _fptr(out)))
@@ -361,6 +380,16 @@ class Patch:
 
 
 # -------------------------------------------------------------------- drop-ins
+class _TileShape(C.Structure):
+    _fields_ = [("fwd_columns_per_wave", C.c_int32), ("bwd_phase_tiles", C.c_int32),
+                ("ms_per_view", C.c_double)]
+
+
+def _shape_dict(t):
+    return {"fwd_columns_per_wave": t.fwd_columns_per_wave, "bwd_phase_tiles": t.bwd_phase_tiles,
+            "ms_per_view": t.ms_per_view}
+
+
 class Batch:
     """Several patches of one PSF run in shared launches (``mlf_batch_*``).
 
@@ -390,6 +419,12 @@ class Batch:
 
     def run(self, iterations: int, filter: bool = True):
         _check(self._lib.mlf_batch_run(self.handle, int(iterations), int(bool(filter))))
+
+    def autotune(self, views: int = 8):
+        """Time every tile shape on the attached patches; keeps the fastest."""
+        t = _TileShape()
+        _check(self._lib.mlf_batch_autotune(self.handle, int(views), C.byref(t)))
+        return _shape_dict(t)
 
     def last_run_ms(self):
         ms = C.c_double()
