@@ -631,6 +631,50 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   #pragma unroll
       for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
 
+    // Fused multiplicative update (recon:130-137), set up before the K loop so
+    // that the first M-tile's loads are in flight while the MFMAs run.  The
+    // taps are the MFMA's A operand and the ratio window its B operand, so the
+    // accumulator tile is phases x lenslets: lane (lr, lg) holds phases
+    // 4lg..4lg+3 of N-tile nt for lenslet a1 = 16 b1 + lr, i.e. 16 contiguous
+    // bytes of X, of the normaliser row and (8 bytes) of the BF16 shadow.
+    // Loads are 16-byte buffer loads from clamped lenslets, issued one M-tile
+    // ahead; only the stores are predicated.  Buffer resources span one plane
+    // (a plane is below 2^31 bytes, checked at patch creation) and return
+    // zero past it.
+    const int C1 = uni(sd.C1[z]), C2 = norm_classes(n2, r);
+    const size_t nlp = (size_t)n2 * n1;  // lenslets per plane
+    const __amdgpu_buffer_rsrc_t xr =
+        __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kP, 0, (int)(nlp * kP * 4), kBufferDword3);
+    const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(sd.toffT[z])), 0, C2 * C1 * kP * 4,
+        kBufferDword3);
+    const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc(
+        Xsh ? Xsh + z * nlp * kShadowRow : nullptr, 0, Xsh ? (int)(nlp * kShadowRow * 2) : 0,
+        kBufferDword3);
+    typedef float floatx4_t __attribute__((ext_vector_type(4)));
+    typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
+    typedef unsigned uintx2 __attribute__((ext_vector_type(2)));
+    const int a1 = 16 * b1 + lr;
+    const bool ok1 = a1 < n1;
+    const int a1c = ok1 ? a1 : n1 - 1;
+    const int c1 = norm_class(a1c, n1, r);
+    int ph0[NT];  // first of this lane's 4 phases in N-tile nt
+  #pragma unroll
+    for (int nt = 0; nt < NT; ++nt) ph0[nt] = grp * 16 * NT + nt * 16 + 4 * lg;
+    floatx4_t xv[2][NT], bv[2][NT];
+    auto load_tile = [&](int mt, int buf) {
+      const int a2 = 16 * b2 + 4 * w + mt;
+      const int a2c = a2 < n2 ? a2 : n2 - 1;
+      const int xo = (a2c * n1 + a1c) * kP;
+      const int to = (norm_class(a2c, n2, r) * C1 + c1) * kP;
+  #pragma unroll
+      for (int nt = 0; nt < NT; ++nt) {
+        xv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0[nt]), 0, 0));
+        bv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0[nt]), 0, 0));
+      }
+    };
+    load_tile(0, 0);
+
     if constexpr (KPL == 1) {
       // FP32: scalar im2col reads, taps e1-fastest
       float* Rw = reinterpret_cast<float*>(smem);                                 // [Wd][Wd]
@@ -669,7 +713,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         for (int mt = 0; mt < MT; ++mt) {
           const typename M::frag af = Rw[base[mt] + o];
   #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
         }
       }
     } else {
@@ -720,86 +764,48 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         for (int mt = 0; mt < MT; ++mt) {
           const typename M::frag af = Rv[base[mt] + o];
   #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(af, bf[nt], acc[mt][nt]);
+          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
         }
       }
     }
 
-    // fused multiplicative update (recon:130-137).  Loads are unconditional
-    // from clamped indices and issued one M-tile ahead (40 in flight per lane),
-    // so HBM latency is paid a few times per workgroup rather than once per
-    // N-tile; only the stores are predicated.  Buffer instructions take 32-bit
-    // offsets from per-plane resources (one VGPR per address; a plane is below
-    // 2^31 bytes, checked at patch creation) and drop any access outside the
-    // plane.
-    const int C1 = uni(sd.C1[z]), C2 = norm_classes(n2, r);
-    const size_t nlp = (size_t)n2 * n1;  // lenslets per plane
-    const __amdgpu_buffer_rsrc_t xr =
-        __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kP, 0, (int)(nlp * kP * 4), kBufferDword3);
-    const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(sd.toffT[z])), 0, C2 * C1 * kP * 4,
-        kBufferDword3);
-    const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc(
-        Xsh ? Xsh + z * nlp * kShadowRow : nullptr, 0, Xsh ? (int)(nlp * kShadowRow * 2) : 0,
-        kBufferDword3);
-    int phc[NT];
-    bool okp[NT];
-  #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) {
-      const int ph = grp * 16 * NT + nt * 16 + lr;
-      okp[nt] = ph < kP;
-      phc[nt] = okp[nt] ? ph : kP - 1;
-    }
-    int a1c[4], c1v[4];
-    bool ok1[4];
-  #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int a1 = 16 * b1 + 4 * lg + j;
-      ok1[j] = a1 < n1;
-      a1c[j] = ok1[j] ? a1 : n1 - 1;
-      c1v[j] = norm_class(a1c[j], n1, r);
-    }
-    float xv[2][NT][4], bv[2][NT][4];
-    auto load_tile = [&](int mt, int buf) {
-      const int a2 = 16 * b2 + 4 * w + mt;
-      const int a2c = a2 < n2 ? a2 : n2 - 1;
-      const int c2 = norm_class(a2c, n2, r);
-  #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int xo = (a2c * n1 + a1c[j]) * kP;
-        const int to = (c2 * C1 + c1v[j]) * kP;
-  #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          xv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(xr, 4 * (xo + phc[nt]), 0, 0));
-          bv[buf][nt][j] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(tr, 4 * (to + phc[nt]), 0, 0));
-        }
-      }
-    };
-    load_tile(0, 0);
+    // fused multiplicative update (recon:130-137), continued from the loads
+    // issued before the K loop.
   #pragma unroll
     for (int mt = 0; mt < MT; ++mt) {
       const int buf = mt & 1;
       if (mt + 1 < MT) load_tile(mt + 1, buf ^ 1);
       const int a2 = 16 * b2 + 4 * w + mt;
-      const bool ok2 = a2 < n2;
-      const int a2c = ok2 ? a2 : n2 - 1;
+      const bool ok = ok1 && a2 < n2;
+      const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
   #pragma unroll
-      for (int j = 0; j < 4; ++j) {
-        const int l = a2c * n1 + a1c[j];
+      for (int nt = 0; nt < NT; ++nt) {
+        floatx4_t xn;
   #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
+        for (int j = 0; j < 4; ++j) {
           const float x = xv[buf][nt][j];
           float C = x * acc[mt][nt][j] / bv[buf][nt][j];
           // NaN, +-Inf and negatives -> 0, as a select (no branches)
           C = (C >= 0.f && C <= 3.402823466e38f) ? C : 0.f;
-          float xn = C * wgt + (1.f - wgt) * x;
-          xn = xn < 0.f ? 0.f : xn;
-          if (ok2 && okp[nt] && ok1[j]) {
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn), xr, 4 * (l * kP + phc[nt]), 0, 0);
-            if (Xsh)
-              __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)xn), sr,
-                                                    2 * (l * kShadowRow + phc[nt]), 0, 0);
+          const float v = C * wgt + (1.f - wgt) * x;
+          xn[j] = v < 0.f ? 0.f : v;
+        }
+        const int xo = 4 * (l * kP + ph0[nt]);
+        const int so = 2 * (l * kShadowRow + ph0[nt]);
+        if (ok && ph0[nt] + 4 <= kP) {
+          __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
+          if (Xsh) {
+            uintx2 h;
+            h[0] = (unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[0]) |
+                   ((unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[1]) << 16);
+            h[1] = (unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[2]) |
+                   ((unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[3]) << 16);
+            __builtin_amdgcn_raw_buffer_store_b64(h, sr, so, 0, 0);
           }
+        } else if (ok && ph0[nt] < kP) {  // phase 224: the last of 225
+          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
+          if (Xsh)
+            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)xn[0]), sr, so, 0, 0);
         }
       }
     }
