@@ -289,7 +289,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16"])
+    ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
     ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5"],
                     help="cfg4: batched movie of 12 uneven patches; cfg5: serving loop")
     ap.add_argument("--patches", type=int, default=0, help="cfg4/cfg5: patch count (12 / 36)")

@@ -126,11 +126,12 @@ mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st, int nviews = 0) {
   mlf_psf_s* ps = b->psf;
   const PrecPack& pk = ps->pack[(int)b->kp];
   const size_t eb = elem_bytes(b->kp);
-  const FwdGeo g{b->d_slots.ptr, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, b->d_fsched.ptr,
-                 b->d_counts.ptr + 0};
   const RatioGeo rg{b->d_rsched.ptr, b->d_counts.ptr + 2};
   const int nv = (nviews <= 0 || nviews > ps->V) ? ps->V : nviews;
   for (int k = 0; k < nv; ++k) {
+    const FwdGeo g{b->d_slots.ptr, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, b->d_fsched.ptr,
+                   b->d_counts.ptr + 0,
+                   pk.d_finv.ptr ? pk.d_finv.ptr + (size_t)k * ps->Z : nullptr};
     const int u = ps->uv[2 * k], v = ps->uv[2 * k + 1];
     const void* Wk = pk.fwd.ptr + (size_t)k * pk.pvf * eb;
     if (b->tpz) {
@@ -179,6 +180,11 @@ mlf_status detach_locked(mlf_batch_s* b, mlf_patch_s* p) {
 }  // namespace
 
 namespace mlf {
+void batch_mark_dirty(mlf_batch_s* b) {
+  std::lock_guard<std::mutex> lk(b->mu);
+  b->dirty = true;
+}
+
 // Called by mlf_patch_destroy.
 mlf_status batch_forget_patch(mlf_patch_s* p) {
   mlf_batch_s* b = p->batch;
@@ -196,7 +202,7 @@ mlf_status mlf_batch_create(mlf_psf psf, mlf_precision prec, int32_t capacity, m
   *out = nullptr;
   if (capacity < 1 || capacity > 4096)
     return set_error(MLF_ERR_INVALID_ARGUMENT, "batch capacity %d out of range", (int)capacity);
-  if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16)
+  if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16 && prec != MLF_PRECISION_FP16)
     return set_error(MLF_ERR_UNSUPPORTED, "precision %d not supported", (int)prec);
   MLF_CHECK(ctx_activate(psf->ctx));
   {
@@ -215,7 +221,7 @@ mlf_status mlf_batch_create(mlf_psf psf, mlf_precision prec, int32_t capacity, m
       }
     }
   } undo{b.get()};
-  b->kp = prec == MLF_PRECISION_BF16 ? Precision::kBF16 : Precision::kF32;
+  b->kp = kernel_precision(prec);
   b->capacity = capacity;
   MLF_CHECK(b->d_slots.alloc((size_t)capacity));
   MLF_CHECK(b->d_counts.alloc(3));

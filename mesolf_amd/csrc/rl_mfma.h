@@ -8,7 +8,8 @@
 
 namespace mlf {
 
-enum class Precision { kF32 = 0, kBF16 = 1 };
+enum class Precision { kF32 = 0, kBF16 = 1, kF16 = 2 };
+constexpr int kPrecisions = 3;
 
 constexpr int kPhiPadF32 = 228;   // 225 phases padded to the f32 MFMA K-step (4)
 constexpr int kPhiPadBF16 = 256;  // 225 phases padded to the bf16 MFMA K-step (32)
@@ -23,6 +24,7 @@ constexpr int kTpzMTBF16 = 4;     // Toeplitz forward: default input columns per
 constexpr int kShadowRow = 256;   // BF16 volume shadow: phases per lenslet (zero beyond 225)
 
 inline int fwd_bm2(Precision p) { return 4 * (p == Precision::kF32 ? kFwdMTF32 : kFwdMTBF16); }
+// FP16 pre-scales the PSF tables (per view and plane) and the volume shadow.
 inline int fwd_phi(Precision p) { return p == Precision::kF32 ? kPhiPadF32 : kPhiPadBF16; }
 inline size_t elem_bytes(Precision p) { return p == Precision::kF32 ? 4 : 2; }
 
@@ -31,7 +33,7 @@ inline size_t elem_bytes(Precision p) { return p == Precision::kF32 ? 4 : 2; }
 // slot, so patches of different lenslet grids share one launch.
 struct SlotDesc {
   float* X;               // FP32 volume [z][n2][n1][225]
-  __bf16* Xb;             // BF16 shadow [z][n2][n1][256], or null
+  void* Xb;               // 16-bit shadow [z][n2][n1][256] (BF16, or FP16 of X * xs), or null
   const float* LF;        // measured image, column-major (15 n1) x (15 n2)
   float* R;               // ratio image [n2][n1]
   float* partial;         // forward partial windows
@@ -41,6 +43,7 @@ struct SlotDesc {
   const int64_t* toffT;   // [Z] plane offset inside one view's normaliser table
   const int32_t* C1;      // [Z] normaliser classes along axis 0
   int n1, n2;
+  float xs, xs_inv;       // FP16 shadow scale (1 otherwise)
 };
 
 // Work entries are processed by a grid-stride loop over *count entries, so a
@@ -53,6 +56,7 @@ struct FwdGeo {
   const int32_t* nchunk;  // [Z] number of 64-tap chunks (Q-scatter kernel)
   const int32_t* sched;   // [entries][4] = (slot, z, b1, b2)
   const int32_t* count;   // number of entries
+  const float* finv;      // FP16: [Z] inverse PSF scale of the view (null otherwise)
 };
 
 struct RatioGeo {
@@ -76,7 +80,8 @@ struct PackGeo {
   const int64_t* src_off;  // [Z] offset in the ABI view block
   const int64_t* dst_off;  // [Z] offset in the packed view block
   int64_t src_view, dst_view;
-  int row8;                // backward pack: tap rows padded to a multiple of 8 (BF16)
+  int row8;                // backward pack: tap rows padded to a multiple of 8 (16-bit types)
+  const float* scale;      // [V][Z] factor applied to the taps (FP16), or null
 };
 
 struct NormGeo {
@@ -87,6 +92,7 @@ struct NormGeo {
   const int64_t* src_off;  // [Z] plane offset in the ABI (backward) view block
   const int64_t* s_off;    // [Z] FP64 scratch offsets
   const int64_t* t_off;    // [Z] normaliser table offsets within the view
+  const float* scale;      // [Z] FP16 tap scale of the view, or null
 };
 
 // Normaliser classes along one axis of an n-lenslet grid for a plane of
@@ -110,7 +116,12 @@ int tpz_max_radius();
 // mt: input columns per wave (f32: 1 or 2; bf16: 2 or 4)
 hipError_t launch_fwd_tpz(Precision p, int mt, const void* W, FwdGeo g, int grid, size_t smem,
                           hipStream_t st);
-hipError_t launch_make_shadow(const float* X, void* Xb, size_t nlens, hipStream_t st);
+hipError_t launch_make_shadow(Precision p, const float* X, void* Xb, size_t nlens, float xs,
+                              hipStream_t st);
+// scale[v][z] = 1 / max|tap| of the (view, plane) block (1 if empty); inv = max.
+hipError_t launch_block_scale(const float* src, const int64_t* off, const int32_t* r,
+                              int64_t per_view, int Z, int V, float* scale, float* inv,
+                              hipStream_t st);
 // y = sum of the forward partials; with use_lf, R = LF(u::15, v::15) ./ y
 // (non-finite -> 0) into each slot's R; y_out (single-slot launches) gets y.
 hipError_t launch_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int grid, int Z, int cw, int u, int v,

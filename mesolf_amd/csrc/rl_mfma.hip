@@ -25,6 +25,7 @@ namespace {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 halfx8 __attribute__((ext_vector_type(8)));
 
 template <typename T>
 struct Mma;
@@ -52,6 +53,32 @@ struct Mma<__bf16> {
   }
   __device__ static __forceinline__ frag load(const __bf16* p) { return *(const bf16x8*)p; }
 };
+
+// FP16 operands: same shape as BF16, 3 more mantissa bits, narrower range
+// (the PSF tables are pre-scaled per view and plane, the volume shadow by a
+// per-patch factor, so operands stay well inside it).
+template <>
+struct Mma<_Float16> {
+  static constexpr int KSTEP = 32, KPL = 8;
+  static constexpr int PHI = kPhiPadBF16;
+  static constexpr int LDB = kPhiPadBF16;
+  typedef halfx8 frag;
+  __device__ static __forceinline__ floatx4 mma(frag a, frag b, floatx4 c) {
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+  }
+  __device__ static __forceinline__ frag load(const _Float16* p) { return *(const halfx8*)p; }
+};
+
+// 16-bit shadow encoding of a value (FP32 kernels keep no shadow; BF16 stands in).
+template <typename T>
+__device__ __forceinline__ unsigned short to_bits16(float v) {
+  if constexpr (sizeof(T) == 2) return __builtin_bit_cast(unsigned short, (T)v);
+  else return __builtin_bit_cast(unsigned short, (__bf16)v);
+}
+template <typename T>
+constexpr bool is_f16() {
+  return __is_same(T, _Float16);
+}
 
 __device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
@@ -86,6 +113,8 @@ __device__ __forceinline__ SlotDesc load_slot(const SlotDesc* slots, int i) {
   o.C1 = uni_ptr(d.C1);
   o.n1 = uni(d.n1);
   o.n2 = uni(d.n2);
+  o.xs = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, d.xs)));
+  o.xs_inv = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, d.xs_inv)));
   return o;
 }
 // s_waitcnt vmcnt(<= n) without waiting on LDS/export counters (gfx9 encoding:
@@ -313,7 +342,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     const int* s = g.sched + 4 * ent;
     const SlotDesc sd = load_slot(g.slots, uni(s[0]));
     const float* __restrict__ X = sd.X;
-    const __bf16* __restrict__ Xb = sd.Xb;
+    const T* __restrict__ Xb = reinterpret_cast<const T*>(sd.Xb);
     const int n1 = sd.n1, n2 = sd.n2;
     const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]);
     const int r = uni(g.r[z]);
@@ -344,8 +373,9 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
           a[mt][ks] = (ok && ph < kP) ? v : 0.f;
         } else {
           // BF16 shadow rows are 256 phases (zero beyond 225), 16-byte aligned
-          const bf16x8 v = *reinterpret_cast<const bf16x8*>(Xb + lens * kShadowRow + ks * KSTEP + lg * KPL);
-          a[mt][ks] = ok ? v : bf16x8{};
+          const typename M::frag v =
+              *reinterpret_cast<const typename M::frag*>(Xb + lens * kShadowRow + ks * KSTEP + lg * KPL);
+          a[mt][ks] = ok ? v : typename M::frag{};
         }
       }
     }
@@ -460,21 +490,24 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
                                                      __builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF,
                                                      false));
             } else {
-              const bf16x8 v = *reinterpret_cast<const bf16x8*>(rp + ks * KSTEP);
-              const bf16x8 inj = ok ? v : bf16x8{};
+              const typename M::frag v = *reinterpret_cast<const typename M::frag*>(rp + ks * KSTEP);
+              const typename M::frag inj = ok ? v : typename M::frag{};
               typedef int intx4 __attribute__((ext_vector_type(4)));
               const intx4 iv = __builtin_bit_cast(intx4, inj);
               intx4 cv = __builtin_bit_cast(intx4, a[mt][ks]);
   #pragma unroll
               for (int q = 0; q < 4; ++q) cv[q] = __builtin_amdgcn_update_dpp(iv[q], cv[q], 0x111, 0xF, 0xF, false);
-              a[mt][ks] = __builtin_bit_cast(bf16x8, cv);
+              a[mt][ks] = __builtin_bit_cast(typename M::frag, cv);
             }
           }
         }
       }
     }
 
-    // P tile -> this wave's window (rows i1 - 16*b1, columns c2 + d2 - (c2_0 + MT*w - r))
+    // P tile -> this wave's window (rows i1 - 16*b1, columns c2 + d2 - (c2_0 + MT*w - r));
+    // FP16 undoes the PSF and volume pre-scaling here (one plane per entry)
+    float unscale = 1.f;
+    if constexpr (is_f16<T>()) unscale = g.finv[z] * sd.xs_inv;
   #pragma unroll
     for (int nt = 0; nt < kTpzNT; ++nt) {
       const int e2 = nt * 16 + lr;
@@ -482,7 +515,11 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
   #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
   #pragma unroll
-        for (int j = 0; j < 4; ++j) atomicAdd(yw + (4 * lg + j) * WW + mt + e2, acc[mt][nt][j]);
+        for (int j = 0; j < 4; ++j) {
+          float v = acc[mt][nt][j];
+          if constexpr (is_f16<T>()) v *= unscale;
+          atomicAdd(yw + (4 * lg + j) * WW + mt + e2, v);
+        }
     }
     __syncthreads();
     const int Wc2 = CW + 2 * r;
@@ -547,6 +584,7 @@ __global__ void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int Z, int CW, int u, 
 template <typename T>
 __global__ void k_pack_fwd_tpz(const float* __restrict__ src, T* __restrict__ dst, PackGeo pg) {
   const int z = blockIdx.y, v = blockIdx.z;
+  const float sc = pg.scale ? pg.scale[(size_t)v * gridDim.y + z] : 1.f;
   const int K = 2 * pg.r[z] + 1;
   const int nrow = 16 * ((K + 15) / 16);
   const int PHI = Mma<T>::PHI;
@@ -558,7 +596,7 @@ __global__ void k_pack_fwd_tpz(const float* __restrict__ src, T* __restrict__ ds
     const int ph = (int)(o % PHI);
     const int e2 = (int)((o / PHI) % nrow);
     const int e1 = (int)(o / ((size_t)PHI * nrow));
-    d[o] = (T)((e2 < K && ph < kP) ? s[((size_t)e2 * K + e1) * kP + ph] : 0.f);
+    d[o] = (T)((e2 < K && ph < kP) ? s[((size_t)e2 * K + e1) * kP + ph] * sc : 0.f);
   }
 }
 
@@ -614,7 +652,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const int* s = g.sched + 5 * ent;
     const SlotDesc sd = load_slot(g.slots, uni(s[0]));
     float* __restrict__ X = sd.X;
-    __bf16* __restrict__ Xsh = sd.Xb;
+    T* __restrict__ Xsh = reinterpret_cast<T*>(sd.Xb);
     const float* __restrict__ R = sd.R;
     const int n1 = sd.n1, n2 = sd.n2;
     const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]), grp = uni(s[4]);
@@ -723,8 +761,9 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // fragment is a single 16-byte LDS read.
       const int Kp8 = (K + 7) & ~7;
       float* Rs = reinterpret_cast<float*>(smem);                                  // [Wd][Wd]
-      bf16x8* Rv = reinterpret_cast<bf16x8*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // [Wd][Wd]
-      int* toff8 = reinterpret_cast<int*>(reinterpret_cast<char*>(Rv) + sizeof(bf16x8) * Wd * Wd);
+      using frag_t = typename M::frag;
+      frag_t* Rv = reinterpret_cast<frag_t*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // [Wd][Wd]
+      int* toff8 = reinterpret_cast<int*>(reinterpret_cast<char*>(Rv) + sizeof(frag_t) * Wd * Wd);
       for (int o = tid; o < Wd * Wd; o += 256) {
         const int x = o % Wd, y = o / Wd;
         const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
@@ -738,9 +777,12 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       __syncthreads();
       for (int o = tid; o < Wd * Wd; o += 256) {
         const int x = o % Wd;
-        bf16x8 v;
+        frag_t v;
   #pragma unroll
-        for (int j = 0; j < 8; ++j) v[j] = (__bf16)(x - j >= 0 ? Rs[o - j] : 0.f);
+        for (int j = 0; j < 8; ++j) {
+          const float rv = x - j >= 0 ? Rs[o - j] : 0.f;
+          v[j] = (T)(is_f16<T>() ? fminf(rv, 65504.f) : rv);  // FP16: clamp to the largest finite value
+        }
         Rv[o] = v;
       }
       __syncthreads();
@@ -795,17 +837,18 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         if (ok && ph0[nt] + 4 <= kP) {
           __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
           if (Xsh) {
+            floatx4_t sv = xn;
+            if constexpr (is_f16<T>()) sv *= sd.xs;
             uintx2 h;
-            h[0] = (unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[0]) |
-                   ((unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[1]) << 16);
-            h[1] = (unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[2]) |
-                   ((unsigned)__builtin_bit_cast(unsigned short, (__bf16)xn[3]) << 16);
+            h[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
+            h[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
             __builtin_amdgcn_raw_buffer_store_b64(h, sr, so, 0, 0);
           }
         } else if (ok && ph0[nt] < kP) {  // phase 224: the last of 225
           __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
           if (Xsh)
-            __builtin_amdgcn_raw_buffer_store_b16(__builtin_bit_cast(unsigned short, (__bf16)xn[0]), sr, so, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b16(to_bits16<T>(is_f16<T>() ? xn[0] * sd.xs : xn[0]), sr, so,
+                                                  0, 0);
         }
       }
     }
@@ -838,6 +881,7 @@ __global__ void k_pack_fwd(const float* __restrict__ src, T* __restrict__ dst, P
 template <typename T>
 __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, PackGeo pg) {
   const int z = blockIdx.y, v = blockIdx.z;
+  const float sc = pg.scale ? pg.scale[(size_t)v * gridDim.y + z] : 1.f;
   const int K = 2 * pg.r[z] + 1, ntaps = K * K;
   const size_t epad = (size_t)pg.rows[z];
   const float* s = src + (size_t)v * pg.src_view + pg.src_off[z];
@@ -855,7 +899,7 @@ __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, P
       e1 = t - e2 * K;
     }
     const bool ok = ph < kP && e2 < K && e1 < K;
-    d[o] = (T)(ok ? s[((size_t)e2 * K + e1) * kP + ph] : 0.f);
+    d[o] = (T)(ok ? s[((size_t)e2 * K + e1) * kP + ph] * sc : 0.f);
   }
 }
 
@@ -863,6 +907,7 @@ __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, P
 template <typename T>
 __global__ void k_norm_pass1(const float* __restrict__ src, double* __restrict__ S, NormGeo ng) {
   const int z = blockIdx.y;
+  const float sc = ng.scale ? ng.scale[z] : 1.f;  // the kernels' pre-scaled taps
   const int r = ng.r[z], K = 2 * r + 1;
   const int C1 = ng.C1[z];
   const size_t total = (size_t)K * C1 * kP;
@@ -876,7 +921,7 @@ __global__ void k_norm_pass1(const float* __restrict__ src, double* __restrict__
     int lo, hi;
     norm_range(c1, ng.n1, r, &lo, &hi);
     double acc = 0.0;
-    for (int d1 = lo; d1 <= hi; ++d1) acc += (double)(float)(T)s[((size_t)e2 * K + (d1 + r)) * kP + ph];
+    for (int d1 = lo; d1 <= hi; ++d1) acc += (double)(float)(T)(s[((size_t)e2 * K + (d1 + r)) * kP + ph] * sc);
     d[o] = acc;
   }
 }
@@ -948,6 +993,11 @@ hipError_t launch_fwd_tpz(Precision p, int mt, const void* W, FwdGeo g, int grid
       hipLaunchKernelGGL((k_fwd_tpz<float, 1>), dim3(grid), dim3(256), smem, st, (const float*)W, g);
     else
       hipLaunchKernelGGL((k_fwd_tpz<float, 2>), dim3(grid), dim3(256), smem, st, (const float*)W, g);
+  } else if (p == Precision::kF16) {
+    if (mt == 2)
+      hipLaunchKernelGGL((k_fwd_tpz<_Float16, 2>), dim3(grid), dim3(256), smem, st, (const _Float16*)W, g);
+    else
+      hipLaunchKernelGGL((k_fwd_tpz<_Float16, 4>), dim3(grid), dim3(256), smem, st, (const _Float16*)W, g);
   } else {
     if (mt == 2)
       hipLaunchKernelGGL((k_fwd_tpz<__bf16, 2>), dim3(grid), dim3(256), smem, st, (const __bf16*)W, g);
@@ -957,19 +1007,56 @@ hipError_t launch_fwd_tpz(Precision p, int mt, const void* W, FwdGeo g, int grid
   return hipGetLastError();
 }
 
-__global__ void k_make_shadow(const float* __restrict__ X, __bf16* __restrict__ Xb, size_t nlens) {
+template <typename T>
+__global__ void k_make_shadow(const float* __restrict__ X, T* __restrict__ Xb, size_t nlens, float xs) {
   for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < nlens * kShadowRow;
        o += (size_t)gridDim.x * blockDim.x) {
     const size_t l = o / kShadowRow;
     const int ph = (int)(o - l * kShadowRow);
-    Xb[o] = (__bf16)(ph < kP ? X[l * kP + ph] : 0.f);
+    const float v = ph < kP ? X[l * kP + ph] : 0.f;
+    Xb[o] = (T)(is_f16<T>() ? v * xs : v);
   }
 }
 
-hipError_t launch_make_shadow(const float* X, void* Xb, size_t nlens, hipStream_t st) {
+hipError_t launch_make_shadow(Precision p, const float* X, void* Xb, size_t nlens, float xs,
+                              hipStream_t st) {
   size_t g = (nlens * kShadowRow + 255) / 256;
   if (g > 65536) g = 65536;
-  hipLaunchKernelGGL(k_make_shadow, dim3((unsigned)g), dim3(256), 0, st, X, (__bf16*)Xb, nlens);
+  if (p == Precision::kF16)
+    hipLaunchKernelGGL(k_make_shadow<_Float16>, dim3((unsigned)g), dim3(256), 0, st, X,
+                       (_Float16*)Xb, nlens, xs);
+  else
+    hipLaunchKernelGGL(k_make_shadow<__bf16>, dim3((unsigned)g), dim3(256), 0, st, X, (__bf16*)Xb,
+                       nlens, xs);
+  return hipGetLastError();
+}
+
+// Largest |tap| of each (view, plane) block of an ABI table -> 1 / max (1 for
+// an empty block): the FP16 pre-scaling factors.
+__global__ void k_block_scale(const float* __restrict__ src, const int64_t* __restrict__ off,
+                              const int32_t* __restrict__ r, int64_t per_view, float* __restrict__ scale,
+                              float* __restrict__ inv) {
+  const int z = blockIdx.x, v = blockIdx.y, Z = gridDim.x;
+  const int K = 2 * r[z] + 1;
+  const size_t n = (size_t)K * K * kP;
+  const float* s = src + (size_t)v * per_view + off[z];
+  float m = 0.f;
+  for (size_t o = threadIdx.x; o < n; o += blockDim.x) m = fmaxf(m, fabsf(s[o]));
+  for (int d = 32; d > 0; d >>= 1) m = fmaxf(m, __shfl_down(m, d, 64));
+  __shared__ float sh[4];
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const float mx = fmaxf(fmaxf(sh[0], sh[1]), fmaxf(sh[2], sh[3]));
+    scale[(size_t)v * Z + z] = mx > 0.f ? 1.f / mx : 1.f;
+    if (inv) inv[(size_t)v * Z + z] = mx > 0.f ? mx : 1.f;
+  }
+}
+
+hipError_t launch_block_scale(const float* src, const int64_t* off, const int32_t* r,
+                              int64_t per_view, int Z, int V, float* scale, float* inv,
+                              hipStream_t st) {
+  hipLaunchKernelGGL(k_block_scale, dim3(Z, V), dim3(256), 0, st, src, off, r, per_view, scale, inv);
   return hipGetLastError();
 }
 
@@ -985,6 +1072,8 @@ hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo
   dim3 grid(64, Z, V);
   if (p == Precision::kF32)
     hipLaunchKernelGGL(k_pack_fwd_tpz<float>, grid, dim3(256), 0, st, src, (float*)dst, pg);
+  else if (p == Precision::kF16)
+    hipLaunchKernelGGL(k_pack_fwd_tpz<_Float16>, grid, dim3(256), 0, st, src, (_Float16*)dst, pg);
   else
     hipLaunchKernelGGL(k_pack_fwd_tpz<__bf16>, grid, dim3(256), 0, st, src, (__bf16*)dst, pg);
   return hipGetLastError();
@@ -1004,6 +1093,11 @@ hipError_t launch_bwd_mfma(Precision p, int nt, const void* Wb, BwdGeo g, int gr
       hipLaunchKernelGGL((k_bwd_mfma<float, 3>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
     else
       hipLaunchKernelGGL((k_bwd_mfma<float, 5>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
+  } else if (p == Precision::kF16) {
+    if (nt == 3)
+      hipLaunchKernelGGL((k_bwd_mfma<_Float16, 3>), dim3(grid), dim3(256), smem, st, (const _Float16*)Wb, g, w);
+    else
+      hipLaunchKernelGGL((k_bwd_mfma<_Float16, 5>), dim3(grid), dim3(256), smem, st, (const _Float16*)Wb, g, w);
   } else {
     if (nt == 3)
       hipLaunchKernelGGL((k_bwd_mfma<__bf16, 3>), dim3(grid), dim3(256), smem, st, (const __bf16*)Wb, g, w);
@@ -1028,6 +1122,8 @@ hipError_t launch_pack_bwd(Precision p, const float* src, void* dst, PackGeo pg,
   dim3 grid(64, Z, V);
   if (p == Precision::kF32)
     hipLaunchKernelGGL(k_pack_bwd<float>, grid, dim3(256), 0, st, src, (float*)dst, pg);
+  else if (p == Precision::kF16)
+    hipLaunchKernelGGL(k_pack_bwd<_Float16>, grid, dim3(256), 0, st, src, (_Float16*)dst, pg);
   else
     hipLaunchKernelGGL(k_pack_bwd<__bf16>, grid, dim3(256), 0, st, src, (__bf16*)dst, pg);
   return hipGetLastError();
@@ -1038,6 +1134,8 @@ hipError_t launch_norm_table(Precision p, const float* src_view, double* S, floa
   dim3 grid(128, Z);
   if (p == Precision::kF32)
     hipLaunchKernelGGL(k_norm_pass1<float>, grid, dim3(256), 0, st, src_view, S, ng);
+  else if (p == Precision::kF16)
+    hipLaunchKernelGGL(k_norm_pass1<_Float16>, grid, dim3(256), 0, st, src_view, S, ng);
   else
     hipLaunchKernelGGL(k_norm_pass1<__bf16>, grid, dim3(256), 0, st, src_view, S, ng);
   hipLaunchKernelGGL(k_norm_pass2, grid, dim3(256), 0, st, S, Tn_view, ng);
@@ -1058,6 +1156,10 @@ hipError_t mfma_kernel_attributes() {
       reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 4>),
       reinterpret_cast<const void*>(&k_bwd_mfma<float, 3>),
       reinterpret_cast<const void*>(&k_bwd_mfma<float, 5>),
+      reinterpret_cast<const void*>(&k_fwd_tpz<_Float16, 2>),
+      reinterpret_cast<const void*>(&k_fwd_tpz<_Float16, 4>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 5>),
       reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3>),
       reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5>),
   };

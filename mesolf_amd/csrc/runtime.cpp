@@ -305,7 +305,7 @@ static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
     const int Kf = 2 * ps->rf[z] + 1;
     rows_f[z] = pk.tpz ? Kf * 16 * ((Kf + 15) / 16) : pk.nchunk[z] * kFwdChunk;
     const int Kb = 2 * ps->rb[z] + 1;
-    const int nt_b_pad = prec == Precision::kBF16 ? Kb * ((Kb + 7) & ~7) : nt_b;
+    const int nt_b_pad = prec != Precision::kF32 ? Kb * ((Kb + 7) & ~7) : nt_b;
     pk.epad[z] = (nt_b_pad + kBwdTapPad - 1) / kBwdTapPad * kBwdTapPad;
     rows_b[z] = pk.epad[z];
     pk.woff[z] = of;
@@ -326,12 +326,24 @@ static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
   MLF_CHECK(upload_vec(d_rows_f, rows_f));
   MLF_CHECK(upload_vec(d_rows_b, rows_b));
   hipStream_t st = ps->ctx->stream;
-  PackGeo gf{ps->d_rf.ptr, d_rows_f.ptr, ps->d_offf.ptr, pk.d_woff.ptr, ps->per_view_f, pk.pvf, 0};
+  if (prec == Precision::kF16) {
+    // pre-scale every (view, plane) block so its largest tap is 1
+    if (!pk.tpz) return set_error(MLF_ERR_UNSUPPORTED, "FP16 needs PSF radii <= %d", tpz_max_radius());
+    MLF_CHECK(pk.d_fscale.alloc((size_t)V * Z));
+    MLF_CHECK(pk.d_finv.alloc((size_t)V * Z));
+    MLF_CHECK(pk.d_bscale.alloc((size_t)V * Z));
+    MLF_LAUNCH(launch_block_scale(ps->fwd.ptr, ps->d_offf.ptr, ps->d_rf.ptr, ps->per_view_f, Z, V,
+                                  pk.d_fscale.ptr, pk.d_finv.ptr, ps->ctx->stream));
+    MLF_LAUNCH(launch_block_scale(ps->bwd.ptr, ps->d_offb.ptr, ps->d_rb.ptr, ps->per_view_b, Z, V,
+                                  pk.d_bscale.ptr, nullptr, ps->ctx->stream));
+  }
+  PackGeo gf{ps->d_rf.ptr, d_rows_f.ptr, ps->d_offf.ptr, pk.d_woff.ptr, ps->per_view_f, pk.pvf, 0,
+             pk.d_fscale.ptr};
   hipError_t e = pk.tpz ? launch_pack_fwd_tpz(prec, ps->fwd.ptr, pk.fwd.ptr, gf, Z, V, st)
                         : launch_pack_fwd(prec, ps->fwd.ptr, pk.fwd.ptr, gf, Z, V, st);
   if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "pack_fwd: %s", hipGetErrorString(e));
   PackGeo gb{ps->d_rb.ptr, d_rows_b.ptr, ps->d_offb.ptr, pk.d_boff.ptr, ps->per_view_b, pk.pvb,
-             prec == Precision::kBF16 ? 1 : 0};
+             prec != Precision::kF32 ? 1 : 0, pk.d_bscale.ptr};
   e = launch_pack_bwd(prec, ps->bwd.ptr, pk.bwd.ptr, gb, Z, V, st);
   if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "pack_bwd: %s", hipGetErrorString(e));
   MLF_HIP(hipStreamSynchronize(st));
@@ -381,9 +393,12 @@ static mlf_status psf_ensure_norm(mlf_psf_s* ps, Precision prec, int n1, int n2,
   MLF_CHECK(upload_vec(d_soff, soff));
   DevBuf<double> S;
   MLF_CHECK(S.alloc((size_t)so));
-  NormGeo ng{g1, g2, ps->d_rb.ptr, nt->d_C1.ptr, d_C2.ptr, ps->d_offb.ptr, d_soff.ptr, nt->d_toff.ptr};
+  NormGeo ng{g1, g2, ps->d_rb.ptr, nt->d_C1.ptr, d_C2.ptr, ps->d_offb.ptr, d_soff.ptr, nt->d_toff.ptr,
+             nullptr};
   hipStream_t st = ps->ctx->stream;
+  const float* bscale = ps->pack[(int)prec].d_bscale.ptr;  // FP16: taps as packed
   for (int v = 0; v < ps->V; ++v) {
+    ng.scale = bscale ? bscale + (size_t)v * Z : nullptr;
     hipError_t e = launch_norm_table(prec, ps->bwd.ptr + (size_t)v * ps->per_view_b, S.ptr,
                                      nt->T.ptr + (size_t)v * to, ng, Z, st);
     if (e != hipSuccess) return set_error(MLF_ERR_DEVICE, "norm table: %s", hipGetErrorString(e));
@@ -763,11 +778,12 @@ static FilterPlan filter_plan_of(mlf_patch_s* p) {
 }
 
 
-static FwdGeo fwd_geo(mlf_patch_s* p) {
+static FwdGeo fwd_geo(mlf_patch_s* p, int k) {
   mlf_psf_s* ps = p->psf;
   const PrecPack& pk = ps->pack[(int)p->kp];
   return FwdGeo{p->d_slot.ptr, ps->d_rf.ptr, pk.d_woff.ptr, pk.d_nchunk.ptr, p->d_fsched.ptr,
-                p->d_counts.ptr + 0};
+                p->d_counts.ptr + 0,
+                pk.d_finv.ptr ? pk.d_finv.ptr + (size_t)k * ps->Z : nullptr};
 }
 
 static BwdGeo bwd_geo(mlf_patch_s* p, int k) {
@@ -794,7 +810,7 @@ static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, fl
     return MLF_OK;
   }
   const PrecPack& pk = ps->pack[(int)p->kp];
-  const FwdGeo g = fwd_geo(p);
+  const FwdGeo g = fwd_geo(p, k);
   const void* Wk = pk.fwd.ptr + (size_t)k * pk.pvf * elem_bytes(p->kp);
   const RatioGeo rg{p->d_rsched.ptr, p->d_counts.ptr + 2};
   const int use_lf = LF != nullptr;
@@ -927,7 +943,7 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   MLF_CHECK(upload_vec(p->d_rsched, rs));
   MLF_CHECK(upload_vec(p->d_counts, std::vector<int32_t>{p->nblk_f, p->nblk_b, p->nblk_r}));
   MLF_CHECK(p->partial.alloc((size_t)po));
-  if (p->tpz && p->kp == Precision::kBF16)
+  if (p->tpz && p->kp != Precision::kF32)
     MLF_CHECK(p->Xb.alloc((size_t)p->n1 * p->n2 * Z * kShadowRow));
   SlotDesc& sd = p->slot;
   sd.X = p->X.ptr;
@@ -942,6 +958,8 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   sd.C1 = p->norm->d_C1.ptr;
   sd.n1 = p->n1;
   sd.n2 = p->n2;
+  sd.xs = p->xs;
+  sd.xs_inv = 1.f / p->xs;
   MLF_CHECK(upload_vec(p->d_slot, std::vector<SlotDesc>{sd}));
   p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax, p->fwd_mt) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
   p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax);
@@ -954,7 +972,7 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
 // Re-derive the BF16 shadow after X changed outside the backprojection.
 mlf_status mlf::patch_refresh_shadow(mlf_patch_s* p) {
   if (!p->Xb.ptr) return MLF_OK;
-  MLF_LAUNCH(launch_make_shadow(p->X.ptr, p->Xb.ptr, (size_t)p->n1 * p->n2 * p->Z,
+  MLF_LAUNCH(launch_make_shadow(p->kp, p->X.ptr, p->Xb.ptr, (size_t)p->n1 * p->n2 * p->Z, p->slot.xs,
                                 p->psf->ctx->stream));
   return MLF_OK;
 }
@@ -981,7 +999,7 @@ size_t mlf::patch_device_bytes(const mlf_patch_s* p) {
 }
 
 size_t mlf::patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int n2) {
-  const Precision kp = prec == MLF_PRECISION_BF16 ? Precision::kBF16 : Precision::kF32;
+  const Precision kp = kernel_precision(prec);
   int rfmax = 0;
   for (int z = 0; z < ps->Z; ++z) rfmax = std::max(rfmax, ps->rf[z]);
   const bool tpz = rfmax <= tpz_max_radius();
@@ -990,7 +1008,7 @@ size_t mlf::patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int 
   size_t b = 4 * n * Z * kP * 2;  // volume + sample_filter temporary
   b += 4 * n * kP + 4 * n * Z + 8 * n;
   b += 4 * (size_t)partial_elems(ps, tpz, cw, fwd_bm2(kp), n1, n2, nullptr);
-  if (kp == Precision::kBF16 && tpz) b += 2 * n * Z * kShadowRow;
+  if (kp != Precision::kF32 && tpz) b += 2 * n * Z * kShadowRow;
   // entry lists and filter taps: < 1 MB
   return b + (1u << 20);
 }
@@ -1005,7 +1023,7 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   if (n1 < 1 || n2 < 1 || n1 > 4096 || n2 > 4096)
     return set_error(MLF_ERR_INVALID_ARGUMENT, "bad lenslet grid %lld x %lld", (long long)n1,
                      (long long)n2);
-  if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16)
+  if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16 && prec != MLF_PRECISION_FP16)
     return set_error(MLF_ERR_UNSUPPORTED, "precision %d not supported", (int)prec);
   // kernels address one plane of the volume with 32-bit byte offsets
   if ((double)n1 * n2 * kP * sizeof(float) >= 2147483648.0)
@@ -1042,7 +1060,7 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   MLF_CHECK(build_filter_plan(p.get()));
   const char* ref = std::getenv("MLF_REFERENCE_KERNELS");
   p->reference = ref && ref[0] == '1' && prec == MLF_PRECISION_FP32;
-  p->kp = prec == MLF_PRECISION_BF16 ? Precision::kBF16 : Precision::kF32;
+  p->kp = kernel_precision(prec);
   if (!p->reference) {
     p->fwd_mt = p->kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
     p->bwd_nt = kBwdNTiles;
@@ -1085,6 +1103,21 @@ mlf_status mlf_patch_set_image(mlf_patch p, const float* lfimg) {
   return MLF_OK;
 }
 
+// FP16 shadow scale: X * xs ~ 1 for a volume of mean `mean` (kernels in
+// flight read the slot descriptor, so the stream is synchronised first).
+static mlf_status patch_set_scale(mlf_patch_s* p, double mean) {
+  if (p->kp != Precision::kF16 || p->reference) return MLF_OK;
+  const float xs = (mean > 0.0 && std::isfinite(mean)) ? (float)(1.0 / mean) : 1.f;
+  if (xs == p->xs) return MLF_OK;
+  MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+  p->xs = xs;
+  p->slot.xs = xs;
+  p->slot.xs_inv = 1.f / xs;
+  MLF_CHECK(upload_vec(p->d_slot, std::vector<SlotDesc>{p->slot}));
+  if (p->batch) batch_mark_dirty(p->batch);
+  return MLF_OK;
+}
+
 mlf_status mlf_patch_init_volume(mlf_patch p) {
   clear_error();
   if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
@@ -1096,6 +1129,12 @@ mlf_status mlf_patch_init_volume(mlf_patch p) {
   const double numel = (double)p->n1 * kN * p->n2 * kN * p->Z;
   MLF_LAUNCH(launch_init_volume(p->LF.ptr, p->LF.count, p->dsum.ptr, p->X.ptr, p->X.count,
                                 1.0 / numel, p->psf->ctx->stream));
+  if (p->kp == Precision::kF16) {
+    double sum = 0.0;
+    MLF_HIP(hipMemcpyAsync(&sum, p->dsum.ptr, sizeof(double), hipMemcpyDeviceToHost, p->psf->ctx->stream));
+    MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
+    MLF_CHECK(patch_set_scale(p, sum / numel));
+  }
   MLF_CHECK(patch_refresh_shadow(p));
   p->volume_set = true;
   return MLF_OK;
@@ -1111,6 +1150,11 @@ mlf_status mlf_patch_set_volume(mlf_patch p, const float* X) {
   MLF_CHECK(tmp.alloc(p->X.count));
   MLF_HIP(hipMemcpyAsync(tmp.ptr, X, tmp.bytes(), hipMemcpyHostToDevice, st));
   MLF_LAUNCH(launch_to_blocked(tmp.ptr, p->X.ptr, p->n1, p->n2, p->Z, st));
+  if (p->kp == Precision::kF16) {
+    double sum = 0.0;
+    for (size_t i = 0; i < p->X.count; ++i) sum += X[i];
+    MLF_CHECK(patch_set_scale(p, sum / (double)p->X.count));
+  }
   MLF_CHECK(patch_refresh_shadow(p));
   MLF_HIP(hipStreamSynchronize(st));
   p->volume_set = true;

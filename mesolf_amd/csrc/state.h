@@ -47,6 +47,7 @@ struct PrecPack {
   std::vector<int32_t> nchunk, epad;
   DevBuf<int64_t> d_woff, d_boff;
   DevBuf<int32_t> d_nchunk, d_epad;
+  DevBuf<float> d_fscale, d_finv, d_bscale;  // FP16: [V][Z] tap scales (empty otherwise)
 };
 
 // Backprojection-of-ones table for one grid-size class (FP64 sums of the taps
@@ -74,8 +75,8 @@ struct mlf_psf_s {
   mlf::DevBuf<float> fwd, bwd;        // ABI layout, FP32
   mlf::DevBuf<int32_t> d_rf, d_rb;
   mlf::DevBuf<int64_t> d_offf, d_offb;
-  mlf::PrecPack pack[2];
-  std::vector<std::unique_ptr<mlf::NormTable>> norms[2];
+  mlf::PrecPack pack[mlf::kPrecisions];
+  std::vector<std::unique_ptr<mlf::NormTable>> norms[mlf::kPrecisions];
   std::mutex mu;
   int refs = 0;         // live patches
   bool closed = false;
@@ -102,6 +103,7 @@ struct mlf_patch_s {
   bool tpz = false;                    // Toeplitz forward kernel
   int cw = 16;                         // Toeplitz: input columns per workgroup (4 * fwd_mt)
   int fwd_mt = 0, bwd_nt = 0;          // tile shape: forward columns per wave, backward N-tiles
+  float xs = 1.f;                      // FP16: shadow = X * xs (chosen so that X * xs ~ 1)
   mlf::DevBuf<float> partial;               // forward window partials
   mlf::DevBuf<uint16_t> Xb;                 // BF16 shadow of X, [lenslet][kShadowRow] (BF16 Toeplitz)
   mlf::DevBuf<int64_t> d_poff;
@@ -134,6 +136,12 @@ namespace mlf {
 void psf_free(mlf_psf psf);  // called with g_life_mu held
 mlf_status patch_filter(mlf_patch_s* p);
 mlf_status patch_refresh_shadow(mlf_patch_s* p);
+// Precision of the kernels for an ABI precision.
+inline Precision kernel_precision(mlf_precision p) {
+  return p == MLF_PRECISION_BF16 ? Precision::kBF16
+                                 : (p == MLF_PRECISION_FP16 ? Precision::kF16 : Precision::kF32);
+}
+void batch_mark_dirty(mlf_batch_s* b);  // a patch's slot descriptor changed
 // Device bytes a patch holds (its own buffers, not the PSF-wide tables), and
 // the same figure before creating one of that grid.
 size_t patch_device_bytes(const mlf_patch_s* p);

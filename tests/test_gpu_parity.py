@@ -18,7 +18,7 @@ pytestmark = pytest.mark.gpu
 FWD_TOL = 1e-5
 VOL_TOL = 1e-4
 # BF16 operands (FP32 accumulation, FP32 update): looser per-step bounds
-TOL = {"fp32": (FWD_TOL, VOL_TOL), "bf16": (4e-3, 2e-2)}
+TOL = {"fp32": (FWD_TOL, VOL_TOL), "bf16": (4e-3, 2e-2), "fp16": (1e-3, 5e-3)}
 
 
 @pytest.fixture(scope="module")
@@ -37,7 +37,7 @@ def _psf(ctx, tabs):
     return mesolf_amd.PSF.from_compact(ctx, tabs["uv"], tabs["weights"], tabs["rf"], tabs["fwd"])
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 @pytest.mark.parametrize("wl", [synth.TINY, synth.SMALL, synth.Workload("odd", 37, 21, (-20, 0, 12), 1, 1)],
                          ids=lambda w: w.name)
 def test_forward_projection_parity(ctx, wl, prec):
@@ -66,7 +66,7 @@ def test_volume_roundtrip_layout(ctx):
     assert np.array_equal(p.get_volume(f64=True), X.astype(np.float64))
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 def test_view_steps_parity(ctx, small_problem, prec):
     tabs, cp, truth, LF = small_problem
     wl = synth.SMALL
@@ -93,7 +93,7 @@ def test_sample_filter_parity(ctx):
     assert np.allclose(got.sum(axis=(0, 1)), X.astype(np.float64).sum(axis=(0, 1)), rtol=1e-6)
 
 
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 def test_rl_parity_small_two_iterations(ctx, small_problem, prec):
     tabs, cp, truth, LF = small_problem
     wl = synth.SMALL
@@ -215,16 +215,18 @@ def test_mex_operator_wide_kernel_is_a_shift():
 def cfg1_reference():
     wl = synth.CFG1
     tabs, cp, truth, LF = synthetic_problem(wl, seed=1)
-    ref = O.rl_compact(LF.astype(np.float64), cp, 8)
-    return tabs, LF, ref
+    inter = {}
+    ref = O.rl_compact(LF.astype(np.float64), cp, 8,
+                       callback=lambda it, X: inter.__setitem__(it + 1, X.copy()))
+    return tabs, LF, ref, inter
 
 
 @pytest.mark.slow
-@pytest.mark.parametrize("prec", ["fp32", "bf16"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 def test_rl_parity_cfg1_eight_iterations(ctx, cfg1_reference, prec):
     """cfg1 shapes (33x33 lenslets, 11 planes), 8 RL iterations."""
     wl = synth.CFG1
-    tabs, LF, ref = cfg1_reference
+    tabs, LF, ref, _ = cfg1_reference
     p = mesolf_amd.Patch(_psf(ctx, tabs), wl.n1, wl.n2, precision=prec)
     p.set_image(LF)
     p.init_volume()
@@ -232,7 +234,34 @@ def test_rl_parity_cfg1_eight_iterations(ctx, cfg1_reference, prec):
     got = p.get_volume(f64=True)
     err = rel_l2(got, ref)
     print(f"cfg1 8-iteration {prec} rel L2 vs FP64 oracle: {err:.3e}")
-    assert err <= {"fp32": VOL_TOL, "bf16": 1e-3}[prec]
+    assert err <= {"fp32": VOL_TOL, "bf16": 1e-3, "fp16": 1e-3}[prec]
+
+
+@pytest.mark.slow
+def test_precision_error_after_1_4_8_iterations(ctx, cfg1_reference):
+    """Relative L2 error vs the FP64 oracle after 1, 4 and 8 iterations for
+    every precision, side by side (33x33 lenslets, 11 planes)."""
+    wl = synth.CFG1
+    tabs, LF, _, inter = cfg1_reference
+    psf = _psf(ctx, tabs)
+    table = {}
+    for prec in ("fp32", "bf16", "fp16"):
+        p = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision=prec)
+        p.set_image(LF)
+        p.init_volume()
+        done = 0
+        for it in (1, 4, 8):
+            p.run(it - done)
+            done = it
+            table[(prec, it)] = rel_l2(p.get_volume(f64=True), inter[it])
+        p.close()
+    print("rel L2 vs FP64 oracle   1 it      4 it      8 it")
+    for prec in ("fp32", "bf16", "fp16"):
+        print(f"  {prec:5s}              " + "  ".join(f"{table[(prec, it)]:.2e}" for it in (1, 4, 8)))
+    for it in (1, 4, 8):
+        assert table[("fp32", it)] <= VOL_TOL
+        assert table[("bf16", it)] <= 1e-3
+        assert table[("fp16", it)] <= table[("bf16", it)]  # 3 more mantissa bits
 
 
 def test_handle_lifetimes_in_any_order():
