@@ -256,6 +256,23 @@ mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap
                                           int64_t height, int64_t n, const float* d_kernels,
                                           int64_t kw, int64_t nphase, float* d_out);
 
+/* All MEX projections of the seed-library builder in one call
+ * (gen_seed_library_local_with_ring_bg_same_size.m:263-318 calls the MEX 4
+ * times per plane and accumulates per segment on the host side):
+ *   out[:,:,s] = sum over slices i in [seg_ptr[s], seg_ptr[s+1]) of
+ *                mex_lfm_convolution_vec(slices[:,:,i], kernels[:,:,:,:,plane[i]])
+ * slices [width, height, nslices] single; plane [nslices] 0-based plane of
+ * each slice; seg_ptr [nseg+1] non-decreasing with seg_ptr[0] = 0 and
+ * seg_ptr[nseg] = nslices; kernels [kw, kw, 15, 15, Z] single; out
+ * [width, height, nseg] single (a segment without slices gets zeros).  Each
+ * slice's projection is formed on its own and then added in slice order, as
+ * the reference adds per-plane results.  Host pointers, synchronous. */
+mlf_status mlf_lfm_convolution_segments(const float* slices, int64_t width, int64_t height,
+                                        int64_t nslices, const int32_t* plane,
+                                        const int64_t* seg_ptr, int64_t nseg,
+                                        const float* kernels, int64_t kw, int64_t nphase,
+                                        int64_t Z, float* out, int32_t device);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,7 +6,8 @@ host-side mirror of the reference's MATLAB-facing interface.
 """
 from .lib import (  # noqa: F401
     Batch, BudgetExceeded, Context, InvalidInput, MesolfError, Patch, PSF, Server, Unsupported,
-    compact_block_layout, exported_symbols, load_library, mex_lfm_convolution_vec,
+    compact_block_layout, exported_symbols, lfm_convolution_segments, load_library,
+    mex_lfm_convolution_vec,
     recon_theory_phase_space_LFM_with_filter,
 )
 
@@ -14,4 +15,5 @@ __all__ = [
     "Context", "PSF", "Patch", "Batch", "Server", "MesolfError", "InvalidInput", "Unsupported", "BudgetExceeded",
     "load_library", "exported_symbols", "compact_block_layout",
     "recon_theory_phase_space_LFM_with_filter", "mex_lfm_convolution_vec",
+    "lfm_convolution_segments",
 ]

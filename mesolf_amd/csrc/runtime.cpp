@@ -1504,4 +1504,51 @@ mlf_status mlf_lfm_convolution_vec(const float* bitmap, int64_t width, int64_t h
   return MLF_OK;
 }
 
+mlf_status mlf_lfm_convolution_segments(const float* slices, int64_t width, int64_t height,
+                                        int64_t nslices, const int32_t* plane,
+                                        const int64_t* seg_ptr, int64_t nseg,
+                                        const float* kernels, int64_t kw, int64_t nphase,
+                                        int64_t Z, float* out, int32_t device) {
+  clear_error();
+  if (!plane || !seg_ptr || !kernels || !out || (nslices > 0 && !slices))
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (width < 1 || height < 1 || nslices < 0 || nseg < 1 || Z < 1)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad dimensions");
+  if (nphase != kN) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernels must be [kw,kw,15,15,Z]");
+  if (kw < 1 || kw > 2048) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernel width out of [1,2048]");
+  if (nseg > 65535) return set_error(MLF_ERR_INVALID_ARGUMENT, "too many segments (> 65535)");
+  if (seg_ptr[0] != 0 || seg_ptr[nseg] != nslices)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "seg_ptr must start at 0 and end at nslices");
+  for (int64_t s = 0; s < nseg; ++s)
+    if (seg_ptr[s + 1] < seg_ptr[s]) return set_error(MLF_ERR_INVALID_ARGUMENT, "seg_ptr decreases");
+  for (int64_t i = 0; i < nslices; ++i)
+    if (plane[i] < 0 || plane[i] >= Z)
+      return set_error(MLF_ERR_INVALID_ARGUMENT, "slice %lld: plane %d outside [0,%lld)", (long long)i,
+                       (int)plane[i], (long long)Z);
+  mlf_context ctx = nullptr;
+  MLF_CHECK(mlf_context_create(device, &ctx));
+  std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> guard(ctx, mlf_context_destroy);
+  hipStream_t st = ctx->stream;
+  const size_t img = (size_t)width * height;
+  DevBuf<float> din, dk, dout;
+  DevBuf<int32_t> dplane;
+  DevBuf<int64_t> dptr;
+  MLF_CHECK(din.alloc(std::max<size_t>(1, img * nslices)));
+  MLF_CHECK(dk.alloc((size_t)kw * kw * kP * Z));
+  MLF_CHECK(dout.alloc(img * nseg));
+  MLF_CHECK(dplane.alloc(std::max<int64_t>(1, nslices)));
+  MLF_CHECK(dptr.alloc(nseg + 1));
+  if (nslices > 0) {
+    MLF_HIP(hipMemcpyAsync(din.ptr, slices, img * nslices * sizeof(float), hipMemcpyHostToDevice, st));
+    MLF_HIP(hipMemcpyAsync(dplane.ptr, plane, nslices * sizeof(int32_t), hipMemcpyHostToDevice, st));
+  }
+  MLF_HIP(hipMemcpyAsync(dk.ptr, kernels, dk.bytes(), hipMemcpyHostToDevice, st));
+  MLF_HIP(hipMemcpyAsync(dptr.ptr, seg_ptr, dptr.bytes(), hipMemcpyHostToDevice, st));
+  MLF_LAUNCH(launch_mex_conv_segments(din.ptr, width, height, dplane.ptr, dptr.ptr, nseg, dk.ptr,
+                                      (int)kw, dout.ptr, st));
+  MLF_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, st));
+  MLF_HIP(hipStreamSynchronize(st));
+  return MLF_OK;
+}
+
 }  // extern "C"

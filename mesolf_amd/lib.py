@@ -121,6 +121,8 @@ _SIGNATURES = {
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
     "mlf_lfm_convolution_vec_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P]),
+    "mlf_lfm_convolution_segments": (C.c_int, [_F32P, _I64, _I64, _I64, _I32P, _I64P, _I64, _F32P,
+                                               _I64, _I64, _I64, _F32P, _I32]),
 }
 
 
@@ -571,4 +573,32 @@ def mex_lfm_convolution_vec(bitmap, kernels, device: int = 0):
     out = np.empty(b.shape, np.float32, order="F")
     _check(lib.mlf_lfm_convolution_vec(_fptr(b), W, Hh, n, _fptr(k), k.shape[0], 15, _fptr(out),
                                        int(device)))
+    return out
+
+
+def lfm_convolution_segments(slices, planes, seg_ptr, kernels, device: int = 0):
+    """All MEX projections of the seed-library builder in one call
+    (``gen_seed_library_local_with_ring_bg_same_size.m:263-318``).
+
+    slices: single [W, H, nslices]; planes: 0-based plane per slice;
+    seg_ptr: [nseg+1] slice ranges per segment; kernels: single
+    [kw, kw, 15, 15, Z].  Returns [W, H, nseg]: per segment, the sum over its
+    slices of ``mex_lfm_convolution_vec(slice, kernels[..., plane])``."""
+    lib = load_library()
+    s = _col(slices, np.float32)
+    if s.ndim == 2:
+        s = s[:, :, None]
+    K = _col(kernels, np.float32)
+    if K.ndim == 4:
+        K = K[..., None]
+    pl = np.ascontiguousarray(planes, dtype=np.int32)
+    sp = np.ascontiguousarray(seg_ptr, dtype=np.int64)
+    if s.ndim != 3 or K.ndim != 5 or K.shape[0] != K.shape[1] or pl.size != s.shape[2] or sp.ndim != 1:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "bad argument shapes")
+    nseg = sp.size - 1
+    out = np.empty((s.shape[0], s.shape[1], max(nseg, 0)), np.float32, order="F")
+    _check(lib.mlf_lfm_convolution_segments(_fptr(s), s.shape[0], s.shape[1], s.shape[2],
+                                            _fptr(pl, C.c_int32), _fptr(sp, C.c_int64), nseg,
+                                            _fptr(K), K.shape[0], K.shape[2], K.shape[4],
+                                            _fptr(out), int(device)))
     return out

@@ -65,6 +65,17 @@ def test_c_abi_validates_before_device():
     assert b"Xguessv" in lib.mlf_last_error()
 
 
+def test_segment_projection_validates_before_device():
+    s = np.zeros((8, 8, 3), np.float32)
+    K = np.zeros((3, 3, 15, 15, 2), np.float32)
+    with pytest.raises(mesolf_amd.InvalidInput, match="seg_ptr"):
+        mesolf_amd.lfm_convolution_segments(s, [0, 1, 1], [0, 2, 1, 3], K)   # decreasing
+    with pytest.raises(mesolf_amd.InvalidInput, match="seg_ptr"):
+        mesolf_amd.lfm_convolution_segments(s, [0, 1, 1], [0, 2], K)         # does not end at 3
+    with pytest.raises(mesolf_amd.InvalidInput, match="plane"):
+        mesolf_amd.lfm_convolution_segments(s, [0, 2, 1], [0, 3], K)         # plane 2 of Z = 2
+
+
 def test_no_device_is_an_error_not_a_fallback():
     """Without a GPU every compute entry point fails loudly."""
     try:

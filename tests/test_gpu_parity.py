@@ -198,6 +198,32 @@ def test_mex_operator_parity(W, H, n, kw):
     assert rel_l2(got, ref) <= 1e-5
 
 
+def test_segment_projection_matches_summed_mex_calls():
+    """One call for all segments and planes == per-plane MEX calls summed per
+    segment (gen_seed_library_local_with_ring_bg_same_size.m:263-318)."""
+    rng = np.random.default_rng(12)
+    W, H, Z, kw = 45, 38, 4, 9
+    K = rng.random((kw, kw, 15, 15, Z)).astype(np.float32)
+    planes = [0, 2, 3, 1, 2, 0, 1]
+    seg_ptr = [0, 3, 3, 5, 7]               # segment 1 has no slices
+    sl = rng.random((W, H, len(planes))).astype(np.float32)
+    got = mesolf_amd.lfm_convolution_segments(sl, planes, seg_ptr, K)
+    assert got.shape == (W, H, 4)
+    for sgi in range(4):
+        ref = np.zeros((W, H))
+        for i in range(seg_ptr[sgi], seg_ptr[sgi + 1]):
+            ref += O.mex_op(sl[:, :, i].astype(np.float64), K[..., planes[i]].astype(np.float64))
+        if seg_ptr[sgi] == seg_ptr[sgi + 1]:
+            assert np.all(got[:, :, sgi] == 0)
+        else:
+            assert rel_l2(got[:, :, sgi], ref) <= 1e-5
+        # and equal to the drop-in MEX called per slice, added in order
+        acc = np.zeros((W, H), np.float32)
+        for i in range(seg_ptr[sgi], seg_ptr[sgi + 1]):
+            acc += mesolf_amd.mex_lfm_convolution_vec(sl[:, :, i], K[..., planes[i]])
+        assert np.array_equal(got[:, :, sgi], acc)
+
+
 def test_mex_operator_wide_kernel_is_a_shift():
     """kw = 401 (> the reference's 341 limit): a single-tap kernel shifts."""
     kw, h = 401, 200
