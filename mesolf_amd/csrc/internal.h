@@ -120,12 +120,14 @@ struct FilterPlan {
   const int64_t* off_2 = nullptr;
   const float* taps_2 = nullptr;
 };
+// maxlen: the longest filter (taps) of the plan, sizes the LDS line buffer.
 hipError_t launch_filter_pass1(const float* X, float* tmp, FilterPlan fp, int n1, int n2,
-                               double* part_before, int nblk, hipStream_t st);
+                               double* part_before, int maxlen, hipStream_t st);
 hipError_t launch_filter_pass2(const float* tmp, float* X, FilterPlan fp, int n1, int n2,
-                               double* part_after, int nblk, hipStream_t st);
+                               double* part_after, int maxlen, hipStream_t st);
 hipError_t launch_filter_scale(float* X, FilterPlan fp, int n1, int n2, const double* part_before,
-                               const double* part_after, int nblk, hipStream_t st);
+                               const double* part_after, hipStream_t st);
+// FP64 partial-sum slots per filtered plane (workgroups of the larger pass).
 int filter_partial_blocks(int n1, int n2);
 
 // MEX operator (mex:43-266).

@@ -222,77 +222,102 @@ __device__ __forceinline__ double block_sum_f64(double v, double* sh) {
 
 // Pass 1: tmp[q][p2][p1] = sum_j k1[j] * X[(p1 - n0 - j) mod I1, p2] for each
 // filtered plane q; also per-block FP64 partial sums of the input plane.
-__global__ void k_filter_pass1(const float* __restrict__ X, float* __restrict__ tmp, FilterPlan fp,
-                               int n1, int n2, double* __restrict__ part_before) {
-  __shared__ double sh[kFilterThreads / 64];
-  const int q = blockIdx.y;
+// One separable pass of sample_filter (recon:222-240) along axis AXIS (1: the
+// first, contiguous image axis; 2: the second), circular:
+//   out[p] = sum_j k[j] * in[(p - n0 - j) mod I]
+// A workgroup owns kLineGroup lenslets along the axis at one lenslet position
+// of the other axis: it copies the lenslet blocks it needs (whole 225-float
+// blocks, including the wrap-around halo) into LDS and each thread produces 8
+// consecutive outputs of one phase row with a sliding register window (one
+// LDS read per tap).  Taps are summed in the same order as the direct form.
+// Pass 1 reads the volume plane and writes the temporary (same lenslet-blocked
+// layout) and sums its input; pass 2 reads the temporary, writes the volume
+// and sums its output; both sums are FP64 per workgroup.
+constexpr int kLineGroup = 8;  // output lenslets per workgroup along the axis
+
+__device__ __forceinline__ int floor_div(int a, int b) { return (a >= 0) ? a / b : -((-a + b - 1) / b); }
+
+template <int AXIS>
+__global__ __launch_bounds__(256) void k_filter_line(float* __restrict__ X, float* __restrict__ tmp,
+                                                     FilterPlan fp, int n1, int n2,
+                                                     double* __restrict__ part) {
+  extern __shared__ float lds[];  // [nl][225]
+  __shared__ double sh[4];
+  const int q = blockIdx.z;
   const int z = fp.plane[q];
-  const int I1 = n1 * kN, I2 = n2 * kN;
-  const size_t plane = (size_t)I1 * I2;
-  const int n0 = fp.n0_1[q], len = fp.len_1[q];
-  const float* k1 = fp.taps_1 + fp.off_1[q];
-  double own = 0.0;
-  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane;
-       o += (size_t)gridDim.x * blockDim.x) {
-    const int p1 = o % I1, p2 = (int)(o / I1);
-    own += (double)X[blk_index(p1, p2, z, n1, n2)];
-    int src = p1 - n0;
-    src %= I1;
-    if (src < 0) src += I1;
-    float acc = 0.f;
-    for (int j = 0; j < len; ++j) {
-      acc = fmaf(k1[j], X[blk_index(src, p2, z, n1, n2)], acc);
-      src = (src == 0) ? I1 - 1 : src - 1;
-    }
-    tmp[(size_t)q * plane + o] = acc;
+  const size_t plane = (size_t)n1 * n2 * kP;
+  const float* src = AXIS == 1 ? X + (size_t)z * plane : tmp + (size_t)q * plane;
+  float* dst = AXIS == 1 ? tmp + (size_t)q * plane : X + (size_t)z * plane;
+  const int na = AXIS == 1 ? n1 : n2;
+  const int Ia = na * kN;
+  const int lo = blockIdx.y;  // lenslet on the other axis
+  const int n0 = AXIS == 1 ? fp.n0_1[q] : fp.n0_2[q];
+  const int len = AXIS == 1 ? fp.len_1[q] : fp.len_2[q];
+  const float* __restrict__ taps = AXIS == 1 ? fp.taps_1 + fp.off_1[q] : fp.taps_2 + fp.off_2[q];
+  const int P0 = blockIdx.x * kLineGroup * kN;
+  const int P1 = min(P0 + kLineGroup * kN, Ia);
+  const int L0 = floor_div(P0 - n0 - len + 1, kN);
+  const int nl = floor_div(P1 - 1 - n0, kN) - L0 + 1;
+  auto lens_of = [&](int L) -> size_t {
+    L %= na;
+    if (L < 0) L += na;
+    return AXIS == 1 ? (size_t)lo * n1 + L : (size_t)L * n1 + lo;
+  };
+  for (int o = threadIdx.x; o < nl * kP; o += blockDim.x) {
+    const int li = o / kP;
+    lds[o] = src[lens_of(L0 + li) * kP + (o - li * kP)];
   }
-  const double s = block_sum_f64(own, sh);
-  if (threadIdx.x == 0) part_before[(size_t)q * gridDim.x + blockIdx.x] = s;
+  __syncthreads();
+  // unwrapped pixel u (from lenslet L0) at other-axis phase so -> LDS
+  const int so = threadIdx.x % kN, g = threadIdx.x / kN;
+  const int umax = nl * kN - 1;
+  auto val = [&](int u) {
+    u = u < 0 ? 0 : (u > umax ? umax : u);
+    const int li = u / kN, sa = u - li * kN;
+    return lds[li * kP + (AXIS == 1 ? so * kN + sa : sa * kN + so)];
+  };
+  double own = 0.0;
+  if (g < kLineGroup * kN / 8) {
+    const int pb = P0 + 8 * g;
+    const int ub = pb - n0 - L0 * kN;
+    float w[8], acc[8];
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      w[m] = val(ub + m);
+      acc[m] = 0.f;
+    }
+    for (int j = 0; j < len; ++j) {
+      const float kj = taps[j];
+#pragma unroll
+      for (int m = 0; m < 8; ++m) acc[m] = fmaf(kj, w[m], acc[m]);
+#pragma unroll
+      for (int m = 7; m > 0; --m) w[m] = w[m - 1];
+      w[0] = val(ub - j - 1);
+    }
+#pragma unroll
+    for (int m = 0; m < 8; ++m) {
+      const int p = pb + m;
+      if (p >= P1) break;
+      const int L = p / kN, sa = p - L * kN;
+      const size_t e = lens_of(L) * kP + (AXIS == 1 ? so * kN + sa : sa * kN + so);
+      own += AXIS == 1 ? (double)src[e] : (double)acc[m];
+      dst[e] = acc[m];
+    }
+  }
+  const double sum = block_sum_f64(own, sh);
+  if (threadIdx.x == 0) part[(size_t)q * gridDim.x * gridDim.y + blockIdx.y * gridDim.x + blockIdx.x] = sum;
 }
 
-// Pass 2: X[z][p1,p2] = sum_j k2[j] * tmp[q][p1, (p2 - n0 - j) mod I2].
-__global__ void k_filter_pass2(const float* __restrict__ tmp, float* __restrict__ X, FilterPlan fp,
-                               int n1, int n2, double* __restrict__ part_after) {
-  __shared__ double sh[kFilterThreads / 64];
-  const int q = blockIdx.y;
-  const int z = fp.plane[q];
-  const int I1 = n1 * kN, I2 = n2 * kN;
-  const size_t plane = (size_t)I1 * I2;
-  const int n0 = fp.n0_2[q], len = fp.len_2[q];
-  const float* k2 = fp.taps_2 + fp.off_2[q];
-  const float* tq = tmp + (size_t)q * plane;
-  double own = 0.0;
-  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane;
-       o += (size_t)gridDim.x * blockDim.x) {
-    const int p1 = o % I1, p2 = (int)(o / I1);
-    int src = p2 - n0;
-    src %= I2;
-    if (src < 0) src += I2;
-    float acc = 0.f;
-    for (int j = 0; j < len; ++j) {
-      acc = fmaf(k2[j], tq[(size_t)p1 + (size_t)I1 * src], acc);
-      src = (src == 0) ? I2 - 1 : src - 1;
-    }
-    X[blk_index(p1, p2, z, n1, n2)] = acc;
-    own += (double)acc;
-  }
-  const double s = block_sum_f64(own, sh);
-  if (threadIdx.x == 0) part_after[(size_t)q * gridDim.x + blockIdx.x] = s;
-}
-
-// Pass 3: X[z] *= sum_before / sum_after (FP64 plane energies, fixed order).
 __global__ void k_filter_scale(float* __restrict__ X, FilterPlan fp, int n1, int n2,
                                const double* __restrict__ pb, const double* __restrict__ pa,
-                               int nblk) {
+                               int nb_before, int nb_after) {
   __shared__ double scale;
   const int q = blockIdx.y;
   const int z = fp.plane[q];
   if (threadIdx.x == 0) {
     double b = 0.0, a = 0.0;
-    for (int k = 0; k < nblk; ++k) {
-      b += pb[(size_t)q * nblk + k];
-      a += pa[(size_t)q * nblk + k];
-    }
+    for (int k = 0; k < nb_before; ++k) b += pb[(size_t)q * nb_before + k];
+    for (int k = 0; k < nb_after; ++k) a += pa[(size_t)q * nb_after + k];
     scale = b / a;
   }
   __syncthreads();
@@ -382,29 +407,35 @@ hipError_t launch_backward_update_fp32(float* X, const float* Wb, PlaneTables t,
   return hipGetLastError();
 }
 
+static int filter_groups(int n) { return (n + kLineGroup - 1) / kLineGroup; }
+
 int filter_partial_blocks(int n1, int n2) {
-  const size_t plane = (size_t)n1 * n2 * kP;
-  return grid_for(plane, kFilterThreads, 512);
+  return std::max(filter_groups(n1) * n2, filter_groups(n2) * n1);
+}
+
+static size_t filter_lds(const FilterPlan& fp, int maxlen) {
+  const int nl = kLineGroup + (maxlen + 2 * kN) / kN + 1;
+  return sizeof(float) * (size_t)nl * kP;
 }
 
 hipError_t launch_filter_pass1(const float* X, float* tmp, FilterPlan fp, int n1, int n2,
-                               double* part_before, int nblk, hipStream_t st) {
-  hipLaunchKernelGGL(k_filter_pass1, dim3(nblk, fp.nplanes), dim3(kFilterThreads), 0, st, X, tmp,
-                     fp, n1, n2, part_before);
+                               double* part_before, int maxlen, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter_line<1>, dim3(filter_groups(n1), n2, fp.nplanes), dim3(256),
+                     filter_lds(fp, maxlen), st, const_cast<float*>(X), tmp, fp, n1, n2, part_before);
   return hipGetLastError();
 }
 
 hipError_t launch_filter_pass2(const float* tmp, float* X, FilterPlan fp, int n1, int n2,
-                               double* part_after, int nblk, hipStream_t st) {
-  hipLaunchKernelGGL(k_filter_pass2, dim3(nblk, fp.nplanes), dim3(kFilterThreads), 0, st, tmp, X,
-                     fp, n1, n2, part_after);
+                               double* part_after, int maxlen, hipStream_t st) {
+  hipLaunchKernelGGL(k_filter_line<2>, dim3(filter_groups(n2), n1, fp.nplanes), dim3(256),
+                     filter_lds(fp, maxlen), st, X, const_cast<float*>(tmp), fp, n1, n2, part_after);
   return hipGetLastError();
 }
 
 hipError_t launch_filter_scale(float* X, FilterPlan fp, int n1, int n2, const double* part_before,
-                               const double* part_after, int nblk, hipStream_t st) {
+                               const double* part_after, hipStream_t st) {
   hipLaunchKernelGGL(k_filter_scale, dim3(256, fp.nplanes), dim3(256), 0, st, X, fp, n1, n2,
-                     part_before, part_after, nblk);
+                     part_before, part_after, filter_groups(n1) * n2, filter_groups(n2) * n1);
   return hipGetLastError();
 }
 

@@ -732,6 +732,7 @@ static mlf_status build_filter_plan(mlf_patch_s* p) {
     t1.insert(t1.end(), ta.begin(), ta.end());
     n02.push_back(b0);
     len2.push_back((int32_t)tbv.size());
+    p->f_maxlen = std::max(p->f_maxlen, (int)std::max(ta.size(), tbv.size()));
     off2.push_back((int64_t)t2.size());
     t2.insert(t2.end(), tbv.begin(), tbv.end());
   }
@@ -982,9 +983,9 @@ mlf_status mlf::patch_filter(mlf_patch_s* p) {
   hipStream_t st = p->psf->ctx->stream;
   const FilterPlan fp = filter_plan_of(p);
   MLF_CHECK(tim_begin(p, 3));
-  MLF_LAUNCH(launch_filter_pass1(p->X.ptr, p->tmp.ptr, fp, p->n1, p->n2, p->pb.ptr, p->nblk, st));
-  MLF_LAUNCH(launch_filter_pass2(p->tmp.ptr, p->X.ptr, fp, p->n1, p->n2, p->pa.ptr, p->nblk, st));
-  MLF_LAUNCH(launch_filter_scale(p->X.ptr, fp, p->n1, p->n2, p->pb.ptr, p->pa.ptr, p->nblk, st));
+  MLF_LAUNCH(launch_filter_pass1(p->X.ptr, p->tmp.ptr, fp, p->n1, p->n2, p->pb.ptr, p->f_maxlen, st));
+  MLF_LAUNCH(launch_filter_pass2(p->tmp.ptr, p->X.ptr, fp, p->n1, p->n2, p->pa.ptr, p->f_maxlen, st));
+  MLF_LAUNCH(launch_filter_scale(p->X.ptr, fp, p->n1, p->n2, p->pb.ptr, p->pa.ptr, st));
   MLF_CHECK(patch_refresh_shadow(p));
   MLF_CHECK(tim_end(p));
   return MLF_OK;

@@ -91,7 +91,7 @@ struct mlf_patch_s {
   mlf::DevBuf<float> X, LF, part, R, y, tmp;
   mlf::DevBuf<double> pb, pa, dsum;
   // filter plan
-  int f_nplanes = 0, nblk = 0;
+  int f_nplanes = 0, nblk = 0, f_maxlen = 1;
   mlf::DevBuf<int32_t> f_plane, f_n0_1, f_len_1, f_n0_2, f_len_2;
   mlf::DevBuf<int64_t> f_off_1, f_off_2;
   mlf::DevBuf<float> f_taps_1, f_taps_2;
