@@ -133,6 +133,12 @@ int filter_partial_blocks(int n1, int n2);
 // MEX operator (mex:43-266).
 hipError_t launch_mex_conv(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
                            float* out, hipStream_t st);
+// MFMA form of the MEX operator for odd kw (clipping == zero padding); Kq is
+// a scratch of mex_kq_floats(kw) floats (the per-phase-row GEMM operand).
+constexpr int kMexMfmaMaxKw = 511;
+size_t mex_kq_floats(int kw);
+hipError_t launch_mex_mfma(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
+                           float* Kq, float* out, hipStream_t st);
 // out[s] = sum_{i in [seg_ptr[s], seg_ptr[s+1])} MEX(in[i], K[plane[i]]); K is [Z][15][15][kw][kw].
 hipError_t launch_mex_conv_segments(const float* in, int64_t W, int64_t H, const int32_t* plane,
                                     const int64_t* seg_ptr, int64_t nseg, const float* K, int kw,

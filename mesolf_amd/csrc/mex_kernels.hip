@@ -109,7 +109,106 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_mex_conv_segments(
   if (valid) out[(size_t)seg * W * H + (size_t)y * W + x] = total;
 }
 
+// ---------------------------------------------------------------- MFMA form
+// For odd kw the clipping equals zero padding, and grouping outputs by phase
+// turns the operator into GEMMs (FP32 MFMA, FP32 accumulation).  For output
+// row y (q = y % 15), image n and x = 15 i + p:
+//   out = sum_{ky, t} in[15 i + t - h, y - h + ky, n] * Kq[q][ky][t][p],
+//   Kq[q][ky][t][p] = K[t - p, ky, p, q] for 0 <= t - p < kw (else 0),
+// i.e. M = lenslet columns i, N = the 15 phases p (padded to 16), K = (ky, t).
+// A workgroup takes one output row and kMexImg images (a wave each) over 32
+// lenslet columns (two M-tiles); input rows are staged in LDS kMexCK at a
+// time, Kq fragments stream from L2 (every workgroup of the row's q reads the
+// same 64 consecutive floats per K-step).
+typedef float mex_floatx4 __attribute__((ext_vector_type(4)));
+constexpr int kMexImg = 4;
+constexpr int kMexCK = 2;
+constexpr int kMexCols = 32;  // lenslet columns per workgroup (2 M-tiles)
+
+__global__ void k_mex_kq(const float* __restrict__ K, int kw, int T4, float* __restrict__ Kq) {
+  const size_t total = (size_t)kN * kw * T4 * 16;
+  for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total;
+       o += (size_t)gridDim.x * blockDim.x) {
+    const int p = (int)(o % 16);
+    const int t = (int)((o / 16) % T4);
+    const int ky = (int)((o / (16 * (size_t)T4)) % kw);
+    const int q = (int)(o / (16 * (size_t)T4 * kw));
+    const int kx = t - p;
+    Kq[o] = (p < kN && kx >= 0 && kx < kw)
+                ? K[(size_t)kx + (size_t)kw * ky + (size_t)kw * kw * (p + kN * q)]
+                : 0.f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_mex_mfma(const float* __restrict__ in, int W, int H, int n,
+                                                  const float* __restrict__ Kq, int kw, int T4,
+                                                  float* __restrict__ out) {
+  extern __shared__ float xs[];  // [kMexCK][kMexImg][XL]
+  const int y = blockIdx.x, ng = blockIdx.y, ib = blockIdx.z;
+  const int q = y % kN, h = kw / 2;
+  const int i0 = kMexCols * ib;
+  const int xlo = kN * i0 - h;
+  const int XL = kN * kMexCols + T4;
+  const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+  const int img = ng * kMexImg + w;
+  mex_floatx4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+  const float* Kqq = Kq + (size_t)q * kw * T4 * 16;
+  for (int c0 = 0; c0 < kw; c0 += kMexCK) {
+    __syncthreads();  // previous chunk consumed
+    for (int o = tid; o < kMexCK * kMexImg * XL; o += 256) {
+      const int xi = o % XL;
+      const int im = (o / XL) % kMexImg;
+      const int kk = o / (XL * kMexImg);
+      const int r = y - h + c0 + kk, x = xlo + xi, nn = ng * kMexImg + im;
+      float v = 0.f;
+      if (c0 + kk < kw && r >= 0 && r < H && x >= 0 && x < W && nn < n)
+        v = in[(size_t)nn * W * H + (size_t)r * W + x];
+      xs[o] = v;
+    }
+    __syncthreads();
+    for (int kk = 0; kk < kMexCK && c0 + kk < kw; ++kk) {
+      const float* Bq = Kqq + (size_t)(c0 + kk) * T4 * 16 + lg * 16 + lr;
+      const float* Ar = xs + (kk * kMexImg + w) * XL + kN * lr + lg;
+#pragma unroll 8
+      for (int s4 = 0; s4 < T4; s4 += 4) {
+        const float b = Bq[s4 * 16];
+        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[s4], b, acc[0], 0, 0, 0);
+        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[s4 + kN * 16], b, acc[1], 0, 0, 0);
+      }
+    }
+  }
+  if (img >= n || lr >= kN) return;
+#pragma unroll
+  for (int it = 0; it < 2; ++it)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int x = kN * (i0 + 16 * it + 4 * lg + j) + lr;
+      if (x < W) out[(size_t)img * W * H + (size_t)y * W + x] = acc[it][j];
+    }
+}
+
 }  // namespace
+
+size_t mex_kq_floats(int kw) {
+  const int T4 = (kw + kN - 1 + 3) / 4 * 4;
+  return (size_t)kN * kw * T4 * 16;
+}
+
+hipError_t launch_mex_mfma(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
+                           float* Kq, float* out, hipStream_t st) {
+  const int T4 = (kw + kN - 1 + 3) / 4 * 4;
+  hipLaunchKernelGGL(k_mex_kq, dim3(1024), dim3(256), 0, st, K, kw, T4, Kq);
+  const int XL = kN * kMexCols + T4;
+  const size_t shmem = sizeof(float) * kMexCK * kMexImg * XL;
+  static hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mex_mfma),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (attr != hipSuccess) return attr;
+  dim3 grid((unsigned)H, (unsigned)((n + kMexImg - 1) / kMexImg),
+            (unsigned)((W / kN + 1 + kMexCols - 1) / kMexCols));
+  hipLaunchKernelGGL(k_mex_mfma, grid, dim3(256), shmem, st, in, (int)W, (int)H, (int)n, Kq, kw, T4,
+                     out);
+  return hipGetLastError();
+}
 
 hipError_t launch_mex_conv_segments(const float* in, int64_t W, int64_t H, const int32_t* plane,
                                     const int64_t* seg_ptr, int64_t nseg, const float* K, int kw,

@@ -1474,6 +1474,18 @@ mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap
     return set_error(MLF_ERR_INVALID_ARGUMENT, "bitmap too large");
   if (n > 65535) return set_error(MLF_ERR_INVALID_ARGUMENT, "batch too large (n > 65535)");
   MLF_CHECK(ctx_activate(ctx));
+  if (kw % 2 == 1 && kw <= kMexMfmaMaxKw) {
+    // odd kernels: the MFMA form; its operand scratch lives in the context
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const size_t need = mex_kq_floats((int)kw);
+    if (ctx->mex_kq.count < need) {
+      MLF_HIP(hipStreamSynchronize(ctx->stream));  // earlier calls may still read it
+      MLF_CHECK(ctx->mex_kq.alloc(need));
+    }
+    MLF_LAUNCH(launch_mex_mfma(d_bitmap, width, height, n, d_kernels, (int)kw, ctx->mex_kq.ptr,
+                               d_out, ctx->stream));
+    return MLF_OK;
+  }
   MLF_LAUNCH(launch_mex_conv(d_bitmap, width, height, n, d_kernels, (int)kw, d_out, ctx->stream));
   return MLF_OK;
 }

@@ -17,6 +17,7 @@
 struct mlf_context_s {
   int device = 0;
   hipStream_t stream = nullptr;
+  mlf::DevBuf<float> mex_kq;  // MEX operator GEMM operand, reused across calls (guarded by mu)
   std::mutex mu;
   int refs = 0;         // live PSFs created on this context
   bool closed = false;  // destroy requested; freed when refs reaches 0
