@@ -284,14 +284,46 @@ def bench_serving(args):
     ctx.close()
 
 
+def bench_mex(args):
+    """The MEX operator at the reference demo's shape (mex:276; 396 x 396
+    bitmaps, 144 per call, kw = 241): host arrays in and out, as the MATLAB
+    gateway calls it."""
+    import mesolf_amd
+
+    W = H = 396
+    n, kw = 144, 241
+    rng = np.random.default_rng(11)
+    bitmap = (rng.random((W, H, n)) < 0.005).astype(np.float32) * rng.random((W, H, n)).astype(np.float32)
+    K = rng.random((kw, kw, 15, 15)).astype(np.float32)
+    mesolf_amd.mex_lfm_convolution_vec(bitmap[:, :, :4], K)  # warm-up (operand table, code)
+    times = []
+    for _ in range(max(1, args.steps)):
+        t = time.perf_counter()
+        mesolf_amd.mex_lfm_convolution_vec(bitmap, K)
+        times.append(time.perf_counter() - t)
+    dt = min(times)
+    mac = W * H * n * kw * kw
+    print(json.dumps({
+        "metric": "MEX operator throughput, 396x396 bitmaps x 144, kw = 241 (host arrays in/out)",
+        "value": mac / dt / 1e12, "unit": "TMAC/s", "n_gpus": 1, "steps": len(times), "warmup": 1,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (MFMA, f32 accumulate)", "data": "synthetic sparse bitmaps, random kernels",
+        "config": {"workload": "mex_demo_396x396x144_kw241"},
+        "roofline": {"bound": "mfma", "achieved": 2 * mac / dt / 1e12, "peak": PEAK["fp32"] / 1e12,
+                     "unit": "TFLOP/s", "frac": 2 * mac / dt / PEAK["fp32"],
+                     "note": "call time incl. 2 x 90 MB host transfers"},
+    }), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5"],
-                    help="cfg4: batched movie of 12 uneven patches; cfg5: serving loop")
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5", "mex"],
+                    help="cfg4: batched movie of 12 uneven patches; cfg5: serving loop; "
+                         "mex: the MEX operator at the demo shape")
     ap.add_argument("--patches", type=int, default=0, help="cfg4/cfg5: patch count (12 / 36)")
     ap.add_argument("--frames", type=int, default=64, help="cfg4: movie length")
     ap.add_argument("--timed-frames", type=int, default=6, help="cfg4: warm frames timed")
@@ -315,6 +347,8 @@ def main():
                           "(sharding across GPUs is described in DESIGN.md, not implemented)"}))
         sys.exit(2)
 
+    if args.workload == "mex":
+        return bench_mex(args)
     if args.workload == "cfg4":
         args.patches = args.patches or 12
         return bench_batched(args)

@@ -312,14 +312,16 @@ __global__ void k_filter_scale(float* __restrict__ X, FilterPlan fp, int n1, int
                                const double* __restrict__ pb, const double* __restrict__ pa,
                                int nb_before, int nb_after) {
   __shared__ double scale;
+  __shared__ double sh[4];
   const int q = blockIdx.y;
   const int z = fp.plane[q];
-  if (threadIdx.x == 0) {
-    double b = 0.0, a = 0.0;
-    for (int k = 0; k < nb_before; ++k) b += pb[(size_t)q * nb_before + k];
-    for (int k = 0; k < nb_after; ++k) a += pa[(size_t)q * nb_after + k];
-    scale = b / a;
-  }
+  // plane energies: strided partial sums per thread, then a fixed-order tree
+  double b = 0.0, a = 0.0;
+  for (int k = threadIdx.x; k < nb_before; k += blockDim.x) b += pb[(size_t)q * nb_before + k];
+  for (int k = threadIdx.x; k < nb_after; k += blockDim.x) a += pa[(size_t)q * nb_after + k];
+  b = block_sum_f64(b, sh);
+  a = block_sum_f64(a, sh);
+  if (threadIdx.x == 0) scale = b / a;
   __syncthreads();
   const double s = scale;
   const size_t plane = (size_t)n1 * n2 * kP;
