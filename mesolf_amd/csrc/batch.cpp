@@ -104,8 +104,18 @@ mlf_status rebuild(mlf_batch_s* b) {
   std::stable_sort(fe.begin(), fe.end(), by_r);
   std::stable_sort(be.begin(), be.end(), by_r);
   std::vector<int32_t> fs, bs;
-  for (const Ent& e : fe) fs.insert(fs.end(), e.v, e.v + 4);
-  for (const Ent& e : be) bs.insert(bs.end(), e.v, e.v + 5);
+  std::vector<int> fg, bg;  // XCD groups: (slot, plane) and (slot, plane, phase group)
+  const int ngrp = kPhiPadBwd / (16 * std::max(1, b->bwd_nt));
+  for (const Ent& e : fe) {
+    fs.insert(fs.end(), e.v, e.v + 4);
+    fg.push_back(e.v[0] * ps->Z + e.v[1]);
+  }
+  for (const Ent& e : be) {
+    bs.insert(bs.end(), e.v, e.v + 5);
+    bg.push_back((e.v[0] * ps->Z + e.v[1]) * ngrp + e.v[4]);
+  }
+  xcd_interleave(fs, 4, fg);
+  xcd_interleave(bs, 5, bg);
   b->n_f = (int)fe.size();
   b->n_b = (int)be.size();
   b->n_r = (int)(rs.size() / 2);

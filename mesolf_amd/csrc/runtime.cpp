@@ -868,6 +868,33 @@ static int64_t partial_elems(const mlf_psf_s* ps, bool tpz, int cw, int bm2, int
   return po;
 }
 
+void mlf::xcd_interleave(std::vector<int32_t>& ent, int width, const std::vector<int>& group) {
+  const size_t n = ent.size() / width;
+  std::vector<std::vector<size_t>> lists(kXcds);
+  std::vector<int> rank_of;  // group id -> rank, by first appearance
+  std::vector<int> seen;
+  for (size_t e = 0; e < n; ++e) {
+    const int gid = group[e];
+    if (gid >= (int)rank_of.size()) rank_of.resize(gid + 1, -1);
+    if (rank_of[gid] < 0) rank_of[gid] = (int)seen.size(), seen.push_back(gid);
+    const int g = rank_of[gid];
+    const int x = ((g / kXcds) % 2 == 0) ? g % kXcds : kXcds - 1 - g % kXcds;
+    lists[x].push_back(e);
+  }
+  std::vector<int32_t> out;
+  out.reserve(ent.size());
+  for (size_t k = 0;; ++k) {
+    bool any = false;
+    for (int x = 0; x < kXcds; ++x)
+      if (k < lists[x].size()) {
+        any = true;
+        out.insert(out.end(), ent.begin() + lists[x][k] * width, ent.begin() + (lists[x][k] + 1) * width);
+      }
+    if (!any) break;
+  }
+  ent.swap(out);
+}
+
 // Prefix each `width`-int entry of `src` with a slot index.
 static void append_entries(const std::vector<int32_t>& src, int width, int slot,
                            std::vector<int32_t>* dst) {
@@ -914,6 +941,11 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
         p->fent.push_back(b2);
       }
   p->nblk_f = (int)(p->fent.size() / 3);
+  {  // one plane per XCD at a time: its blocks share the plane's taps and halo rows
+    std::vector<int> grp(p->nblk_f);
+    for (int e = 0; e < p->nblk_f; ++e) grp[e] = p->fent[3 * e];
+    xcd_interleave(p->fent, 3, grp);
+  }
   std::vector<int> border(Z);
   for (int z = 0; z < Z; ++z) border[z] = z;
   std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return ps->rb[a] > ps->rb[b]; });
@@ -930,6 +962,11 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
           p->bent.push_back(gq);
         }
   p->nblk_b = (int)(p->bent.size() / 4);
+  {  // one (plane, phase group) per XCD at a time: shared tap table
+    std::vector<int> grp(p->nblk_b);
+    for (int e = 0; e < p->nblk_b; ++e) grp[e] = p->bent[4 * e] * ngrp + p->bent[4 * e + 3];
+    xcd_interleave(p->bent, 4, grp);
+  }
   p->nblk_r = (p->n1 * p->n2 + 255) / 256;
   std::vector<int32_t> fs, bs, rs;
   append_entries(p->fent, 3, 0, &fs);

@@ -148,6 +148,15 @@ void batch_mark_dirty(mlf_batch_s* b);  // a patch's slot descriptor changed
 size_t patch_device_bytes(const mlf_patch_s* p);
 size_t patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int n2);
 mlf_status batch_forget_patch(mlf_patch_s* p);  // detach before destruction
+// Workgroups go to the 8 XCDs (each with its own L2) round-robin by id.
+// Reorder `width`-int entries so that each group of entries (same `group[e]`,
+// groups ranked in order of first appearance) runs on one XCD: group rank g
+// goes to XCD g % 8 for even rounds g / 8 and 7 - g % 8 for odd ones (a
+// snake, so heavy and light groups spread evenly), and the k-th entry of
+// XCD x's list lands at position x + 8 k.  Entries keep their relative order
+// within an XCD.
+constexpr int kXcds = 8;
+void xcd_interleave(std::vector<int32_t>& ent, int width, const std::vector<int>& group);
 // (Re)build a patch's work entries, partial buffers and slot descriptor for
 // its tile shape (fwd_mt, bwd_nt); the caller synchronises the stream first.
 mlf_status patch_setup_tiles(mlf_patch_s* p);
