@@ -16,6 +16,8 @@
 //
 // 16x16 MFMA operand layout used throughout (lane l, r = l % 16, g = l / 16):
 //   A: row r, k = g*KPL + j;  B: column r, k = g*KPL + j;  C: rows 4g + j, column r.
+#include <type_traits>
+
 #include "internal.h"
 #include "rl_mfma.h"
 
@@ -310,6 +312,10 @@ void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
 // MFMAs run.  The sum over e1 happens inside the MFMA accumulators, so only
 // the final P tile is scattered (a few LDS atomics per wave and plane).
 constexpr int kTpzNT = 3;      // N-tiles (taps along axis 1): K <= 48, i.e. r <= 23
+// Zero source for injection-ring rows outside the grid (read by the DMA, never
+// written), so the shift needs no per-element select.
+__device__ __align__(16) float g_zero_row[256] = {};  // >= one ring row (FP32: 225 floats)
+
 // W/ring buffers: the DMA runs NB-1 steps ahead.  Two buffers keep the BF16
 // kernel at 75 KB of LDS (two workgroups per CU, which measured 1.9x faster
 // than three buffers at one workgroup per CU); FP32 W buffers are 45 KB each.
@@ -404,19 +410,23 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     // shadow row in 2 passes of 64 dwords.
     constexpr int RPASS = (KPL == 1) ? 4 : 2;
     auto issue_inj = [&](int t, int slot_buf) {
-      const int i1 = min(max(16 * b1 + r - t, 0), n1 - 1);
+      const int i1r = 16 * b1 + r - t;
+      const bool rowok = i1r >= 0 && i1r < n1;
+      const int i1 = min(max(i1r, 0), n1 - 1);
       T* dst = ring + slot_buf * CW * kRingRow;
       for (int q = w; q < CW * RPASS; q += 4) {
         const int c = q / RPASS, part = q - c * RPASS;
+        const bool ok = rowok && c2_0 + c < n2;
         const int c2 = min(c2_0 + c, n2 - 1);
         const size_t lens = ((size_t)z * n2 + c2) * n1 + i1;
         const int d = part * 64 + lane;  // dword within the row
         const void* src;
         if constexpr (KPL == 1) {
           if (d >= kP) continue;
-          src = X + lens * kP + d;
+          src = ok ? (const void*)(X + lens * kP + d) : (const void*)(g_zero_row + d);
         } else {
-          src = reinterpret_cast<const char*>(Xb + lens * kShadowRow) + 4 * d;
+          src = ok ? (const void*)(reinterpret_cast<const char*>(Xb + lens * kShadowRow) + 4 * d)
+                   : (const void*)(g_zero_row + d);
         }
         __builtin_amdgcn_global_load_lds(
             (const __attribute__((address_space(1))) void*)src,
@@ -456,42 +466,47 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
           return M::load(Wc + row * PHI + ks * KSTEP + lg * KPL);
         }
       };
+      // N-tile outer, K-step inner: each tile's loads and MFMAs are
+      // straight-line code, and the tile count is tested twice per step
+      // rather than once per group of MFMAs (a branch makes the MFMAs after
+      // it wait for every LDS read issued before it)
+      auto tile = [&](auto ntc) __attribute__((always_inline)) {
+        constexpr int nt = decltype(ntc)::value;
   #pragma unroll
-      for (int ks = 0; ks < NKS; ++ks) {
-  #pragma unroll
-        for (int nt = 0; nt < kTpzNT; ++nt) {
-          if (nt >= NT) break;
+        for (int ks = 0; ks < NKS; ++ks) {
           const typename M::frag bf = load_b(ks, nt);
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
         }
-      }
+      };
+      static_assert(kTpzNT == 3, "three N-tiles");
+      tile(std::integral_constant<int, 0>{});
+      if (NT > 1) tile(std::integral_constant<int, 1>{});
+      if (NT > 2) tile(std::integral_constant<int, 2>{});
       // batch t+1 landed; later batches may still be in flight
       wait_vm_le(max(0, min(K, t + NB) - t - 2) * n_batch);
       __syncthreads();
       if (t + 1 < K) {
         const int i1n = 16 * b1 + r - (t + 1);
+        (void)i1n;
         const T* rg = ring + ((t + 1) % NB) * CW * kRingRow;
-        const bool rowok = (i1n >= 0) && (i1n < n1);
   #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
           const int c = MT * w + mt;
-          const bool ok = rowok && (c2_0 + c < n2);
           // every lane reads its row group's slot (broadcast); only lane 0 of
-          // each 16-lane row keeps it (the DPP 'old' operand)
+          // each 16-lane row keeps it (the DPP 'old' operand); rows outside the
+          // grid arrived as zeros
           const T* rp = rg + c * kRingRow + lg * KPL;
   #pragma unroll
           for (int ks = 0; ks < NKS; ++ks) {
             if constexpr (KPL == 1) {
-              const float v = rp[ks * KSTEP];
-              const float inj = ok ? v : 0.f;
+              const float inj = rp[ks * KSTEP];
               a[mt][ks] = __builtin_bit_cast(
                   float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, inj),
                                                      __builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF,
                                                      false));
             } else {
-              const typename M::frag v = *reinterpret_cast<const typename M::frag*>(rp + ks * KSTEP);
-              const typename M::frag inj = ok ? v : typename M::frag{};
+              const typename M::frag inj = *reinterpret_cast<const typename M::frag*>(rp + ks * KSTEP);
               typedef int intx4 __attribute__((ext_vector_type(4)));
               const intx4 iv = __builtin_bit_cast(intx4, inj);
               intx4 cv = __builtin_bit_cast(intx4, a[mt][ks]);
