@@ -97,7 +97,7 @@ mlf_status rebuild(mlf_batch_s* b) {
     }
     for (int j = 0; j < p->nblk_r; ++j) {
       rs.push_back((int32_t)i);
-      rs.push_back(256 * j);
+      rs.push_back(kRatioLens * j);
     }
   }
   auto by_r = [](const Ent& a, const Ent& c) { return a.r > c.r; };

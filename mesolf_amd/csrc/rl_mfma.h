@@ -18,7 +18,10 @@ constexpr int kFwdMTF32 = 2;      // M-tiles per wave, forward, f32 (block 16 x 
 constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x 16 lenslets)
 constexpr int kBwdNTiles = 5;     // default N-tiles per backward workgroup (80 phases; 3 also built)
 constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases (or 5 of 48)
-constexpr int kBwdTapPad = 32;    // backward taps padded to a multiple of 32
+constexpr int kRatioLens = 32;    // lenslets per ratio work entry
+constexpr int kRatioSlices = 8;   // plane slices summed in parallel per entry (x kRatioLens = 256 threads)
+constexpr int kBwdTapPad = 32;    // FP32 backward taps padded to a multiple of 32
+constexpr int kBwdTapPad16 = 64;  // 16-bit: two K-steps of 32 (the K loop runs in pairs)
 constexpr int kTpzMTF32 = 2;      // Toeplitz forward: default input columns per wave, f32 (1 also built)
 constexpr int kTpzMTBF16 = 4;     // Toeplitz forward: default input columns per wave, bf16 (2 also built)
 constexpr int kShadowRow = 256;   // BF16 volume shadow: phases per lenslet (zero beyond 225)
@@ -68,7 +71,7 @@ struct BwdGeo {
   const SlotDesc* slots;
   const int32_t* r;       // [Z] backward radius
   const int64_t* boff;    // [Z] element offset of the plane block in the packed view
-  const int32_t* epad;    // [Z] padded tap count (multiple of kBwdTapPad)
+  const int32_t* epad;    // [Z] padded tap count (multiple of kBwdTapPad / kBwdTapPad16)
   const int32_t* sched;   // [entries][5] = (slot, z, b1, b2, phase group)
   const int32_t* count;
   int view;               // selects the normaliser table of the view

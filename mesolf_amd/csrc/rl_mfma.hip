@@ -12,7 +12,8 @@
 //     B[a, phase] = sum_e R[a - e] * Wt[z][e][phase]             (GEMM, K = taps)
 //   A operand = im2col of the ratio image (LDS window), B operand = the
 //   transposed PSF table (global, taps contiguous).  The backprojection of
-//   ones (the normaliser) is read from a precomputed FP64-summed table.
+//   ones (the normaliser B1) is read from a precomputed table of 1 / B1
+//   (summed and inverted in FP64).
 //
 // 16x16 MFMA operand layout used throughout (lane l, r = l % 16, g = l / 16):
 //   A: row r, k = g*KPL + j;  B: column r, k = g*KPL + j;  C: rows 4g + j, column r.
@@ -569,29 +570,46 @@ __device__ __forceinline__ void ratio_out(const SlotDesc& sd, int i, int i1, int
 }
 
 // Gather for the Toeplitz partials: y[i] = sum_z sum_{b2} window(b1 = i1/16, b2)
-// in a fixed order (FP64), then the ratio.
-__global__ void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int Z, int CW, int u, int v, int use_lf,
-                                   float* __restrict__ y_out) {
+// in FP64, then the ratio.  A workgroup takes kRatioLens lenslets; its
+// kRatioSlices thread groups sum disjoint plane ranges, and the slice sums
+// are added in slice order (a fixed order: the result does not depend on
+// timing or on the other patches of a batch).
+__global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int Z, int CW, int u,
+                                                          int v, int use_lf, float* __restrict__ y_out) {
+  static_assert(kRatioLens * kRatioSlices == 256, "one thread per (lenslet, slice)");
+  __shared__ double part[kRatioSlices][kRatioLens];
+  const int li = threadIdx.x % kRatioLens, sl = threadIdx.x / kRatioLens;
+  const int z0 = sl * Z / kRatioSlices, z1 = (sl + 1) * Z / kRatioSlices;
   const int nent = uni(*rg.count);
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
     const SlotDesc sd = load_slot(g.slots, uni(rg.sched[2 * ent]));
     const int n1 = sd.n1, n2 = sd.n2;
-    const int i = uni(rg.sched[2 * ent + 1]) + threadIdx.x;
-    if (i >= n1 * n2) continue;
-    const int i1 = i % n1, i2 = i / n1;
+    const int i = uni(rg.sched[2 * ent + 1]) + li;
+    const bool ok = i < n1 * n2;
+    const int i1 = ok ? i % n1 : 0, i2 = ok ? i / n1 : 0;
     const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, CW);
     const int b1 = i1 / 16;
     double y = 0.0;
-    for (int z = 0; z < Z; ++z) {
-      const int r = uni(g.r[z]);
-      const int Wc2 = CW + 2 * r;
-      const int b2h = min(nb2 - 1, (i2 + r) / CW);
-      const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
-      const float* pz = sd.partial + uni64(sd.poff[z]);
-      for (int b2 = b2l; b2 <= b2h; ++b2)
-        y += (double)pz[(size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r)];
+    if (ok) {
+      for (int z = z0; z < z1; ++z) {
+        const int r = g.r[z];  // the plane slice differs between the half-waves
+        const int Wc2 = CW + 2 * r;
+        const int b2h = min(nb2 - 1, (i2 + r) / CW);
+        const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
+        const float* pz = sd.partial + sd.poff[z];
+        for (int b2 = b2l; b2 <= b2h; ++b2)
+          y += (double)pz[(size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r)];
+      }
     }
-    ratio_out(sd, i, i1, i2, y, u, v, use_lf, y_out);
+    part[sl][li] = y;
+    __syncthreads();
+    if (sl == 0 && ok) {
+      double t = part[0][li];
+  #pragma unroll
+      for (int q = 1; q < kRatioSlices; ++q) t += part[q][li];
+      ratio_out(sd, i, i1, i2, t, u, v, use_lf, y_out);
+    }
+    __syncthreads();  // part is reused by the next entry
   }
 }
 
@@ -615,33 +633,47 @@ __global__ void k_pack_fwd_tpz(const float* __restrict__ src, T* __restrict__ ds
   }
 }
 
-// y[i] = sum over planes and covering blocks of the window partials (fixed
-// order, FP64), then the ratio (Q-scatter forward).
-__global__ void k_ratio_gather(FwdGeo g, RatioGeo rg, int Z, int BM2, int u, int v, int use_lf,
-                               float* __restrict__ y_out) {
+// y[i] = sum over planes and covering blocks of the window partials (FP64,
+// fixed order: plane slices as in k_ratio_gather_tpz), then the ratio
+// (Q-scatter forward).
+__global__ __launch_bounds__(256) void k_ratio_gather(FwdGeo g, RatioGeo rg, int Z, int BM2, int u, int v,
+                                                      int use_lf, float* __restrict__ y_out) {
+  __shared__ double part[kRatioSlices][kRatioLens];
+  const int li = threadIdx.x % kRatioLens, sl = threadIdx.x / kRatioLens;
+  const int z0 = sl * Z / kRatioSlices, z1 = (sl + 1) * Z / kRatioSlices;
   const int nent = uni(*rg.count);
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
     const SlotDesc sd = load_slot(g.slots, uni(rg.sched[2 * ent]));
     const int n1 = sd.n1, n2 = sd.n2;
-    const int i = uni(rg.sched[2 * ent + 1]) + threadIdx.x;
-    if (i >= n1 * n2) continue;
-    const int i1 = i % n1, i2 = i / n1;
+    const int i = uni(rg.sched[2 * ent + 1]) + li;
+    const bool ok = i < n1 * n2;
+    const int i1 = ok ? i % n1 : 0, i2 = ok ? i / n1 : 0;
     const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, BM2);
     double y = 0.0;
-    for (int z = 0; z < Z; ++z) {
-      const int r = uni(g.r[z]);
-      const int W1 = 16 + 2 * r, W2 = BM2 + 2 * r;
-      const int b1h = min(nb1 - 1, (i1 + r) / 16);
-      const int b1l = (i1 - r - 15 <= 0) ? 0 : (i1 - r - 15 + 15) / 16;
-      const int b2h = min(nb2 - 1, (i2 + r) / BM2);
-      const int b2l = (i2 - r - BM2 + 1 <= 0) ? 0 : (i2 - r - BM2 + 1 + BM2 - 1) / BM2;
-      const float* pz = sd.partial + uni64(sd.poff[z]);
-      for (int b2 = b2l; b2 <= b2h; ++b2)
-        for (int b1 = b1l; b1 <= b1h; ++b1)
-          y += (double)pz[(size_t)(b2 * nb1 + b1) * W1 * W2 + (i2 - BM2 * b2 + r) * W1 +
-                          (i1 - 16 * b1 + r)];
+    if (ok) {
+      for (int z = z0; z < z1; ++z) {
+        const int r = g.r[z];  // the plane slice differs between the half-waves
+        const int W1 = 16 + 2 * r, W2 = BM2 + 2 * r;
+        const int b1h = min(nb1 - 1, (i1 + r) / 16);
+        const int b1l = (i1 - r - 15 <= 0) ? 0 : (i1 - r - 15 + 15) / 16;
+        const int b2h = min(nb2 - 1, (i2 + r) / BM2);
+        const int b2l = (i2 - r - BM2 + 1 <= 0) ? 0 : (i2 - r - BM2 + 1 + BM2 - 1) / BM2;
+        const float* pz = sd.partial + sd.poff[z];
+        for (int b2 = b2l; b2 <= b2h; ++b2)
+          for (int b1 = b1l; b1 <= b1h; ++b1)
+            y += (double)pz[(size_t)(b2 * nb1 + b1) * W1 * W2 + (i2 - BM2 * b2 + r) * W1 +
+                            (i1 - 16 * b1 + r)];
+      }
     }
-    ratio_out(sd, i, i1, i2, y, u, v, use_lf, y_out);
+    part[sl][li] = y;
+    __syncthreads();
+    if (sl == 0 && ok) {
+      double t = part[0][li];
+  #pragma unroll
+      for (int q = 1; q < kRatioSlices; ++q) t += part[q][li];
+      ratio_out(sd, i, i1, i2, t, u, v, use_lf, y_out);
+    }
+    __syncthreads();  // part is reused by the next entry
   }
 }
 
@@ -676,13 +708,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const int epad = uni(g.epad[z]);
     const int Wd = 16 + 2 * r;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
-    const T* Bz = Wb + uni64(g.boff[z]) + (size_t)(grp * 16 * NT) * epad;
     const int nks = epad / KSTEP;
-    floatx4 acc[MT][NT];
-  #pragma unroll
-    for (int mt = 0; mt < MT; ++mt)
-  #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
 
     // Fused multiplicative update (recon:130-137), set up before the K loop so
     // that the first M-tile's loads are in flight while the MFMAs run.  The
@@ -711,27 +737,17 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const bool ok1 = a1 < n1;
     const int a1c = ok1 ? a1 : n1 - 1;
     const int c1 = norm_class(a1c, n1, r);
-    int ph0[NT];  // first of this lane's 4 phases in N-tile nt
+    int base[MT];  // window cell of this lane's lenslet in M-tile mt
   #pragma unroll
-    for (int nt = 0; nt < NT; ++nt) ph0[nt] = grp * 16 * NT + nt * 16 + 4 * lg;
-    floatx4_t xv[2][NT], bv[2][NT];
-    auto load_tile = [&](int mt, int buf) {
-      const int a2 = 16 * b2 + 4 * w + mt;
-      const int a2c = a2 < n2 ? a2 : n2 - 1;
-      const int xo = (a2c * n1 + a1c) * kP;
-      const int to = (norm_class(a2c, n2, r) * C1 + c1) * kP;
-  #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        xv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0[nt]), 0, 0));
-        bv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0[nt]), 0, 0));
-      }
-    };
-    load_tile(0, 0);
+    for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
 
+    // ratio window
+    float* Rw = reinterpret_cast<float*>(smem);  // FP32: [Wd][Wd]
+    int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // FP32: [epad]
+    using frag_t = typename M::frag;
+    frag_t* Rv = reinterpret_cast<frag_t*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // 16-bit: [Wd][Wd]
     if constexpr (KPL == 1) {
       // FP32: scalar im2col reads, taps e1-fastest
-      float* Rw = reinterpret_cast<float*>(smem);                                 // [Wd][Wd]
-      int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // [epad]
       for (int o = tid; o < Wd * Wd; o += 256) {
         const int x = o % Wd, y = o / Wd;
         const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
@@ -746,48 +762,16 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         toff[e] = off;
       }
       __syncthreads();
-      int base[MT];
-  #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
-      typename M::frag bn[NT];
-  #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
-      for (int ks = 0; ks < nks; ++ks) {
-        typename M::frag bf[NT];
-  #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
-        if (ks + 1 < nks) {
-  #pragma unroll
-          for (int nt = 0; nt < NT; ++nt)
-            bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
-        }
-        const int o = toff[ks * KSTEP + lg];
-  #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          const typename M::frag af = Rw[base[mt] + o];
-  #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
-        }
-      }
     } else {
-      // BF16: taps ordered (e2, e1) with e1 padded to a multiple of 8, so a
+      // 16-bit: taps ordered (e2, e1) with e1 padded to a multiple of 8, so a
       // lane's 8 taps of a K-step are 8 consecutive e1 in one row.  Cell (x, y)
       // of the pre-shifted window holds R(x-j, y) for j = 0..7, so each A
       // fragment is a single 16-byte LDS read.
-      const int Kp8 = (K + 7) & ~7;
-      float* Rs = reinterpret_cast<float*>(smem);                                  // [Wd][Wd]
-      using frag_t = typename M::frag;
-      frag_t* Rv = reinterpret_cast<frag_t*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // [Wd][Wd]
-      int* toff8 = reinterpret_cast<int*>(reinterpret_cast<char*>(Rv) + sizeof(frag_t) * Wd * Wd);
+      float* Rs = reinterpret_cast<float*>(smem);  // [Wd][Wd]
       for (int o = tid; o < Wd * Wd; o += 256) {
         const int x = o % Wd, y = o / Wd;
         const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
         Rs[o] = (a1 >= 0 && a1 < n1 && a2 >= 0 && a2 < n2) ? R[a1 + n1 * a2] : 0.f;
-      }
-      for (int q = tid; q < epad / 8; q += 256) {
-        const int t0 = 8 * q;
-        const int e2 = t0 / Kp8, e10 = t0 - e2 * Kp8;
-        toff8[q] = (e2 < K) ? (r - e10) - (e2 - r) * Wd : 0;
       }
       __syncthreads();
       for (int o = tid; o < Wd * Wd; o += 256) {
@@ -801,69 +785,150 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         Rv[o] = v;
       }
       __syncthreads();
-      int base[MT];
-  #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
-      typename M::frag bn[NT];
-  #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
-      for (int ks = 0; ks < nks; ++ks) {
-        typename M::frag bf[NT];
-  #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
-        if (ks + 1 < nks) {
-  #pragma unroll
-          for (int nt = 0; nt < NT; ++nt)
-            bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
-        }
-        const int o = toff8[ks * 4 + lg];
-  #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          const typename M::frag af = Rv[base[mt] + o];
-  #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
-        }
-      }
     }
 
-    // fused multiplicative update (recon:130-137), continued from the loads
-    // issued before the K loop.
+    {  // the entry's phase group: 16 * NT phases from 16 * NT * grp
+      const T* Bz = Wb + uni64(g.boff[z]) + (size_t)(grp * 16 * NT) * epad;
+      floatx4 acc[MT][NT];
   #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) {
-      const int buf = mt & 1;
-      if (mt + 1 < MT) load_tile(mt + 1, buf ^ 1);
-      const int a2 = 16 * b2 + 4 * w + mt;
-      const bool ok = ok1 && a2 < n2;
-      const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
+      for (int mt = 0; mt < MT; ++mt)
   #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) {
-        floatx4_t xn;
+        for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
+      int ph0[NT];  // first of this lane's 4 phases in N-tile nt
   #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          const float x = xv[buf][nt][j];
-          float C = x * acc[mt][nt][j] / bv[buf][nt][j];
-          // NaN, +-Inf and negatives -> 0, as a select (no branches)
-          C = (C >= 0.f && C <= 3.402823466e38f) ? C : 0.f;
-          const float v = C * wgt + (1.f - wgt) * x;
-          xn[j] = v < 0.f ? 0.f : v;
+      for (int nt = 0; nt < NT; ++nt) ph0[nt] = grp * 16 * NT + nt * 16 + 4 * lg;
+      floatx4_t xv[2][NT], bv[2][NT];
+      auto load_tile = [&](int mt, int buf) {
+        const int a2 = 16 * b2 + 4 * w + mt;
+        const int a2c = a2 < n2 ? a2 : n2 - 1;
+        const int xo = (a2c * n1 + a1c) * kP;
+        const int to = (norm_class(a2c, n2, r) * C1 + c1) * kP;
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          xv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0[nt]), 0, 0));
+          bv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0[nt]), 0, 0));
         }
-        const int xo = 4 * (l * kP + ph0[nt]);
-        const int so = 2 * (l * kShadowRow + ph0[nt]);
-        if (ok && ph0[nt] + 4 <= kP) {
-          __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
-          if (Xsh) {
-            floatx4_t sv = xn;
-            if constexpr (is_f16<T>()) sv *= sd.xs;
-            uintx2 h;
-            h[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
-            h[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
-            __builtin_amdgcn_raw_buffer_store_b64(h, sr, so, 0, 0);
+      };
+      load_tile(0, 0);
+
+      if constexpr (KPL == 1) {
+        typename M::frag bn[NT];
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
+        for (int ks = 0; ks < nks; ++ks) {
+          typename M::frag bf[NT];
+  #pragma unroll
+          for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
+          if (ks + 1 < nks) {
+  #pragma unroll
+            for (int nt = 0; nt < NT; ++nt)
+              bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
           }
-        } else if (ok && ph0[nt] < kP) {  // phase 224: the last of 225
-          __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
-          if (Xsh)
-            __builtin_amdgcn_raw_buffer_store_b16(to_bits16<T>(is_f16<T>() ? xn[0] * sd.xs : xn[0]), sr, so,
-                                                  0, 0);
+          const int o = toff[ks * KSTEP + lg];
+  #pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const typename M::frag af = Rw[base[mt] + o];
+  #pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
+          }
+        }
+      } else {
+        // Taps are read straight from global memory (the four waves read the
+        // same rows, which the L1 serves) two K-steps ahead, through a buffer
+        // resource so that the step offset is a scalar (the prefetches of the
+        // last two steps re-read the last step).  The window offset of a
+        // lane's 8 taps is computed, not looked up: t0 = 32 ks + 8 lg ->
+        // (e2, e1) = divmod(t0, Kp8) with a 20-bit reciprocal (exact for
+        // t0 * Kp8 < 2^20).
+        const int Kp8 = (K + 7) & ~7;
+        const __amdgpu_buffer_rsrc_t br = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<T*>(Bz), 0, (int)((size_t)NT * 16 * epad * sizeof(T)), kBufferDword3);
+        int boffv[NT];
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) boffv[nt] = (int)(((nt * 16 + lr) * epad + lg * KPL) * sizeof(T));
+        auto load_bk = [&](int ks, int nt) {
+          return __builtin_bit_cast(frag_t, __builtin_amdgcn_raw_buffer_load_b128(
+                                                br, boffv[nt], ks * KSTEP * (int)sizeof(T), 0));
+        };
+        const unsigned rcp = ((1u << 20) + Kp8 - 1) / Kp8;
+        // one K-step: MFMAs on stage bq, then bq is refilled with step ks + 2
+        auto step = [&](int ks, frag_t (&bq)[NT]) __attribute__((always_inline)) {
+          const unsigned t0 = 32u * ks + 8u * lg;
+          const int e2 = (int)((t0 * rcp) >> 20);
+          const int e10 = (int)t0 - e2 * Kp8;
+          const int o = ((r - e10) - (e2 - r) * Wd) & -(int)(e2 < K);
+  #pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            const frag_t af = Rv[base[mt] + o];
+  #pragma unroll
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bq[nt], af, acc[mt][nt]);
+          }
+  #pragma unroll
+          for (int nt = 0; nt < NT; ++nt) bq[nt] = load_bk(min(ks + 2, nks - 1), nt);
+          // keep the refill here: scheduled after the next step's MFMAs it
+          // would be only one step ahead
+          __builtin_amdgcn_sched_barrier(0);
+        };
+        // (all of stage 0, then all of stage 1: the order the loop issues
+        // them in, so the wait counts at the loop head stay tight)
+        frag_t bs0[NT], bs1[NT];
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bs0[nt] = load_bk(0, nt);
+        __builtin_amdgcn_sched_barrier(0);
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) bs1[nt] = load_bk(1, nt);
+        __builtin_amdgcn_sched_barrier(0);
+        // two stages that alternate, so no fragment is copied (a copy would
+        // wait for the younger stage's loads); the tap count is a multiple of
+        // two K-steps (at least one pair: epad >= 64)
+        int ks = 0;
+        do {
+          step(ks, bs0);
+          step(ks + 1, bs1);
+          ks += 2;
+        } while (ks < nks);
+      }
+
+      // fused multiplicative update (recon:130-137), continued from the loads
+      // issued before the K loop.
+  #pragma unroll
+      for (int mt = 0; mt < MT; ++mt) {
+        const int buf = mt & 1;
+        if (mt + 1 < MT) load_tile(mt + 1, buf ^ 1);
+        const int a2 = 16 * b2 + 4 * w + mt;
+        const bool ok = ok1 && a2 < n2;
+        const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt) {
+          floatx4_t xn;
+  #pragma unroll
+          for (int j = 0; j < 4; ++j) {
+            const float x = xv[buf][nt][j];
+            float C = x * acc[mt][nt][j] * bv[buf][nt][j];  // bv: 1 / B1
+            // NaN, +-Inf and negatives -> 0: keep C only in the classes
+            // -0, +0, +denormal, +normal (one class test and a select)
+            C = __builtin_amdgcn_classf(C, 0x1E0) ? C : 0.f;
+            const float v = C * wgt + (1.f - wgt) * x;
+            xn[j] = v < 0.f ? 0.f : v;
+          }
+          const int xo = 4 * (l * kP + ph0[nt]);
+          const int so = 2 * (l * kShadowRow + ph0[nt]);
+          if (ok && ph0[nt] + 4 <= kP) {
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
+            if (Xsh) {
+              floatx4_t sv = xn;
+              if constexpr (is_f16<T>()) sv *= sd.xs;
+              uintx2 h;
+              h[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
+              h[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
+              __builtin_amdgcn_raw_buffer_store_b64(h, sr, so, 0, 0);
+            }
+          } else if (ok && ph0[nt] < kP) {  // phase 224: the last of 225
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
+            if (Xsh)
+              __builtin_amdgcn_raw_buffer_store_b16(to_bits16<T>(is_f16<T>() ? xn[0] * sd.xs : xn[0]), sr,
+                                                    so, 0, 0);
+          }
         }
       }
     }
@@ -941,7 +1006,9 @@ __global__ void k_norm_pass1(const float* __restrict__ src, double* __restrict__
   }
 }
 
-// pass 2: T[z][c2][c1][phase] = sum_{d2 in rng(c2)} S (rounded once to FP32).
+// pass 2: T[z][c2][c1][phase] = 1 / sum_{d2 in rng(c2)} S, the reciprocal taken
+// in FP64 and rounded once to FP32.  A zero sum gives +Inf, so the update's
+// X * B * T is non-finite (and set to 0) where B1 = 0, as X * B / B1 is.
 __global__ void k_norm_pass2(const double* __restrict__ S, float* __restrict__ Tn, NormGeo ng) {
   const int z = blockIdx.y;
   const int r = ng.r[z];
@@ -958,7 +1025,7 @@ __global__ void k_norm_pass2(const double* __restrict__ S, float* __restrict__ T
     norm_range(c2, ng.n2, r, &lo, &hi);
     double acc = 0.0;
     for (int d2 = lo; d2 <= hi; ++d2) acc += s[((size_t)(d2 + r) * C1 + c1) * kP + ph];
-    d[o] = (float)acc;
+    d[o] = (float)(1.0 / acc);
   }
 }
 
@@ -975,7 +1042,7 @@ size_t fwd_smem_bytes(Precision p, int rmax, int bm2) {
 size_t bwd_smem_bytes(Precision p, int rmax, int epad_max) {
   const int Wd = 16 + 2 * rmax;
   if (p == Precision::kF32) return sizeof(float) * ((Wd * Wd + 7) & ~7) + sizeof(int) * epad_max;
-  return sizeof(float) * ((Wd * Wd + 3) & ~3) + 16 * (size_t)Wd * Wd + sizeof(int) * (epad_max / 8);
+  return sizeof(float) * ((Wd * Wd + 3) & ~3) + 16 * (size_t)Wd * Wd;
 }
 
 hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem,

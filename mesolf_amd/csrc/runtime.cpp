@@ -306,7 +306,8 @@ static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
     rows_f[z] = pk.tpz ? Kf * 16 * ((Kf + 15) / 16) : pk.nchunk[z] * kFwdChunk;
     const int Kb = 2 * ps->rb[z] + 1;
     const int nt_b_pad = prec != Precision::kF32 ? Kb * ((Kb + 7) & ~7) : nt_b;
-    pk.epad[z] = (nt_b_pad + kBwdTapPad - 1) / kBwdTapPad * kBwdTapPad;
+    const int tpad = prec != Precision::kF32 ? kBwdTapPad16 : kBwdTapPad;
+    pk.epad[z] = (nt_b_pad + tpad - 1) / tpad * tpad;
     rows_b[z] = pk.epad[z];
     pk.woff[z] = of;
     of += (int64_t)rows_f[z] * phi;
@@ -967,13 +968,13 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
     for (int e = 0; e < p->nblk_b; ++e) grp[e] = p->bent[4 * e] * ngrp + p->bent[4 * e + 3];
     xcd_interleave(p->bent, 4, grp);
   }
-  p->nblk_r = (p->n1 * p->n2 + 255) / 256;
+  p->nblk_r = (p->n1 * p->n2 + kRatioLens - 1) / kRatioLens;
   std::vector<int32_t> fs, bs, rs;
   append_entries(p->fent, 3, 0, &fs);
   append_entries(p->bent, 4, 0, &bs);
   for (int i = 0; i < p->nblk_r; ++i) {
     rs.push_back(0);
-    rs.push_back(256 * i);
+    rs.push_back(kRatioLens * i);
   }
   MLF_CHECK(upload_vec(p->d_poff, poff));
   MLF_CHECK(upload_vec(p->d_fsched, fs));
