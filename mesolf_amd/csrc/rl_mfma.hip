@@ -494,26 +494,29 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
   #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
           const int c = MT * w + mt;
-          // every lane reads its row group's slot (broadcast); only lane 0 of
-          // each 16-lane row keeps it (the DPP 'old' operand); rows outside the
-          // grid arrived as zeros
-          const T* rp = rg + c * kRingRow + lg * KPL;
+          // shift every 16-lane row down by one lane, then lane 0 of each row
+          // alone reads the entering lenslet from the ring (the other lanes'
+          // LDS reads would return data nobody keeps); rows outside the grid
+          // arrived as zeros
   #pragma unroll
           for (int ks = 0; ks < NKS; ++ks) {
             if constexpr (KPL == 1) {
-              const float inj = rp[ks * KSTEP];
               a[mt][ks] = __builtin_bit_cast(
-                  float, __builtin_amdgcn_update_dpp(__builtin_bit_cast(int, inj),
-                                                     __builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF,
-                                                     false));
+                  float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, a[mt][ks]), 0x111, 0xF, 0xF, true));
             } else {
-              const typename M::frag inj = *reinterpret_cast<const typename M::frag*>(rp + ks * KSTEP);
               typedef int intx4 __attribute__((ext_vector_type(4)));
-              const intx4 iv = __builtin_bit_cast(intx4, inj);
               intx4 cv = __builtin_bit_cast(intx4, a[mt][ks]);
   #pragma unroll
-              for (int q = 0; q < 4; ++q) cv[q] = __builtin_amdgcn_update_dpp(iv[q], cv[q], 0x111, 0xF, 0xF, false);
+              for (int q = 0; q < 4; ++q) cv[q] = __builtin_amdgcn_mov_dpp(cv[q], 0x111, 0xF, 0xF, true);
               a[mt][ks] = __builtin_bit_cast(typename M::frag, cv);
+            }
+          }
+          if (lr == 0) {
+            const T* rp = rg + c * kRingRow + lg * KPL;
+  #pragma unroll
+            for (int ks = 0; ks < NKS; ++ks) {
+              if constexpr (KPL == 1) a[mt][ks] = rp[ks * KSTEP];
+              else a[mt][ks] = *reinterpret_cast<const typename M::frag*>(rp + ks * KSTEP);
             }
           }
         }
