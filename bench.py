@@ -460,7 +460,11 @@ def main():
                 "avg_launch_ms": avg_s * 1e3}
     tfile = os.path.join(ROOT, "profiles", "hbm_traffic_cfg2_101x101x51.json")
     if args.workload == "cfg2" and os.path.exists(tfile):
+        # measured bytes per launch of the kernel instance this run used
         tr = json.load(open(tfile))["bytes_per_launch"].get(args.precision, {}).get(dominant)
+        fmt, bnt = work.tile_shape()
+        if tr and "read" not in tr:
+            tr = tr.get(f"fwd_columns_per_wave={fmt}" if dominant == "forward" else f"bwd_phase_tiles={bnt}")
         if tr:
             roof["traffic"] = tr["read"] + tr["write"]
             roof["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE, profiles/"
