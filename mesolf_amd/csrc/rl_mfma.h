@@ -18,6 +18,15 @@ constexpr int kFwdMTF32 = 2;      // M-tiles per wave, forward, f32 (block 16 x 
 constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x 16 lenslets)
 constexpr int kBwdNTiles = 5;     // default N-tiles per backward workgroup (80 phases; 3 also built)
 constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases (or 5 of 48)
+constexpr int kBwdStages = 4;     // 16-bit backprojection: tap ring depth (copies run 3 K-steps ahead)
+// Bytes before the 16-bit backprojection's shifted window: the FP32 staging
+// window [Wd][Wd] and the tap ring (kBwdStages slices of [16 nt][32] taps),
+// which alias (the staging window is dead once the shifted one is built).
+inline __host__ __device__ int bwd_ring_bytes(int Wd, int nt, int elem_bytes) {
+  const int rs = 4 * ((Wd * Wd + 3) & ~3);
+  const int ring = kBwdStages * nt * 16 * 32 * elem_bytes;
+  return rs > ring ? rs : ring;
+}
 constexpr int kRatioLens = 32;    // lenslets per ratio work entry
 constexpr int kRatioSlices = 8;   // plane slices summed in parallel per entry (x kRatioLens = 256 threads)
 constexpr int kBwdTapPad = 32;    // FP32 backward taps padded to a multiple of 32
@@ -131,7 +140,7 @@ hipError_t launch_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int grid, int Z, int c
                                    int use_lf, float* y_out, hipStream_t st);
 hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                                hipStream_t st);
-size_t bwd_smem_bytes(Precision p, int rmax, int epad_max);
+size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt);
 
 hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
                            hipStream_t st);

@@ -748,7 +748,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     float* Rw = reinterpret_cast<float*>(smem);  // FP32: [Wd][Wd]
     int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // FP32: [epad]
     using frag_t = typename M::frag;
-    frag_t* Rv = reinterpret_cast<frag_t*>(smem + sizeof(float) * ((Wd * Wd + 3) & ~3));  // 16-bit: [Wd][Wd]
+    // 16-bit: [Wd][Wd] after the staging window Rs / tap ring (which alias)
+    frag_t* Rv = reinterpret_cast<frag_t*>(smem + bwd_ring_bytes(Wd, NT, (int)sizeof(T)));
     if constexpr (KPL == 1) {
       // FP32: scalar im2col reads, taps e1-fastest
       for (int o = tid; o < Wd * Wd; o += 256) {
@@ -836,60 +837,50 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         }
       } else {
-        // Taps are read straight from global memory (the four waves read the
-        // same rows, which the L1 serves) two K-steps ahead, through a buffer
-        // resource so that the step offset is a scalar (the prefetches of the
-        // last two steps re-read the last step).  The window offset of a
-        // lane's 8 taps is computed, not looked up: t0 = 32 ks + 8 lg ->
-        // (e2, e1) = divmod(t0, Kp8) with a 20-bit reciprocal (exact for
-        // t0 * Kp8 < 2^20).
+        // Taps go global -> LDS by DMA (global_load_lds, one copy per
+        // workgroup instead of one per wave through the L1) into a ring of
+        // kBwdStages K-step slices [16 NT rows][32 taps], kBwdStages - 1 steps
+        // ahead; slices past the last step re-load the last one, so every wave
+        // issues the same number of copies per step and waits with a fixed
+        // count.  The window offset of a lane's 8 taps is computed, not looked
+        // up: t0 = 32 ks + 8 lg -> (e2, e1) = divmod(t0, Kp8) with a 20-bit
+        // reciprocal (exact for t0 * Kp8 < 2^20).
         const int Kp8 = (K + 7) & ~7;
-        const __amdgpu_buffer_rsrc_t br = __builtin_amdgcn_make_buffer_rsrc(
-            const_cast<T*>(Bz), 0, (int)((size_t)NT * 16 * epad * sizeof(T)), kBufferDword3);
-        int boffv[NT];
-  #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) boffv[nt] = (int)(((nt * 16 + lr) * epad + lg * KPL) * sizeof(T));
-        auto load_bk = [&](int ks, int nt) {
-          return __builtin_bit_cast(frag_t, __builtin_amdgcn_raw_buffer_load_b128(
-                                                br, boffv[nt], ks * KSTEP * (int)sizeof(T), 0));
-        };
         const unsigned rcp = ((1u << 20) + Kp8 - 1) / Kp8;
-        // one K-step: MFMAs on stage bq, then bq is refilled with step ks + 2
-        auto step = [&](int ks, frag_t (&bq)[NT]) __attribute__((always_inline)) {
+        T* Bq = reinterpret_cast<T*>(smem);  // ring (aliases the dead staging window Rs)
+        constexpr int kSlice = NT * 16 * 32;  // elements per K-step slice
+        const int ncopy = w < NT ? (NT - w + 3) / 4 : 0;  // copies per step issued by this wave
+        auto issue_taps = [&](int ks, int slot) {
+          const int kc = min(ks, nks - 1);
+          char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
+          for (int nt = w; nt < NT; nt += 4) {
+            const T* src = Bz + (size_t)(nt * 16 + (lane >> 2)) * epad + kc * 32 + (lane & 3) * 8;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                             (__attribute__((address_space(3))) void*)(dst + nt * 1024),
+                                             16, 0, 0);
+          }
+        };
+  #pragma unroll
+        for (int q = 0; q < kBwdStages - 1; ++q) issue_taps(q, q);
+        for (int ks = 0; ks < nks; ++ks) {
+          wait_vm_le((kBwdStages - 2) * ncopy);  // this wave's copies of step ks landed
+          __syncthreads();  // everyone's copies landed; everyone is done with step ks - 1
+          issue_taps(ks + kBwdStages - 1, (ks + kBwdStages - 1) % kBwdStages);
+          const T* Bs = Bq + (ks % kBwdStages) * kSlice;
           const unsigned t0 = 32u * ks + 8u * lg;
           const int e2 = (int)((t0 * rcp) >> 20);
           const int e10 = (int)t0 - e2 * Kp8;
           const int o = ((r - e10) - (e2 - r) * Wd) & -(int)(e2 < K);
+          frag_t bf[NT];
+  #pragma unroll
+          for (int nt = 0; nt < NT; ++nt) bf[nt] = *reinterpret_cast<const frag_t*>(Bs + (nt * 16 + lr) * 32 + lg * 8);
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
             const frag_t af = Rv[base[mt] + o];
   #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bq[nt], af, acc[mt][nt]);
+            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
           }
-  #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) bq[nt] = load_bk(min(ks + 2, nks - 1), nt);
-          // keep the refill here: scheduled after the next step's MFMAs it
-          // would be only one step ahead
-          __builtin_amdgcn_sched_barrier(0);
-        };
-        // (all of stage 0, then all of stage 1: the order the loop issues
-        // them in, so the wait counts at the loop head stay tight)
-        frag_t bs0[NT], bs1[NT];
-  #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) bs0[nt] = load_bk(0, nt);
-        __builtin_amdgcn_sched_barrier(0);
-  #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) bs1[nt] = load_bk(1, nt);
-        __builtin_amdgcn_sched_barrier(0);
-        // two stages that alternate, so no fragment is copied (a copy would
-        // wait for the younger stage's loads); the tap count is a multiple of
-        // two K-steps (at least one pair: epad >= 64)
-        int ks = 0;
-        do {
-          step(ks, bs0);
-          step(ks + 1, bs1);
-          ks += 2;
-        } while (ks < nks);
+        }
       }
 
       // fused multiplicative update (recon:130-137), continued from the loads
@@ -1042,10 +1033,10 @@ size_t fwd_smem_bytes(Precision p, int rmax, int bm2) {
   return b + sizeof(float) * 4 * (16 + 2 * rmax) * (bm2 / 4 + 2 * rmax);
 }
 
-size_t bwd_smem_bytes(Precision p, int rmax, int epad_max) {
+size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt) {
   const int Wd = 16 + 2 * rmax;
   if (p == Precision::kF32) return sizeof(float) * ((Wd * Wd + 7) & ~7) + sizeof(int) * epad_max;
-  return sizeof(float) * ((Wd * Wd + 3) & ~3) + 16 * (size_t)Wd * Wd;
+  return bwd_ring_bytes(Wd, nt, 2) + 16 * (size_t)Wd * Wd;
 }
 
 hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem,

@@ -1001,7 +1001,7 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   sd.xs_inv = 1.f / p->xs;
   MLF_CHECK(upload_vec(p->d_slot, std::vector<SlotDesc>{sd}));
   p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax, p->fwd_mt) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
-  p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax);
+  p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax, p->bwd_nt);
   if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
     return set_error(MLF_ERR_UNSUPPORTED,
                      "PSF radius too large for the LDS tiles (rf_max=%d, rb_max=%d)", rfmax, rbmax);
