@@ -18,7 +18,7 @@ constexpr int kFwdMTF32 = 2;      // M-tiles per wave, forward, f32 (block 16 x 
 constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x 16 lenslets)
 constexpr int kBwdNTiles = 5;     // default N-tiles per backward workgroup (80 phases; 3 also built)
 constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases (or 5 of 48)
-constexpr int kBwdStages = 4;     // 16-bit backprojection: tap ring depth (copies run 3 K-steps ahead)
+constexpr int kBwdStages = 6;     // 16-bit backprojection: tap ring depth (copies run 5 K-steps ahead)
 // Bytes before the 16-bit backprojection's shifted window: the FP32 staging
 // window [Wd][Wd] and the tap ring (kBwdStages slices of [16 nt][32] taps),
 // which alias (the staging window is dead once the shifted one is built).

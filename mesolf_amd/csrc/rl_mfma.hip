@@ -681,6 +681,33 @@ __global__ __launch_bounds__(256) void k_ratio_gather(FwdGeo g, RatioGeo rg, int
 }
 
 // --------------------------------------------------------------- backward
+// dst[y][x] = R(a1_0 + x, a2_0 + y) for a Wd x Wd window (0 outside the
+// grid), 256 threads.  The reads of up to 16 cells per thread are issued
+// before any is waited for (clamped addresses, the zero applied by a select),
+// so a window costs about one L2 round trip instead of one per 256 cells.
+__device__ __forceinline__ void gather_window(const float* __restrict__ R, int n1, int n2, int a1_0,
+                                              int a2_0, int Wd, float* __restrict__ dst) {
+  constexpr int kPer = 16;
+  const int tid = threadIdx.x, ncell = Wd * Wd;
+  for (int c0 = 0; c0 < ncell; c0 += 256 * kPer) {
+    float v[kPer];
+  #pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int o = min(c0 + tid + 256 * q, ncell - 1);
+      const int y = o / Wd, x = o - y * Wd;
+      const int a1 = a1_0 + x, a2 = a2_0 + y;
+      const bool in = a1 >= 0 && a1 < n1 && a2 >= 0 && a2 < n2;
+      const float t = R[min(max(a1, 0), n1 - 1) + n1 * min(max(a2, 0), n2 - 1)];
+      v[q] = in ? t : 0.f;
+    }
+  #pragma unroll
+    for (int q = 0; q < kPer; ++q) {
+      const int o = c0 + tid + 256 * q;
+      if (o < ncell) dst[o] = v[q];
+    }
+  }
+}
+
 __device__ __forceinline__ int norm_class(int a, int n, int r) {
   if (n < 2 * r + 1) return a;  // small grid: one class per lenslet
   return a < r ? a : (a > n - 1 - r ? 2 * r - (n - 1 - a) : r);
@@ -752,11 +779,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     frag_t* Rv = reinterpret_cast<frag_t*>(smem + bwd_ring_bytes(Wd, NT, (int)sizeof(T)));
     if constexpr (KPL == 1) {
       // FP32: scalar im2col reads, taps e1-fastest
-      for (int o = tid; o < Wd * Wd; o += 256) {
-        const int x = o % Wd, y = o / Wd;
-        const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
-        Rw[o] = (a1 >= 0 && a1 < n1 && a2 >= 0 && a2 < n2) ? R[a1 + n1 * a2] : 0.f;
-      }
+      gather_window(R, n1, n2, 16 * b1 - r, 16 * b2 - r, Wd, Rw);
       for (int e = tid; e < epad; e += 256) {
         int off = 0;  // padded taps have zero weights; any in-window read is fine
         if (e < ntaps) {
@@ -772,21 +795,32 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // of the pre-shifted window holds R(x-j, y) for j = 0..7, so each A
       // fragment is a single 16-byte LDS read.
       float* Rs = reinterpret_cast<float*>(smem);  // [Wd][Wd]
-      for (int o = tid; o < Wd * Wd; o += 256) {
-        const int x = o % Wd, y = o / Wd;
-        const int a1 = 16 * b1 - r + x, a2 = 16 * b2 - r + y;
-        Rs[o] = (a1 >= 0 && a1 < n1 && a2 >= 0 && a2 < n2) ? R[a1 + n1 * a2] : 0.f;
-      }
+      gather_window(R, n1, n2, 16 * b1 - r, 16 * b2 - r, Wd, Rs);
       __syncthreads();
-      for (int o = tid; o < Wd * Wd; o += 256) {
-        const int x = o % Wd;
-        frag_t v;
+      // four cells per thread per pass, their 32 reads issued together
+      for (int c0 = 0; c0 < Wd * Wd; c0 += 4 * 256) {
+        float rv[4][8];
   #pragma unroll
-        for (int j = 0; j < 8; ++j) {
-          const float rv = x - j >= 0 ? Rs[o - j] : 0.f;
-          v[j] = (T)(is_f16<T>() ? fminf(rv, 65504.f) : rv);  // FP16: clamp to the largest finite value
+        for (int q = 0; q < 4; ++q) {
+          const int o = min(c0 + tid + 256 * q, Wd * Wd - 1);
+          const int x = o % Wd;
+  #pragma unroll
+          for (int j = 0; j < 8; ++j) {
+            const float t = Rs[max(o - j, 0)];
+            rv[q][j] = x - j >= 0 ? t : 0.f;
+          }
         }
-        Rv[o] = v;
+  #pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int o = c0 + tid + 256 * q;
+          if (o < Wd * Wd) {
+            frag_t v;
+  #pragma unroll
+            for (int j = 0; j < 8; ++j)
+              v[j] = (T)(is_f16<T>() ? fminf(rv[q][j], 65504.f) : rv[q][j]);  // FP16: clamp to the largest finite value
+            Rv[o] = v;
+          }
+        }
       }
       __syncthreads();
     }
