@@ -389,14 +389,18 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     __syncthreads();  // ring zeroed before any DMA lands
 
     const T* Wz = W + uni64(g.woff[z]);  // [K][nrow][PHI]
-    // Every wave issues the same number of DMA instructions per step (W: WSLOT
-    // slots / 256 threads; ring: CW*RPASS passes / 4 waves), so a step can wait
-    // for an earlier batch with a fixed vmcnt.
-    const int nslot = nrow * RV;
+    // Every wave issues the same number of DMA instructions per step (W: the
+    // entry's nw_inst; ring: CW*RPASS passes / 4 waves), so a step can wait
+    // for an earlier batch with a fixed vmcnt.  Only the K tap rows e2 < K
+    // are copied: rows K..16*NT-1 of the buffer feed accumulator columns that
+    // the flush drops, so whatever they hold does not matter.
+    const int nslot = K * RV;
+    const int nw_inst = (nslot + 255) / 256;
     auto issue_w = [&](int t, int buf) {
       const T* src = Wz + (size_t)t * nrow * PHI;
       char* dst = reinterpret_cast<char*>(Wb) + (size_t)buf * WSLOT * 16;
-      for (int base = w * 64; base < WSLOT; base += 256) {
+      for (int i = 0; i < nw_inst; ++i) {
+        const int base = 256 * i + 64 * w;
         const int slot = min(base + lane, nslot - 1);  // tail slots re-read the last vector
         const int row = slot / RV, pos = slot - row * RV;
         const int col = SWZ ? (pos ^ (row & 31)) : pos;
@@ -435,7 +439,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
             4, 0, 0);
       }
     };
-    const int n_batch = WSLOT / 256 + CW * RPASS / 4;  // DMA instructions per wave, step t >= 1
+    const int n_batch = nw_inst + CW * RPASS / 4;  // DMA instructions per wave, step t >= 1
 
     floatx4 acc[MT][kTpzNT];
   #pragma unroll
