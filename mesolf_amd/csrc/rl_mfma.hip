@@ -771,9 +771,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const bool ok1 = a1 < n1;
     const int a1c = ok1 ? a1 : n1 - 1;
     const int c1 = norm_class(a1c, n1, r);
-    int base[MT];  // window cell of this lane's lenslet in M-tile mt
-  #pragma unroll
-    for (int mt = 0; mt < MT; ++mt) base[mt] = (lr + r) + (4 * w + mt + r) * Wd;
+    // window cell of this lane's lenslet in M-tile mt: base + mt * Wd
+    const int base = (lr + r) + (4 * w + r) * Wd;
 
     // ratio window
     float* Rw = reinterpret_cast<float*>(smem);  // FP32: [Wd][Wd]
@@ -836,24 +835,37 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       for (int mt = 0; mt < MT; ++mt)
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-      int ph0[NT];  // first of this lane's 4 phases in N-tile nt
+      // first of this lane's 4 phases in N-tile nt: ph0 + 16 nt
+      const int ph0 = grp * 16 * NT + 4 * lg;
+      // X of M-tiles 0 .. MT-2 is loaded before the K loop (its HBM latency
+      // hides under the MFMAs; the last tile's registers are taken by the
+      // loop's operands and it is loaded when the loop ends); the 1/B1 rows
+      // of tile 0 too, those of later tiles (L2-resident) when the tile is
+      // reached
+      floatx4_t xv[MT][NT], bv[NT];
+      auto load_x = [&](int mt0, int mt1) {
   #pragma unroll
-      for (int nt = 0; nt < NT; ++nt) ph0[nt] = grp * 16 * NT + nt * 16 + 4 * lg;
-      floatx4_t xv[2][NT], bv[2][NT];
-      auto load_tile = [&](int mt, int buf) {
-        const int a2 = 16 * b2 + 4 * w + mt;
-        const int a2c = a2 < n2 ? a2 : n2 - 1;
-        const int xo = (a2c * n1 + a1c) * kP;
-        const int to = (norm_class(a2c, n2, r) * C1 + c1) * kP;
+        for (int mt = 0; mt < MT; ++mt) {
+          if (mt < mt0 || mt >= mt1) continue;
+          const int a2 = 16 * b2 + 4 * w + mt;
+          const int xo = ((a2 < n2 ? a2 : n2 - 1) * n1 + a1c) * kP;
   #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) {
-          xv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0[nt]), 0, 0));
-          bv[buf][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0[nt]), 0, 0));
+          for (int nt = 0; nt < NT; ++nt)
+            xv[mt][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0) + 64 * nt, 0, 0));
         }
       };
-      load_tile(0, 0);
+      auto load_t = [&](int mt) {
+        const int a2 = 16 * b2 + 4 * w + mt;
+        const int to = (norm_class(a2 < n2 ? a2 : n2 - 1, n2, r) * C1 + c1) * kP;
+  #pragma unroll
+        for (int nt = 0; nt < NT; ++nt)
+          bv[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0) + 64 * nt, 0, 0));
+      };
+      constexpr int kEpiLoads = (MT - 1) * NT + NT;  // load_x(0, MT - 1) + load_t(0)
 
       if constexpr (KPL == 1) {
+        load_x(0, MT - 1);
+        load_t(0);
         typename M::frag bn[NT];
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
@@ -869,7 +881,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           const int o = toff[ks * KSTEP + lg];
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
-            const typename M::frag af = Rw[base[mt] + o];
+            const typename M::frag af = Rw[base + mt * Wd + o];
   #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
           }
@@ -900,8 +912,12 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         };
   #pragma unroll
         for (int q = 0; q < kBwdStages - 1; ++q) issue_taps(q, q);
+        load_x(0, MT - 1);  // after the first slices: their waits need not cover these loads
+        load_t(0);
         for (int ks = 0; ks < nks; ++ks) {
-          wait_vm_le((kBwdStages - 2) * ncopy);  // this wave's copies of step ks landed
+          // this wave's copies of step ks landed (the epilogue loads are
+          // younger than the slices of steps 0 .. kBwdStages - 2)
+          wait_vm_le((kBwdStages - 2) * ncopy + (ks < kBwdStages - 1 ? kEpiLoads : 0));
           __syncthreads();  // everyone's copies landed; everyone is done with step ks - 1
           issue_taps(ks + kBwdStages - 1, (ks + kBwdStages - 1) % kBwdStages);
           const T* Bs = Bq + (ks % kBwdStages) * kSlice;
@@ -914,7 +930,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           for (int nt = 0; nt < NT; ++nt) bf[nt] = *reinterpret_cast<const frag_t*>(Bs + (nt * 16 + lr) * 32 + lg * 8);
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
-            const frag_t af = Rv[base[mt] + o];
+            const frag_t af = Rv[base + mt * Wd + o];
   #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
           }
@@ -923,10 +939,10 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
 
       // fused multiplicative update (recon:130-137), continued from the loads
       // issued before the K loop.
+      load_x(MT - 1, MT);
   #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
-        const int buf = mt & 1;
-        if (mt + 1 < MT) load_tile(mt + 1, buf ^ 1);
+        if (mt > 0) load_t(mt);
         const int a2 = 16 * b2 + 4 * w + mt;
         const bool ok = ok1 && a2 < n2;
         const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
@@ -935,17 +951,18 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           floatx4_t xn;
   #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const float x = xv[buf][nt][j];
-            float C = x * acc[mt][nt][j] * bv[buf][nt][j];  // bv: 1 / B1
+            const float x = xv[mt][nt][j];
+            float C = x * acc[mt][nt][j] * bv[nt][j];  // bv: 1 / B1
             // NaN, +-Inf and negatives -> 0: keep C only in the classes
             // -0, +0, +denormal, +normal (one class test and a select)
             C = __builtin_amdgcn_classf(C, 0x1E0) ? C : 0.f;
             const float v = C * wgt + (1.f - wgt) * x;
             xn[j] = v < 0.f ? 0.f : v;
           }
-          const int xo = 4 * (l * kP + ph0[nt]);
-          const int so = 2 * (l * kShadowRow + ph0[nt]);
-          if (ok && ph0[nt] + 4 <= kP) {
+          const int ph = ph0 + 16 * nt;
+          const int xo = 4 * (l * kP + ph);
+          const int so = 2 * (l * kShadowRow + ph);
+          if (ok && ph + 4 <= kP) {
             __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
             if (Xsh) {
               floatx4_t sv = xn;
@@ -955,7 +972,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
               h[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
               __builtin_amdgcn_raw_buffer_store_b64(h, sr, so, 0, 0);
             }
-          } else if (ok && ph0[nt] < kP) {  // phase 224: the last of 225
+          } else if (ok && ph < kP) {  // phase 224: the last of 225
             __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
             if (Xsh)
               __builtin_amdgcn_raw_buffer_store_b16(to_bits16<T>(is_f16<T>() ? xn[0] * sd.xs : xn[0]), sr,
