@@ -85,6 +85,32 @@ constexpr bool is_f16() {
 
 __device__ __forceinline__ int ceil_div(int a, int b) { return (a + b - 1) / b; }
 
+// Global-memory access through a generic pointer.  The compiler emits flat
+// instructions for generic pointers it cannot prove global, and a flat
+// instruction counts on the LDS counter too, so a later LDS wait or barrier
+// also waits for the memory round trip.  Every pointer passed here points
+// into device memory.
+template <typename V>
+__device__ __forceinline__ V gload(const V* p) {
+  return *(const __attribute__((address_space(1))) V*)p;
+}
+template <typename V>
+__device__ __forceinline__ void gstore(V* p, V v) {
+  *(__attribute__((address_space(1))) V*)p = v;
+}
+
+// Lane i of each 16-lane row takes lane (i - n) mod 16 of the same row (DPP
+// row_ror:n), n = 0 .. 3.
+__device__ __forceinline__ float row_ror(float v, int n) {
+  const int i = __builtin_bit_cast(int, v);
+  switch (n) {
+    case 1: return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(i, 0x121, 0xF, 0xF, false));
+    case 2: return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(i, 0x122, 0xF, 0xF, false));
+    case 3: return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(i, 0x123, 0xF, 0xF, false));
+    default: return v;
+  }
+}
+
 // Work entries and slot descriptors are read from memory the kernel also
 // writes through other pointers, so the compiler cannot keep them in scalar
 // registers by itself; they are wave-uniform by construction.  Values that
@@ -359,9 +385,14 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     const int WW = MT + 2 * r;
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
     const int c2_0 = CW * b2;
+    // partial-window output of the entry and (FP16) the unscaling factor, read
+    // here so that their latency stays off the flush
+    float* const out =
+        sd.partial + uni64(gload(sd.poff + z)) + (size_t)(b2 * ceil_div(n1, 16) + b1) * 16 * (CW + 2 * r);
+    float unscale = 1.f;
+    if constexpr (is_f16<T>()) unscale = gload(g.finv + z) * sd.xs_inv;
     const float* Xz = X + (size_t)z * n2 * n1 * kP;
-    float* yw = ywin + w * 16 * WW;
-    for (int o = tid; o < 4 * 16 * WW; o += 256) ywin[o] = 0.f;
+    float* yw = ywin + w * 16 * WW;  // every cell is written by the flush
     for (int o = tid; o < NB * CW * kRingRow; o += 256) ring[o] = (T)0.f;  // pads stay zero
 
     // A operand at e1 = 0 (d1 = -r): lane row lr holds lenslet i1 = 16*b1 + lr + r
@@ -376,12 +407,12 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
       for (int ks = 0; ks < NKS; ++ks) {
         if constexpr (KPL == 1) {
           const int ph = ks * KSTEP + lg;
-          const float v = X[lens * kP + (ph < kP ? ph : kP - 1)];
+          const float v = gload(X + lens * kP + (ph < kP ? ph : kP - 1));
           a[mt][ks] = (ok && ph < kP) ? v : 0.f;
         } else {
           // BF16 shadow rows are 256 phases (zero beyond 225), 16-byte aligned
           const typename M::frag v =
-              *reinterpret_cast<const typename M::frag*>(Xb + lens * kShadowRow + ks * KSTEP + lg * KPL);
+              gload(reinterpret_cast<const typename M::frag*>(Xb + lens * kShadowRow + ks * KSTEP + lg * KPL));
           a[mt][ks] = ok ? v : typename M::frag{};
         }
       }
@@ -527,27 +558,42 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
       }
     }
 
-    // P tile -> this wave's window (rows i1 - 16*b1, columns c2 + d2 - (c2_0 + MT*w - r));
-    // FP16 undoes the PSF and volume pre-scaling here (one plane per entry)
-    float unscale = 1.f;
-    if constexpr (is_f16<T>()) unscale = g.finv[z] * sd.xs_inv;
+    // P tiles -> this wave's window: y[i1][mt + e2] = sum_mt P_mt[i1, e2] (rows
+    // i1 - 16*b1, columns c2 + d2 - (c2_0 + MT*w - r)).  Lane (lr, lg) holds
+    // P_mt[4 lg + j, 16 nt + lr].  Rotating tile mt right by mt lanes within
+    // each 16-lane row brings the terms of column 16 nt + lr to lane lr when
+    // mt <= lr; the terms that wrapped (lr < mt) belong to column
+    // 16 (nt + 1) + lr and are carried into the next tile.  Columns e2 >= K
+    // (padded taps) are zeroed first.  Each window cell then has one writer:
+    // plain LDS stores, summed in a fixed order, and no clearing of the window.
+    // FP16 undoes the PSF and volume pre-scaling here (one plane per entry).
+    static_assert(MT <= 4, "row rotations of up to 3 lanes");
+    floatx4 carry = {0.f, 0.f, 0.f, 0.f};
   #pragma unroll
-    for (int nt = 0; nt < kTpzNT; ++nt) {
-      const int e2 = nt * 16 + lr;
-      if (nt >= NT || e2 >= K) continue;
+    for (int nt = 0; nt <= kTpzNT; ++nt) {
+      if (nt > NT) continue;
+      floatx4 col = carry;
+      carry = floatx4{0.f, 0.f, 0.f, 0.f};
+      if (nt < kTpzNT && nt < NT) {
+        const bool ev = nt * 16 + lr < K;
   #pragma unroll
-      for (int mt = 0; mt < MT; ++mt)
+        for (int mt = 0; mt < MT; ++mt)
   #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-          float v = acc[mt][nt][j];
-          if constexpr (is_f16<T>()) v *= unscale;
-          atomicAdd(yw + (4 * lg + j) * WW + mt + e2, v);
-        }
+          for (int j = 0; j < 4; ++j) {
+            const float v = ev ? acc[mt][nt][j] : 0.f;
+            const float x = row_ror(v, mt);
+            col[j] += lr >= mt ? x : 0.f;
+            carry[j] += lr >= mt ? 0.f : x;
+          }
+      }
+      const int c = nt * 16 + lr;
+      if (c < WW) {
+  #pragma unroll
+        for (int j = 0; j < 4; ++j) yw[(4 * lg + j) * WW + c] = is_f16<T>() ? col[j] * unscale : col[j];
+      }
     }
     __syncthreads();
     const int Wc2 = CW + 2 * r;
-    const int nb1 = ceil_div(n1, 16);
-    float* out = sd.partial + uni64(sd.poff[z]) + (size_t)(b2 * nb1 + b1) * 16 * Wc2;
     for (int o = tid; o < 16 * Wc2; o += 256) {
       const int i = o / Wc2, c = o - i * Wc2;
       float sum = 0.f;
@@ -556,7 +602,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
         const int cc = c - MT * q;
         if (cc >= 0 && cc < WW) sum += ywin[(q * 16 + i) * WW + cc];
       }
-      out[o] = sum;
+      gstore(out + o, sum);
     }
     __syncthreads();  // LDS is reused by the next entry
   }
