@@ -337,7 +337,8 @@ void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
 // lenslet from an LDS ring filled by direct-to-LDS loads.  The B operand (one
 // e1 column of taps) is copied global -> LDS while the previous column's
 // MFMAs run.  The sum over e1 happens inside the MFMA accumulators, so only
-// the final P tile is scattered (a few LDS atomics per wave and plane).
+// the final P tile is scattered, once per entry, by in-register diagonal sums
+// and plain LDS stores.
 constexpr int kTpzNT = 3;      // N-tiles (taps along axis 1): K <= 48, i.e. r <= 23
 // Zero source for injection-ring rows outside the grid (read by the DMA, never
 // written), so the shift needs no per-element select.
