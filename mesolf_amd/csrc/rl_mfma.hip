@@ -384,7 +384,8 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     const int NT = (K + 15) / 16;
     const int nrow = 16 * NT;
     const int WW = MT + 2 * r;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+    // w: wave index, wave-uniform (readfirstlane), so loops and bases over it run on the scalar unit
+    const int tid = threadIdx.x, lane = tid & 63, w = uni(tid >> 6), lr = lane & 15, lg = lane >> 4;
     const int c2_0 = CW * b2;
     // partial-window output of the entry and (FP16) the unscaling factor, read
     // here so that their latency stays off the flush
@@ -393,6 +394,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     float unscale = 1.f;
     if constexpr (is_f16<T>()) unscale = gload(g.finv + z) * sd.xs_inv;
     const float* Xz = X + (size_t)z * n2 * n1 * kP;
+    const T* Xbz = KPL > 1 ? Xb + (size_t)z * n2 * n1 * kShadowRow : nullptr;  // BF16 / FP16 shadow plane
     float* yw = ywin + w * 16 * WW;  // every cell is written by the flush
     for (int o = tid; o < NB * CW * kRingRow; o += 256) ring[o] = (T)0.f;  // pads stay zero
 
@@ -428,17 +430,27 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     // the flush drops, so whatever they hold does not matter.
     const int nslot = K * RV;
     const int nw_inst = (nslot + 255) / 256;
+    // Copy addresses: a wave-uniform (scalar) base plus the lane's 32-bit byte
+    // offset, which for the W copies does not depend on the step and is
+    // computed once per entry.
+    constexpr int kWInst = (16 * kTpzNT * RV + 255) / 256;  // copies per wave for the widest plane
+    int woff[kWInst];
+  #pragma unroll
+    for (int i = 0; i < kWInst; ++i) {
+      const int slot = min(256 * i + 64 * w + lane, nslot - 1);  // tail slots re-read the last vector
+      const int row = (unsigned)slot / RV, pos = (unsigned)slot % RV;
+      const int col = SWZ ? (pos ^ (row & 31)) : pos;
+      woff[i] = (row * PHI + col * VEC) * (int)sizeof(T);
+    }
     auto issue_w = [&](int t, int buf) {
-      const T* src = Wz + (size_t)t * nrow * PHI;
+      const char* src = reinterpret_cast<const char*>(Wz + (size_t)t * nrow * PHI);
       char* dst = reinterpret_cast<char*>(Wb) + (size_t)buf * WSLOT * 16;
-      for (int i = 0; i < nw_inst; ++i) {
-        const int base = 256 * i + 64 * w;
-        const int slot = min(base + lane, nslot - 1);  // tail slots re-read the last vector
-        const int row = slot / RV, pos = slot - row * RV;
-        const int col = SWZ ? (pos ^ (row & 31)) : pos;
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)(src + (size_t)row * PHI + col * VEC),
-            (__attribute__((address_space(3))) void*)(dst + (size_t)base * 16), 16, 0, 0);
+  #pragma unroll
+      for (int i = 0; i < kWInst; ++i) {
+        if (i >= nw_inst) break;
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(src + woff[i]),
+                                         (__attribute__((address_space(3))) void*)(dst + (256 * i + 64 * w) * 16),
+                                         16, 0, 0);
       }
     };
     // injection row needed when entering step t (t >= 1): i1 = 16*b1 + r - t
@@ -450,25 +462,22 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
       const int i1r = 16 * b1 + r - t;
       const bool rowok = i1r >= 0 && i1r < n1;
       const int i1 = min(max(i1r, 0), n1 - 1);
-      T* dst = ring + slot_buf * CW * kRingRow;
+      char* dst = reinterpret_cast<char*>(ring + slot_buf * CW * kRingRow);
       for (int q = w; q < CW * RPASS; q += 4) {
         const int c = q / RPASS, part = q - c * RPASS;
         const bool ok = rowok && c2_0 + c < n2;
         const int c2 = min(c2_0 + c, n2 - 1);
-        const size_t lens = ((size_t)z * n2 + c2) * n1 + i1;
-        const int d = part * 64 + lane;  // dword within the row
-        const void* src;
+        const size_t lens = (size_t)c2 * n1 + i1;  // within the plane
+        const int d = part * 64 + lane;          // dword within the row
         if constexpr (KPL == 1) {
           if (d >= kP) continue;
-          src = ok ? (const void*)(X + lens * kP + d) : (const void*)(g_zero_row + d);
-        } else {
-          src = ok ? (const void*)(reinterpret_cast<const char*>(Xb + lens * kShadowRow) + 4 * d)
-                   : (const void*)(g_zero_row + d);
         }
-        __builtin_amdgcn_global_load_lds(
-            (const __attribute__((address_space(1))) void*)src,
-            (__attribute__((address_space(3))) void*)(reinterpret_cast<char*>(dst + c * kRingRow) + 4 * part * 64),
-            4, 0, 0);
+        const char* row = ok ? (KPL == 1 ? reinterpret_cast<const char*>(Xz + lens * kP)
+                                         : reinterpret_cast<const char*>(Xbz + lens * kShadowRow))
+                             : reinterpret_cast<const char*>(g_zero_row);
+        __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(row + 4 * d),
+                                         (__attribute__((address_space(3))) void*)(dst + (c * kRingRow) * (int)sizeof(T) + 4 * part * 64),
+                                         4, 0, 0);
       }
     };
     const int n_batch = nw_inst + CW * RPASS / 4;  // DMA instructions per wave, step t >= 1
@@ -788,7 +797,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const int K = 2 * r + 1, ntaps = K * K;
     const int epad = uni(g.epad[z]);
     const int Wd = 16 + 2 * r;
-    const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
+    // w: wave index, wave-uniform (readfirstlane), so loops and bases over it run on the scalar unit
+    const int tid = threadIdx.x, lane = tid & 63, w = uni(tid >> 6), lr = lane & 15, lg = lane >> 4;
     const int nks = epad / KSTEP;
 
     // Fused multiplicative update (recon:130-137), set up before the K loop so
@@ -947,11 +957,13 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         T* Bq = reinterpret_cast<T*>(smem);  // ring (aliases the dead staging window Rs)
         constexpr int kSlice = NT * 16 * 32;  // elements per K-step slice
         const int ncopy = w < NT ? (NT - w + 3) / 4 : 0;  // copies per step issued by this wave
+        // the lane's byte offset within a copy: row lane / 4, 8 taps at (lane % 4) * 8
+        const int tap_lane = ((lane >> 2) * epad + (lane & 3) * 8) * (int)sizeof(T);
         auto issue_taps = [&](int ks, int slot) {
           const int kc = min(ks, nks - 1);
           char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
           for (int nt = w; nt < NT; nt += 4) {
-            const T* src = Bz + (size_t)(nt * 16 + (lane >> 2)) * epad + kc * 32 + (lane & 3) * 8;
+            const char* src = reinterpret_cast<const char*>(Bz + (size_t)nt * 16 * epad + kc * 32) + tap_lane;
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                              (__attribute__((address_space(3))) void*)(dst + nt * 1024),
                                              16, 0, 0);
