@@ -99,6 +99,10 @@ __device__ __forceinline__ void gstore(V* p, V v) {
   *(__attribute__((address_space(1))) V*)p = v;
 }
 
+// Dword 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
+// no swizzle, no index stride).
+constexpr int kBufferDword3 = 0x00020000;
+
 // Lane i of each 16-lane row takes lane (i - n) mod 16 of the same row (DPP
 // row_ror:n), n = 0 .. 3.
 __device__ __forceinline__ float row_ror(float v, int n) {
@@ -395,6 +399,9 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     if constexpr (is_f16<T>()) unscale = gload(g.finv + z) * sd.xs_inv;
     const float* Xz = X + (size_t)z * n2 * n1 * kP;
     const T* Xbz = KPL > 1 ? Xb + (size_t)z * n2 * n1 * kShadowRow : nullptr;  // BF16 / FP16 shadow plane
+    // the shadow plane as a buffer (a plane is below 2^31 bytes, checked at patch creation)
+    const __amdgpu_buffer_rsrc_t shr = __builtin_amdgcn_make_buffer_rsrc(
+        const_cast<T*>(Xbz), 0, KPL > 1 ? (int)((size_t)n2 * n1 * kShadowRow * sizeof(T)) : 0, kBufferDword3);
     float* yw = ywin + w * 16 * WW;  // every cell is written by the flush
     for (int o = tid; o < NB * CW * kRingRow; o += 256) ring[o] = (T)0.f;  // pads stay zero
 
@@ -480,7 +487,8 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
                                          4, 0, 0);
       }
     };
-    const int n_batch = nw_inst + CW * RPASS / 4;  // DMA instructions per wave, step t >= 1
+    // DMA instructions per wave of a batch t >= 1 (16-bit: every W slot copy is issued)
+    const int n_batch = (KPL == 1 ? nw_inst : kWInst) + CW * RPASS / 4;
 
     floatx4 acc[MT][kTpzNT];
   #pragma unroll
@@ -497,11 +505,47 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     }
     wait_vm_le(max(0, min(K, NB - 1) - 1) * n_batch);  // batch 0 landed
     __syncthreads();
+    // 16-bit: the copies of batch t + NB - 1 are spread over tile 0's K-steps
+    // (one injection copy and at most one W copy per K-step, no branches), so
+    // the eight waves of a CU do not queue their copies at the address unit
+    // all at once.  The last step re-copies the last column and a row past the
+    // grid (zeros) into the free buffer, which nothing reads.
+    constexpr int kInj = CW * RPASS / 4;  // injection copies per wave and step
+    static_assert(KPL == 1 || (kInj <= NKS && kWInst <= NKS), "one copy of each kind per K-step");
     for (int t = 0; t < K; ++t) {
-      if (t + NB - 1 < K) {
-        issue_w(t + NB - 1, (t + NB - 1) % NB);
-        issue_inj(t + NB - 1, (t + NB - 1) % NB);
+      const int tn = min(t + NB - 1, K - 1), bn = (t + NB - 1) % NB;
+      if constexpr (KPL == 1) {
+        if (t + NB - 1 < K) {
+          issue_w(tn, bn);
+          issue_inj(tn, bn);
+        }
       }
+      const char* w_src = reinterpret_cast<const char*>(Wz + (size_t)tn * nrow * PHI);
+      char* w_dst = reinterpret_cast<char*>(Wb) + (size_t)bn * WSLOT * 16;
+      const int i1r_n = 16 * b1 + r - tn;
+      const bool rowok_n = i1r_n >= 0 && i1r_n < n1;
+      const int i1_n = min(max(i1r_n, 0), n1 - 1);
+      char* inj_dst = reinterpret_cast<char*>(ring + bn * CW * kRingRow);
+      auto issue_piece = [&](int k) __attribute__((always_inline)) {
+        if constexpr (KPL > 1) {
+          if (k < kWInst)
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(w_src + woff[k]),
+                                             (__attribute__((address_space(3))) void*)(w_dst + (256 * k + 64 * w) * 16),
+                                             16, 0, 0);
+          if (k < kInj) {
+            // rows outside the grid: an offset past the plane, which the
+            // buffer load returns as zeros (no branch, no zero-row pointer)
+            const int q = w + 4 * k, c = q / RPASS, part = q - c * RPASS;
+            const bool ok = rowok_n && c2_0 + c < n2;
+            const int c2 = min(c2_0 + c, n2 - 1);
+            const unsigned voff = ok ? (unsigned)(((c2 * n1 + i1_n) * kShadowRow + part * 128) * 2) + 4u * lane
+                                     : 0x80000000u;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                shr, (__attribute__((address_space(3))) void*)(inj_dst + (c * kRingRow) * (int)sizeof(T) + 4 * part * 64),
+                4, voff, 0, 0, 0);
+          }
+        }
+      };
       const T* Wc = Wb + (size_t)(t % NB) * WSLOT * VEC;
       auto load_b = [&](int ks, int nt) {
         const int row = nt * 16 + lr;
@@ -523,6 +567,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
           const typename M::frag bf = load_b(ks, nt);
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
+          if constexpr (nt == 0) issue_piece(ks);
         }
       };
       static_assert(kTpzNT == 3, "three N-tiles");
@@ -773,9 +818,6 @@ __device__ __forceinline__ int norm_class(int a, int n, int r) {
   return a < r ? a : (a > n - 1 - r ? 2 * r - (n - 1 - a) : r);
 }
 
-// Dword 3 of a raw buffer resource on gfx9-family parts (32-bit data format,
-// no swizzle, no index stride).
-constexpr int kBufferDword3 = 0x00020000;
 
 template <typename T, int NT>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
