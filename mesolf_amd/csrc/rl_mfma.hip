@@ -1040,6 +1040,10 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
 
       // fused multiplicative update (recon:130-137), continued from the loads
       // issued before the K loop.
+      // Stores take no branch except in the one N-tile that reaches past phase
+      // 224 (the last of 225): lanes outside the grid store at an offset past
+      // the plane, which the buffer store drops (the plane is below 2^31
+      // bytes, so 2^31 + a row offset is out of range and does not wrap).
       load_x(MT - 1, MT);
   #pragma unroll
       for (int mt = 0; mt < MT; ++mt) {
@@ -1047,6 +1051,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         const int a2 = 16 * b2 + 4 * w + mt;
         const bool ok = ok1 && a2 < n2;
         const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
+        const unsigned xbase = ok ? 4u * (unsigned)(l * kP) : 0x80000000u;
+        const unsigned sbase = ok ? 2u * (unsigned)(l * kShadowRow) : 0x80000000u;
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
           floatx4_t xn;
@@ -1061,6 +1067,18 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
             xn[j] = v < 0.f ? 0.f : v;
           }
           const int ph = ph0 + 16 * nt;
+          if (grp * 16 * NT + 16 * nt + 16 <= kP) {  // wave-uniform: the whole N-tile is in the row
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, (int)(xbase + 4u * ph), 0, 0);
+            if constexpr (KPL > 1) {  // the 16-bit paths always carry the shadow
+              floatx4_t sv = xn;
+              if constexpr (is_f16<T>()) sv *= sd.xs;
+              uintx2 h;
+              h[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
+              h[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
+              __builtin_amdgcn_raw_buffer_store_b64(h, sr, (int)(sbase + 2u * ph), 0, 0);
+            }
+            continue;
+          }
           const int xo = 4 * (l * kP + ph);
           const int so = 2 * (l * kShadowRow + ph);
           if (ok && ph + 4 <= kP) {
