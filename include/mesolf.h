@@ -126,8 +126,10 @@ mlf_status mlf_patch_project_all(mlf_patch p, float* sensor);
 mlf_status mlf_patch_sample_filter(mlf_patch p);
 /* Time of the last mlf_patch_run on the device, milliseconds (HIP events). */
 mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms);
-/* Per-kernel timing: while enabled, every launch of mlf_patch_run is
- * bracketed by HIP events on the context stream.  mlf_patch_kernel_times
+/* Per-kernel timing: while enabled, the launches of mlf_patch_run are
+ * bracketed by HIP events on the context stream: enable = 1 times every view
+ * step, enable = N > 1 every N-th view step (fewer events in the stream; the
+ * sample filter is always timed), 0 turns timing off.  mlf_patch_kernel_times
  * returns, per kernel class, the summed milliseconds and the launch count
  * since the previous read, then resets them.  Classes (MLF_KERNEL_CLASSES):
  * 0 forward projection, 1 ratio, 2 backprojection + update, 3 sample_filter. */

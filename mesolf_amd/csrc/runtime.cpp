@@ -688,7 +688,7 @@ mlf_status mlf_psf_destroy(mlf_psf psf) {
 
 // Bracket a launch with a timing event pair (no-op unless timing is on).
 static mlf_status tim_begin(mlf_patch_s* p, int cls) {
-  if (!p->timing) return MLF_OK;
+  if (!p->tim_now) return MLF_OK;
   if (p->ev_used + 2 > p->ev_pool.size()) {
     for (int k = 0; k < 2; ++k) {
       hipEvent_t e;
@@ -703,7 +703,7 @@ static mlf_status tim_begin(mlf_patch_s* p, int cls) {
 }
 
 static mlf_status tim_end(mlf_patch_s* p) {
-  if (!p->timing) return MLF_OK;
+  if (!p->tim_now) return MLF_OK;
   MLF_HIP(hipEventRecord(p->ev_pool[p->ev_used + 1], p->psf->ctx->stream));
   p->ev_used += 2;
   return MLF_OK;
@@ -834,12 +834,14 @@ static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, fl
 }
 
 static mlf_status patch_forward(mlf_patch_s* p, int k) {
+  p->tim_now = p->timing;
   return patch_forward_ratio(p, k, nullptr, p->y.ptr);
 }
 
 static mlf_status patch_view_step(mlf_patch_s* p, int k) {
   mlf_psf_s* ps = p->psf;
   hipStream_t st = ps->ctx->stream;
+  p->tim_now = p->timing && p->tim_seq++ % p->timing_stride == 0;
   MLF_CHECK(patch_forward_ratio(p, k, p->LF.ptr, nullptr));
   MLF_CHECK(tim_begin(p, 2));
   if (p->reference) {
@@ -1020,6 +1022,7 @@ mlf_status mlf::patch_filter(mlf_patch_s* p) {
   if (p->f_nplanes == 0) return MLF_OK;
   hipStream_t st = p->psf->ctx->stream;
   const FilterPlan fp = filter_plan_of(p);
+  p->tim_now = p->timing;
   MLF_CHECK(tim_begin(p, 3));
   MLF_LAUNCH(launch_filter_pass1(p->X.ptr, p->tmp.ptr, fp, p->n1, p->n2, p->pb.ptr, p->f_maxlen, st));
   MLF_LAUNCH(launch_filter_pass2(p->tmp.ptr, p->X.ptr, fp, p->n1, p->n2, p->pa.ptr, p->f_maxlen, st));
@@ -1260,8 +1263,12 @@ mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms) {
 mlf_status mlf_patch_set_timing(mlf_patch p, int32_t enable) {
   clear_error();
   if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (enable < 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "negative timing stride");
   std::lock_guard<std::mutex> lk(p->mu);
   p->timing = enable != 0;
+  p->timing_stride = enable > 1 ? enable : 1;
+  p->tim_seq = 0;
+  p->tim_now = false;
   p->ev_used = 0;
   return MLF_OK;
 }

@@ -119,8 +119,11 @@ struct mlf_patch_s {
   mlf::NormTable* norm = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
-  // per-kernel timing
+  // per-kernel timing (of every timing_stride-th view step; the filter always)
   bool timing = false;
+  int timing_stride = 1;
+  int64_t tim_seq = 0;   // view steps since timing was enabled
+  bool tim_now = false;  // the launches being enqueued are timed
   std::vector<hipEvent_t> ev_pool;
   std::vector<int> ev_class;  // class of the pair (2k, 2k+1)
   size_t ev_used = 0;

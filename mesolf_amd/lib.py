@@ -331,8 +331,12 @@ class Patch:
 
     KERNEL_CLASSES = ("forward", "ratio", "backward_update", "sample_filter")
 
-    def set_timing(self, enable: bool = True):
-        _check(self._lib.mlf_patch_set_timing(self.handle, int(bool(enable))))
+    def set_timing(self, enable: bool = True, stride: int = 1):
+        """Bracket launches with HIP events: every `stride`-th view step (the
+        sample filter always) while enabled."""
+        if stride < 1:
+            raise ValueError("stride must be >= 1")
+        _check(self._lib.mlf_patch_set_timing(self.handle, int(stride) if enable else 0))
 
     def kernel_times(self):
         """{class: (total_ms, launches)} since the previous read."""

@@ -138,6 +138,32 @@ def test_determinism_bitwise(ctx, small_problem):
     assert np.array_equal(outs[0], outs[1])
 
 
+def test_kernel_timing_stride(ctx, small_problem):
+    """Timing every 4th view step brackets 4 of 16 view steps; timing changes
+    no result."""
+    tabs, cp, truth, LF = small_problem
+    wl = synth.SMALL
+    psf = _psf(ctx, tabs)
+    outs = []
+    for stride in (0, 1, 4):
+        p = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision="bf16")
+        p.set_image(LF)
+        p.init_volume()
+        p.set_timing(stride > 0, stride=max(stride, 1))
+        p.kernel_times()
+        p.run(1, views_limit=16, filter=False)
+        kt = p.kernel_times()
+        p.set_timing(False)
+        want = 0 if stride == 0 else 16 // stride
+        for c in ("forward", "ratio", "backward_update"):
+            assert kt[c][1] == want, (stride, c, kt[c])
+            assert (kt[c][0] > 0) == (want > 0)
+        outs.append(p.get_volume())
+    assert np.array_equal(outs[0], outs[1]) and np.array_equal(outs[0], outs[2])
+    with pytest.raises(ValueError):
+        p.set_timing(True, stride=0)
+
+
 def test_project_all_parity(ctx):
     wl = synth.TINY
     tabs = tables.synthetic_compact(wl)
