@@ -34,11 +34,12 @@ def oracle_compact_from_abi(wl, tabs):
     return cp
 
 
-def synthetic_problem(wl, seed=1):
-    """(tabs, oracle CompactPSF, truth, LF) for a synthetic workload; the
-    measurement is made with the FP64 oracle forward."""
+def synthetic_problem(wl, seed=1, views=None):
+    """(tabs, oracle CompactPSF, truth, LF) for a synthetic workload (all 177
+    aperture views, or the subset ``views``); the measurement is made with the
+    FP64 oracle forward."""
     rng = np.random.default_rng(seed)
-    tabs = tables.synthetic_compact(wl)
+    tabs = tables.synthetic_compact(wl, views=views)
     cp = oracle_compact_from_abi(wl, tabs)
     truth = synth.sphere_truth(wl, rng, dtype=np.float64)
     Xl = O.to_lenslet(truth, 15)

@@ -56,6 +56,44 @@ def test_forward_projection_parity(ctx, wl, prec):
         assert rel_l2(y, ref) <= TOL[prec][0], (k, rel_l2(y, ref))
 
 
+WIDE = synth.Workload("wide_37x21", 37, 21, (-100, -84, -40, 0), 1, 1)
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_forward_projection_parity_wide_radii(ctx, prec):
+    """Planes of radius 17, 15, 8 and 2 (K = 35, 31, 17, 5 taps along an axis):
+    every N-tile count of the 16x16 path and both ends of the 32x32 path, on a
+    grid smaller than the widest kernel."""
+    views = synth.aperture_views()[::30]
+    tabs = tables.synthetic_compact(WIDE, views=views)
+    assert sorted(int(r) for r in tabs["rf"]) == [2, 8, 15, 17]
+    cp = oracle_compact_from_abi(WIDE, tabs)
+    psf = _psf(ctx, tabs)
+    p = mesolf_amd.Patch(psf, WIDE.n1, WIDE.n2, precision=prec)
+    rng = np.random.default_rng(12)
+    X = rng.random((WIDE.I1, WIDE.I2, WIDE.Z)).astype(np.float32)
+    p.set_volume(X)
+    Xl = O.to_lenslet(X.astype(np.float64), 15)
+    for k in range(psf.nviews):
+        y = p.forward_view(k)
+        ref = O.forward_compact(Xl, cp.fwd[k], cp.rf)
+        assert rel_l2(y, ref) <= TOL[prec][0], (k, rel_l2(y, ref))
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_view_steps_parity_wide_radii(ctx, prec):
+    """Forward, ratio and backprojection + update of all six views of the
+    wide-radius PSF (windows of up to 50 x 50 lenslets, up to 44 K-steps)."""
+    tabs, cp, truth, LF = synthetic_problem(WIDE, seed=5, views=synth.aperture_views()[::30])
+    p = mesolf_amd.Patch(_psf(ctx, tabs), WIDE.n1, WIDE.n2, precision=prec)
+    p.set_image(LF)
+    p.init_volume()
+    p.run(1, filter=False)
+    got = p.get_volume(f64=True)
+    ref = O.rl_compact(LF.astype(np.float64), cp, 1, filter_planes=False)
+    assert rel_l2(got, ref) <= TOL[prec][1] / 4
+
+
 def test_volume_roundtrip_layout(ctx):
     wl = synth.TINY
     psf = _psf(ctx, tables.synthetic_compact(wl))
