@@ -697,16 +697,49 @@ __global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg,
     const int i1 = ok ? i % n1 : 0, i2 = ok ? i / n1 : 0;
     const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, CW);
     const int b1 = i1 / 16;
+    // The slice's planes go in chunks of kZC: the radii and offsets of a chunk
+    // are read together, then up to kW windows per plane (all of them for
+    // 16-column blocks, r <= 23), and only then summed, in plane then window
+    // order (FP64; an unused slot adds +0.0, which changes no sum).  A plane
+    // with more windows (narrower blocks) reads the rest one by one.
     double y = 0.0;
     if (ok) {
-      for (int z = z0; z < z1; ++z) {
-        const int r = g.r[z];  // the plane slice differs between the half-waves
-        const int Wc2 = CW + 2 * r;
-        const int b2h = min(nb2 - 1, (i2 + r) / CW);
-        const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
-        const float* pz = sd.partial + sd.poff[z];
-        for (int b2 = b2l; b2 <= b2h; ++b2)
-          y += (double)pz[(size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r)];
+      constexpr int kZC = 8, kW = 4;
+      for (int zc = z0; zc < z1; zc += kZC) {
+        int rr[kZC];
+        const float* pz[kZC];
+  #pragma unroll
+        for (int q = 0; q < kZC; ++q) {
+          const int zq = min(zc + q, z1 - 1);  // the plane slice differs between the half-waves
+          rr[q] = gload(g.r + zq);
+          pz[q] = sd.partial + gload(sd.poff + zq);
+        }
+        float vv[kZC][kW];
+  #pragma unroll
+        for (int q = 0; q < kZC; ++q) {
+          const int r = rr[q], Wc2 = CW + 2 * r;
+          const int b2h = min(nb2 - 1, (i2 + r) / CW);
+          const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
+  #pragma unroll
+          for (int wi = 0; wi < kW; ++wi) {
+            const int b2 = b2l + wi;
+            const bool use = zc + q < z1 && b2 <= b2h;
+            const size_t idx = use ? (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r) : 0;
+            const float t = gload(pz[q] + idx);
+            vv[q][wi] = use ? t : 0.f;
+          }
+        }
+  #pragma unroll
+        for (int q = 0; q < kZC; ++q) {
+          if (zc + q >= z1) break;
+  #pragma unroll
+          for (int wi = 0; wi < kW; ++wi) y += (double)vv[q][wi];
+          const int r = rr[q], Wc2 = CW + 2 * r;
+          const int b2h = min(nb2 - 1, (i2 + r) / CW);
+          const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
+          for (int b2 = b2l + kW; b2 <= b2h; ++b2)
+            y += (double)gload(pz[q] + (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r));
+        }
       }
     }
     part[sl][li] = y;
