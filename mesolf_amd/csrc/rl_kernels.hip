@@ -1,8 +1,8 @@
 // Phase-space Richardson-Lucy kernels (FP32 reference kernels).
 //
 // Volume layout in HBM ("lenslet-blocked"): X[z][L2][L1][s2][s1], i.e. the
-// 225 phases of one lenslet are contiguous; pixel (p1, p2) of the MATLAB
-// array is (15*L1 + s1, 15*L2 + s2).  The compact PSF tables are
+// 225 phases of one lenslet are contiguous, in rows of kXRow floats whose
+// pad stays zero; pixel (p1, p2) of the MATLAB array is (15*L1 + s1, 15*L2 + s2).  The compact PSF tables are
 // [view][z][e][phase] with e = (d2 + r)*(2r+1) + (d1 + r).
 #include "internal.h"
 
@@ -13,7 +13,7 @@ namespace {
 __device__ __forceinline__ size_t blk_index(int p1, int p2, int z, int n1, int n2) {
   const int L1 = p1 / kN, s1 = p1 - L1 * kN;
   const int L2 = p2 / kN, s2 = p2 - L2 * kN;
-  return ((((size_t)z * n2 + L2) * n1 + L1) * kN + s2) * kN + s1;
+  return (((size_t)z * n2 + L2) * n1 + L1) * kXRow + s2 * kN + s1;
 }
 
 __global__ void k_to_blocked(const float* __restrict__ src, float* __restrict__ dst, int n1, int n2,
@@ -29,7 +29,7 @@ __global__ void k_to_blocked(const float* __restrict__ src, float* __restrict__ 
     const int L2 = t % n2; t /= n2;
     const int z = (int)t;
     const int p1 = L1 * kN + s1, p2 = L2 * kN + s2;
-    dst[o] = src[(size_t)p1 + (size_t)I1 * (p2 + (size_t)I2 * z)];
+    dst[(o / kP) * kXRow + s2 * kN + s1] = src[(size_t)p1 + (size_t)I1 * (p2 + (size_t)I2 * z)];
   }
 }
 
@@ -109,7 +109,7 @@ __global__ void k_fill_from_sum(float* __restrict__ dst, size_t n, const double*
   const float v = (float)(inv_numel * (*sum));
   for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < n;
        o += (size_t)gridDim.x * blockDim.x)
-    dst[o] = v;
+    dst[o] = o % kXRow < kP ? v : 0.f;  // volume rows: the pad stays zero
 }
 
 // One thread per (output lenslet i, plane z):
@@ -130,7 +130,7 @@ __global__ void k_forward_fp32(const float* __restrict__ X, const float* __restr
     for (int d1 = -r; d1 <= r; ++d1) {
       const int L1 = i1 - d1;
       if (L1 < 0 || L1 >= n1) continue;
-      const float* x = X + (((size_t)z * n2 + L2) * n1 + L1) * kP;
+      const float* x = X + (((size_t)z * n2 + L2) * n1 + L1) * kXRow;
       const float* w = Wz + ((size_t)(d2 + r) * K + (d1 + r)) * kP;
 #pragma unroll 5
       for (int p = 0; p < kP; ++p) acc = fmaf(x[p], w[p], acc);
@@ -196,7 +196,7 @@ __global__ void k_backward_update_fp32(float* __restrict__ X, const float* __res
       B1 += wv;
     }
   }
-  const size_t idx = (((size_t)z * n2 + a2) * n1 + a1) * kP + p;
+  const size_t idx = (((size_t)z * n2 + a2) * n1 + a1) * kXRow + p;
   const float x = X[idx];
   float C = x * B / B1;
   if (isnan(C) || isinf(C) || C < 0.f) C = 0.f;
@@ -245,7 +245,7 @@ __global__ __launch_bounds__(256) void k_filter_line(float* __restrict__ X, floa
   __shared__ double sh[4];
   const int q = blockIdx.z;
   const int z = fp.plane[q];
-  const size_t plane = (size_t)n1 * n2 * kP;
+  const size_t plane = (size_t)n1 * n2 * kXRow;  // X and tmp share the row layout
   const float* src = AXIS == 1 ? X + (size_t)z * plane : tmp + (size_t)q * plane;
   float* dst = AXIS == 1 ? tmp + (size_t)q * plane : X + (size_t)z * plane;
   const int na = AXIS == 1 ? n1 : n2;
@@ -265,7 +265,7 @@ __global__ __launch_bounds__(256) void k_filter_line(float* __restrict__ X, floa
   };
   for (int o = threadIdx.x; o < nl * kP; o += blockDim.x) {
     const int li = o / kP;
-    lds[o] = src[lens_of(L0 + li) * kP + (o - li * kP)];
+    lds[o] = src[lens_of(L0 + li) * kXRow + (o - li * kP)];
   }
   __syncthreads();
   // unwrapped pixel u (from lenslet L0) at other-axis phase so -> LDS
@@ -299,7 +299,7 @@ __global__ __launch_bounds__(256) void k_filter_line(float* __restrict__ X, floa
       const int p = pb + m;
       if (p >= P1) break;
       const int L = p / kN, sa = p - L * kN;
-      const size_t e = lens_of(L) * kP + (AXIS == 1 ? so * kN + sa : sa * kN + so);
+      const size_t e = lens_of(L) * kXRow + (AXIS == 1 ? so * kN + sa : sa * kN + so);
       own += AXIS == 1 ? (double)src[e] : (double)acc[m];
       dst[e] = acc[m];
     }
@@ -324,11 +324,11 @@ __global__ void k_filter_scale(float* __restrict__ X, FilterPlan fp, int n1, int
   if (threadIdx.x == 0) scale = b / a;
   __syncthreads();
   const double s = scale;
-  const size_t plane = (size_t)n1 * n2 * kP;
+  const size_t plane = (size_t)n1 * n2 * kXRow;  // rows of kXRow; the pad is left alone
   float* Xz = X + (size_t)z * plane;
   for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < plane;
        o += (size_t)gridDim.x * blockDim.x)
-    Xz[o] = (float)((double)Xz[o] * s);
+    if (o % kXRow < kP) Xz[o] = (float)((double)Xz[o] * s);
 }
 
 inline int grid_for(size_t n, int threads, int cap = 65536) {

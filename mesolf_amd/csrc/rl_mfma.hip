@@ -221,7 +221,7 @@ void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
       for (int mt = 0; mt < MT; ++mt) {
         const int a2 = BM2 * b2 + MT * w + mt;
         const bool ok = a1 < n1 && a2 < n2;
-        const float* xp = X + (((size_t)z * n2 + (ok ? a2 : 0)) * n1 + (ok ? a1 : 0)) * kP;
+        const float* xp = X + (((size_t)z * n2 + (ok ? a2 : 0)) * n1 + (ok ? a1 : 0)) * kXRow;
         // branch-free: load from a clamped in-row phase, select zero afterwards
   #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
@@ -397,7 +397,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
         sd.partial + uni64(gload(sd.poff + z)) + (size_t)(b2 * ceil_div(n1, 16) + b1) * 16 * (CW + 2 * r);
     float unscale = 1.f;
     if constexpr (is_f16<T>()) unscale = gload(g.finv + z) * sd.xs_inv;
-    const float* Xz = X + (size_t)z * n2 * n1 * kP;
+    const float* Xz = X + (size_t)z * n2 * n1 * kXRow;
     const T* Xbz = KPL > 1 ? Xb + (size_t)z * n2 * n1 * kShadowRow : nullptr;  // BF16 / FP16 shadow plane
     // the shadow plane as a buffer (a plane is below 2^31 bytes, checked at patch creation)
     const __amdgpu_buffer_rsrc_t shr = __builtin_amdgcn_make_buffer_rsrc(
@@ -417,7 +417,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
       for (int ks = 0; ks < NKS; ++ks) {
         if constexpr (KPL == 1) {
           const int ph = ks * KSTEP + lg;
-          const float v = gload(X + lens * kP + (ph < kP ? ph : kP - 1));
+          const float v = gload(X + lens * kXRow + (ph < kP ? ph : kP - 1));
           a[mt][ks] = (ok && ph < kP) ? v : 0.f;
         } else {
           // BF16 shadow rows are 256 phases (zero beyond 225), 16-byte aligned
@@ -479,7 +479,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
         if constexpr (KPL == 1) {
           if (d >= kP) continue;
         }
-        const char* row = ok ? (KPL == 1 ? reinterpret_cast<const char*>(Xz + lens * kP)
+        const char* row = ok ? (KPL == 1 ? reinterpret_cast<const char*>(Xz + lens * kXRow)
                                          : reinterpret_cast<const char*>(Xbz + lens * kShadowRow))
                              : reinterpret_cast<const char*>(g_zero_row);
         __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)(row + 4 * d),
@@ -889,7 +889,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const int C1 = uni(sd.C1[z]), C2 = norm_classes(n2, r);
     const size_t nlp = (size_t)n2 * n1;  // lenslets per plane
     const __amdgpu_buffer_rsrc_t xr =
-        __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kP, 0, (int)(nlp * kP * 4), kBufferDword3);
+        __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kXRow, 0, (int)(nlp * kXRow * 4), kBufferDword3);
     const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
         const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(sd.toffT[z])), 0, C2 * C1 * kP * 4,
         kBufferDword3);
@@ -980,7 +980,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         for (int mt = 0; mt < MT; ++mt) {
           if (mt < mt0 || mt >= mt1) continue;
           const int a2 = 16 * b2 + 4 * w + mt;
-          const int xo = ((a2 < n2 ? a2 : n2 - 1) * n1 + a1c) * kP;
+          const int xo = ((a2 < n2 ? a2 : n2 - 1) * n1 + a1c) * kXRow;
   #pragma unroll
           for (int nt = 0; nt < NT; ++nt)
             xv[mt][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0) + 64 * nt, 0, 0));
@@ -1084,7 +1084,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         const int a2 = 16 * b2 + 4 * w + mt;
         const bool ok = ok1 && a2 < n2;
         const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
-        const unsigned xbase = ok ? 4u * (unsigned)(l * kP) : 0x80000000u;
+        const unsigned xbase = ok ? 4u * (unsigned)(l * kXRow) : 0x80000000u;
         const unsigned sbase = ok ? 2u * (unsigned)(l * kShadowRow) : 0x80000000u;
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
@@ -1112,7 +1112,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
             }
             continue;
           }
-          const int xo = 4 * (l * kP + ph);
+          const int xo = 4 * (l * kXRow + ph);
           const int so = 2 * (l * kShadowRow + ph);
           if (ok && ph + 4 <= kP) {
             __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
@@ -1296,7 +1296,7 @@ __global__ void k_make_shadow(const float* __restrict__ X, T* __restrict__ Xb, s
        o += (size_t)gridDim.x * blockDim.x) {
     const size_t l = o / kShadowRow;
     const int ph = (int)(o - l * kShadowRow);
-    const float v = ph < kP ? X[l * kP + ph] : 0.f;
+    const float v = ph < kP ? X[l * kXRow + ph] : 0.f;
     Xb[o] = (T)(is_f16<T>() ? v * xs : v);
   }
 }

@@ -760,7 +760,8 @@ static mlf_status build_filter_plan(mlf_patch_s* p) {
   p->nblk = filter_partial_blocks(p->n1, p->n2);
   MLF_CHECK(p->pb.alloc((size_t)p->nblk * p->f_nplanes));
   MLF_CHECK(p->pa.alloc((size_t)p->nblk * p->f_nplanes));
-  MLF_CHECK(p->tmp.alloc((size_t)p->f_nplanes * I1 * I2));
+  // same row layout as the volume (rows of kXRow floats; the pad is never read)
+  MLF_CHECK(p->tmp.alloc((size_t)p->f_nplanes * p->n1 * p->n2 * kXRow));
   return MLF_OK;
 }
 
@@ -1040,6 +1041,9 @@ size_t mlf::patch_device_bytes(const mlf_patch_s* p) {
   return b;
 }
 
+// voxels of the patch's volume (the MATLAB array's element count)
+static size_t patch_numel(const mlf_patch_s* p) { return (size_t)p->n1 * p->n2 * p->Z * kP; }
+
 size_t mlf::patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int n2) {
   const Precision kp = kernel_precision(prec);
   int rfmax = 0;
@@ -1047,7 +1051,7 @@ size_t mlf::patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int 
   const bool tpz = rfmax <= tpz_max_radius();
   const int cw = 4 * (kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16);
   const size_t n = (size_t)n1 * n2, Z = (size_t)ps->Z;
-  size_t b = 4 * n * Z * kP * 2;  // volume + sample_filter temporary
+  size_t b = 4 * n * Z * kXRow * 2;  // volume + sample_filter temporary
   b += 4 * n * kP + 4 * n * Z + 8 * n;
   b += 4 * (size_t)partial_elems(ps, tpz, cw, fwd_bm2(kp), n1, n2, nullptr);
   if (kp != Precision::kF32 && tpz) b += 2 * n * Z * kShadowRow;
@@ -1068,7 +1072,7 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   if (prec != MLF_PRECISION_FP32 && prec != MLF_PRECISION_BF16 && prec != MLF_PRECISION_FP16)
     return set_error(MLF_ERR_UNSUPPORTED, "precision %d not supported", (int)prec);
   // kernels address one plane of the volume with 32-bit byte offsets
-  if ((double)n1 * n2 * kP * sizeof(float) >= 2147483648.0)
+  if ((double)n1 * n2 * kXRow * sizeof(float) >= 2147483648.0)
     return set_error(MLF_ERR_UNSUPPORTED, "lenslet grid %lld x %lld: a plane exceeds 2 GiB",
                      (long long)n1, (long long)n2);
   MLF_CHECK(ctx_activate(psf->ctx));
@@ -1093,7 +1097,10 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   p->Z = psf->Z;
   p->prec = prec;
   const size_t n = (size_t)n1 * n2;
-  MLF_CHECK(p->X.alloc(n * p->Z * kP));
+  // volume rows are kXRow floats; the pad is zeroed here and never written again
+  MLF_CHECK(p->X.alloc(n * p->Z * kXRow));
+  MLF_HIP(hipMemsetAsync(p->X.ptr, 0, p->X.bytes(), psf->ctx->stream));
+  MLF_HIP(hipStreamSynchronize(psf->ctx->stream));
   MLF_CHECK(p->LF.alloc(n * kP));
   MLF_CHECK(p->part.alloc(n * p->Z));
   MLF_CHECK(p->R.alloc(n));
@@ -1188,14 +1195,15 @@ mlf_status mlf_patch_set_volume(mlf_patch p, const float* X) {
   std::lock_guard<std::mutex> lk(p->mu);
   MLF_CHECK(ctx_activate(p->psf->ctx));
   hipStream_t st = p->psf->ctx->stream;
+  const size_t numel = patch_numel(p);
   DevBuf<float> tmp;
-  MLF_CHECK(tmp.alloc(p->X.count));
+  MLF_CHECK(tmp.alloc(numel));
   MLF_HIP(hipMemcpyAsync(tmp.ptr, X, tmp.bytes(), hipMemcpyHostToDevice, st));
   MLF_LAUNCH(launch_to_blocked(tmp.ptr, p->X.ptr, p->n1, p->n2, p->Z, st));
   if (p->kp == Precision::kF16) {
     double sum = 0.0;
-    for (size_t i = 0; i < p->X.count; ++i) sum += X[i];
-    MLF_CHECK(patch_set_scale(p, sum / (double)p->X.count));
+    for (size_t i = 0; i < numel; ++i) sum += X[i];
+    MLF_CHECK(patch_set_scale(p, sum / (double)numel));
   }
   MLF_CHECK(patch_refresh_shadow(p));
   MLF_HIP(hipStreamSynchronize(st));
@@ -1211,13 +1219,13 @@ static mlf_status get_volume_impl(mlf_patch p, void* out, bool f64) {
   hipStream_t st = p->psf->ctx->stream;
   if (f64) {
     DevBuf<double> tmp;
-    MLF_CHECK(tmp.alloc(p->X.count));
+    MLF_CHECK(tmp.alloc(patch_numel(p)));
     MLF_LAUNCH(launch_from_blocked_f64(p->X.ptr, tmp.ptr, p->n1, p->n2, p->Z, st));
     MLF_HIP(hipMemcpyAsync(out, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, st));
     MLF_HIP(hipStreamSynchronize(st));
   } else {
     DevBuf<float> tmp;
-    MLF_CHECK(tmp.alloc(p->X.count));
+    MLF_CHECK(tmp.alloc(patch_numel(p)));
     MLF_LAUNCH(launch_from_blocked_f32(p->X.ptr, tmp.ptr, p->n1, p->n2, p->Z, st));
     MLF_HIP(hipMemcpyAsync(out, tmp.ptr, tmp.bytes(), hipMemcpyDeviceToHost, st));
     MLF_HIP(hipStreamSynchronize(st));
