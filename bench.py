@@ -473,6 +473,24 @@ def main():
     per_kernel = {c: {"total_ms": round(ms, 3), "launches": n,
                       "avg_ms": round(ms / n, 4) if n else None} for c, (ms, n) in ktimes.items()}
 
+    # Post-reconstruction step of the reconstruction module (top-layer balance,
+    # max normalisations, uint16 image, central-plane crop) on the reconstructed
+    # volume: outside the volumes/s figure; device time from its event pair (the
+    # launches only, not the copies to the host), after one untimed call.
+    vr = wl.Z if wl.Z % 2 else wl.Z - 1
+    work.postprocess(vr)
+    work.set_timing(True)
+    work.kernel_times()
+    work.postprocess(vr)
+    pp_ms = work.kernel_times()["postprocess"][0]
+    work.set_timing(False)
+    nvox = float(wl.I1) * wl.I2 * wl.Z
+    pp_bytes = 4.0 * nvox + 8.0 * wl.I1 * wl.I2 * vr + 2.0 * nvox  # read X; write stack + image
+    post = {"ms": round(pp_ms, 4), "valid_range": vr, "algorithmic_bytes": pp_bytes,
+            "achieved_GB/s": round(pp_bytes / (pp_ms * 1e-3) / 1e9, 1),
+            "frac_of_hbm_peak": round(pp_bytes / (pp_ms * 1e-3) / HBM_PEAK, 3)}
+    log(f"postprocess {pp_ms:.3f} ms")
+
     compare = None
     if not args.no_compare and args.precision != "fp32" and not partial:
         Xb = work.get_volume()
@@ -520,6 +538,7 @@ def main():
                    "partial_profile_run": partial},
         "roofline": roof,
         "kernels": per_kernel,
+        "postprocess": post,
         "fp32_comparison": compare,
         "cpu_baseline": cpu,
     }

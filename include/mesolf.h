@@ -124,6 +124,21 @@ mlf_status mlf_patch_forward_view(mlf_patch p, int32_t view, float* y);
 mlf_status mlf_patch_project_all(mlf_patch p, float* sensor);
 /* sample_filter (recon:222-240) on the current volume, FP64 plane energies. */
 mlf_status mlf_patch_sample_filter(mlf_patch p);
+/* Replaces the post-reconstruction lines of LFM_reconstruction_module.m:45-53
+ * (top-layer balance: plane 1 times max(plane 2) / max(plane 1); each plane
+ * divided by its max) and :60-62 (the stack divided by its max; the TIFF
+ * image im2uint16(stack / max(stack))), then mesolf.m:101-102 (keep planes
+ * ceil(Z/2) - floor(valid_range/2) .. ceil(Z/2) + floor(valid_range/2),
+ * 1-based).  FP64 arithmetic on the patch's volume in HBM, in MATLAB's
+ * evaluation order; maxima omit NaN.  The volume itself is not changed.
+ *   valid_range: positive and odd (mesolf_set_params.m:138), the planes within
+ *                1..Z; 0 keeps all Z planes (the module's output before the crop)
+ *   out:  double [15*n1, 15*n2, valid_range or Z] column-major, host, or NULL
+ *   tiff: uint16 [15*n1, 15*n2, Z] column-major, host, or NULL
+ * Needs Z >= 2 (plane 2 is read).  The optional vessel mask
+ * (LFM_reconstruction_module.m:56-58, off by default) is not applied.
+ * Synchronous. */
+mlf_status mlf_patch_postprocess(mlf_patch p, int32_t valid_range, double* out, uint16_t* tiff);
 /* Time of the last mlf_patch_run on the device, milliseconds (HIP events). */
 mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms);
 /* Per-kernel timing: while enabled, the launches of mlf_patch_run are
@@ -132,8 +147,9 @@ mlf_status mlf_patch_last_run_ms(mlf_patch p, double* ms);
  * sample filter is always timed), 0 turns timing off.  mlf_patch_kernel_times
  * returns, per kernel class, the summed milliseconds and the launch count
  * since the previous read, then resets them.  Classes (MLF_KERNEL_CLASSES):
- * 0 forward projection, 1 ratio, 2 backprojection + update, 3 sample_filter. */
-#define MLF_KERNEL_CLASSES 4
+ * 0 forward projection, 1 ratio, 2 backprojection + update, 3 sample_filter,
+ * 4 post-processing (mlf_patch_postprocess, one event pair per call). */
+#define MLF_KERNEL_CLASSES 5
 mlf_status mlf_patch_set_timing(mlf_patch p, int32_t enable);
 mlf_status mlf_patch_kernel_times(mlf_patch p, double* total_ms, int64_t* launches);
 

@@ -27,8 +27,8 @@ COMMON = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "runtime.cpp", "batch.cpp",
-           "server.cpp", "tune.cpp"]
+SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "post.hip", "runtime.cpp",
+           "batch.cpp", "server.cpp", "tune.cpp"]
 
 
 def _compile(src: str) -> str:

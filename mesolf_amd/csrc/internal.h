@@ -135,6 +135,14 @@ hipError_t launch_filter_scale(float* X, FilterPlan fp, int n1, int n2, const do
 // FP64 partial-sum slots per filtered plane (workgroups of the larger pass).
 int filter_partial_blocks(int n1, int n2);
 
+// Post-reconstruction normalisation (LFM_reconstruction_module.m:45-53,60-62)
+// and crop (mesolf.m:101-102).  coef: 4*Z doubles, parts: kPostBlocks*Z
+// doubles.  out: double [I1, I2, zhi - zlo] column-major (planes zlo..zhi-1)
+// or null; tiff: uint16 [I1, I2, Z] column-major or null.  Z >= 2.
+constexpr int kPostBlocks = 64;  // workgroups per plane in the max passes
+hipError_t launch_postprocess(const float* X, int n1, int n2, int Z, double* coef, double* parts,
+                              int zlo, int zhi, double* out, uint16_t* tiff, hipStream_t st);
+
 // MEX operator (mex:43-266).
 hipError_t launch_mex_conv(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
                            float* out, hipStream_t st);

@@ -1346,6 +1346,46 @@ mlf_status mlf_patch_sample_filter(mlf_patch p) {
   return MLF_OK;
 }
 
+mlf_status mlf_patch_postprocess(mlf_patch p, int32_t valid_range, double* out, uint16_t* tiff) {
+  clear_error();
+  if (!p) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (!p->volume_set) return set_error(MLF_ERR_INVALID_ARGUMENT, "volume not set");
+  const int Z = p->Z;
+  // LFM_reconstruction_module.m:46 reads planes 1 and 2
+  if (Z < 2)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "post-processing needs at least 2 planes, got %d", Z);
+  // mesolf_set_params.m:138: a positive odd number of planes, centred at
+  // ceil(Z/2); 0 keeps every plane (the module's own output, before the crop)
+  if (valid_range < 0 || (valid_range > 0 && valid_range % 2 == 0))
+    return set_error(MLF_ERR_INVALID_ARGUMENT,
+                     "valid_range must be a positive odd integer (or 0: all planes), got %d",
+                     (int)valid_range);
+  const int zc = (Z + 1) / 2, h = valid_range / 2;
+  const int zlo = valid_range ? zc - h - 1 : 0, zhi = valid_range ? zc + h : Z;  // 0-based, [zlo, zhi)
+  if (zlo < 0 || zhi > Z)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "valid_range %d selects planes %d..%d outside 1..%d",
+                     (int)valid_range, zc - h, zc + h, Z);
+  std::lock_guard<std::mutex> lk(p->mu);
+  MLF_CHECK(ctx_activate(p->psf->ctx));
+  hipStream_t st = p->psf->ctx->stream;
+  const size_t plane = (size_t)p->n1 * kN * p->n2 * kN;
+  DevBuf<double> coef, parts, dout;
+  DevBuf<uint16_t> dtiff;
+  MLF_CHECK(coef.alloc(4 * (size_t)Z));
+  MLF_CHECK(parts.alloc((size_t)kPostBlocks * Z));
+  if (out) MLF_CHECK(dout.alloc(plane * (zhi - zlo)));
+  if (tiff) MLF_CHECK(dtiff.alloc(plane * Z));
+  p->tim_now = p->timing;
+  MLF_CHECK(tim_begin(p, 4));
+  MLF_LAUNCH(launch_postprocess(p->X.ptr, p->n1, p->n2, Z, coef.ptr, parts.ptr, zlo, zhi, dout.ptr,
+                                dtiff.ptr, st));
+  MLF_CHECK(tim_end(p));
+  if (out) MLF_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, st));
+  if (tiff) MLF_HIP(hipMemcpyAsync(tiff, dtiff.ptr, dtiff.bytes(), hipMemcpyDeviceToHost, st));
+  MLF_HIP(hipStreamSynchronize(st));
+  return MLF_OK;
+}
+
 mlf_status mlf_patch_tile_shape(mlf_patch p, mlf_tile_shape* cur) {
   clear_error();
   if (!p || !cur) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");

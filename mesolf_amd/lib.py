@@ -96,6 +96,7 @@ _SIGNATURES = {
     "mlf_patch_forward_view": (C.c_int, [_P, _I32, _F32P]),
     "mlf_patch_project_all": (C.c_int, [_P, _F32P]),
     "mlf_patch_sample_filter": (C.c_int, [_P]),
+    "mlf_patch_postprocess": (C.c_int, [_P, _I32, _F64P, C.POINTER(C.c_uint16)]),
     "mlf_patch_last_run_ms": (C.c_int, [_P, _F64P]),
     "mlf_patch_set_timing": (C.c_int, [_P, _I32]),
     "mlf_patch_kernel_times": (C.c_int, [_P, _F64P, _I64P]),
@@ -329,7 +330,7 @@ class Patch:
         _check(self._lib.mlf_patch_last_run_ms(self.handle, C.byref(ms)))
         return ms.value
 
-    KERNEL_CLASSES = ("forward", "ratio", "backward_update", "sample_filter")
+    KERNEL_CLASSES = ("forward", "ratio", "backward_update", "sample_filter", "postprocess")
 
     def set_timing(self, enable: bool = True, stride: int = 1):
         """Bracket launches with HIP events: every `stride`-th view step (the
@@ -340,8 +341,8 @@ class Patch:
 
     def kernel_times(self):
         """{class: (total_ms, launches)} since the previous read."""
-        ms = np.zeros(4, np.float64)
-        n = np.zeros(4, np.int64)
+        ms = np.zeros(len(self.KERNEL_CLASSES), np.float64)
+        n = np.zeros(len(self.KERNEL_CLASSES), np.int64)
         _check(self._lib.mlf_patch_kernel_times(self.handle, _fptr(ms, C.c_double), _fptr(n, C.c_int64)))
         return {c: (float(ms[i]), int(n[i])) for i, c in enumerate(self.KERNEL_CLASSES)}
 
@@ -372,6 +373,27 @@ class Patch:
 
     def sample_filter(self):
         _check(self._lib.mlf_patch_sample_filter(self.handle))
+
+    def postprocess(self, valid_range: int, stack: bool = True, tiff: bool = True):
+        """Post-reconstruction step of LFM_reconstruction_module.m:45-53,60-62
+        and the central-plane crop of mesolf.m:101-102, on the device volume.
+
+        Returns (stack, tiff): the normalised stack's central `valid_range`
+        planes (all Z planes for valid_range = 0), float64 [I1, I2, planes],
+        and the uint16 image of the whole normalised stack [I1, I2, Z] (what
+        the module writes with saveastiff); either is None when not asked for.
+        """
+        I1, I2, Z = self.shape
+        nz = int(valid_range) if valid_range else Z
+        if nz < 0:
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"valid_range {valid_range} < 0")
+        out = np.empty((I1, I2, nz), np.float64, order="F") if stack else None
+        img = np.empty((I1, I2, Z), np.uint16, order="F") if tiff else None
+        _check(self._lib.mlf_patch_postprocess(
+            self.handle, int(valid_range),
+            _fptr(out, C.c_double) if stack else None,
+            img.ctypes.data_as(C.POINTER(C.c_uint16)) if tiff else None))
+        return out, img
 
     def close(self):
         if getattr(self, "handle", None):
@@ -552,6 +574,59 @@ def recon_theory_phase_space_LFM_with_filter(LFIMG, H, Ht, maxIter, gpuFlag=True
         stats["iteration_seconds"] = np.frombuffer(raw[16:16 + 8 * 64], np.float64)[:it].tolist()
         stats["relative_change"] = np.frombuffer(raw[16 + 8 * 64:16 + 16 * 64], np.float64)[:it].tolist()
     return out
+
+
+def _si_get(SI, name, default=None):
+    if isinstance(SI, dict):
+        return SI.get(name, default)
+    return getattr(SI, name, default)
+
+
+def LFM_reconstruction_module(summary_image, H, Ht, SI, precision="fp32", valid_range=0):
+    """Drop-in for LFM_reconstruction_module.m:1-62.
+
+    Reads SI.recon_max_iter, SI.gpu_ids (1-based), SI.worker_ix,
+    SI.recon_mode and SI.vessel_mask_enable like the module (:19-33).  Runs the
+    phase-space reconstruction (:37-39), then the top-layer balance and the
+    per-layer and global max normalisation (:45-53, :61) on the device.
+    Returns (recon_stack, tiff): float64 [I1, I2, Z] and the uint16 image the
+    module writes with saveastiff (:62), returned rather than written.  With
+    valid_range > 0 the stack is cropped to the central planes as mesolf.m:101-102
+    does after the module returns.
+    """
+    mode = _si_get(SI, "recon_mode", "phase_space")
+    if mode == "phase_space_peeling":
+        raise Unsupported(MLF_ERR_UNSUPPORTED, "recon_mode 'phase_space_peeling' is not built")
+    if mode != "phase_space":
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "Unsupported value of SI.recon_mode")
+    if _si_get(SI, "vessel_mask_enable", False):
+        raise Unsupported(MLF_ERR_UNSUPPORTED, "the blood-vessel mask (BCOSFIRE) is not built")
+    gpu_ids = list(np.atleast_1d(_si_get(SI, "gpu_ids", [])))
+    if not gpu_ids:
+        raise Unsupported(MLF_ERR_UNSUPPORTED, "the CPU branch (empty SI.gpu_ids) is not built")
+    wix = _si_get(SI, "worker_ix", None)
+    gpu_id = int(gpu_ids[int(wix) % len(gpu_ids)]) if wix is not None else int(gpu_ids[0])
+    lf = _col(summary_image, np.float32)
+    Hc = _col(H, np.float32)
+    Htc = _col(Ht, np.float32)
+    if lf.ndim != 2 or Hc.ndim != 5 or Hc.shape != Htc.shape:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "summary_image must be 2-D and H, Ht [S,S,N,N,Z]")
+    N = Hc.shape[2]
+    if lf.shape[0] % N or lf.shape[1] % N:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"image {lf.shape} is not whole lenslets of {N} px")
+    with Context(gpu_id - 1) as ctx:
+        psf = PSF.from_dense(ctx, Hc, Htc)
+        try:
+            p = Patch(psf, lf.shape[0] // N, lf.shape[1] // N, precision=precision)
+            try:
+                p.set_image(lf)
+                p.init_volume()
+                p.run(int(_si_get(SI, "recon_max_iter")))
+                return p.postprocess(valid_range)
+            finally:
+                p.close()
+        finally:
+            psf.close()
 
 
 def mex_lfm_convolution_vec(bitmap, kernels, device: int = 0):

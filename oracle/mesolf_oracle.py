@@ -32,6 +32,8 @@ forward/backward_compact    literal forms (tiny), brute-force loops
 rl_literal / rl_compact     each other at tiny sizes
 mex_op                      brute-force loop over the kernel's own clipping
                             rules; adjoint of forward_scatter (odd kw)
+post_recon / im2uint16      a 2x2x3 stack and its outputs computed by hand;
+                            NaN planes; crop indices worked out for Z = 51, 4
 ==========================  ==================================================
 
 There is no reference test data to pin against: the reference ships no
@@ -660,3 +662,60 @@ def forward_scatter(X: np.ndarray, K: np.ndarray, nphase: int = 15) -> np.ndarra
             t[a::nphase, b::nphase] = X[a::nphase, b::nphase]
             out += conv2_same(t, K[:, :, a, b])
     return out
+
+
+# ---------------------------------------------------------------------------
+# post-reconstruction step -- LFM_reconstruction_module.m:45-53,60-62 and
+# mesolf.m:101-102
+# ---------------------------------------------------------------------------
+
+
+def matlab_max(a) -> float:
+    """max(a, [], 'all'): NaN entries are ignored; NaN only if all are NaN."""
+    a = np.asarray(a, np.float64)
+    return float("nan") if np.all(np.isnan(a)) else float(np.nanmax(a))
+
+
+def im2uint16(x) -> np.ndarray:
+    """im2uint16 of a double array: 65535*x rounded half away from zero,
+    saturated to [0, 65535]; NaN -> 0."""
+    v = np.asarray(x, np.float64)
+    with np.errstate(invalid="ignore"):
+        s = np.where(v >= 1.0, 65535.0, np.where(v > 0.0, v * 65535.0, 0.0))
+        fl = np.floor(s)
+        r = fl + (s - fl >= 0.5)
+    return r.astype(np.uint16)
+
+
+def valid_planes(Z: int, valid_range: int):
+    """0-based [lo, hi) of recon_stack(:,:, ceil(Z/2)-floor(v/2) : ceil(Z/2)+floor(v/2))
+    (mesolf.m:101-102); valid_range = 0 keeps every plane."""
+    if valid_range == 0:
+        return 0, Z
+    if valid_range < 0 or valid_range % 2 == 0:
+        raise ValueError("valid_range must be positive and odd")
+    c = -(-Z // 2)
+    lo, hi = c - valid_range // 2, c + valid_range // 2  # 1-based, inclusive
+    if lo < 1 or hi > Z:
+        raise ValueError("valid_range selects planes outside the stack")
+    return lo - 1, hi
+
+
+def post_recon(X: np.ndarray, valid_range: int = 0):
+    """Top-layer balance and max normalisations of LFM_reconstruction_module.m
+    (:46 plane 1 scaled by max(plane 2)/max(plane 1), left to right; :49-53 each
+    plane divided by its max; :61 the stack divided by its max), then the TIFF
+    image of :62 (im2uint16(stack / max(stack))) and the crop of mesolf.m:101-102.
+
+    Returns (cropped stack float64, uint16 image of the whole stack)."""
+    R = np.array(X, np.float64, copy=True)
+    Z = R.shape[2]
+    with np.errstate(invalid="ignore", divide="ignore"):
+        R[:, :, 0] = R[:, :, 0] * matlab_max(R[:, :, 1]) / matlab_max(R[:, :, 0])
+        for i in range(Z):
+            layer = R[:, :, i]
+            R[:, :, i] = layer / matlab_max(layer)
+        R = R / matlab_max(R)
+        tiff = im2uint16(R / matlab_max(R))
+    lo, hi = valid_planes(Z, valid_range)
+    return R[:, :, lo:hi], tiff

@@ -293,3 +293,58 @@ def test_mex_op_batch_axis():
     out = O.mex_op(b, K)
     for n in range(3):
         assert np.allclose(out[:, :, n], O.mex_op(b[:, :, n], K))
+
+
+# --------------------------------------------------- post-reconstruction step
+
+
+def test_post_recon_by_hand():
+    # planes as [row i][column j]; stack[:, :, z]
+    p1 = [[1.0, 2.0], [3.0, 4.0]]    # max 4
+    p2 = [[0.0, 8.0], [2.0, 6.0]]    # max 8
+    p3 = [[1.0, 1.0], [1.0, 0.5]]    # max 1
+    X = np.stack([np.array(p1), np.array(p2), np.array(p3)], axis=2)
+    out, tiff = O.post_recon(X)
+    # plane 1 * 8 / 4 = [[2, 4], [6, 8]], then / 8; plane 2 / 8; plane 3 / 1; global max 1
+    want = np.stack([np.array([[0.25, 0.5], [0.75, 1.0]]),
+                     np.array([[0.0, 1.0], [0.25, 0.75]]),
+                     np.array([[1.0, 1.0], [1.0, 0.5]])], axis=2)
+    assert np.array_equal(out, want)
+    # 65535 * [0.25, 0.5, 0.75] = 16383.75, 32767.5, 49151.25 -> 16384, 32768, 49151
+    want_u16 = np.stack([np.array([[16384, 32768], [49151, 65535]]),
+                         np.array([[0, 65535], [16384, 49151]]),
+                         np.array([[65535, 65535], [65535, 32768]])], axis=2)
+    assert tiff.dtype == np.uint16 and np.array_equal(tiff, want_u16)
+    # central plane of 3 (mesolf.m:101: ceil(3/2) = 2)
+    mid, _ = O.post_recon(X, 1)
+    assert np.array_equal(mid, want[:, :, 1:2])
+
+
+def test_post_recon_zero_plane_and_nan():
+    X = np.zeros((2, 3, 3))
+    X[:, :, 0] = [[1, 2, 3], [4, 5, 6]]        # max 6
+    X[:, :, 1] = [[9, 3, 0], [np.nan, 3, 0]]   # max 9 (NaN ignored)
+    out, tiff = O.post_recon(X)                # plane 3 all zero: 0/0 -> NaN
+    assert np.all(np.isnan(out[:, :, 2])) and np.all(tiff[:, :, 2] == 0)
+    # plane 1 * 9 / 6 -> max 9 -> / 9: the original plane / 6
+    assert np.array_equal(out[:, :, 0], (X[:, :, 0] * 9 / 6) / 9)
+    assert np.isnan(out[1, 0, 1]) and tiff[1, 0, 1] == 0
+    assert out[0, 0, 1] == 1.0 and tiff[0, 0, 1] == 65535
+
+
+def test_im2uint16_values():
+    v = np.array([-1.0, 0.0, 1e-300, 0.25, 0.5, 1.0, 2.0, np.inf, np.nan, 1 / 65535, 0.5 / 65535 * 0.999])
+    assert O.im2uint16(v).tolist() == [0, 0, 0, 16384, 32768, 65535, 65535, 65535, 0, 1, 0]
+
+
+@pytest.mark.parametrize("Z,v,lo,hi", [(51, 51, 0, 51), (51, 31, 10, 41), (51, 1, 25, 26),
+                                       (4, 3, 0, 3), (4, 1, 1, 2), (7, 0, 0, 7)])
+def test_valid_planes(Z, v, lo, hi):
+    # mesolf.m:101-102 with ceil(Z/2) +- floor(v/2), 1-based inclusive
+    assert O.valid_planes(Z, v) == (lo, hi)
+
+
+@pytest.mark.parametrize("Z,v", [(51, 53), (4, 5), (51, 2), (51, -1)])
+def test_valid_planes_rejects(Z, v):
+    with pytest.raises(ValueError):
+        O.valid_planes(Z, v)
