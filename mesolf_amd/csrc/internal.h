@@ -136,7 +136,7 @@ hipError_t launch_filter_scale(float* X, FilterPlan fp, int n1, int n2, const do
 int filter_partial_blocks(int n1, int n2);
 
 // Post-reconstruction normalisation (LFM_reconstruction_module.m:45-53,60-62)
-// and crop (mesolf.m:101-102).  coef: 4*Z doubles, parts: kPostBlocks*Z
+// and crop (mesolf.m:101-102).  coef: 4*Z + 2 doubles, parts: kPostBlocks*Z
 // doubles.  out: double [I1, I2, zhi - zlo] column-major (planes zlo..zhi-1)
 // or null; tiff: uint16 [I1, I2, Z] column-major or null.  Z >= 2.
 constexpr int kPostBlocks = 64;  // workgroups per plane in the max passes

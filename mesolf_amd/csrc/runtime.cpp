@@ -1371,7 +1371,7 @@ mlf_status mlf_patch_postprocess(mlf_patch p, int32_t valid_range, double* out, 
   const size_t plane = (size_t)p->n1 * kN * p->n2 * kN;
   DevBuf<double> coef, parts, dout;
   DevBuf<uint16_t> dtiff;
-  MLF_CHECK(coef.alloc(4 * (size_t)Z));
+  MLF_CHECK(coef.alloc(4 * (size_t)Z + 2));
   MLF_CHECK(parts.alloc((size_t)kPostBlocks * Z));
   if (out) MLF_CHECK(dout.alloc(plane * (zhi - zlo)));
   if (tiff) MLF_CHECK(dtiff.alloc(plane * Z));
