@@ -350,13 +350,74 @@ def bench_mex(args):
     }), flush=True)
 
 
+def bench_summary(args):
+    """Summary image + rank-1 background (summary_image_generation_module.m,
+    'pixelwise' detrend, defaults 100 / 20 / 3 / 5) of a 1515 x 1515-px movie
+    of 400 frames resident in HBM, through mlf_summary_image_device."""
+    import ctypes as C
+
+    import torch
+
+    import mesolf_amd
+    from mesolf_amd import lib as L
+
+    h = w = 1515
+    T = 400
+    npix = h * w
+    ctx = mesolf_amd.Context(0)
+    g = torch.Generator(device="cuda").manual_seed(5)
+    movie = torch.rand((T, npix), device="cuda", generator=g, dtype=torch.float32) * 100 + 10
+    img = torch.empty(npix, device="cuda", dtype=torch.float64)
+    bs = torch.empty(npix, device="cuda", dtype=torch.float64)
+    bt = torch.empty(T, device="cuda", dtype=torch.float64)
+    torch.cuda.synchronize()
+    lib = L.load_library()
+
+    def call():
+        L._check(lib.mlf_summary_image_device(ctx.handle, C.c_void_p(movie.data_ptr()), h, w, T, 100, 20,
+                                              3, 5, C.c_void_p(img.data_ptr()), C.c_void_p(bs.data_ptr()),
+                                              C.c_void_p(bt.data_ptr())))
+
+    for _ in range(max(1, args.warmup)):
+        call()
+    times = []
+    for _ in range(max(1, args.steps)):
+        t = time.perf_counter()
+        call()  # returns when the device work is done
+        times.append(time.perf_counter() - t)
+    dt = min(times)
+    nbytes = 4.0 * npix * T * (4 + 2 * 5)  # max, 5 x (s'Y, Y t), detrend read + write, smooth read
+    cpu = None
+    if not args.no_cpu_baseline:
+        from oracle import mesolf_oracle as O
+        sub = 20000
+        sample = movie[:, :sub].double().cpu().numpy().T.copy()
+        t = time.perf_counter()
+        O.summary_pixelwise(sample, 100, 20, 3, 5)
+        tc = (time.perf_counter() - t) * npix / sub
+        cpu = {"value": T / tc, "unit": "frames/s", "cores": os.cpu_count(), "kind": "port",
+               "sample": f"FP64 numpy oracle on {sub} of {npix} pixels x {T} frames, scaled by the pixel count"}
+    print(json.dumps({
+        "metric": "summary image + rank-1 background, 1515x1515 px x 400 frames (pixelwise detrend)",
+        "value": T / dt, "unit": "frames/s", "n_gpus": 1, "steps": len(times), "warmup": args.warmup,
+        "ms_per_step": dt * 1e3, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64 (f32 samples, f32 detrended movie)", "data": "synthetic uniform movie",
+        "config": {"workload": "summary_1515x1515x400"},
+        "roofline": {"bound": "hbm", "achieved": nbytes / dt / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": nbytes / dt / HBM_PEAK, "traffic": None,
+                     "note": "algorithmic bytes 56 per sample; call time incl. scratch allocation"},
+        "cpu_baseline": cpu,
+    }), flush=True)
+    ctx.close()
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5", "mex"],
+    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5", "mex", "summary"],
                     help="cfg4: batched movie of 12 uneven patches; cfg5: serving loop; "
                          "mex: the MEX operator at the demo shape")
     ap.add_argument("--patches", type=int, default=0, help="cfg4/cfg5: patch count (12 / 36)")
@@ -382,6 +443,8 @@ def main():
                           "(sharding across GPUs is described in DESIGN.md, not implemented)"}))
         sys.exit(2)
 
+    if args.workload == "summary":
+        return bench_summary(args)
     if args.workload == "mex":
         return bench_mex(args)
     if args.workload == "cfg4":

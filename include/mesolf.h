@@ -274,6 +274,32 @@ mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap
                                           int64_t height, int64_t n, const float* d_kernels,
                                           int64_t kw, int64_t nphase, float* d_out);
 
+/* Replaces summary_image_generation_module.m:62-96 and :130 for the default
+ * detrend mode 'pixelwise' (mesolf_set_params.m:114): the rank-1 background
+ * of rank_1_factorization.m (bg_iter rounds, divisor max(movie)) and the
+ * summary image: x = movie / max(movie); d = x - movmean(x, window_1);
+ * y = movmean(d, window_2) .^ poly_index over frames; compute_std_image.m
+ * with zero factors, sqrt(sum(y.^2)/T - mean(y).^2); divided by its max.
+ * movmean is MATLAB's (window [j-a, j+b], a = b = (k-1)/2 for odd k, a = k/2,
+ * b = k/2-1 for even k, 'shrink' endpoints).  FP64 arithmetic on the single
+ * samples; the detrended movie is held in FP32.
+ *   movie:       single [height*width, frames] column-major (pixel fastest), host
+ *   summary:     double [height, width]
+ *   bg_spatial:  double [height, width] or NULL;  bg_temporal: double [frames] or NULL
+ * Defaults of the reference: window_1 = 100, window_2 = 20, poly_index = 3,
+ * bg_iter = 5 (mesolf_set_params.m:120-132).  The 'global' detrend mode
+ * (Savitzky-Golay baseline) is not built.  Synchronous. */
+mlf_status mlf_summary_image(const float* movie, int64_t height, int64_t width, int64_t frames,
+                             int32_t window_1, int32_t window_2, int32_t poly_index,
+                             int32_t bg_iter, double* summary, double* bg_spatial,
+                             double* bg_temporal, int32_t device);
+/* Same on device pointers (bg_spatial / bg_temporal may be NULL), on ctx's
+ * stream; returns when done (its scratch, about one FP32 movie, is released). */
+mlf_status mlf_summary_image_device(mlf_context ctx, const float* d_movie, int64_t height,
+                                    int64_t width, int64_t frames, int32_t window_1,
+                                    int32_t window_2, int32_t poly_index, int32_t bg_iter,
+                                    double* d_summary, double* d_bg_spatial, double* d_bg_temporal);
+
 /* All MEX projections of the seed-library builder in one call
  * (gen_seed_library_local_with_ring_bg_same_size.m:263-318 calls the MEX 4
  * times per plane and accumulates per segment on the host side):

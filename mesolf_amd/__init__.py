@@ -7,7 +7,8 @@ host-side mirror of the reference's MATLAB-facing interface.
 from .lib import (  # noqa: F401
     Batch, BudgetExceeded, Context, InvalidInput, MesolfError, Patch, PSF, Server, Unsupported,
     compact_block_layout, exported_symbols, lfm_convolution_segments, load_library,
-    mex_lfm_convolution_vec, LFM_reconstruction_module,
+    mex_lfm_convolution_vec, LFM_reconstruction_module, summary_image_generation_module,
+    summary_image_pixelwise,
     recon_theory_phase_space_LFM_with_filter,
 )
 
@@ -15,5 +16,6 @@ __all__ = [
     "Context", "PSF", "Patch", "Batch", "Server", "MesolfError", "InvalidInput", "Unsupported", "BudgetExceeded",
     "load_library", "exported_symbols", "compact_block_layout",
     "recon_theory_phase_space_LFM_with_filter", "mex_lfm_convolution_vec",
-    "lfm_convolution_segments", "LFM_reconstruction_module",
+    "lfm_convolution_segments", "LFM_reconstruction_module", "summary_image_generation_module",
+    "summary_image_pixelwise",
 ]

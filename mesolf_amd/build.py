@@ -27,7 +27,7 @@ COMMON = [
     "-I", os.path.join(HERE, "..", "include"),
 ]
 
-SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "post.hip", "runtime.cpp",
+SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "post.hip", "summary.hip", "runtime.cpp",
            "batch.cpp", "server.cpp", "tune.cpp"]
 
 

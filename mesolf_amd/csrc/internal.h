@@ -140,6 +140,16 @@ int filter_partial_blocks(int n1, int n2);
 // doubles.  out: double [I1, I2, zhi - zlo] column-major (planes zlo..zhi-1)
 // or null; tiff: uint16 [I1, I2, Z] column-major or null.  Z >= 2.
 constexpr int kPostBlocks = 64;  // workgroups per plane in the max passes
+
+// Summary image, 'pixelwise' detrend (summary_image_generation_module.m:62-96)
+// and rank-1 background (rank_1_factorization.m) of a movie [npix, T] single,
+// column-major.  scratch: summary_scratch_doubles(npix, T) doubles; dbuf:
+// npix * T floats; img, s: npix doubles (summary image normalised by its max,
+// bg_spatial); bg_temporal is left at summary_temporal(scratch) [T].
+size_t summary_scratch_doubles(size_t npix, int T);
+hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, int p, int bg_iter,
+                          double* scratch, float* dbuf, double* img, double* s, hipStream_t st);
+const double* summary_temporal(const double* scratch);
 hipError_t launch_postprocess(const float* X, int n1, int n2, int Z, double* coef, double* parts,
                               int zlo, int zhi, double* out, uint16_t* tiff, hipStream_t st);
 

@@ -1620,6 +1620,72 @@ mlf_status mlf_lfm_convolution_vec(const float* bitmap, int64_t width, int64_t h
   return MLF_OK;
 }
 
+mlf_status mlf_summary_image_device(mlf_context ctx, const float* d_movie, int64_t height,
+                                   int64_t width, int64_t frames, int32_t window_1,
+                                   int32_t window_2, int32_t poly_index, int32_t bg_iter,
+                                   double* d_summary, double* d_bg_spatial, double* d_bg_temporal) {
+  clear_error();
+  if (!ctx || !d_movie || !d_summary) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (height < 1 || width < 1 || frames < 1 || frames > 0x7fffffff)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad movie dims %lld x %lld x %lld", (long long)height,
+                     (long long)width, (long long)frames);
+  // mesolf_set_params.m:120-132: windows and the exponent are non-negative
+  // integers; a window of 0 selects nothing, and bg_iter = 0 leaves the
+  // temporal background undefined in rank_1_factorization.m
+  if (window_1 < 1 || window_2 < 1 || poly_index < 0 || bg_iter < 1)
+    return set_error(MLF_ERR_INVALID_ARGUMENT,
+                     "window_1, window_2 and bg_iter must be >= 1 and poly_index >= 0");
+  MLF_CHECK(ctx_activate(ctx));
+  const size_t npix = (size_t)height * width;
+  const int T = (int)frames;
+  DevBuf<double> scratch, s_own;
+  DevBuf<float> dbuf;
+  MLF_CHECK(scratch.alloc(summary_scratch_doubles(npix, T)));
+  MLF_CHECK(dbuf.alloc(npix * (size_t)T));
+  double* s = d_bg_spatial;
+  if (!s) {
+    MLF_CHECK(s_own.alloc(npix));
+    s = s_own.ptr;
+  }
+  MLF_LAUNCH(launch_summary(d_movie, npix, T, window_1, window_2, poly_index, bg_iter, scratch.ptr,
+                            dbuf.ptr, d_summary, s, ctx->stream));
+  if (d_bg_temporal)
+    MLF_HIP(hipMemcpyAsync(d_bg_temporal, summary_temporal(scratch.ptr), sizeof(double) * T,
+                           hipMemcpyDeviceToDevice, ctx->stream));
+  MLF_HIP(hipStreamSynchronize(ctx->stream));  // before the scratch is released
+  return MLF_OK;
+}
+
+mlf_status mlf_summary_image(const float* movie, int64_t height, int64_t width, int64_t frames,
+                             int32_t window_1, int32_t window_2, int32_t poly_index,
+                             int32_t bg_iter, double* summary, double* bg_spatial,
+                             double* bg_temporal, int32_t device) {
+  clear_error();
+  if (!movie || !summary) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (height < 1 || width < 1 || frames < 1 || frames > 0x7fffffff)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad movie dims");
+  mlf_context ctx = nullptr;
+  MLF_CHECK(mlf_context_create(device, &ctx));
+  std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> guard(ctx, mlf_context_destroy);
+  const size_t npix = (size_t)height * width;
+  DevBuf<float> dm;
+  DevBuf<double> dimg, ds, dt;
+  MLF_CHECK(dm.alloc(npix * (size_t)frames));
+  MLF_CHECK(dimg.alloc(npix));
+  MLF_CHECK(ds.alloc(npix));
+  MLF_CHECK(dt.alloc((size_t)frames));
+  MLF_HIP(hipMemcpyAsync(dm.ptr, movie, dm.bytes(), hipMemcpyHostToDevice, ctx->stream));
+  MLF_CHECK(mlf_summary_image_device(ctx, dm.ptr, height, width, frames, window_1, window_2,
+                                     poly_index, bg_iter, dimg.ptr, ds.ptr, dt.ptr));
+  MLF_HIP(hipMemcpyAsync(summary, dimg.ptr, dimg.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+  if (bg_spatial)
+    MLF_HIP(hipMemcpyAsync(bg_spatial, ds.ptr, ds.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+  if (bg_temporal)
+    MLF_HIP(hipMemcpyAsync(bg_temporal, dt.ptr, dt.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+  MLF_HIP(hipStreamSynchronize(ctx->stream));
+  return MLF_OK;
+}
+
 mlf_status mlf_lfm_convolution_segments(const float* slices, int64_t width, int64_t height,
                                         int64_t nslices, const int32_t* plane,
                                         const int64_t* seg_ptr, int64_t nseg,

@@ -1,0 +1,289 @@
+// Summary image of a sensor movie, 'pixelwise' detrend mode
+// (summary_image_generation_module.m:62-96, :130), with the rank-1 background
+// of rank_1_factorization.m:31-41 and compute_std_image.m.
+//
+// The movie is MATLAB column-major [npix, T] single (pixel fastest), so a
+// thread per pixel walking the frames reads 64 consecutive pixels per wave
+// per frame.  Arithmetic is FP64 on the FP32 samples.
+//   mx = max(movie);  x = movie / mx
+//   d  = x - movmean(x, window_1)            (pass 1, stored FP32)
+//   y  = movmean(d, window_2) .^ poly_index  (pass 2, fused with the sums)
+//   img = sqrt(sum(y.^2) / T - mean(y).^2);  summary = img / max(img)
+// movmean over the window [j - a, j + b] clipped to [0, T) ('shrink'),
+// a = b = (k-1)/2 for odd k, a = k/2, b = k/2 - 1 for even k, kept as a
+// running FP64 sum (one sample enters and one leaves per frame; the leaving
+// sample is read again, from cache).
+#include "internal.h"
+
+namespace mlf {
+
+namespace {
+
+__device__ __forceinline__ double nmax(double a, double b) {
+  if (a != a) return b;
+  if (b != b) return a;
+  return a > b ? a : b;
+}
+
+template <typename F>
+__device__ double block_reduce(double v, double* sh, F op) {
+  for (int off = 32; off > 0; off >>= 1) v = op(v, __shfl_down(v, off, 64));
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  __syncthreads();  // sh may still be read by a previous reduction
+  if (lane == 0) sh[wid] = v;
+  __syncthreads();
+  double r = sh[0];
+  for (int k = 1; k < (int)(blockDim.x >> 6); ++k) r = op(r, sh[k]);
+  return r;  // every thread
+}
+
+constexpr int kThreads = 256;
+
+// per-block maxima of a float array (NaN ignored)
+__global__ __launch_bounds__(kThreads) void k_max_f32(const float* __restrict__ a, size_t n,
+                                                      double* __restrict__ parts) {
+  __shared__ double sh[kThreads / 64];
+  double m = __builtin_nan("");
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    m = nmax(m, (double)a[i]);
+  m = block_reduce(m, sh, nmax);
+  if (threadIdx.x == 0) parts[blockIdx.x] = m;
+}
+
+// same over a 16-byte aligned array: n4 float4 and a tail of ntail floats
+__global__ __launch_bounds__(kThreads) void k_max_f32x4(const float4* __restrict__ a4, size_t n4,
+                                                        const float* __restrict__ tail, int ntail,
+                                                        double* __restrict__ parts) {
+  __shared__ double sh[kThreads / 64];
+  double m = __builtin_nan("");
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n4; i += (size_t)gridDim.x * blockDim.x) {
+    const float4 v = a4[i];
+    m = nmax(nmax(m, (double)v.x), nmax(nmax((double)v.y, (double)v.z), (double)v.w));
+  }
+  if (blockIdx.x == 0 && (int)threadIdx.x < ntail) m = nmax(m, (double)tail[threadIdx.x]);
+  m = block_reduce(m, sh, nmax);
+  if (threadIdx.x == 0) parts[blockIdx.x] = m;
+}
+
+// scalar[0] = max of parts (one workgroup)
+__global__ __launch_bounds__(kThreads) void k_fold_max(const double* __restrict__ parts, int n,
+                                                       double* __restrict__ out) {
+  __shared__ double sh[kThreads / 64];
+  double m = __builtin_nan("");
+  for (int k = threadIdx.x; k < n; k += blockDim.x) m = nmax(m, parts[k]);
+  m = block_reduce(m, sh, nmax);
+  if (threadIdx.x == 0) *out = m;
+}
+
+// rank-1 step 1, t~ = s' * Y: parts[j * nchunk + c] = sum over chunk c of s_i Y_ij.
+// A thread keeps the s of its kChunk / kThreads pixels in registers for the
+// kTdotFrames frames of its workgroup.
+constexpr int kChunk = kThreads * 16;  // pixels per chunk
+constexpr int kTdotFrames = 16;
+__global__ __launch_bounds__(kThreads) void k_r1_tdot(const float* __restrict__ Y, size_t npix, int T,
+                                                      const double* __restrict__ s,
+                                                      double* __restrict__ parts) {
+  __shared__ double sh[kThreads / 64];
+  constexpr int kPer = kChunk / kThreads;
+  const size_t i0 = (size_t)blockIdx.x * kChunk + threadIdx.x;
+  const int nchunk = gridDim.x;
+  double sv[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; ++q) {
+    const size_t i = i0 + (size_t)q * kThreads;
+    sv[q] = i < npix ? s[i] : 0.0;
+  }
+  for (int j0 = blockIdx.y * kTdotFrames; j0 < T; j0 += gridDim.y * kTdotFrames) {
+    for (int j = j0; j < j0 + kTdotFrames && j < T; ++j) {
+      const float* col = Y + (size_t)j * npix;
+      double acc = 0.0;
+#pragma unroll
+      for (int q = 0; q < kPer; ++q) {
+        const size_t i = i0 + (size_t)q * kThreads;
+        if (i < npix) acc = fma((double)col[i], sv[q], acc);
+      }
+      acc = block_reduce(acc, sh, [](double a, double b) { return a + b; });
+      if (threadIdx.x == 0) parts[(size_t)j * nchunk + blockIdx.x] = acc;
+    }
+  }
+}
+
+// rank-1 step 2a, one workgroup per frame j: t_j = (sum_c parts) / div
+__global__ __launch_bounds__(kThreads) void k_r1_tsum(const double* __restrict__ parts, int nchunk, int T,
+                                                      const double* __restrict__ mx,
+                                                      double* __restrict__ t) {
+  __shared__ double sh[kThreads / 64];
+  for (int j = blockIdx.x; j < T; j += gridDim.x) {
+    double v = 0.0;
+    for (int c = threadIdx.x; c < nchunk; c += blockDim.x) v += parts[(size_t)j * nchunk + c];
+    v = block_reduce(v, sh, [](double a, double b) { return a + b; });
+    if (threadIdx.x == 0) t[j] = v / *mx;
+  }
+}
+
+// rank-1 step 2b (one workgroup): t /= ||t||
+__global__ __launch_bounds__(kThreads) void k_r1_tnorm(int T, double* __restrict__ t) {
+  __shared__ double sh[kThreads / 64];
+  double ss = 0.0;
+  for (int j = threadIdx.x; j < T; j += blockDim.x) ss = fma(t[j], t[j], ss);
+  const double nrm = sqrt(block_reduce(ss, sh, [](double a, double b) { return a + b; }));
+  for (int j = threadIdx.x; j < T; j += blockDim.x) t[j] /= nrm;
+}
+
+// rank-1 step 3: s_i = (Y t)_i / div; per-block partial sums of s_i^2
+__global__ __launch_bounds__(kThreads) void k_r1_sdot(const float* __restrict__ Y, size_t npix, int T,
+                                                      const double* __restrict__ t,
+                                                      const double* __restrict__ mx,
+                                                      double* __restrict__ s, double* __restrict__ parts) {
+  __shared__ double sh[kThreads / 64];
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double v = 0.0;
+  if (i < npix) {
+#pragma unroll 8
+    for (int j = 0; j < T; ++j) v = fma((double)Y[i + (size_t)j * npix], t[j], v);
+    v /= *mx;
+    s[i] = v;
+  }
+  const double ss = block_reduce(v * v, sh, [](double a, double b) { return a + b; });
+  if (threadIdx.x == 0) parts[blockIdx.x] = ss;
+}
+
+// scalar[0] = sqrt(sum of parts) (one workgroup, fixed order per thread)
+__global__ __launch_bounds__(kThreads) void k_fold_norm(const double* __restrict__ parts, int n,
+                                                        double* __restrict__ out) {
+  __shared__ double sh[kThreads / 64];
+  double v = 0.0;
+  for (int k = threadIdx.x; k < n; k += blockDim.x) v += parts[k];
+  v = block_reduce(v, sh, [](double a, double b) { return a + b; });
+  if (threadIdx.x == 0) *out = sqrt(v);
+}
+
+__global__ void k_fill_f64(double* __restrict__ v, size_t n, double value) {
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v[i] = value;
+}
+
+__global__ void k_scale_div(double* __restrict__ v, size_t n, const double* __restrict__ d) {
+  const double dv = *d;
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+    v[i] /= dv;
+}
+
+__device__ __forceinline__ void window_of(int k, int* a, int* b) {
+  if (k & 1) {
+    *a = *b = (k - 1) / 2;
+  } else {
+    *a = k / 2;
+    *b = k / 2 - 1;
+  }
+}
+
+// pass 1: d = x - movmean(x, w1), x = Y / mx
+__global__ __launch_bounds__(kThreads) void k_detrend(const float* __restrict__ Y, size_t npix, int T,
+                                                      int w1, const double* __restrict__ mx,
+                                                      float* __restrict__ d) {
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  if (i >= npix) return;
+  const double m = *mx;
+  int a, b;
+  window_of(w1, &a, &b);
+  auto x = [&](int j) { return (double)Y[i + (size_t)j * npix] / m; };
+  double S = 0.0;
+  const int h0 = min(b, T - 1);
+  for (int k = 0; k <= h0; ++k) S += x(k);
+  for (int j = 0; j < T; ++j) {
+    const int lo = max(j - a, 0), hi = min(j + b, T - 1);
+    d[i + (size_t)j * npix] = (float)(x(j) - S / (double)(hi - lo + 1));
+    if (j + b + 1 <= T - 1) S += x(j + b + 1);
+    if (j - a >= 0) S -= x(j - a);
+  }
+}
+
+// pass 2: y = movmean(d, w2) .^ p; img = sqrt(sum(y^2)/T - mean(y)^2)
+__global__ __launch_bounds__(kThreads) void k_smooth_std(const float* __restrict__ d, size_t npix, int T,
+                                                         int w2, int p, double* __restrict__ img,
+                                                         double* __restrict__ parts) {
+  __shared__ double sh[kThreads / 64];
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double v = __builtin_nan("");
+  if (i < npix) {
+    int a, b;
+    window_of(w2, &a, &b);
+    auto dv = [&](int j) { return (double)d[i + (size_t)j * npix]; };
+    double S = 0.0, A = 0.0, B = 0.0;
+    const int h0 = min(b, T - 1);
+    for (int k = 0; k <= h0; ++k) S += dv(k);
+    for (int j = 0; j < T; ++j) {
+      const int lo = max(j - a, 0), hi = min(j + b, T - 1);
+      const double sm = S / (double)(hi - lo + 1);
+      double y = 1.0;
+      for (int q = 0; q < p; ++q) y *= sm;
+      A = fma(y, y, A);
+      B += y;
+      if (j + b + 1 <= T - 1) S += dv(j + b + 1);
+      if (j - a >= 0) S -= dv(j - a);
+    }
+    const double mean = B / (double)T;
+    v = sqrt(A / (double)T - mean * mean);
+    img[i] = v;
+  }
+  v = block_reduce(v, sh, nmax);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v;
+}
+
+inline unsigned blocks_for(size_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
+
+}  // namespace
+
+size_t summary_scratch_doubles(size_t npix, int T) {
+  const size_t nchunk = (npix + kChunk - 1) / kChunk;
+  const size_t nblk = (npix + kThreads - 1) / kThreads;
+  // mx, norm, t[T], tdot parts[T * nchunk], per-block parts[max(nblk, 1024)]
+  return 2 + (size_t)T + (size_t)T * nchunk + (nblk > 1024 ? nblk : 1024);
+}
+
+hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, int p, int bg_iter,
+                          double* scratch, float* dbuf, double* img, double* s, hipStream_t st) {
+  const size_t nchunk = (npix + kChunk - 1) / kChunk;
+  const unsigned nblk = blocks_for(npix);
+  double* mx = scratch;
+  double* nrm = scratch + 1;
+  double* t = scratch + 2;
+  double* tparts = t + T;
+  double* parts = tparts + (size_t)T * nchunk;
+  // movie max (the divisor of the background and the scale of x)
+  const size_t n = npix * (size_t)T;
+  const unsigned gmax = 1024;
+  if (((uintptr_t)Y & 15) == 0)
+    hipLaunchKernelGGL(k_max_f32x4, dim3(gmax), dim3(kThreads), 0, st, reinterpret_cast<const float4*>(Y),
+                       n / 4, Y + (n & ~(size_t)3), (int)(n & 3), parts);
+  else
+    hipLaunchKernelGGL(k_max_f32, dim3(gmax), dim3(kThreads), 0, st, Y, n, parts);
+  hipLaunchKernelGGL(k_fold_max, dim3(1), dim3(kThreads), 0, st, parts, (int)gmax, mx);
+  // rank-1 background: s = ones / sqrt(npix), bg_iter rounds
+  hipLaunchKernelGGL(k_fill_f64, dim3(nblk < 65535 ? nblk : 65535), dim3(kThreads), 0, st, s, npix,
+                     1.0 / sqrt((double)npix));
+  for (int it = 1; it <= bg_iter; ++it) {
+    const int fy = (T + kTdotFrames - 1) / kTdotFrames;
+    hipLaunchKernelGGL(k_r1_tdot, dim3((unsigned)nchunk, fy < 65535 ? fy : 65535), dim3(kThreads), 0, st,
+                       Y, npix, T, s, tparts);
+    hipLaunchKernelGGL(k_r1_tsum, dim3(T < 65535 ? T : 65535), dim3(kThreads), 0, st, tparts, (int)nchunk,
+                       T, mx, t);
+    hipLaunchKernelGGL(k_r1_tnorm, dim3(1), dim3(kThreads), 0, st, T, t);
+    hipLaunchKernelGGL(k_r1_sdot, dim3(nblk), dim3(kThreads), 0, st, Y, npix, T, t, mx, s, parts);
+    if (it < bg_iter) {
+      hipLaunchKernelGGL(k_fold_norm, dim3(1), dim3(kThreads), 0, st, parts, (int)nblk, nrm);
+      hipLaunchKernelGGL(k_scale_div, dim3(nblk < 65535 ? nblk : 65535), dim3(kThreads), 0, st, s, npix, nrm);
+    }
+  }
+  // summary image
+  hipLaunchKernelGGL(k_detrend, dim3(nblk), dim3(kThreads), 0, st, Y, npix, T, w1, mx, dbuf);
+  hipLaunchKernelGGL(k_smooth_std, dim3(nblk), dim3(kThreads), 0, st, dbuf, npix, T, w2, p, img, parts);
+  hipLaunchKernelGGL(k_fold_max, dim3(1), dim3(kThreads), 0, st, parts, (int)nblk, nrm);
+  hipLaunchKernelGGL(k_scale_div, dim3(nblk < 65535 ? nblk : 65535), dim3(kThreads), 0, st, img, npix, nrm);
+  return hipGetLastError();
+}
+
+const double* summary_temporal(const double* scratch) { return scratch + 2; }
+
+}  // namespace mlf

@@ -122,6 +122,10 @@ _SIGNATURES = {
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
     "mlf_lfm_convolution_vec_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P]),
+    "mlf_summary_image": (C.c_int, [_F32P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _F64P, _F64P,
+                                    _F64P, _I32]),
+    "mlf_summary_image_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _P, _P,
+                                           _P]),
     "mlf_lfm_convolution_segments": (C.c_int, [_F32P, _I64, _I64, _I64, _I32P, _I64P, _I64, _F32P,
                                                _I64, _I64, _I64, _F32P, _I32]),
 }
@@ -627,6 +631,51 @@ def LFM_reconstruction_module(summary_image, H, Ht, SI, precision="fp32", valid_
                 p.close()
         finally:
             psf.close()
+
+
+def summary_image_pixelwise(movie, height: int, width: int, window_1: int = 100, window_2: int = 20,
+                            poly_index: int = 3, bg_iter: int = 5, device: int = 0):
+    """Summary image and rank-1 background of a movie (summary_image_generation_module.m:62-96,
+    :130, detrend mode 'pixelwise').
+
+    movie: single [height*width, frames] (or [height, width, frames]), column-major.
+    Returns (summary [height, width], bg_spatial [height, width], bg_temporal [1, frames]),
+    float64, the summary normalised by its max.
+    """
+    lib = load_library()
+    m = _col(movie, np.float32)
+    if m.ndim == 3:
+        if m.shape[:2] != (height, width):
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"movie {m.shape} is not [{height}, {width}, T]")
+        m = m.reshape(height * width, m.shape[2], order="F")
+    if m.ndim != 2 or m.shape[0] != height * width:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"movie {m.shape} is not [{height}*{width}, T]")
+    T = m.shape[1]
+    img = np.empty((height, width), np.float64, order="F")
+    bs = np.empty((height, width), np.float64, order="F")
+    bt = np.empty((1, T), np.float64, order="F")
+    _check(lib.mlf_summary_image(_fptr(m), height, width, T, int(window_1), int(window_2),
+                                 int(poly_index), int(bg_iter), _fptr(img, C.c_double),
+                                 _fptr(bs, C.c_double), _fptr(bt, C.c_double), int(device)))
+    return img, bs, bt
+
+
+def summary_image_generation_module(sensor_movie, patch_info, SI, device: int = 0):
+    """Drop-in for summary_image_generation_module.m:1-130 (detrend mode
+    'pixelwise', the default): returns (summary_image, bg_spatial_init,
+    bg_temporal_init).  The images and plots the module saves are left out;
+    the 'global' mode (Savitzky-Golay baseline) raises Unsupported."""
+    mode = _si_get(SI, "detrend_mode", "pixelwise")
+    if mode == "global":
+        raise Unsupported(MLF_ERR_UNSUPPORTED, "detrend_mode 'global' is not built")
+    if mode != "pixelwise":
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, 'Invalid value for param "detrend_mode"')
+    size = _si_get(patch_info, "size")
+    h, w = int(size[0]), int(size[1])
+    return summary_image_pixelwise(sensor_movie, h, w, _si_get(SI, "pixelwise_window_1", 100),
+                                   _si_get(SI, "pixelwise_window_2", 20),
+                                   _si_get(SI, "pixelwise_poly_index", 3), _si_get(SI, "bg_iter", 5),
+                                   device)
 
 
 def mex_lfm_convolution_vec(bitmap, kernels, device: int = 0):

@@ -34,6 +34,12 @@ mex_op                      brute-force loop over the kernel's own clipping
                             rules; adjoint of forward_scatter (odd kw)
 post_recon / im2uint16      a 2x2x3 stack and its outputs computed by hand;
                             NaN planes; crop indices worked out for Z = 51, 4
+movmean                     MATLAB's documented movmean(A, 3) example; brute
+                            force over odd and even windows
+rank_1_factorization        rank-1 movies (closed form); the leading singular
+                            vectors (numpy SVD) after many iterations
+std_image                   numpy's population std of the same rows
+summary_pixelwise           a per-pixel loop composition of brute-force pieces
 ==========================  ==================================================
 
 There is no reference test data to pin against: the reference ships no
@@ -719,3 +725,64 @@ def post_recon(X: np.ndarray, valid_range: int = 0):
         tiff = im2uint16(R / matlab_max(R))
     lo, hi = valid_planes(Z, valid_range)
     return R[:, :, lo:hi], tiff
+
+
+# ---------------------------------------------------------------------------
+# summary image -- summary_image_generation_module.m:62-96 ('pixelwise',
+# the default detrend mode), rank_1_factorization.m, compute_std_image.m
+# ---------------------------------------------------------------------------
+
+
+def movmean(x: np.ndarray, k: int, axis: int = -1) -> np.ndarray:
+    """smoothdata(x, dim, 'movmean', k) / movmean(x, k) with the default
+    'shrink' endpoints: the mean over the window [j - a, j + b] clipped to the
+    array, a = b = (k-1)/2 for odd k, a = k/2, b = k/2 - 1 for even k."""
+    x = np.moveaxis(np.asarray(x, np.float64), axis, -1)
+    T = x.shape[-1]
+    a, b = ((k - 1) // 2, (k - 1) // 2) if k % 2 else (k // 2, k // 2 - 1)
+    P = np.concatenate([np.zeros(x.shape[:-1] + (1,)), np.cumsum(x, axis=-1)], axis=-1)
+    j = np.arange(T)
+    lo = np.maximum(j - a, 0)
+    hi = np.minimum(j + b, T - 1)
+    out = (P[..., hi + 1] - P[..., lo]) / (hi - lo + 1)
+    return np.moveaxis(out, -1, axis)
+
+
+def rank_1_factorization(Y: np.ndarray, max_iter: int = 1, divisor: float = 1.0):
+    """rank_1_factorization.m:31-41 on Y [pixels, frames]: returns
+    (bg_spatial [pixels], bg_temporal [frames])."""
+    Y = np.asarray(Y, np.float64)
+    s = np.ones(Y.shape[0]) / math.sqrt(Y.shape[0])
+    t = None
+    for it in range(1, max_iter + 1):
+        t = (s @ Y) / divisor
+        t = t / np.linalg.norm(t)
+        s = (Y @ t) / divisor
+        if it < max_iter:
+            s = s / np.linalg.norm(s)
+    return s, t
+
+
+def std_image(Y: np.ndarray) -> np.ndarray:
+    """compute_std_image.m with y_1 = 0, y_2 = zeros(1, T):
+    sqrt(sum(Y.^2, 2) / T - mean(Y, 2).^2)."""
+    Y = np.asarray(Y, np.float64)
+    A = np.sum(Y * Y, axis=1)
+    B = np.mean(Y, axis=1)
+    with np.errstate(invalid="ignore"):
+        return np.sqrt(A / Y.shape[1] - B * B)
+
+
+def summary_pixelwise(movie: np.ndarray, window_1: int = 100, window_2: int = 20,
+                      poly_index: int = 3, bg_iter: int = 5):
+    """summary_image_generation_module.m:62-96 and :130 for movie [pixels,
+    frames]: returns (summary image normalised by its max, bg_spatial,
+    bg_temporal)."""
+    Y = np.asarray(movie, np.float64)
+    mx = float(np.max(Y))
+    bs, bt = rank_1_factorization(Y, bg_iter, mx)
+    x = Y / mx
+    d = x - movmean(x, window_1, axis=1)
+    y = movmean(d, window_2, axis=1) ** poly_index
+    img = std_image(y)
+    return img / matlab_max(img), bs, bt

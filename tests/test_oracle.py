@@ -348,3 +348,70 @@ def test_valid_planes(Z, v, lo, hi):
 def test_valid_planes_rejects(Z, v):
     with pytest.raises(ValueError):
         O.valid_planes(Z, v)
+
+
+# ------------------------------------------------------------- summary image
+
+
+def test_movmean_documented_example():
+    # MATLAB's movmean documentation: movmean(A, 3) of this vector
+    A = np.array([4, 8, 6, -1, -2, -3, -1, 3, 4, 5], float)
+    want = [6, 6, 13 / 3, 1, -2, -2, -1 / 3, 2, 4, 4.5]
+    assert np.allclose(O.movmean(A, 3), want, rtol=0, atol=1e-12)
+
+
+@pytest.mark.parametrize("k", [1, 2, 3, 4, 7, 20, 23, 40])
+def test_movmean_bruteforce(k):
+    rng = np.random.default_rng(k)
+    x = rng.standard_normal((3, 17))
+    got = O.movmean(x, k, axis=1)
+    T = x.shape[1]
+    before, after = (k // 2, k // 2 - 1) if k % 2 == 0 else ((k - 1) // 2, (k - 1) // 2)
+    for i in range(3):
+        for j in range(T):
+            win = [x[i, m] for m in range(T) if j - before <= m <= j + after]
+            assert abs(got[i, j] - sum(win) / len(win)) < 1e-12
+
+
+def test_movmean_even_window_by_hand():
+    # k = 4 covers [j-2, j+1]; k = 2 covers [j-1, j]
+    x = np.array([1.0, 2, 3, 4, 5])
+    assert np.allclose(O.movmean(x, 4), [1.5, 2, 2.5, 3.5, 4])
+    assert np.allclose(O.movmean(x, 2), [1, 1.5, 2.5, 3.5, 4.5])
+
+
+def test_rank_1_factorization_closed_form_and_svd():
+    rng = np.random.default_rng(5)
+    u = rng.random(30) + 0.1
+    v = rng.random(12) + 0.1
+    s, t = O.rank_1_factorization(np.outer(u, v), 1, 2.0)
+    # t = v / |v| exactly in direction; s = Y t / 2 = u |v| / 2
+    assert np.allclose(t, v / np.linalg.norm(v), rtol=1e-12)
+    assert np.allclose(s, u * np.linalg.norm(v) / 2.0, rtol=1e-12)
+    Y = rng.random((40, 15))
+    s, t = O.rank_1_factorization(Y, 200, 1.0)
+    U, S, Vt = np.linalg.svd(Y)
+    assert np.allclose(t, Vt[0] * np.sign(Vt[0, 0]), rtol=1e-9)
+    assert np.allclose(s, U[:, 0] * S[0] * np.sign(U[0, 0]), rtol=1e-9)
+
+
+def test_std_image_is_population_std():
+    Y = np.random.default_rng(6).standard_normal((9, 31))
+    assert np.allclose(O.std_image(Y), np.std(Y, axis=1), rtol=1e-12)
+
+
+def test_summary_pixelwise_by_loops():
+    rng = np.random.default_rng(7)
+    npix, T, w1, w2, pw = 6, 25, 6, 3, 3
+    movie = rng.random((npix, T)) * 50 + 10
+    img, bs, bt = O.summary_pixelwise(movie, w1, w2, pw, 2)
+    mx = movie.max()
+    want = np.zeros(npix)
+    for i in range(npix):
+        x = movie[i] / mx
+        d = [x[j] - np.mean(x[max(0, j - w1 // 2):min(T, j + w1 // 2)]) for j in range(T)]
+        y = [np.mean(d[max(0, j - (w2 - 1) // 2):min(T, j + (w2 - 1) // 2 + 1)]) ** pw for j in range(T)]
+        want[i] = np.std(y)
+    assert np.allclose(img, want / want.max(), rtol=1e-10)
+    s0, t0 = O.rank_1_factorization(movie, 2, mx)
+    assert np.array_equal(bs, s0) and np.array_equal(bt, t0)
