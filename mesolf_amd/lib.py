@@ -4,11 +4,14 @@ Arrays follow MATLAB semantics: a numpy array ``A`` with ``A[x, y, ...]``
 is passed in column-major (Fortran) order, so ``A[:, :, z]`` is the MATLAB
 slice ``A(:, :, z)``.
 
-The two drop-in functions keep the reference's argument order and meaning:
+The drop-in functions keep the reference's argument order and meaning:
 
 * :func:`recon_theory_phase_space_LFM_with_filter` --
   ``src/recon/recon_theory_phase_space_LFM_with_filter.m:10``
 * :func:`mex_lfm_convolution_vec` -- ``src/cuda/mex_lfm_convolution_vec.cu:276``
+* :func:`LFM_reconstruction_module` -- ``src/module/LFM_reconstruction_module.m:1``
+* :func:`summary_image_generation_module` --
+  ``src/module/summary_image_generation_module.m:1`` (detrend mode 'pixelwise')
 
 There is no CPU fallback: if the shared library is missing every call raises.
 """
