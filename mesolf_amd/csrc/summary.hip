@@ -178,25 +178,50 @@ __device__ __forceinline__ void window_of(int k, int* a, int* b) {
   }
 }
 
+// Frames j of a window walk: [0, j1) and [j2, T) take the clipped window
+// (cnt varies, the updates are guarded); [j1, j2) is the steady part, where
+// the window has all k samples and one enters and one leaves every frame.
+// The steady loop is unrolled so that its loads (independent of the running
+// sum) are issued ahead; the order of the additions is the same everywhere.
+__device__ __forceinline__ void walk_bounds(int a, int b, int T, int* j1, int* j2) {
+  *j1 = min(a, T);
+  *j2 = max(*j1, T - b - 1);
+}
+
 // pass 1: d = x - movmean(x, w1), x = Y / mx
 __global__ __launch_bounds__(kThreads) void k_detrend(const float* __restrict__ Y, size_t npix, int T,
                                                       int w1, const double* __restrict__ mx,
                                                       float* __restrict__ d) {
   const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
   if (i >= npix) return;
-  const double m = *mx;
+  // x = Y / max as a product with the reciprocal, and the steady mean as a
+  // product with 1/k: within 1 ulp of the quotients, without the FP64
+  // divide sequence per sample
+  const double minv = 1.0 / *mx;
   int a, b;
   window_of(w1, &a, &b);
-  auto x = [&](int j) { return (double)Y[i + (size_t)j * npix] / m; };
+  auto x = [&](int j) { return (double)Y[i + (size_t)j * npix] * minv; };
   double S = 0.0;
   const int h0 = min(b, T - 1);
   for (int k = 0; k <= h0; ++k) S += x(k);
-  for (int j = 0; j < T; ++j) {
+  auto step = [&](int j) {
     const int lo = max(j - a, 0), hi = min(j + b, T - 1);
     d[i + (size_t)j * npix] = (float)(x(j) - S / (double)(hi - lo + 1));
     if (j + b + 1 <= T - 1) S += x(j + b + 1);
     if (j - a >= 0) S -= x(j - a);
+  };
+  int j1, j2;
+  walk_bounds(a, b, T, &j1, &j2);
+  for (int j = 0; j < j1; ++j) step(j);
+  const double kinv = 1.0 / (double)(a + b + 1);
+#pragma unroll 8
+  for (int j = j1; j < j2; ++j) {
+    const double xc = x(j), xn = x(j + b + 1), xo = x(j - a);
+    d[i + (size_t)j * npix] = (float)(xc - S * kinv);
+    S += xn;
+    S -= xo;
   }
+  for (int j = j2; j < T; ++j) step(j);
 }
 
 // pass 2: y = movmean(d, w2) .^ p; img = sqrt(sum(y^2)/T - mean(y)^2)
@@ -213,16 +238,30 @@ __global__ __launch_bounds__(kThreads) void k_smooth_std(const float* __restrict
     double S = 0.0, A = 0.0, B = 0.0;
     const int h0 = min(b, T - 1);
     for (int k = 0; k <= h0; ++k) S += dv(k);
-    for (int j = 0; j < T; ++j) {
-      const int lo = max(j - a, 0), hi = min(j + b, T - 1);
-      const double sm = S / (double)(hi - lo + 1);
+    auto acc = [&](double sm) {
       double y = 1.0;
       for (int q = 0; q < p; ++q) y *= sm;
       A = fma(y, y, A);
       B += y;
+    };
+    auto step = [&](int j) {
+      const int lo = max(j - a, 0), hi = min(j + b, T - 1);
+      acc(S / (double)(hi - lo + 1));
       if (j + b + 1 <= T - 1) S += dv(j + b + 1);
       if (j - a >= 0) S -= dv(j - a);
+    };
+    int j1, j2;
+    walk_bounds(a, b, T, &j1, &j2);
+    for (int j = 0; j < j1; ++j) step(j);
+    const double kinv = 1.0 / (double)(a + b + 1);
+#pragma unroll 8
+    for (int j = j1; j < j2; ++j) {
+      const double dn = dv(j + b + 1), dol = dv(j - a);
+      acc(S * kinv);
+      S += dn;
+      S -= dol;
     }
+    for (int j = j2; j < T; ++j) step(j);
     const double mean = B / (double)T;
     v = sqrt(A / (double)T - mean * mean);
     img[i] = v;
