@@ -83,7 +83,6 @@ constexpr int kTdotFrames = 16;
 __global__ __launch_bounds__(kThreads) void k_r1_tdot(const float* __restrict__ Y, size_t npix, int T,
                                                       const double* __restrict__ s,
                                                       double* __restrict__ parts) {
-  __shared__ double sh[kThreads / 64];
   constexpr int kPer = kChunk / kThreads;
   const size_t i0 = (size_t)blockIdx.x * kChunk + threadIdx.x;
   const int nchunk = gridDim.x;
@@ -93,17 +92,38 @@ __global__ __launch_bounds__(kThreads) void k_r1_tdot(const float* __restrict__ 
     const size_t i = i0 + (size_t)q * kThreads;
     sv[q] = i < npix ? s[i] : 0.0;
   }
+  // frames in groups of kG: the group's loads are in flight together, one
+  // pair of barriers per group
+  constexpr int kG = 4;
+  __shared__ double shf[kG][kThreads / 64];
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
   for (int j0 = blockIdx.y * kTdotFrames; j0 < T; j0 += gridDim.y * kTdotFrames) {
-    for (int j = j0; j < j0 + kTdotFrames && j < T; ++j) {
-      const float* col = Y + (size_t)j * npix;
-      double acc = 0.0;
+    for (int jg = j0; jg < j0 + kTdotFrames && jg < T; jg += kG) {
+      const int nf = min(kG, T - jg);
+      double acc[kG];
+#pragma unroll
+      for (int f = 0; f < kG; ++f) acc[f] = 0.0;
 #pragma unroll
       for (int q = 0; q < kPer; ++q) {
         const size_t i = i0 + (size_t)q * kThreads;
-        if (i < npix) acc = fma((double)col[i], sv[q], acc);
+        if (i >= npix) continue;
+#pragma unroll
+        for (int f = 0; f < kG; ++f)
+          if (f < nf) acc[f] = fma((double)Y[i + (size_t)(jg + f) * npix], sv[q], acc[f]);
       }
-      acc = block_reduce(acc, sh, [](double a, double b) { return a + b; });
-      if (threadIdx.x == 0) parts[(size_t)j * nchunk + blockIdx.x] = acc;
+#pragma unroll
+      for (int f = 0; f < kG; ++f) {
+        double v = acc[f];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+        if (lane == 0) shf[f][wid] = v;
+      }
+      __syncthreads();
+      if ((int)threadIdx.x < nf) {
+        double r = shf[threadIdx.x][0];
+        for (int k = 1; k < kThreads / 64; ++k) r += shf[threadIdx.x][k];
+        parts[(size_t)(jg + threadIdx.x) * nchunk + blockIdx.x] = r;
+      }
+      __syncthreads();  // shf is rewritten by the next group
     }
   }
 }
