@@ -274,6 +274,24 @@ mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap
                                           int64_t height, int64_t n, const float* d_kernels,
                                           int64_t kw, int64_t nphase, float* d_out);
 
+/* Replaces the 'carpet' branch of determine_patch_info_aug.m:103-143: an
+ * image of global_h x global_w px in n_h x n_w patches of floor(size / n)
+ * rounded up to a multiple of M (the last row and column end at the image
+ * edge); each patch's height and width are cut down to a multiple of M and a
+ * patch below min_size (50 in the reference) in either is discarded.
+ * corners: int64 [n_h * n_w][4] = top, left, bottom, right (1-based,
+ * inclusive) in MATLAB's ind2sub order (n_h index fastest); *count = patches
+ * kept.  Host only. */
+mlf_status mlf_carpet_patches(int64_t global_h, int64_t global_w, int32_t n_h, int32_t n_w,
+                              int32_t M, int32_t min_size, int64_t* corners, int32_t* count);
+/* sensor_movie(top : top+h-1, left : left+w-1, :) of a single movie
+ * [H, W, frames] on the device (column-major, 1-based corner) into d_out
+ * [h, w, frames]: the patch movie of mesolf.m's per-patch loop, the input of
+ * mlf_summary_image_device.  Asynchronous on ctx's stream; w <= 65535. */
+mlf_status mlf_crop_movie_device(mlf_context ctx, const float* d_movie, int64_t H, int64_t W,
+                                 int64_t frames, int64_t top, int64_t left, int64_t h, int64_t w,
+                                 float* d_out);
+
 /* Replaces summary_image_generation_module.m:62-96 and :130 for the default
  * detrend mode 'pixelwise' (mesolf_set_params.m:114): the rank-1 background
  * of rank_1_factorization.m (bg_iter rounds, divisor max(movie)) and the

@@ -1620,6 +1620,50 @@ mlf_status mlf_lfm_convolution_vec(const float* bitmap, int64_t width, int64_t h
   return MLF_OK;
 }
 
+mlf_status mlf_carpet_patches(int64_t global_h, int64_t global_w, int32_t n_h, int32_t n_w,
+                              int32_t M, int32_t min_size, int64_t* corners, int32_t* count) {
+  clear_error();
+  if (!corners || !count) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (global_h < 1 || global_w < 1 || n_h < 1 || n_w < 1 || M < 1)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad image size, patch counts or M");
+  // determine_patch_info_aug.m:106-112: floor(size / n), rounded up to a multiple of M
+  const int64_t ph = (global_h / n_h + M - 1) / M * M, pw = (global_w / n_w + M - 1) / M * M;
+  int32_t k = 0;
+  for (int64_t i = 0; i < (int64_t)n_h * n_w; ++i) {
+    const int64_t ci = i % n_h + 1, cj = i / n_h + 1;  // ind2sub, first index fastest (:115)
+    const int64_t top = (ci - 1) * ph + 1, left = (cj - 1) * pw + 1;
+    const int64_t bottom = std::min(ci * ph, global_h), right = std::min(cj * pw, global_w);
+    int64_t h = bottom - top + 1, w = right - left + 1;
+    h -= h % M;  // :126-127
+    w -= w % M;
+    if (h < min_size || w < min_size) continue;  // :129-132, discarded
+    int64_t* c = corners + 4 * (size_t)k++;
+    c[0] = top;
+    c[1] = left;
+    c[2] = top + h - 1;
+    c[3] = left + w - 1;
+  }
+  *count = k;
+  return MLF_OK;
+}
+
+mlf_status mlf_crop_movie_device(mlf_context ctx, const float* d_movie, int64_t H, int64_t W,
+                                 int64_t frames, int64_t top, int64_t left, int64_t h, int64_t w,
+                                 float* d_out) {
+  clear_error();
+  if (!ctx || !d_movie || !d_out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (H < 1 || W < 1 || frames < 1 || frames > 0x7fffffff || h < 1 || w < 1 || h > 0x7fffffff ||
+      w > 65535 || top < 1 || left < 1 || top + h - 1 > H || left + w - 1 > W)
+    return set_error(MLF_ERR_INVALID_ARGUMENT,
+                     "patch rows %lld..%lld, columns %lld..%lld outside the %lld x %lld movie",
+                     (long long)top, (long long)(top + h - 1), (long long)left,
+                     (long long)(left + w - 1), (long long)H, (long long)W);
+  MLF_CHECK(ctx_activate(ctx));
+  MLF_LAUNCH(launch_crop_movie(d_movie, H, W, top - 1, left - 1, (int)h, (int)w, (int)frames, d_out,
+                               ctx->stream));
+  return MLF_OK;
+}
+
 mlf_status mlf_summary_image_device(mlf_context ctx, const float* d_movie, int64_t height,
                                    int64_t width, int64_t frames, int32_t window_1,
                                    int32_t window_2, int32_t poly_index, int32_t bg_iter,

@@ -290,6 +290,19 @@ __global__ __launch_bounds__(kThreads) void k_smooth_std(const float* __restrict
   if (threadIdx.x == 0) parts[blockIdx.x] = v;
 }
 
+// Patch movie from a sensor movie [H, W, T] (column-major): out[r + h*c + h*w*t]
+// = in[(top + r) + H*(left + c) + H*W*t], one patch column per workgroup row.
+__global__ __launch_bounds__(kThreads) void k_crop_movie(const float* __restrict__ in, int64_t H,
+                                                         int64_t W, int64_t top, int64_t left, int h,
+                                                         int w, int T, float* __restrict__ out) {
+  const int c = blockIdx.y;
+  for (int t = blockIdx.z; t < T; t += gridDim.z) {
+    const float* src = in + (size_t)t * H * W + (size_t)(left + c) * H + top;
+    float* dst = out + (size_t)t * h * w + (size_t)c * h;
+    for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < h; r += gridDim.x * blockDim.x) dst[r] = src[r];
+  }
+}
+
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
 }  // namespace
@@ -344,5 +357,13 @@ hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, in
 }
 
 const double* summary_temporal(const double* scratch) { return scratch + 2; }
+
+hipError_t launch_crop_movie(const float* in, int64_t H, int64_t W, int64_t top, int64_t left, int h,
+                             int w, int T, float* out, hipStream_t st) {
+  const unsigned gx = (unsigned)((h + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_crop_movie, dim3(gx, (unsigned)w, (unsigned)(T < 65535 ? T : 65535)), dim3(kThreads), 0,
+                     st, in, H, W, top, left, h, w, T, out);
+  return hipGetLastError();
+}
 
 }  // namespace mlf

@@ -125,6 +125,8 @@ _SIGNATURES = {
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
     "mlf_lfm_convolution_vec_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P]),
+    "mlf_carpet_patches": (C.c_int, [_I64, _I64, _I32, _I32, _I32, _I32, _I64P, _I32P]),
+    "mlf_crop_movie_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
     "mlf_summary_image": (C.c_int, [_F32P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _F64P, _F64P,
                                     _F64P, _I32]),
     "mlf_summary_image_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _P, _P,
@@ -634,6 +636,35 @@ def LFM_reconstruction_module(summary_image, H, Ht, SI, precision="fp32", valid_
                 p.close()
         finally:
             psf.close()
+
+
+def carpet_patch_info(global_h: int, global_w: int, n_patches, M: int = 15, min_size: int = 50):
+    """The 'carpet' patches of determine_patch_info_aug.m:103-143 as patch_info
+    dicts: {"location": [[top, left], [bottom, right]] (1-based, inclusive),
+    "size": [h, w]}, in MATLAB's ind2sub order."""
+    lib = load_library()
+    n = list(np.atleast_1d(n_patches))
+    if len(n) == 1:
+        n = [n[0], n[0]]
+    corners = np.zeros((int(n[0]) * int(n[1]), 4), np.int64)
+    cnt = C.c_int32()
+    _check(lib.mlf_carpet_patches(int(global_h), int(global_w), int(n[0]), int(n[1]), int(M),
+                                  int(min_size), corners.ctypes.data_as(_I64P), C.byref(cnt)))
+    out = []
+    for t, l, b, r in corners[:cnt.value].tolist():
+        out.append({"location": [[t, l], [b, r]], "size": [b - t + 1, r - l + 1]})
+    return out
+
+
+def crop_movie_device(ctx: "Context", d_movie: int, H: int, W: int, frames: int, patch_info,
+                      d_out: int):
+    """Patch movie sensor_movie(top:bottom, left:right, :) on the device: raw
+    device pointers (ints) to a single [H, W, frames] movie and a
+    [h, w, frames] output; asynchronous on ctx's stream."""
+    (t, l), (b, r) = patch_info["location"]
+    _check(ctx._lib.mlf_crop_movie_device(ctx.handle, C.c_void_p(d_movie), int(H), int(W), int(frames),
+                                          int(t), int(l), int(b - t + 1), int(r - l + 1),
+                                          C.c_void_p(d_out)))
 
 
 def summary_image_pixelwise(movie, height: int, width: int, window_1: int = 100, window_2: int = 20,

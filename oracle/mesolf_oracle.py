@@ -40,6 +40,8 @@ rank_1_factorization        rank-1 movies (closed form); the leading singular
                             vectors (numpy SVD) after many iterations
 std_image                   numpy's population std of the same rows
 summary_pixelwise           a per-pixel loop composition of brute-force pieces
+carpet_patches              tilings of 1000x1000, 100x100 (a discarded patch)
+                            and 700x451 worked out by hand
 ==========================  ==================================================
 
 There is no reference test data to pin against: the reference ships no
@@ -786,3 +788,34 @@ def summary_pixelwise(movie: np.ndarray, window_1: int = 100, window_2: int = 20
     y = movmean(d, window_2, axis=1) ** poly_index
     img = std_image(y)
     return img / matlab_max(img), bs, bt
+
+
+# ---------------------------------------------------------------------------
+# patch tiling -- determine_patch_info_aug.m:26-30 and :103-143 ('carpet')
+# ---------------------------------------------------------------------------
+
+
+def carpet_patches(global_h: int, global_w: int, n_patches, M: int = 15, min_size: int = 50):
+    """Patches of the 'carpet' mode in MATLAB order (ind2sub over
+    [n_patches(1), n_patches(2)], first index fastest): a list of
+    ((top, left), (bottom, right)) 1-based inclusive corners; patches whose
+    height or width, cut down to a multiple of M, is below min_size are
+    discarded."""
+    n = list(np.atleast_1d(n_patches))
+    if len(n) == 1:
+        n = [n[0], n[0]]
+    ph = -(-(global_h // n[0]) // M) * M
+    pw = -(-(global_w // n[1]) // M) * M
+    out = []
+    for k in range(n[0] * n[1]):
+        ci, cj = k % n[0] + 1, k // n[0] + 1
+        top, left = (ci - 1) * ph + 1, (cj - 1) * pw + 1
+        bottom, right = min(ci * ph, global_h), min(cj * pw, global_w)
+        h = bottom - top + 1
+        w = right - left + 1
+        h -= h % M
+        w -= w % M
+        if h < min_size or w < min_size:
+            continue
+        out.append(((top, left), (top + h - 1, left + w - 1)))
+    return out

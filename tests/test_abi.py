@@ -88,3 +88,16 @@ def test_no_device_is_an_error_not_a_fallback():
     b = np.zeros((20, 20), np.float32)
     with pytest.raises(L.MesolfError):
         mesolf_amd.mex_lfm_convolution_vec(b, np.zeros((5, 5, 15, 15), np.float32))
+
+
+
+@pytest.mark.parametrize("gh,gw,n", [(1000, 1000, [3, 3]), (100, 100, 2), (700, 451, [2, 3]),
+                                     (1515, 1515, [4, 5]), (60, 2000, [1, 7]), (333, 77, 1)])
+def test_carpet_tiling_matches_oracle(gh, gw, n):
+    # host-only entry point: runs without a GPU
+    from oracle import mesolf_oracle as O
+    got = mesolf_amd.carpet_patch_info(gh, gw, n)
+    want = O.carpet_patches(gh, gw, n)
+    assert [tuple(map(tuple, g["location"])) for g in got] == want
+    for g in got:
+        assert g["size"][0] % 15 == 0 and g["size"][1] % 15 == 0 and min(g["size"]) >= 50

@@ -10,6 +10,7 @@ import numpy as np
 import pytest
 
 import mesolf_amd
+from mesolf_amd import lib as L
 from oracle import mesolf_oracle as O
 
 pytestmark = pytest.mark.gpu
@@ -69,3 +70,74 @@ def test_summary_module_dropin_and_errors():
     for bad in ((0, 20, 3, 5), (100, 0, 3, 5), (100, 20, -1, 5), (100, 20, 3, 0)):
         with pytest.raises(mesolf_amd.InvalidInput):
             mesolf_amd.summary_image_pixelwise(movie, h, w, *bad)
+
+
+class _Hip:
+    """hipMalloc / hipMemcpy / hipFree of the HIP runtime the library loaded
+    (test plumbing for the device-pointer entry points)."""
+
+    def __init__(self):
+        import ctypes as C
+        L.load_library()
+        path = next(line.split()[-1] for line in open("/proc/self/maps") if "libamdhip64" in line)
+        self.C = C
+        self.h = C.CDLL(path)
+        self.h.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+        self.h.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+        self.h.hipFree.argtypes = [C.c_void_p]
+
+    def alloc(self, nbytes):
+        p = self.C.c_void_p()
+        assert self.h.hipMalloc(self.C.byref(p), nbytes) == 0
+        return p.value
+
+    def put(self, a):
+        d = self.alloc(a.nbytes)
+        assert self.h.hipMemcpy(d, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        return d
+
+    def get(self, d, shape, dtype):
+        out = np.empty(shape, dtype)
+        assert self.h.hipMemcpy(out.ctypes.data, d, out.nbytes, 2) == 0  # device to host
+        return out
+
+    def free(self, d):
+        self.h.hipFree(d)
+
+
+def test_carpet_crop_then_summary_on_device():
+    """Carpet patches of a sensor movie in HBM: each patch movie cut on the
+    device equals the MATLAB slice bit for bit, and its summary image computed
+    from device pointers equals the host entry point's."""
+    import ctypes as C
+
+    H, W, T = 200, 170, 40
+    movie = np.asfortranarray(_movie(H * W, T, seed=9).reshape(H, W, T, order="F"))
+    patches = mesolf_amd.carpet_patch_info(H, W, [2, 2])
+    assert len(patches) == 4
+    ctx = mesolf_amd.Context(0)
+    hip = _Hip()
+    d_movie = hip.put(movie.ravel(order="F").copy())
+    bufs = [d_movie]
+    try:
+        for pi in patches:
+            (t, l), (b, r) = pi["location"]
+            h, w = pi["size"]
+            d_out = hip.alloc(4 * h * w * T)
+            d_img = hip.alloc(8 * h * w)
+            bufs += [d_out, d_img]
+            mesolf_amd.crop_movie_device(ctx, d_movie, H, W, T, pi, d_out)
+            ctx.synchronize()
+            want = movie[t - 1:b, l - 1:r, :]
+            assert np.array_equal(hip.get(d_out, h * w * T, np.float32), want.ravel(order="F"))
+            L._check(ctx._lib.mlf_summary_image_device(ctx.handle, C.c_void_p(d_out), h, w, T, 30, 6, 3, 2,
+                                                       C.c_void_p(d_img), None, None))
+            img = hip.get(d_img, h * w, np.float64).reshape(h, w, order="F")
+            host = mesolf_amd.summary_image_pixelwise(want, h, w, 30, 6, 3, 2)[0]
+            assert np.array_equal(img, host)
+        with pytest.raises(mesolf_amd.InvalidInput):
+            mesolf_amd.crop_movie_device(ctx, d_movie, H, W, T, {"location": [[150, 1], [215, 60]]}, bufs[1])
+    finally:
+        for d in bufs:
+            hip.free(d)
+        ctx.close()

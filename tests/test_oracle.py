@@ -415,3 +415,23 @@ def test_summary_pixelwise_by_loops():
     assert np.allclose(img, want / want.max(), rtol=1e-10)
     s0, t0 = O.rank_1_factorization(movie, 2, mx)
     assert np.array_equal(bs, s0) and np.array_equal(bt, t0)
+
+
+# --------------------------------------------------------------- patch tiling
+
+
+def test_carpet_patches_by_hand():
+    # 1000 / 3 = 333 -> 345 (multiple of 15); the last row/column ends at 1000,
+    # 310 px, cut to 300
+    got = O.carpet_patches(1000, 1000, [3, 3])
+    rows = [(1, 345), (346, 690), (691, 990)]
+    want = [((rows[i][0], rows[j][0]), (rows[i][1], rows[j][1])) for j in range(3) for i in range(3)]
+    assert got == want
+    # 100 / 2 = 50 -> 60; the second patch is 61..100 = 40 px -> 30 < 50: discarded
+    assert O.carpet_patches(100, 100, 2) == [((1, 1), (60, 60))]
+    # 700 x 451 in 2 x 3: heights 350 -> 360 (1..360, 361..700 = 340 -> 330);
+    # widths 150 -> 150 (1..150, 151..300, 301..450; 451 is left out)
+    got = O.carpet_patches(700, 451, [2, 3])
+    assert got == [((1, 1), (360, 150)), ((361, 1), (690, 150)),
+                   ((1, 151), (360, 300)), ((361, 151), (690, 300)),
+                   ((1, 301), (360, 450)), ((361, 301), (690, 450))]
