@@ -274,6 +274,21 @@ mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap
                                           int64_t height, int64_t n, const float* d_kernels,
                                           int64_t kw, int64_t nphase, float* d_out);
 
+/* Replaces img_rectify_mod (image_load_module_aug.m:124-164): frames
+ * [H, W] single (column-major, n = frames of them) resampled at a grid of
+ * step dx/M px through (x_center, y_center) (1-based pixels; x along the
+ * second, y along the first axis), started at a whole lenslet and cut to
+ * whole lenslets, by griddedInterpolant's 'linear' interpolation with linear
+ * extrapolation past the image.  FP64 arithmetic; out single
+ * [out_h, out_w, frames].  mlf_rectify_size gives out_h, out_w (host only).
+ * H, W >= 2; the centre inside the image; out_w <= 65535.  Synchronous. */
+mlf_status mlf_rectify_size(int64_t H, int64_t W, double x_center, double y_center, int32_t M,
+                            double dx, int64_t* out_h, int64_t* out_w);
+mlf_status mlf_rectify(const float* in, int64_t H, int64_t W, int64_t frames, double x_center,
+                       double y_center, int32_t M, double dx, float* out, int32_t device);
+mlf_status mlf_rectify_device(mlf_context ctx, const float* d_in, int64_t H, int64_t W, int64_t frames,
+                              double x_center, double y_center, int32_t M, double dx, float* d_out);
+
 /* Replaces the 'carpet' branch of determine_patch_info_aug.m:103-143: an
  * image of global_h x global_w px in n_h x n_w patches of floor(size / n)
  * rounded up to a multiple of M (the last row and column end at the image

@@ -150,6 +150,10 @@ size_t summary_scratch_doubles(size_t npix, int T);
 hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, int p, int bg_iter,
                           double* scratch, float* dbuf, double* img, double* s, hipStream_t st);
 const double* summary_temporal(const double* scratch);
+// Bilinear (griddedInterpolant 'linear', extrapolating) resample of n frames
+// [H, W] at yq [Hq] x xq [Wq] (device arrays, 1-based pixel coordinates).
+hipError_t launch_rectify(const float* in, int H, int W, int n, const double* yq, int Hq, const double* xq,
+                          int Wq, float* out, hipStream_t st);
 // out [h, w, T] = in(top : top+h-1, left : left+w-1, :) of in [H, W, T], 0-based corner.
 hipError_t launch_crop_movie(const float* in, int64_t H, int64_t W, int64_t top, int64_t left, int h,
                              int w, int T, float* out, hipStream_t st);

@@ -1620,6 +1620,91 @@ mlf_status mlf_lfm_convolution_vec(const float* bitmap, int64_t width, int64_t h
   return MLF_OK;
 }
 
+// image_load_module_aug.m:128-148,160-162 (img_rectify_mod): query
+// coordinates along one axis, step dx/M through `center`, from -dx/M to
+// size + 0.9, started at a whole lenslet and cut to whole lenslets.  MATLAB's
+// a:d:b is taken as a + k*d.
+static std::vector<double> rectify_axis(double center, int64_t size, int M, double dx) {
+  const double s = dx / M;
+  auto colon = [](double a, double d, double b) {
+    const int64_t n = (int64_t)std::floor((b - a) / d + 1e-10) + 1;
+    std::vector<double> v;
+    for (int64_t k = 0; k < n; ++k) v.push_back(a + d * (double)k);
+    return v;
+  };
+  std::vector<double> first = colon(center, -s, -s);
+  std::reverse(first.begin(), first.end());
+  const std::vector<double> second = colon(center + s, s, (double)size + 0.9);
+  std::vector<double> grid(first);
+  grid.insert(grid.end(), second.begin(), second.end());
+  const int64_t qc = (int64_t)first.size() - M / 2;  // find(grid == center) - floor(M/2)
+  const int64_t q0 = qc - (int64_t)M * (int64_t)std::floor((double)qc / M);
+  std::vector<double> q;
+  if (q0 < (int64_t)grid.size()) q.assign(grid.begin() + q0, grid.end());
+  q.resize(q.size() / M * M);
+  return q;
+}
+
+static mlf_status rectify_check(int64_t H, int64_t W, double xc, double yc, int32_t M, double dx) {
+  if (H < 2 || W < 2 || M < 1 || !(dx > 0) || !std::isfinite(xc) || !std::isfinite(yc) || xc < 0 ||
+      yc < 0 || xc > (double)W || yc > (double)H)
+    return set_error(MLF_ERR_INVALID_ARGUMENT,
+                     "rectification needs H, W >= 2, M >= 1, dx > 0 and a centre inside the image");
+  return MLF_OK;
+}
+
+mlf_status mlf_rectify_size(int64_t H, int64_t W, double x_center, double y_center, int32_t M,
+                            double dx, int64_t* out_h, int64_t* out_w) {
+  clear_error();
+  if (!out_h || !out_w) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLF_CHECK(rectify_check(H, W, x_center, y_center, M, dx));
+  *out_h = (int64_t)rectify_axis(y_center, H, M, dx).size();
+  *out_w = (int64_t)rectify_axis(x_center, W, M, dx).size();
+  return MLF_OK;
+}
+
+mlf_status mlf_rectify_device(mlf_context ctx, const float* d_in, int64_t H, int64_t W, int64_t frames,
+                              double x_center, double y_center, int32_t M, double dx, float* d_out) {
+  clear_error();
+  if (!ctx || !d_in || !d_out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLF_CHECK(rectify_check(H, W, x_center, y_center, M, dx));
+  if (frames < 1 || H > 0x7fffffff || W > 0x7fffffff || frames > 0x7fffffff)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad frame count or size");
+  const std::vector<double> yq = rectify_axis(y_center, H, M, dx), xq = rectify_axis(x_center, W, M, dx);
+  if (yq.empty() || xq.empty() || xq.size() > 65535)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "rectified size %zu x %zu", yq.size(), xq.size());
+  MLF_CHECK(ctx_activate(ctx));
+  DevBuf<double> dq;
+  MLF_CHECK(dq.alloc(yq.size() + xq.size()));
+  MLF_HIP(hipMemcpyAsync(dq.ptr, yq.data(), sizeof(double) * yq.size(), hipMemcpyHostToDevice, ctx->stream));
+  MLF_HIP(hipMemcpyAsync(dq.ptr + yq.size(), xq.data(), sizeof(double) * xq.size(), hipMemcpyHostToDevice,
+                         ctx->stream));
+  MLF_LAUNCH(launch_rectify(d_in, (int)H, (int)W, (int)frames, dq.ptr, (int)yq.size(), dq.ptr + yq.size(),
+                            (int)xq.size(), d_out, ctx->stream));
+  MLF_HIP(hipStreamSynchronize(ctx->stream));  // before the grid buffer is released
+  return MLF_OK;
+}
+
+mlf_status mlf_rectify(const float* in, int64_t H, int64_t W, int64_t frames, double x_center,
+                       double y_center, int32_t M, double dx, float* out, int32_t device) {
+  clear_error();
+  if (!in || !out) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  int64_t hq = 0, wq = 0;
+  MLF_CHECK(mlf_rectify_size(H, W, x_center, y_center, M, dx, &hq, &wq));
+  if (frames < 1) return set_error(MLF_ERR_INVALID_ARGUMENT, "bad frame count");
+  mlf_context ctx = nullptr;
+  MLF_CHECK(mlf_context_create(device, &ctx));
+  std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> guard(ctx, mlf_context_destroy);
+  DevBuf<float> din, dout;
+  MLF_CHECK(din.alloc((size_t)H * W * frames));
+  MLF_CHECK(dout.alloc((size_t)hq * wq * frames));
+  MLF_HIP(hipMemcpyAsync(din.ptr, in, din.bytes(), hipMemcpyHostToDevice, ctx->stream));
+  MLF_CHECK(mlf_rectify_device(ctx, din.ptr, H, W, frames, x_center, y_center, M, dx, dout.ptr));
+  MLF_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, ctx->stream));
+  MLF_HIP(hipStreamSynchronize(ctx->stream));
+  return MLF_OK;
+}
+
 mlf_status mlf_carpet_patches(int64_t global_h, int64_t global_w, int32_t n_h, int32_t n_w,
                               int32_t M, int32_t min_size, int64_t* corners, int32_t* count) {
   clear_error();

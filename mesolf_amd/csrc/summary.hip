@@ -303,6 +303,34 @@ __global__ __launch_bounds__(kThreads) void k_crop_movie(const float* __restrict
   }
 }
 
+// Rectification resample of frames [H, W] (column-major single) at the query
+// coordinates yq [Hq] x xq [Wq] (1-based pixels): griddedInterpolant's
+// 'linear' interpolation and extrapolation, i.e. the bilinear form of the
+// grid cell k = clamp(floor(q), 1, size-1), which extends the boundary
+// cells linearly past the image.  FP64 arithmetic.
+__global__ __launch_bounds__(kThreads) void k_rectify(const float* __restrict__ in, int H, int W, int n,
+                                                      const double* __restrict__ yq, int Hq,
+                                                      const double* __restrict__ xq, int Wq,
+                                                      float* __restrict__ out) {
+  const int j = blockIdx.y;
+  const double x = xq[j];
+  const int kx = min(max((int)floor(x), 1), W - 1);
+  const double tx = x - kx;
+  for (int f = blockIdx.z; f < n; f += gridDim.z) {
+    const float* img = in + (size_t)f * H * W;
+    const float* c0 = img + (size_t)(kx - 1) * H;  // column kx (1-based)
+    const float* c1 = c0 + H;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Hq; i += gridDim.x * blockDim.x) {
+      const double y = yq[i];
+      const int ky = min(max((int)floor(y), 1), H - 1);
+      const double ty = y - ky;
+      const double v0 = (1.0 - ty) * (double)c0[ky - 1] + ty * (double)c0[ky];
+      const double v1 = (1.0 - ty) * (double)c1[ky - 1] + ty * (double)c1[ky];
+      out[(size_t)f * Hq * Wq + (size_t)j * Hq + i] = (float)((1.0 - tx) * v0 + tx * v1);
+    }
+  }
+}
+
 inline unsigned blocks_for(size_t n) { return (unsigned)((n + kThreads - 1) / kThreads); }
 
 }  // namespace
@@ -357,6 +385,14 @@ hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, in
 }
 
 const double* summary_temporal(const double* scratch) { return scratch + 2; }
+
+hipError_t launch_rectify(const float* in, int H, int W, int n, const double* yq, int Hq, const double* xq,
+                          int Wq, float* out, hipStream_t st) {
+  const unsigned gx = (unsigned)((Hq + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(k_rectify, dim3(gx, (unsigned)Wq, (unsigned)(n < 65535 ? n : 65535)), dim3(kThreads), 0, st,
+                     in, H, W, n, yq, Hq, xq, Wq, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_crop_movie(const float* in, int64_t H, int64_t W, int64_t top, int64_t left, int h,
                              int w, int T, float* out, hipStream_t st) {

@@ -125,6 +125,11 @@ _SIGNATURES = {
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
     "mlf_lfm_convolution_vec_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P]),
+    "mlf_rectify_size": (C.c_int, [_I64, _I64, C.c_double, C.c_double, _I32, C.c_double, _I64P, _I64P]),
+    "mlf_rectify": (C.c_int, [_F32P, _I64, _I64, _I64, C.c_double, C.c_double, _I32, C.c_double, _F32P,
+                              _I32]),
+    "mlf_rectify_device": (C.c_int, [_P, _P, _I64, _I64, _I64, C.c_double, C.c_double, _I32, C.c_double,
+                                     _P]),
     "mlf_carpet_patches": (C.c_int, [_I64, _I64, _I32, _I32, _I32, _I32, _I64P, _I32P]),
     "mlf_crop_movie_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
     "mlf_summary_image": (C.c_int, [_F32P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _F64P, _F64P,
@@ -636,6 +641,29 @@ def LFM_reconstruction_module(summary_image, H, Ht, SI, precision="fp32", valid_
                 p.close()
         finally:
             psf.close()
+
+
+def rectify_size(H: int, W: int, xCenter: float, yCenter: float, M: int, dx: float):
+    """Rectified frame size (rows, columns) of img_rectify_mod (host only)."""
+    lib = load_library()
+    h, w = C.c_int64(), C.c_int64()
+    _check(lib.mlf_rectify_size(int(H), int(W), float(xCenter), float(yCenter), int(M), float(dx),
+                                C.byref(h), C.byref(w)))
+    return h.value, w.value
+
+
+def img_rectify_mod(IMG_BW, xCenter, yCenter, M, dx, device: int = 0):
+    """Drop-in for img_rectify_mod (image_load_module_aug.m:124-164) on one
+    frame [H, W] or a stack [H, W, n]: single output of the rectified size."""
+    lib = load_library()
+    a = _col(IMG_BW, np.float32)
+    H, W = a.shape[:2]
+    n = int(np.prod(a.shape[2:])) if a.ndim > 2 else 1
+    h, w = rectify_size(H, W, xCenter, yCenter, M, dx)
+    out = np.empty((h, w) + a.shape[2:], np.float32, order="F")
+    _check(lib.mlf_rectify(_fptr(a), H, W, n, float(xCenter), float(yCenter), int(M), float(dx),
+                           _fptr(out), int(device)))
+    return out
 
 
 def carpet_patch_info(global_h: int, global_w: int, n_patches, M: int = 15, min_size: int = 50):

@@ -42,6 +42,9 @@ std_image                   numpy's population std of the same rows
 summary_pixelwise           a per-pixel loop composition of brute-force pieces
 carpet_patches              tilings of 1000x1000, 100x100 (a discarded patch)
                             and 700x451 worked out by hand
+rectify_axis / rectify      a grid worked out by hand; scipy's
+                            RegularGridInterpolator (linear, extrapolating);
+                            planar images reproduced at every query point
 ==========================  ==================================================
 
 There is no reference test data to pin against: the reference ships no
@@ -819,3 +822,48 @@ def carpet_patches(global_h: int, global_w: int, n_patches, M: int = 15, min_siz
             continue
         out.append(((top, left), (top + h - 1, left + w - 1)))
     return out
+
+
+# ---------------------------------------------------------------------------
+# rectification -- image_load_module_aug.m:124-164 (img_rectify_mod)
+# ---------------------------------------------------------------------------
+
+
+def _colon(a: float, d: float, b: float) -> np.ndarray:
+    """a:d:b as a + k*d (MATLAB computes the upper half from the end; the
+    difference is rounding in the last bits of the grid)."""
+    n = int(math.floor((b - a) / d + 1e-10)) + 1
+    return a + d * np.arange(max(n, 0))
+
+
+def rectify_axis(center: float, size: int, M: int, dx: float) -> np.ndarray:
+    """Query coordinates (1-based pixels) along one axis: the resample grid
+    of step dx/M through `center` from -dx/M to size + 0.9, started at a whole
+    lenslet (img_rectify_mod :128-148) and cut to whole lenslets (:160-162)."""
+    s = dx / M
+    first = _colon(center, -s, -s)[::-1]
+    second = _colon(center + s, s, size + 0.9)
+    grid = np.concatenate([first, second])
+    q_center = len(first) - M // 2  # find(grid == center) - floor(M/2), 1-based
+    q_init = q_center - M * math.floor(q_center / M)
+    q = grid[q_init:]
+    return q[:M * (len(q) // M)]
+
+
+def rectify(img: np.ndarray, x_center: float, y_center: float, M: int, dx: float) -> np.ndarray:
+    """griddedInterpolant({1:H, 1:W}, img) (linear, linear extrapolation) at
+    the rectification grid: [len(yq), len(xq)] (frames along a third axis)."""
+    from scipy.interpolate import RegularGridInterpolator
+    img = np.asarray(img, np.float64)
+    H, W = img.shape[:2]
+    xq = rectify_axis(x_center, W, M, dx)
+    yq = rectify_axis(y_center, H, M, dx)
+    Y, X = np.meshgrid(yq, xq, indexing="ij")
+    pts = np.stack([Y.ravel(), X.ravel()], axis=1)
+    frames = img.reshape(H, W, -1)
+    out = np.empty((len(yq), len(xq), frames.shape[2]))
+    for f in range(frames.shape[2]):
+        gi = RegularGridInterpolator((np.arange(1, H + 1), np.arange(1, W + 1)), frames[:, :, f],
+                                     method="linear", bounds_error=False, fill_value=None)
+        out[:, :, f] = gi(pts).reshape(len(yq), len(xq))
+    return out.reshape((len(yq), len(xq)) + img.shape[2:])

@@ -101,3 +101,13 @@ def test_carpet_tiling_matches_oracle(gh, gw, n):
     assert [tuple(map(tuple, g["location"])) for g in got] == want
     for g in got:
         assert g["size"][0] % 15 == 0 and g["size"][1] % 15 == 0 and min(g["size"]) >= 50
+
+
+
+@pytest.mark.parametrize("H,W,xc,yc,M,dx", [(20, 20, 8.0, 8.0, 15, 15.0), (40, 33, 3.5, 10.25, 15, 7.5),
+                                             (1515, 1400, 700.3, 651.9, 15, 14.87), (61, 47, 10.3, 7.6, 15, 14.2)])
+def test_rectify_size_matches_oracle(H, W, xc, yc, M, dx):
+    # host-only entry point: runs without a GPU
+    from oracle import mesolf_oracle as O
+    assert mesolf_amd.rectify_size(H, W, xc, yc, M, dx) == (len(O.rectify_axis(yc, H, M, dx)),
+                                                            len(O.rectify_axis(xc, W, M, dx)))

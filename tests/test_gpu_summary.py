@@ -141,3 +141,21 @@ def test_carpet_crop_then_summary_on_device():
         for d in bufs:
             hip.free(d)
         ctx.close()
+
+
+
+@pytest.mark.parametrize("H,W,n,xc,yc,dx", [(61, 47, 3, 10.3, 7.6, 14.2), (40, 33, 1, 3.5, 10.25, 7.5),
+                                             (90, 75, 2, 44.0, 45.0, 15.0)])
+def test_rectify_vs_oracle(H, W, n, xc, yc, dx):
+    """img_rectify_mod against scipy's linear interpolation with linear
+    extrapolation (FP64 on both sides, single output: within 2e-7 relative)."""
+    rng = np.random.default_rng(H + n)
+    frames = (rng.random((H, W, n)) * 1000).astype(np.float32)
+    got = mesolf_amd.img_rectify_mod(frames, xc, yc, 15, dx)
+    want = O.rectify(frames.astype(np.float64), xc, yc, 15, dx)
+    assert got.shape == want.shape and got.shape[0] % 15 == 0 and got.shape[1] % 15 == 0
+    assert np.max(np.abs(got - want)) <= 2e-7 * np.max(np.abs(want))
+    one = mesolf_amd.img_rectify_mod(frames[:, :, 0], xc, yc, 15, dx)
+    assert np.array_equal(one, got[:, :, 0])
+    with pytest.raises(mesolf_amd.InvalidInput):
+        mesolf_amd.img_rectify_mod(frames, W + 5.0, yc, 15, dx)  # centre outside the image

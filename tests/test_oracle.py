@@ -435,3 +435,32 @@ def test_carpet_patches_by_hand():
     assert got == [((1, 1), (360, 150)), ((361, 1), (690, 150)),
                    ((1, 151), (360, 300)), ((361, 151), (690, 300)),
                    ((1, 301), (360, 450)), ((361, 301), (690, 450))]
+
+
+# -------------------------------------------------------------- rectification
+
+
+def test_rectify_axis_by_hand():
+    # center 8, step 15/15 = 1, 20 px: grid -1..20 (22 points), 8 is the 10th;
+    # 10 - floor(15/2) = 3 -> start after 3 points (at 2), 19 left -> cut to 15
+    assert np.array_equal(O.rectify_axis(8.0, 20, 15, 15.0), np.arange(2.0, 17.0))
+    # center 3.5, step 0.5 (dx 7.5): first half 3.5 down to -0.5 (9 points),
+    # 9 - 7 = 2 -> start at 0.5; up to 40.9 -> 0.5..40.5 (81 points) -> 75
+    q = O.rectify_axis(3.5, 40, 15, 7.5)
+    assert len(q) == 75 and q[0] == 0.5 and np.allclose(np.diff(q), 0.5)
+
+
+def test_rectify_reproduces_planes_and_bilinear_by_hand():
+    H, W = 23, 31
+    yy, xx = np.meshgrid(np.arange(1, H + 1), np.arange(1, W + 1), indexing="ij")
+    plane = 0.5 * xx - 0.25 * yy + 3.0
+    out = O.rectify(plane, 10.3, 7.6, 15, 14.2)
+    xq = O.rectify_axis(10.3, W, 15, 14.2)
+    yq = O.rectify_axis(7.6, H, 15, 14.2)
+    Y, X = np.meshgrid(yq, xq, indexing="ij")
+    assert np.allclose(out, 0.5 * X - 0.25 * Y + 3.0, atol=1e-12)  # incl. extrapolated points
+    # one bilinear cell by hand: corners 1, 2 (x), 3, 4 (y) at (y, x) = (1.25, 1.5)
+    img = np.array([[1.0, 2.0], [3.0, 4.0]])
+    from scipy.interpolate import RegularGridInterpolator
+    gi = RegularGridInterpolator(([1, 2], [1, 2]), img, bounds_error=False, fill_value=None)
+    assert abs(gi([[1.25, 1.5]])[0] - (0.75 * 1.5 + 0.25 * 3.5)) < 1e-15
