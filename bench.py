@@ -387,6 +387,40 @@ def bench_summary(args):
         times.append(time.perf_counter() - t)
     dt = min(times)
     nbytes = 4.0 * npix * T * (4 + 2 * 5)  # max, 5 x (s'Y, Y t), detrend read + write, smooth read
+
+    # the producers in front of it: rectification of the raw stack (img_rectify_mod,
+    # 14.8-px lenslet pitch) and the cut of its 2 x 2 carpet patches
+    xc, yc, dx = 8.3, 7.9, 14.8
+    hq, wq = mesolf_amd.rectify_size(h, w, xc, yc, 15, dx)
+    rect = torch.empty(hq * wq * T, device="cuda", dtype=torch.float32)
+    torch.cuda.synchronize()
+
+    def rectify():
+        L._check(lib.mlf_rectify_device(ctx.handle, C.c_void_p(movie.data_ptr()), h, w, T, xc, yc, 15, dx,
+                                        C.c_void_p(rect.data_ptr())))
+
+    rectify()
+    t = time.perf_counter()
+    rectify()
+    t_rect = time.perf_counter() - t
+    patches = mesolf_amd.carpet_patch_info(hq, wq, [2, 2])
+    pbuf = torch.empty(max(p["size"][0] * p["size"][1] for p in patches) * T, device="cuda",
+                       dtype=torch.float32)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    for pi in patches:
+        mesolf_amd.crop_movie_device(ctx, rect.data_ptr(), hq, wq, T, pi, pbuf.data_ptr())
+    ctx.synchronize()
+    t_crop = time.perf_counter() - t
+    crop_px = sum(p["size"][0] * p["size"][1] for p in patches) * T
+    producers = {
+        "rectify": {"ms": round(t_rect * 1e3, 3), "in": [h, w, T], "out": [hq, wq, T],
+                    "GB_per_s": round(4.0 * (h * w + hq * wq) * T / t_rect / 1e9, 1)},
+        "crop_2x2_carpet": {"ms": round(t_crop * 1e3, 3), "patches": len(patches),
+                            "GB_per_s": round(8.0 * crop_px / t_crop / 1e9, 1)},
+        "note": "call times (synchronous entry points); bytes = samples read + written x 4",
+    }
+    del rect, pbuf
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import mesolf_oracle as O
@@ -407,6 +441,7 @@ def bench_summary(args):
                      "frac": nbytes / dt / HBM_PEAK, "traffic": None,
                      "note": "algorithmic bytes 56 per sample; call time incl. scratch allocation"},
         "cpu_baseline": cpu,
+        "producers": producers,
     }), flush=True)
     ctx.close()
 

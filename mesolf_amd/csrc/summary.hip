@@ -312,22 +312,21 @@ __global__ __launch_bounds__(kThreads) void k_rectify(const float* __restrict__ 
                                                       const double* __restrict__ yq, int Hq,
                                                       const double* __restrict__ xq, int Wq,
                                                       float* __restrict__ out) {
+  // one output row i per thread and column j per workgroup row, the cell
+  // and weights computed once; frames strided over the grid's z
   const int j = blockIdx.y;
-  const double x = xq[j];
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Hq) return;
+  const double x = xq[j], y = yq[i];
   const int kx = min(max((int)floor(x), 1), W - 1);
-  const double tx = x - kx;
+  const int ky = min(max((int)floor(y), 1), H - 1);
+  const double tx = x - kx, ty = y - ky;
+  const size_t o0 = (size_t)(kx - 1) * H + (ky - 1);  // (row ky, column kx), 1-based
   for (int f = blockIdx.z; f < n; f += gridDim.z) {
-    const float* img = in + (size_t)f * H * W;
-    const float* c0 = img + (size_t)(kx - 1) * H;  // column kx (1-based)
-    const float* c1 = c0 + H;
-    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Hq; i += gridDim.x * blockDim.x) {
-      const double y = yq[i];
-      const int ky = min(max((int)floor(y), 1), H - 1);
-      const double ty = y - ky;
-      const double v0 = (1.0 - ty) * (double)c0[ky - 1] + ty * (double)c0[ky];
-      const double v1 = (1.0 - ty) * (double)c1[ky - 1] + ty * (double)c1[ky];
-      out[(size_t)f * Hq * Wq + (size_t)j * Hq + i] = (float)((1.0 - tx) * v0 + tx * v1);
-    }
+    const float* img = in + (size_t)f * H * W + o0;
+    const double v0 = (1.0 - ty) * (double)img[0] + ty * (double)img[1];
+    const double v1 = (1.0 - ty) * (double)img[H] + ty * (double)img[H + 1];
+    out[(size_t)f * Hq * Wq + (size_t)j * Hq + i] = (float)((1.0 - tx) * v0 + tx * v1);
   }
 }
 
@@ -389,7 +388,7 @@ const double* summary_temporal(const double* scratch) { return scratch + 2; }
 hipError_t launch_rectify(const float* in, int H, int W, int n, const double* yq, int Hq, const double* xq,
                           int Wq, float* out, hipStream_t st) {
   const unsigned gx = (unsigned)((Hq + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_rectify, dim3(gx, (unsigned)Wq, (unsigned)(n < 65535 ? n : 65535)), dim3(kThreads), 0, st,
+  hipLaunchKernelGGL(k_rectify, dim3(gx, (unsigned)Wq, (unsigned)(n < 16 ? n : 16)), dim3(kThreads), 0, st,
                      in, H, W, n, yq, Hq, xq, Wq, out);
   return hipGetLastError();
 }
@@ -397,7 +396,7 @@ hipError_t launch_rectify(const float* in, int H, int W, int n, const double* yq
 hipError_t launch_crop_movie(const float* in, int64_t H, int64_t W, int64_t top, int64_t left, int h,
                              int w, int T, float* out, hipStream_t st) {
   const unsigned gx = (unsigned)((h + kThreads - 1) / kThreads);
-  hipLaunchKernelGGL(k_crop_movie, dim3(gx, (unsigned)w, (unsigned)(T < 65535 ? T : 65535)), dim3(kThreads), 0,
+  hipLaunchKernelGGL(k_crop_movie, dim3(gx, (unsigned)w, (unsigned)(T < 16 ? T : 16)), dim3(kThreads), 0,
                      st, in, H, W, top, left, h, w, T, out);
   return hipGetLastError();
 }
