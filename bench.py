@@ -475,7 +475,7 @@ def main():
 
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus must be 1 "
-                          "(sharding across GPUs is described in DESIGN.md, not implemented)"}))
+                          "(sharding across GPUs is designed but not implemented)"}))
         sys.exit(2)
 
     if args.workload == "summary":
