@@ -320,8 +320,7 @@ mlf_status mlf_crop_movie_device(mlf_context ctx, const float* d_movie, int64_t 
  *   summary:     double [height, width]
  *   bg_spatial:  double [height, width] or NULL;  bg_temporal: double [frames] or NULL
  * Defaults of the reference: window_1 = 100, window_2 = 20, poly_index = 3,
- * bg_iter = 5 (mesolf_set_params.m:120-132).  The 'global' detrend mode
- * (Savitzky-Golay baseline) is not built.  Synchronous. */
+ * bg_iter = 5 (mesolf_set_params.m:120-132).  Synchronous. */
 mlf_status mlf_summary_image(const float* movie, int64_t height, int64_t width, int64_t frames,
                              int32_t window_1, int32_t window_2, int32_t poly_index,
                              int32_t bg_iter, double* summary, double* bg_spatial,
@@ -332,6 +331,21 @@ mlf_status mlf_summary_image_device(mlf_context ctx, const float* d_movie, int64
                                     int64_t width, int64_t frames, int32_t window_1,
                                     int32_t window_2, int32_t poly_index, int32_t bg_iter,
                                     double* d_summary, double* d_bg_spatial, double* d_bg_temporal);
+
+/* Same for detrend mode 'global' (summary_image_generation_module.m:43-60,
+ * :130): the movie divided by its max, its rank-1 background (bg_iter
+ * rounds, rank_1_factorization.m with divisor 1), and compute_std_image.m as
+ * written with that background, sqrt((A - s.^2)/T - B.^2 + 2 B.*s C - s.^2 C^2),
+ * A = sum(x.^2, 2), B = mean(x, 2), C = mean(bg_temporal); divided by its max.
+ * A negative argument gives NaN (MATLAB would return a complex value).  The
+ * Savitzky-Golay baseline of :45-48 only feeds a plot and is not computed. */
+mlf_status mlf_summary_image_global(const float* movie, int64_t height, int64_t width,
+                                    int64_t frames, int32_t bg_iter, double* summary,
+                                    double* bg_spatial, double* bg_temporal, int32_t device);
+mlf_status mlf_summary_image_global_device(mlf_context ctx, const float* d_movie, int64_t height,
+                                          int64_t width, int64_t frames, int32_t bg_iter,
+                                          double* d_summary, double* d_bg_spatial,
+                                          double* d_bg_temporal);
 
 /* All MEX projections of the seed-library builder in one call
  * (gen_seed_library_local_with_ring_bg_same_size.m:263-318 calls the MEX 4

@@ -147,8 +147,11 @@ constexpr int kPostBlocks = 64;  // workgroups per plane in the max passes
 // npix * T floats; img, s: npix doubles (summary image normalised by its max,
 // bg_spatial); bg_temporal is left at summary_temporal(scratch) [T].
 size_t summary_scratch_doubles(size_t npix, int T);
+// global != 0: the 'global' detrend (std image with the rank-1 background,
+// summary_image_generation_module.m:43-60); dbuf, w1, w2 and p are unused.
 hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, int p, int bg_iter,
-                          double* scratch, float* dbuf, double* img, double* s, hipStream_t st);
+                          int global, double* scratch, float* dbuf, double* img, double* s,
+                          hipStream_t st);
 const double* summary_temporal(const double* scratch);
 // Bilinear (griddedInterpolant 'linear', extrapolating) resample of n frames
 // [H, W] at yq [Hq] x xq [Wq] (device arrays, 1-based pixel coordinates).

@@ -1749,11 +1749,33 @@ mlf_status mlf_crop_movie_device(mlf_context ctx, const float* d_movie, int64_t 
   return MLF_OK;
 }
 
+static mlf_status summary_device(mlf_context ctx, const float* d_movie, int64_t height,
+                                 int64_t width, int64_t frames, int32_t window_1, int32_t window_2,
+                                 int32_t poly_index, int32_t bg_iter, int global, double* d_summary,
+                                 double* d_bg_spatial, double* d_bg_temporal);
+
 mlf_status mlf_summary_image_device(mlf_context ctx, const float* d_movie, int64_t height,
                                    int64_t width, int64_t frames, int32_t window_1,
                                    int32_t window_2, int32_t poly_index, int32_t bg_iter,
                                    double* d_summary, double* d_bg_spatial, double* d_bg_temporal) {
   clear_error();
+  return summary_device(ctx, d_movie, height, width, frames, window_1, window_2, poly_index, bg_iter,
+                        0, d_summary, d_bg_spatial, d_bg_temporal);
+}
+
+mlf_status mlf_summary_image_global_device(mlf_context ctx, const float* d_movie, int64_t height,
+                                          int64_t width, int64_t frames, int32_t bg_iter,
+                                          double* d_summary, double* d_bg_spatial,
+                                          double* d_bg_temporal) {
+  clear_error();
+  return summary_device(ctx, d_movie, height, width, frames, 1, 1, 0, bg_iter, 1, d_summary,
+                        d_bg_spatial, d_bg_temporal);
+}
+
+static mlf_status summary_device(mlf_context ctx, const float* d_movie, int64_t height,
+                                 int64_t width, int64_t frames, int32_t window_1, int32_t window_2,
+                                 int32_t poly_index, int32_t bg_iter, int global, double* d_summary,
+                                 double* d_bg_spatial, double* d_bg_temporal) {
   if (!ctx || !d_movie || !d_summary) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
   if (height < 1 || width < 1 || frames < 1 || frames > 0x7fffffff)
     return set_error(MLF_ERR_INVALID_ARGUMENT, "bad movie dims %lld x %lld x %lld", (long long)height,
@@ -1770,14 +1792,14 @@ mlf_status mlf_summary_image_device(mlf_context ctx, const float* d_movie, int64
   DevBuf<double> scratch, s_own;
   DevBuf<float> dbuf;
   MLF_CHECK(scratch.alloc(summary_scratch_doubles(npix, T)));
-  MLF_CHECK(dbuf.alloc(npix * (size_t)T));
+  if (!global) MLF_CHECK(dbuf.alloc(npix * (size_t)T));
   double* s = d_bg_spatial;
   if (!s) {
     MLF_CHECK(s_own.alloc(npix));
     s = s_own.ptr;
   }
-  MLF_LAUNCH(launch_summary(d_movie, npix, T, window_1, window_2, poly_index, bg_iter, scratch.ptr,
-                            dbuf.ptr, d_summary, s, ctx->stream));
+  MLF_LAUNCH(launch_summary(d_movie, npix, T, window_1, window_2, poly_index, bg_iter, global,
+                            scratch.ptr, dbuf.ptr, d_summary, s, ctx->stream));
   if (d_bg_temporal)
     MLF_HIP(hipMemcpyAsync(d_bg_temporal, summary_temporal(scratch.ptr), sizeof(double) * T,
                            hipMemcpyDeviceToDevice, ctx->stream));
@@ -1785,11 +1807,32 @@ mlf_status mlf_summary_image_device(mlf_context ctx, const float* d_movie, int64
   return MLF_OK;
 }
 
+static mlf_status summary_host(const float* movie, int64_t height, int64_t width, int64_t frames,
+                               int32_t window_1, int32_t window_2, int32_t poly_index,
+                               int32_t bg_iter, int global, double* summary, double* bg_spatial,
+                               double* bg_temporal, int32_t device);
+
 mlf_status mlf_summary_image(const float* movie, int64_t height, int64_t width, int64_t frames,
                              int32_t window_1, int32_t window_2, int32_t poly_index,
                              int32_t bg_iter, double* summary, double* bg_spatial,
                              double* bg_temporal, int32_t device) {
   clear_error();
+  return summary_host(movie, height, width, frames, window_1, window_2, poly_index, bg_iter, 0,
+                      summary, bg_spatial, bg_temporal, device);
+}
+
+mlf_status mlf_summary_image_global(const float* movie, int64_t height, int64_t width,
+                                    int64_t frames, int32_t bg_iter, double* summary,
+                                    double* bg_spatial, double* bg_temporal, int32_t device) {
+  clear_error();
+  return summary_host(movie, height, width, frames, 1, 1, 0, bg_iter, 1, summary, bg_spatial,
+                      bg_temporal, device);
+}
+
+static mlf_status summary_host(const float* movie, int64_t height, int64_t width, int64_t frames,
+                               int32_t window_1, int32_t window_2, int32_t poly_index,
+                               int32_t bg_iter, int global, double* summary, double* bg_spatial,
+                               double* bg_temporal, int32_t device) {
   if (!movie || !summary) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
   if (height < 1 || width < 1 || frames < 1 || frames > 0x7fffffff)
     return set_error(MLF_ERR_INVALID_ARGUMENT, "bad movie dims");
@@ -1804,8 +1847,8 @@ mlf_status mlf_summary_image(const float* movie, int64_t height, int64_t width, 
   MLF_CHECK(ds.alloc(npix));
   MLF_CHECK(dt.alloc((size_t)frames));
   MLF_HIP(hipMemcpyAsync(dm.ptr, movie, dm.bytes(), hipMemcpyHostToDevice, ctx->stream));
-  MLF_CHECK(mlf_summary_image_device(ctx, dm.ptr, height, width, frames, window_1, window_2,
-                                     poly_index, bg_iter, dimg.ptr, ds.ptr, dt.ptr));
+  MLF_CHECK(summary_device(ctx, dm.ptr, height, width, frames, window_1, window_2, poly_index,
+                           bg_iter, global, dimg.ptr, ds.ptr, dt.ptr));
   MLF_HIP(hipMemcpyAsync(summary, dimg.ptr, dimg.bytes(), hipMemcpyDeviceToHost, ctx->stream));
   if (bg_spatial)
     MLF_HIP(hipMemcpyAsync(bg_spatial, ds.ptr, ds.bytes(), hipMemcpyDeviceToHost, ctx->stream));

@@ -303,6 +303,36 @@ __global__ __launch_bounds__(kThreads) void k_crop_movie(const float* __restrict
   }
 }
 
+// 'global' detrend: compute_std_image.m as written with the rank-1
+// background (y_1 = s, y_2 = t) of the movie divided by its max:
+// sqrt((A - s_i^2)/T - B^2 + 2 B s_i C - s_i^2 C^2), A = sum(x^2), B = mean(x),
+// C = mean(t), x = Y / mx.  A negative argument gives NaN.
+__global__ __launch_bounds__(kThreads) void k_std_bg(const float* __restrict__ Y, size_t npix, int T,
+                                                     const double* __restrict__ mx,
+                                                     const double* __restrict__ s,
+                                                     const double* __restrict__ t, double* __restrict__ img,
+                                                     double* __restrict__ parts) {
+  __shared__ double sh[kThreads / 64];
+  const size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+  double v = __builtin_nan("");
+  if (i < npix) {
+    const double m = *mx;
+    double A = 0.0, Bs = 0.0, Cs = 0.0;
+#pragma unroll 8
+    for (int j = 0; j < T; ++j) {
+      const double x = (double)Y[i + (size_t)j * npix] / m;
+      A = fma(x, x, A);
+      Bs += x;
+      Cs += t[j];
+    }
+    const double B = Bs / (double)T, C = Cs / (double)T, y1 = s[i];
+    v = sqrt((A - y1 * y1) / (double)T - B * B + 2.0 * B * y1 * C - y1 * y1 * (C * C));
+    img[i] = v;
+  }
+  v = block_reduce(v, sh, nmax);
+  if (threadIdx.x == 0) parts[blockIdx.x] = v;
+}
+
 // Rectification resample of frames [H, W] (column-major single) at the query
 // coordinates yq [Hq] x xq [Wq] (1-based pixels): griddedInterpolant's
 // 'linear' interpolation and extrapolation, i.e. the bilinear form of the
@@ -342,7 +372,8 @@ size_t summary_scratch_doubles(size_t npix, int T) {
 }
 
 hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, int p, int bg_iter,
-                          double* scratch, float* dbuf, double* img, double* s, hipStream_t st) {
+                          int global, double* scratch, float* dbuf, double* img, double* s,
+                          hipStream_t st) {
   const size_t nchunk = (npix + kChunk - 1) / kChunk;
   const unsigned nblk = blocks_for(npix);
   double* mx = scratch;
@@ -376,8 +407,12 @@ hipError_t launch_summary(const float* Y, size_t npix, int T, int w1, int w2, in
     }
   }
   // summary image
-  hipLaunchKernelGGL(k_detrend, dim3(nblk), dim3(kThreads), 0, st, Y, npix, T, w1, mx, dbuf);
-  hipLaunchKernelGGL(k_smooth_std, dim3(nblk), dim3(kThreads), 0, st, dbuf, npix, T, w2, p, img, parts);
+  if (global) {
+    hipLaunchKernelGGL(k_std_bg, dim3(nblk), dim3(kThreads), 0, st, Y, npix, T, mx, s, t, img, parts);
+  } else {
+    hipLaunchKernelGGL(k_detrend, dim3(nblk), dim3(kThreads), 0, st, Y, npix, T, w1, mx, dbuf);
+    hipLaunchKernelGGL(k_smooth_std, dim3(nblk), dim3(kThreads), 0, st, dbuf, npix, T, w2, p, img, parts);
+  }
   hipLaunchKernelGGL(k_fold_max, dim3(1), dim3(kThreads), 0, st, parts, (int)nblk, nrm);
   hipLaunchKernelGGL(k_scale_div, dim3(nblk < 65535 ? nblk : 65535), dim3(kThreads), 0, st, img, npix, nrm);
   return hipGetLastError();

@@ -134,6 +134,8 @@ _SIGNATURES = {
     "mlf_crop_movie_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
     "mlf_summary_image": (C.c_int, [_F32P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _F64P, _F64P,
                                     _F64P, _I32]),
+    "mlf_summary_image_global": (C.c_int, [_F32P, _I64, _I64, _I64, _I32, _F64P, _F64P, _F64P, _I32]),
+    "mlf_summary_image_global_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I32, _P, _P, _P]),
     "mlf_summary_image_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _P, _P,
                                            _P]),
     "mlf_lfm_convolution_segments": (C.c_int, [_F32P, _I64, _I64, _I64, _I32P, _I64P, _I64, _F32P,
@@ -722,18 +724,39 @@ def summary_image_pixelwise(movie, height: int, width: int, window_1: int = 100,
     return img, bs, bt
 
 
+def summary_image_global(movie, height: int, width: int, bg_iter: int = 5, device: int = 0):
+    """Summary image and rank-1 background for detrend mode 'global'
+    (summary_image_generation_module.m:43-60, :130); arguments and returns as
+    summary_image_pixelwise."""
+    lib = load_library()
+    m = _col(movie, np.float32)
+    if m.ndim == 3:
+        if m.shape[:2] != (height, width):
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"movie {m.shape} is not [{height}, {width}, T]")
+        m = m.reshape(height * width, m.shape[2], order="F")
+    if m.ndim != 2 or m.shape[0] != height * width:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, f"movie {m.shape} is not [{height}*{width}, T]")
+    T = m.shape[1]
+    img = np.empty((height, width), np.float64, order="F")
+    bs = np.empty((height, width), np.float64, order="F")
+    bt = np.empty((1, T), np.float64, order="F")
+    _check(lib.mlf_summary_image_global(_fptr(m), height, width, T, int(bg_iter), _fptr(img, C.c_double),
+                                        _fptr(bs, C.c_double), _fptr(bt, C.c_double), int(device)))
+    return img, bs, bt
+
+
 def summary_image_generation_module(sensor_movie, patch_info, SI, device: int = 0):
-    """Drop-in for summary_image_generation_module.m:1-130 (detrend mode
-    'pixelwise', the default): returns (summary_image, bg_spatial_init,
-    bg_temporal_init).  The images and plots the module saves are left out;
-    the 'global' mode (Savitzky-Golay baseline) raises Unsupported."""
+    """Drop-in for summary_image_generation_module.m:1-130 (detrend modes
+    'pixelwise', the default, and 'global'): returns (summary_image,
+    bg_spatial_init, bg_temporal_init).  The images and plots the module
+    saves are left out (in 'global' mode the smoothed baseline only feeds one)."""
     mode = _si_get(SI, "detrend_mode", "pixelwise")
-    if mode == "global":
-        raise Unsupported(MLF_ERR_UNSUPPORTED, "detrend_mode 'global' is not built")
-    if mode != "pixelwise":
+    if mode not in ("pixelwise", "global"):
         raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, 'Invalid value for param "detrend_mode"')
     size = _si_get(patch_info, "size")
     h, w = int(size[0]), int(size[1])
+    if mode == "global":
+        return summary_image_global(sensor_movie, h, w, _si_get(SI, "bg_iter", 5), device)
     return summary_image_pixelwise(sensor_movie, h, w, _si_get(SI, "pixelwise_window_1", 100),
                                    _si_get(SI, "pixelwise_window_2", 20),
                                    _si_get(SI, "pixelwise_poly_index", 3), _si_get(SI, "bg_iter", 5),

@@ -39,6 +39,8 @@ movmean                     MATLAB's documented movmean(A, 3) example; brute
 rank_1_factorization        rank-1 movies (closed form); the leading singular
                             vectors (numpy SVD) after many iterations
 std_image                   numpy's population std of the same rows
+std_image_bg                a 2x3 case worked out by hand; y_1 = 0 gives std_image
+summary_global              rank-1 of movie/max equals the divisor form
 summary_pixelwise           a per-pixel loop composition of brute-force pieces
 carpet_patches              tilings of 1000x1000, 100x100 (a discarded patch)
                             and 700x451 worked out by hand
@@ -867,3 +869,31 @@ def rectify(img: np.ndarray, x_center: float, y_center: float, M: int, dx: float
                                      method="linear", bounds_error=False, fill_value=None)
         out[:, :, f] = gi(pts).reshape(len(yq), len(xq))
     return out.reshape((len(yq), len(xq)) + img.shape[2:])
+
+
+
+def std_image_bg(Y: np.ndarray, y_1: np.ndarray, y_2: np.ndarray) -> np.ndarray:
+    """compute_std_image.m:23-28 as written:
+    sqrt((A - y_1.^2) / length(y_2) - B.^2 + 2 * B .* y_1 * C - y_1.^2 * C^2),
+    A = sum(Y.^2, 2), B = mean(Y, 2), C = mean(y_2); a negative argument gives
+    NaN here (MATLAB would return a complex value)."""
+    Y = np.asarray(Y, np.float64)
+    y_1 = np.asarray(y_1, np.float64).ravel()
+    y_2 = np.asarray(y_2, np.float64).ravel()
+    A = np.sum(Y * Y, axis=1)
+    B = np.mean(Y, axis=1)
+    C = np.mean(y_2)
+    with np.errstate(invalid="ignore"):
+        return np.sqrt((A - y_1 ** 2) / len(y_2) - B ** 2 + 2 * B * y_1 * C - y_1 ** 2 * C ** 2)
+
+
+def summary_global(movie: np.ndarray, bg_iter: int = 5):
+    """summary_image_generation_module.m:43-60 and :130 ('global'): the movie
+    divided by its max, its rank-1 background (divisor 1) and
+    compute_std_image with that background; the Savitzky-Golay baseline of
+    :45-48 only feeds a plot.  Returns (summary / max, bg_spatial, bg_temporal)."""
+    Y = np.asarray(movie, np.float64)
+    Y = Y / float(np.max(Y))
+    bs, bt = rank_1_factorization(Y, bg_iter)
+    img = std_image_bg(Y, bs, bt)
+    return img / matlab_max(img), bs, bt

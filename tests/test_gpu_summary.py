@@ -63,8 +63,9 @@ def test_summary_module_dropin_and_errors():
     want = mesolf_amd.summary_image_pixelwise(movie, h, w, 30, 6, 3, 4)
     for a, b in zip(got, want):
         assert np.array_equal(a, b)
-    with pytest.raises(mesolf_amd.Unsupported):
-        mesolf_amd.summary_image_generation_module(movie, {"size": [h, w]}, dict(SI, detrend_mode="global"))
+    g = mesolf_amd.summary_image_generation_module(movie, {"size": [h, w]}, dict(SI, detrend_mode="global"))
+    for a, b in zip(g, mesolf_amd.summary_image_global(movie, h, w, 4)):
+        assert np.array_equal(a, b)
     with pytest.raises(mesolf_amd.InvalidInput):
         mesolf_amd.summary_image_generation_module(movie, {"size": [h, w]}, dict(SI, detrend_mode="x"))
     for bad in ((0, 20, 3, 5), (100, 0, 3, 5), (100, 20, -1, 5), (100, 20, 3, 0)):
@@ -159,3 +160,18 @@ def test_rectify_vs_oracle(H, W, n, xc, yc, dx):
     assert np.array_equal(one, got[:, :, 0])
     with pytest.raises(mesolf_amd.InvalidInput):
         mesolf_amd.img_rectify_mod(frames, W + 5.0, yc, 15, dx)  # centre outside the image
+
+
+
+@pytest.mark.parametrize("h,w,T,seed", [(20, 17, 150, 1), (70, 65, 130, 2), (13, 11, 64, 3)])
+def test_summary_global_vs_oracle(h, w, T, seed):
+    """'global' detrend: the std image with the rank-1 background (the square
+    root's argument stays above 0.03 for these movies, so no NaN can flip)."""
+    rng = np.random.default_rng(seed)
+    movie = (rng.random((h * w, T)) * 50 + rng.random((h * w, 1)) * 20).astype(np.float32)
+    img, bs, bt = mesolf_amd.summary_image_global(movie, h, w, 5)
+    ref_img, ref_s, ref_t = O.summary_global(movie, 5)
+    assert not np.isnan(img).any()
+    assert np.max(np.abs(img - ref_img.reshape(h, w, order="F"))) <= IMG_TOL
+    assert np.allclose(bs, ref_s.reshape(h, w, order="F"), rtol=BG_RTOL, atol=0)
+    assert np.allclose(bt[0], ref_t, rtol=BG_RTOL, atol=0)

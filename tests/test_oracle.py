@@ -464,3 +464,22 @@ def test_rectify_reproduces_planes_and_bilinear_by_hand():
     from scipy.interpolate import RegularGridInterpolator
     gi = RegularGridInterpolator(([1, 2], [1, 2]), img, bounds_error=False, fill_value=None)
     assert abs(gi([[1.25, 1.5]])[0] - (0.75 * 1.5 + 0.25 * 3.5)) < 1e-15
+
+
+
+def test_std_image_bg_by_hand():
+    # Y = [[1, 2, 3], [2, 2, 2]], y_1 = [0.5, 1], y_2 = [1, 0, 2]: C = 1
+    # pixel 1: A = 14, B = 2: (14 - 0.25)/3 - 4 + 2*2*0.5*1 - 0.25*1 = 0.33333...
+    # pixel 2: A = 12, B = 2: (12 - 1)/3 - 4 + 2*2*1*1 - 1*1 = 2/3
+    got = O.std_image_bg(np.array([[1.0, 2, 3], [2, 2, 2]]), np.array([0.5, 1.0]), np.array([1.0, 0, 2]))
+    assert np.allclose(got, np.sqrt([13.75 / 3 - 4 + 2 - 0.25, 11 / 3 - 4 + 4 - 1]), rtol=1e-14)
+    Y = np.random.default_rng(3).random((5, 9))
+    assert np.allclose(O.std_image_bg(Y, np.zeros(5), np.random.default_rng(4).random(9)),
+                       O.std_image(Y), rtol=1e-12)
+
+
+def test_summary_global_background_is_divisor_form():
+    movie = np.random.default_rng(8).random((30, 20)) * 100
+    _, s, t = O.summary_global(movie, 3)
+    s2, t2 = O.rank_1_factorization(movie, 3, movie.max())
+    assert np.allclose(s, s2, rtol=1e-12) and np.allclose(t, t2, rtol=1e-12)
