@@ -34,7 +34,8 @@ typedef enum {
   MLF_ERR_DEVICE = 3,           /* HIP runtime error (message carries hipGetErrorString) */
   MLF_ERR_OUT_OF_MEMORY = 4,
   MLF_ERR_BUDGET = 5,           /* request refused by the HBM admission budget */
-  MLF_ERR_INTERNAL = 6
+  MLF_ERR_INTERNAL = 6,
+  MLF_ERR_CANCELLED = 7         /* queued request dropped before it ran (server destroyed) */
 } mlf_status;
 
 typedef enum {
@@ -244,9 +245,21 @@ mlf_status mlf_server_wait(mlf_server s, uint64_t ticket, double* latency_ms);
 /* Block until every submitted request has completed. */
 mlf_status mlf_server_drain(mlf_server s);
 mlf_status mlf_server_get_stats(mlf_server s, mlf_server_stats* st);
-/* Requests still queued are cancelled (their wait returns an error); those
- * in flight complete first. */
+/* Requests still queued are cancelled (their wait returns MLF_ERR_CANCELLED);
+ * those in flight complete first. */
 mlf_status mlf_server_destroy(mlf_server s);
+/* Failure handling.  mlf_server_submit refuses (MLF_ERR_BUDGET) a patch whose
+ * state alone exceeds hbm_budget_bytes.  A request that fails on the device
+ * completes with that error, its patch's resident state is dropped (the next
+ * frame of the patch starts cold) and the loop keeps serving the others.
+ * mlf_server_inject_fault forces such a failure from the host, for tests of
+ * the failure paths: MLF_FAULT_ADMIT fails the admission of the next request
+ * of patch_id; MLF_FAULT_ITERATION fails the next batched iteration as a
+ * device error there would (every request in flight fails; patch_id is
+ * ignored).  No kernel is made to fault. */
+#define MLF_FAULT_ADMIT 1
+#define MLF_FAULT_ITERATION 2
+mlf_status mlf_server_inject_fault(mlf_server s, int32_t kind, int32_t patch_id);
 
 /* ------------------------------------------------------------ drop-ins */
 /* recon_theory_phase_space_LFM_with_filter(LFIMG,H,Ht,maxIter,gpuFlag,gpu_id)

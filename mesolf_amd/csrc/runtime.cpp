@@ -215,6 +215,7 @@ const char* mlf_status_string(mlf_status s) {
     case MLF_ERR_OUT_OF_MEMORY: return "out of device memory";
     case MLF_ERR_BUDGET: return "HBM budget exceeded";
     case MLF_ERR_INTERNAL: return "internal error";
+    case MLF_ERR_CANCELLED: return "cancelled";
   }
   return "unknown status";
 }
@@ -1114,7 +1115,7 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
     p->fwd_mt = p->kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
     p->bwd_nt = kBwdNTiles;
     int mt = 0, nt = 0;
-    if (tune_cache_get(tune_key(p.get(), "patch"), &mt, &nt)) {
+    if (tune_cache_get(tune_key(p.get(), "patch"), p->kp, &mt, &nt)) {
       p->fwd_mt = mt;
       p->bwd_nt = nt;
     }

@@ -6,7 +6,10 @@
 // attached patch as one batch launch sequence, and retires the requests that
 // reached their iteration count.  Patch state (volume, image, work buffers)
 // stays resident per patch id so the next frame starts warm; idle patches are
-// evicted least-recently-used first when a new patch needs room.
+// evicted least-recently-used first when a new patch needs room.  A request
+// whose patch alone exceeds the budget is refused at submission; one that
+// fails on the device fails alone (its patch state is dropped, so the patch's
+// next frame starts cold) and the loop keeps serving.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -71,7 +74,7 @@ struct mlf_server_s {
   std::condition_variable cv_work, cv_done;
   std::deque<std::shared_ptr<Request>> pending;
   std::unordered_map<uint64_t, std::shared_ptr<Request>> tickets;
-  std::map<int32_t, Resident> resident;  // worker thread only, except stats
+  std::map<int32_t, Resident> resident;  // worker thread only
   std::vector<Active> active;            // worker thread only
   uint64_t next_ticket = 1, clock = 0;
   bool stop = false;
@@ -79,8 +82,12 @@ struct mlf_server_s {
   // statistics (under mu)
   std::vector<double> latencies;
   int64_t submitted = 0, completed = 0, evictions = 0, cold_starts = 0, iterations_run = 0;
-  int64_t in_flight = 0;
+  int64_t in_flight = 0, resident_count = 0;
+  int64_t waiters = 0;  // threads inside mlf_server_wait (destroy waits for them)
   uint64_t resident_bytes = 0, peak_resident_bytes = 0;
+  // host-forced failures (mlf_server_inject_fault, under mu)
+  int32_t fault_admit_patch = -1;
+  int32_t fault_runs = 0;
   double slot_sum = 0, t_first = -1, t_last = 0;
 };
 
@@ -157,15 +164,24 @@ mlf_status evict_patch(mlf_server_s* s, int32_t id) {
   auto it = s->resident.find(id);
   if (it == s->resident.end()) return MLF_OK;
   const size_t b = it->second.bytes;
-  MLF_CHECK(mlf_patch_destroy(it->second.p));
+  const mlf_status st = mlf_patch_destroy(it->second.p);
   s->resident.erase(it);
   std::lock_guard<std::mutex> lk(s->mu);
   s->resident_bytes -= b;
+  --s->resident_count;
   ++s->evictions;
-  return MLF_OK;
+  return st;
 }
 
 mlf_status admit_one(mlf_server_s* s, const std::shared_ptr<Request>& r) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->fault_admit_patch == r->patch_id) {
+      s->fault_admit_patch = -1;
+      return set_error(MLF_ERR_DEVICE, "injected device failure at admission of patch %d",
+                       (int)r->patch_id);
+    }
+  }
   auto it = s->resident.find(r->patch_id);
   bool cold = false;
   if (it == s->resident.end()) {
@@ -179,6 +195,7 @@ mlf_status admit_one(mlf_server_s* s, const std::shared_ptr<Request>& r) {
     std::lock_guard<std::mutex> lk(s->mu);
     s->resident_bytes += res.bytes;
     s->peak_resident_bytes = std::max(s->peak_resident_bytes, s->resident_bytes);
+    ++s->resident_count;
     ++s->cold_starts;
   } else if (it->second.n1 != r->n1 || it->second.n2 != r->n2) {
     return set_error(MLF_ERR_INVALID_ARGUMENT, "patch %d was %dx%d, request says %dx%d",
@@ -209,7 +226,7 @@ void worker_main(mlf_server_s* s) {
         std::deque<std::shared_ptr<Request>> dropped;
         dropped.swap(s->pending);
         lk.unlock();
-        for (const auto& r : dropped) finish(s, r, MLF_ERR_INVALID_ARGUMENT, "cancelled: server destroyed");
+        for (const auto& r : dropped) finish(s, r, MLF_ERR_CANCELLED, "cancelled: server destroyed");
         lk.lock();
       }
       if (s->stop && s->active.empty()) return;
@@ -226,15 +243,25 @@ void worker_main(mlf_server_s* s) {
     }
     for (int32_t id : evict) (void)evict_patch(s, id);
     for (const auto& r : admit) {
-      if (admit_one(s, r) != MLF_OK) {
+      const mlf_status as = admit_one(s, r);
+      if (as != MLF_OK) {
         const std::string e = mlf_last_error();
         auto it = s->resident.find(r->patch_id);
         if (it != s->resident.end() && !it->second.busy) (void)evict_patch(s, r->patch_id);
-        finish(s, r, MLF_ERR_DEVICE, e.c_str());
+        finish(s, r, as, e.c_str());
       }
     }
     if (s->active.empty()) continue;
-    mlf_status st = mlf_batch_run(s->batch, 1, s->cfg.filter);
+    bool injected = false;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->fault_runs > 0) {
+        --s->fault_runs;
+        injected = true;
+      }
+    }
+    mlf_status st = injected ? set_error(MLF_ERR_DEVICE, "injected device failure in a batch iteration")
+                             : mlf_batch_run(s->batch, 1, s->cfg.filter);
     if (st == MLF_OK) st = mlf_context_synchronize(s->psf->ctx);
     const std::string err = st == MLF_OK ? std::string() : mlf_last_error();
     {
@@ -243,6 +270,7 @@ void worker_main(mlf_server_s* s) {
       s->slot_sum += (double)s->active.size();
     }
     std::vector<Active> keep;
+    std::vector<int32_t> failed;  // patches whose volume is undefined after a failed iteration
     for (Active& a : s->active) {
       ++a.iters;
       if (st == MLF_OK && a.iters < s->cfg.iterations) {
@@ -262,9 +290,11 @@ void worker_main(mlf_server_s* s) {
       }
       res.busy = false;
       res.last_used = ++s->clock;
+      if (rs != MLF_OK) failed.push_back(a.patch_id);
       finish(s, a.req, rs, rs == MLF_OK ? nullptr : re.c_str());
     }
     s->active.swap(keep);
+    for (int32_t id : failed) (void)evict_patch(s, id);
   }
 }
 
@@ -284,7 +314,11 @@ mlf_status mlf_server_create(mlf_psf psf, mlf_precision prec, const mlf_server_c
   s->prec = prec;
   s->cfg = *cfg;
   MLF_CHECK(mlf_batch_create(psf, prec, cfg->max_slots, &s->batch));
-  MLF_CHECK(mlf_batch_set_graph(s->batch, cfg->use_graph));
+  if (mlf_batch_set_graph(s->batch, cfg->use_graph) != MLF_OK) {
+    const std::string e = mlf_last_error();
+    (void)mlf_batch_destroy(s->batch);
+    return set_error(MLF_ERR_INTERNAL, "%s", e.c_str());
+  }
   s->worker = std::thread(worker_main, s.get());
   *out = s.release();
   return MLF_OK;
@@ -302,6 +336,11 @@ mlf_status mlf_server_submit(mlf_server s, int32_t patch_id, int64_t n1, int64_t
   r->n2 = (int)n2;
   r->image.assign(lfimg, lfimg + (size_t)n1 * n2 * kP);
   r->out = volume_out;
+  const size_t need = patch_bytes_estimate(s->psf, s->prec, r->n1, r->n2);
+  if (need > s->cfg.hbm_budget_bytes)
+    return set_error(MLF_ERR_BUDGET, "patch %d (%lld x %lld lenslets, %zu bytes) exceeds the HBM budget of %llu bytes",
+                     (int)patch_id, (long long)n1, (long long)n2, need,
+                     (unsigned long long)s->cfg.hbm_budget_bytes);
   std::lock_guard<std::mutex> lk(s->mu);
   if (s->stop) return set_error(MLF_ERR_INVALID_ARGUMENT, "server is shutting down");
   r->ticket = s->next_ticket++;
@@ -323,7 +362,10 @@ mlf_status mlf_server_wait(mlf_server s, uint64_t ticket, double* latency_ms) {
   auto it = s->tickets.find(ticket);
   if (it == s->tickets.end()) return set_error(MLF_ERR_INVALID_ARGUMENT, "unknown ticket");
   std::shared_ptr<Request> r = it->second;
+  ++s->waiters;
   s->cv_done.wait(lk, [&] { return r->done; });
+  --s->waiters;
+  s->cv_done.notify_all();
   s->tickets.erase(ticket);
   if (latency_ms) *latency_ms = r->t_done - r->t_submit;
   if (r->status != MLF_OK) return set_error(r->status, "request failed: %s", r->error.c_str());
@@ -334,7 +376,10 @@ mlf_status mlf_server_drain(mlf_server s) {
   clear_error();
   if (!s) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
   std::unique_lock<std::mutex> lk(s->mu);
+  ++s->waiters;
   s->cv_done.wait(lk, [&] { return s->in_flight == 0; });
+  --s->waiters;
+  s->cv_done.notify_all();
   return MLF_OK;
 }
 
@@ -347,7 +392,7 @@ mlf_status mlf_server_get_stats(mlf_server s, mlf_server_stats* st) {
   st->completed = s->completed;
   st->pending = (int64_t)s->pending.size();
   st->in_flight = s->in_flight;
-  st->resident_patches = (int64_t)s->resident.size();
+  st->resident_patches = s->resident_count;
   st->evictions = s->evictions;
   st->cold_starts = s->cold_starts;
   st->iterations_run = s->iterations_run;
@@ -372,6 +417,20 @@ mlf_status mlf_server_get_stats(mlf_server s, mlf_server_stats* st) {
   return MLF_OK;
 }
 
+mlf_status mlf_server_inject_fault(mlf_server s, int32_t kind, int32_t patch_id) {
+  clear_error();
+  if (!s) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (kind == MLF_FAULT_ADMIT) {
+    s->fault_admit_patch = patch_id;
+  } else if (kind == MLF_FAULT_ITERATION) {
+    ++s->fault_runs;
+  } else {
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "unknown fault kind %d", (int)kind);
+  }
+  return MLF_OK;
+}
+
 mlf_status mlf_server_destroy(mlf_server s) {
   clear_error();
   if (!s) return MLF_OK;
@@ -381,6 +440,11 @@ mlf_status mlf_server_destroy(mlf_server s) {
   }
   s->cv_work.notify_all();
   if (s->worker.joinable()) s->worker.join();
+  {
+    // every request has finished; let threads still inside mlf_server_wait return
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->cv_done.wait(lk, [&] { return s->waiters == 0; });
+  }
   (void)ctx_activate(s->psf->ctx);
   for (auto& kv : s->resident) (void)mlf_patch_destroy(kv.second.p);
   s->resident.clear();

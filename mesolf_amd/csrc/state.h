@@ -167,7 +167,7 @@ mlf_status patch_setup_tiles(mlf_patch_s* p);
 std::vector<std::pair<int, int>> tile_candidates(Precision kp);
 // Tile shape cache (per process, optionally persisted in the file named by
 // MLF_TUNE_CACHE): key = precision, grid(s), PSF radii.
-bool tune_cache_get(const std::string& key, int* mt, int* nt);
+bool tune_cache_get(const std::string& key, Precision kp, int* mt, int* nt);
 void tune_cache_put(const std::string& key, int mt, int nt, double ms);
 std::string tune_key(const mlf_patch_s* p, const char* scope);
 }  // namespace mlf

@@ -2,8 +2,12 @@
 //
 // The cache maps a key (scope, precision, lenslet grid(s), PSF radii) to the
 // fastest measured tile shape.  It lives for the process; if MLF_TUNE_CACHE
-// names a file, entries are loaded from it on first use and appended to it
-// when measured, one line per entry: "<key> <fwd_mt> <bwd_nt> <ms per view>".
+// names a file, entries are loaded from it when the name is first seen and
+// appended to it when measured, one line per entry:
+//   "mlf-tune-v2 <key> <fwd_mt> <bwd_nt> <ms per view>".
+// Lines without the format tag (older builds) are ignored, and a cached shape
+// is used only if it is one of the shapes this build has for the patch's
+// precision: a stale or edited entry falls back to the default shape.
 #include <cstdio>
 #include <cstdlib>
 #include <fstream>
@@ -21,23 +25,27 @@
 namespace mlf {
 
 namespace {
+constexpr const char* kTuneTag = "mlf-tune-v2";
 std::mutex g_tune_mu;
 std::map<std::string, std::pair<int, int>> g_tune;
-bool g_tune_loaded = false;
+std::string g_tune_loaded_path;  // cache file whose entries are loaded
 
 void load_locked() {
-  if (g_tune_loaded) return;
-  g_tune_loaded = true;
   const char* path = std::getenv("MLF_TUNE_CACHE");
-  if (!path || !*path) return;
+  if (!path || !*path || g_tune_loaded_path == path) return;
+  g_tune_loaded_path = path;
   std::ifstream f(path);
   std::string line;
   while (std::getline(f, line)) {
     std::istringstream is(line);
-    std::string key;
+    std::string tag, key;
     int mt = 0, nt = 0;
-    if (is >> key >> mt >> nt) g_tune[key] = {mt, nt};
+    if (is >> tag >> key >> mt >> nt && tag == kTuneTag) g_tune[key] = {mt, nt};
   }
+}
+
+const char* precision_name(Precision kp) {
+  return kp == Precision::kF32 ? "fp32" : kp == Precision::kF16 ? "fp16" : "bf16";
 }
 }  // namespace
 
@@ -51,20 +59,28 @@ std::vector<std::pair<int, int>> tile_candidates(Precision kp) {
 
 std::string tune_key(const mlf_patch_s* p, const char* scope) {
   std::ostringstream os;
-  os << scope << "|" << (p->kp == Precision::kF32 ? "fp32" : "bf16") << "|" << p->n1 << "x" << p->n2
+  os << scope << "|" << precision_name(p->kp) << "|" << p->n1 << "x" << p->n2
      << "|rf";
   for (int r : p->psf->rf) os << "," << r;
   return os.str();
 }
 
-bool tune_cache_get(const std::string& key, int* mt, int* nt) {
-  std::lock_guard<std::mutex> lk(g_tune_mu);
-  load_locked();
-  auto it = g_tune.find(key);
-  if (it == g_tune.end()) return false;
-  *mt = it->second.first;
-  *nt = it->second.second;
-  return true;
+bool tune_cache_get(const std::string& key, Precision kp, int* mt, int* nt) {
+  std::pair<int, int> v;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    load_locked();
+    auto it = g_tune.find(key);
+    if (it == g_tune.end()) return false;
+    v = it->second;
+  }
+  for (const auto& c : tile_candidates(kp))
+    if (c == v) {
+      *mt = v.first;
+      *nt = v.second;
+      return true;
+    }
+  return false;  // not a shape of this build: keep the default
 }
 
 void tune_cache_put(const std::string& key, int mt, int nt, double ms) {
@@ -74,7 +90,7 @@ void tune_cache_put(const std::string& key, int mt, int nt, double ms) {
   const char* path = std::getenv("MLF_TUNE_CACHE");
   if (!path || !*path) return;
   std::ofstream f(path, std::ios::app);
-  f << key << " " << mt << " " << nt << " " << ms << "\n";
+  f << kTuneTag << " " << key << " " << mt << " " << nt << " " << ms << "\n";
 }
 
 }  // namespace mlf

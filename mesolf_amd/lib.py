@@ -33,6 +33,11 @@ MLF_ERR_DEVICE = 3
 MLF_ERR_OUT_OF_MEMORY = 4
 MLF_ERR_BUDGET = 5
 MLF_ERR_INTERNAL = 6
+MLF_ERR_CANCELLED = 7
+
+# host-forced server failures (mlf_server_inject_fault)
+MLF_FAULT_ADMIT = 1
+MLF_FAULT_ITERATION = 2
 
 PRECISIONS = {"fp32": 0, "bf16": 1, "fp16": 2}
 
@@ -60,6 +65,10 @@ class Unsupported(MesolfError):
 
 class BudgetExceeded(MesolfError):
     pass
+
+
+class Cancelled(MesolfError):
+    """A queued server request dropped before it ran (server destroyed)."""
 
 
 _lock = threading.Lock()
@@ -121,6 +130,7 @@ _SIGNATURES = {
     "mlf_server_drain": (C.c_int, [_P]),
     "mlf_server_get_stats": (C.c_int, [_P, _P]),
     "mlf_server_destroy": (C.c_int, [_P]),
+    "mlf_server_inject_fault": (C.c_int, [_P, _I32, _I32]),
     "mlf_recon_phase_space": (C.c_int, [_F32P, _I64, _I64, _F32P, _F32P, _I64, _I64, _I64, _I32,
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
@@ -178,6 +188,8 @@ def _check(status: int):
         raise Unsupported(status, msg)
     if status == MLF_ERR_BUDGET:
         raise BudgetExceeded(status, msg)
+    if status == MLF_ERR_CANCELLED:
+        raise Cancelled(status, msg)
     raise MesolfError(status, msg)
 
 
@@ -542,6 +554,13 @@ class Server:
 
     def drain(self):
         _check(self._lib.mlf_server_drain(self.handle))
+
+    def inject_fault(self, kind: str, patch_id: int = -1):
+        """Force a failure from the host (failure-path tests): kind "admit"
+        fails the admission of patch_id's next request, "iteration" fails the
+        next batched iteration (every request then in flight)."""
+        k = {"admit": MLF_FAULT_ADMIT, "iteration": MLF_FAULT_ITERATION}[kind]
+        _check(self._lib.mlf_server_inject_fault(self.handle, k, int(patch_id)))
 
     def stats(self):
         st = _ServerStats()

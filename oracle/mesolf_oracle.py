@@ -32,6 +32,8 @@ forward/backward_compact    literal forms (tiny), brute-force loops
 rl_literal / rl_compact     each other at tiny sizes
 mex_op                      brute-force loop over the kernel's own clipping
                             rules; adjoint of forward_scatter (odd kw)
+mex_op_sparse               mex_op on sparse bitmaps (odd kw, kernel widths
+                            up to the bitmap's and beyond)
 post_recon / im2uint16      a 2x2x3 stack and its outputs computed by hand;
                             NaN planes; crop indices worked out for Z = 51, 4
 movmean                     MATLAB's documented movmean(A, 3) example; brute
@@ -663,6 +665,28 @@ def mex_op(bitmap: np.ndarray, K: np.ndarray, nphase: int = 15) -> np.ndarray:
             src = bd[xi[:, None], yi[None, :], :]
             out += (m * kv)[:, :, None] * src
     return out[:, :, 0] if squeeze else out
+
+
+def mex_op_sparse(W: int, Hh: int, xs, ys, vals, K: np.ndarray, nphase: int = 15) -> np.ndarray:
+    """mex_op of one [W, Hh] bitmap given by its non-zero pixels (xs, ys, vals),
+    odd kw only: mex:55-78 then clips exactly the taps whose input pixel
+    falls outside the bitmap, so each non-zero pixel (p, q) adds
+    ``v * K[p - x + h, q - y + h, x % 15, y % 15]`` to every output (x, y)
+    within h of it.  Cost ~ nnz * kw^2 instead of W * Hh * kw^2."""
+    kw = K.shape[0]
+    if kw % 2 == 0:
+        raise ValueError("mex_op_sparse covers odd kernel widths only")
+    h = kw // 2
+    out = np.zeros((W, Hh))
+    Kd = K.astype(np.float64)
+    for p, q, v in zip(np.asarray(xs).tolist(), np.asarray(ys).tolist(), np.asarray(vals, np.float64).tolist()):
+        x = np.arange(max(0, p - h), min(W, p + h + 1))
+        y = np.arange(max(0, q - h), min(Hh, q + h + 1))
+        kx = p - x + h
+        ky = q - y + h
+        out[x[0]:x[-1] + 1, y[0]:y[-1] + 1] += v * Kd[kx[:, None], ky[None, :],
+                                                      (x % nphase)[:, None], (y % nphase)[None, :]]
+    return out
 
 
 def forward_scatter(X: np.ndarray, K: np.ndarray, nphase: int = 15) -> np.ndarray:

@@ -209,3 +209,94 @@ def test_server_budget_evicts_idle_patches(psf):
     assert st["cold_starts"] == 3 and st["evictions"] == 2 and st["resident_patches"] == 1
     assert st["peak_resident_bytes"] <= budget
     srv.close()
+
+
+def test_server_refuses_request_over_budget_and_keeps_serving(psf):
+    """A patch whose state alone exceeds the HBM budget is refused at
+    submission (MLF_ERR_BUDGET); requests that fit still complete and equal
+    their single-patch runs."""
+    prec = "bf16"
+    g = (20, 20)
+    imgs = [_image(*g, seed=500 + k) for k in range(2)]
+    probe = mesolf_amd.Server(psf, precision=prec, max_slots=1, iterations=1)
+    probe.wait(probe.submit(0, imgs[0]))
+    one = probe.stats()["resident_bytes"]
+    probe.close()
+    srv = mesolf_amd.Server(psf, precision=prec, max_slots=2, iterations=2, hbm_budget_bytes=int(2.5 * one))
+    t0 = srv.submit(0, imgs[0], want_volume=True)
+    with pytest.raises(mesolf_amd.BudgetExceeded):
+        srv.submit(9, _image(60, 60, seed=9))     # 9x the pixels of the budgeted patches
+    t1 = srv.submit(1, imgs[1], want_volume=True)
+    for k, t in ((0, t0), (1, t1)):
+        _, vol = srv.wait(t)
+        assert np.array_equal(vol, _alone(psf, g, imgs[k], 2, prec))
+    st = srv.stats()
+    assert st["submitted"] == 2 and st["completed"] == 2
+    srv.close()
+
+
+def test_server_injected_faults_fail_one_request_and_keep_serving(psf):
+    """Host-forced failures: a failed admission fails that request alone; a
+    failed batched iteration fails the requests then in flight and drops their
+    patch state; later requests run (cold) and match their single-patch runs."""
+    prec = "bf16"
+    frames = {i: [_image(*g, seed=600 + 10 * i + f) for f in range(3)] for i, g in enumerate(GRIDS)}
+    srv = mesolf_amd.Server(psf, precision=prec, max_slots=3, iterations=2)
+    # 1) admission of patch 1 fails; patches 0 and 2 complete
+    srv.inject_fault("admit", 1)
+    ts = [srv.submit(i, frames[i][0], want_volume=True) for i in range(3)]
+    with pytest.raises(mesolf_amd.MesolfError) as ei:
+        srv.wait(ts[1])
+    assert ei.value.status == mesolf_amd.lib.MLF_ERR_DEVICE and "injected" in ei.value.message
+    v0 = srv.wait(ts[0])[1]
+    v2 = srv.wait(ts[2])[1]
+    assert np.array_equal(v0, _alone(psf, GRIDS[0], frames[0][0], 2, prec))
+    assert np.array_equal(v2, _alone(psf, GRIDS[2], frames[2][0], 2, prec))
+    # 2) the next batched iteration fails: the request in flight fails ...
+    srv.inject_fault("iteration")
+    t = srv.submit(0, frames[0][1], want_volume=True)
+    with pytest.raises(mesolf_amd.MesolfError) as ei:
+        srv.wait(t)
+    assert ei.value.status == mesolf_amd.lib.MLF_ERR_DEVICE
+    # ... and patch 0's state was dropped: its next frame starts cold
+    before = srv.stats()["cold_starts"]
+    t = srv.submit(0, frames[0][2], want_volume=True)
+    _, v = srv.wait(t)
+    assert srv.stats()["cold_starts"] == before + 1
+    assert np.array_equal(v, _alone(psf, GRIDS[0], frames[0][2], 2, prec))
+    # patch 2 kept its warm state through patch 0's failure
+    t = srv.submit(2, frames[2][1], want_volume=True)
+    _, v = srv.wait(t)
+    assert np.array_equal(v, _alone(psf, GRIDS[2], frames[2][1], 2, prec, warm=v2))
+    assert srv._lib.mlf_server_inject_fault(srv.handle, 99, 0) == mesolf_amd.lib.MLF_ERR_INVALID_ARGUMENT
+    srv.close()
+
+
+def test_server_destroy_cancels_queued_requests(psf):
+    """Requests still queued when the server is destroyed end with the
+    cancellation status, not an argument error."""
+    prec = "bf16"
+    g = (40, 40)
+    srv = mesolf_amd.Server(psf, precision=prec, max_slots=1, iterations=30)
+    ts = [srv.submit(k, _image(*g, seed=700 + k)) for k in range(4)]
+    lib = srv._lib
+    import ctypes as C
+    import threading
+    results = {}
+
+    def waiter(t):
+        ms = C.c_double()
+        results[t] = lib.mlf_server_wait(srv.handle, C.c_uint64(t), C.byref(ms))
+
+    ths = [threading.Thread(target=waiter, args=(t,)) for t in ts]
+    for th in ths:
+        th.start()
+    h = srv.handle
+    srv.handle = None
+    assert lib.mlf_server_destroy(h) == 0
+    for th in ths:
+        th.join()
+    # the request in flight completes; those behind it (one slot, 30
+    # iterations each) were still queued and are cancelled
+    assert results[ts[-1]] == mesolf_amd.lib.MLF_ERR_CANCELLED
+    assert set(results.values()) <= {0, mesolf_amd.lib.MLF_ERR_CANCELLED}

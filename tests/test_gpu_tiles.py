@@ -116,3 +116,28 @@ def test_tuned_batch_matches_alone(problem):
     b.close()
     for p in ps:
         p.close()
+
+
+def test_tune_cache_file_entries_are_validated(problem, tmp_path, monkeypatch):
+    """Cached shapes are checked against the shapes this build has: a bogus
+    entry (a shape never built) or an untagged line leaves the default; a
+    valid tagged entry is applied; FP16 and BF16 have separate keys."""
+    psf, _, _ = problem
+    rf = ",".join(str(int(r)) for r in psf.radii()[0])
+    path = tmp_path / "tune.txt"
+    path.write_text(
+        f"mlf-tune-v2 patch|bf16|{WL.n1}x{WL.n2}|rf,{rf} 3 7 0.1\n"     # not a built shape
+        f"patch|bf16|16x30|rf,{rf} 2 3 0.1\n"                            # untagged (older format)
+        f"mlf-tune-v2 patch|fp16|{WL.n1}x{WL.n2}|rf,{rf} 2 3 0.1\n"      # valid, fp16 key only
+        f"mlf-tune-v2 patch|fp32|16x30|rf,{rf} 1 3 0.1\n")               # valid fp32 entry
+    monkeypatch.setenv("MLF_TUNE_CACHE", str(path))
+    shapes = {}
+    for prec, grid in (("bf16", (WL.n1, WL.n2)), ("fp16", (WL.n1, WL.n2)), ("bf16", (16, 30)),
+                       ("fp32", (16, 30))):
+        p = mesolf_amd.Patch(psf, *grid, precision=prec)
+        shapes[(prec, grid)] = p.tile_shape()
+        p.close()
+    assert shapes[("bf16", (WL.n1, WL.n2))] == (4, 5)   # bogus entry ignored
+    assert shapes[("fp16", (WL.n1, WL.n2))] == (2, 3)   # its own key
+    assert shapes[("bf16", (16, 30))] == (4, 5)         # untagged line ignored
+    assert shapes[("fp32", (16, 30))] == (1, 3)

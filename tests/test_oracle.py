@@ -114,6 +114,46 @@ def test_weights_degenerate_all_equal_is_nan():
     assert np.all(np.isnan(w))          # NaN views stay "active" (NaN == 0 false)
 
 
+def test_normalize_depth_by_hand():
+    """recon:13-16 divides each depth slab by its own total, not each view:
+    two slabs whose views have unequal sums, worked out by hand."""
+    H = np.zeros((1, 1, 2, 1, 2))
+    H[0, 0, :, 0, 0] = [1.0, 3.0]     # slab 0: total 4
+    H[0, 0, :, 0, 1] = [2.0, 2.0]     # slab 1: total 4 -> but the views differ from slab 0
+    H[0, 0, :, 0, 1] *= 5.0           # slab 1: total 20
+    got = O.normalize_depth(H)
+    assert np.array_equal(got[0, 0, :, 0, 0], [0.25, 0.75])
+    assert np.array_equal(got[0, 0, :, 0, 1], [0.5, 0.5])
+    # per-view sums over depth after the normalisation: 0.75 and 1.25; the
+    # weights of recon:30-38 from them: /2 -> .375, .625; -min -> 0, .25;
+    # /max -> 0, 1; cut; *0.3
+    w = O.view_weights_from_sums(got.sum(axis=(0, 1, 4)))
+    assert np.allclose(w[:, 0], [0.0, 0.3])
+    # a 3-D slab: every entry of slab z divided by the same total
+    rng = np.random.default_rng(0)
+    H3 = rng.random((4, 4, 3, 3, 2))
+    n3 = O.normalize_depth(H3)
+    for z in range(2):
+        tot = sum(float(x) for x in H3[:, :, :, :, z].ravel())
+        assert np.allclose(n3[:, :, :, :, z] * tot, H3[:, :, :, :, z], rtol=1e-14)
+        # per-view sums stay unequal (a per-view normalisation would make them equal)
+        assert np.ptp(n3[:, :, :, :, z].sum(axis=(0, 1))) > 1e-3
+
+
+def test_weights_of_production_psf_cut_and_vary():
+    """A raw PSF whose in-aperture views have unequal sums: one in-disc view
+    falls under the 0.2 cut and the active weights differ (recon:30-38)."""
+    from tests.helpers import random_production_psf
+    H, Ht, info = random_production_psf(np.random.default_rng(3))
+    cp = O.compact_from_dense(H, Ht)
+    assert info["cut_view"] not in cp.views
+    assert len(cp.views) == 176
+    w = np.array(cp.weights)
+    assert w.max() == pytest.approx(0.3) and w.min() >= 0.3 * 0.2 and np.ptp(w) > 0.1
+    assert cp.rf == info["rf"] and cp.rb == info["rb"]
+    assert any(b == f + 1 for f, b in zip(cp.rf, cp.rb))
+
+
 # ------------------------------------------------------------------- conv2
 def test_conv2_same_bruteforce():
     rng = np.random.default_rng(3)
@@ -218,6 +258,46 @@ def test_omega_array_cfg2():
     assert om.min() > 6.4 and om.max() <= 8.0
 
 
+@pytest.mark.parametrize("Z,size,first_axis", [(11, 12, -5), (51, 54, -26), (101, 102, -50)])
+def test_omega_array_closed_form(Z, size, first_axis):
+    """recon:21-26,54-56: omega_k = (N-7) exp(-((k - 3R/2)/100 * 2 sqrt(ln 2))^2)
+    = 8 * 2^(-a^2 / 2500) with a = k - 3R/2 (exp(-4 ln2 a^2/1e4) = 2^(-a^2/2500)),
+    floored at 0.2.  Exact values at named indices; a dropped factor 2 in
+    the exponent gives 8 * 2^(-a^2/10000) and fails."""
+    om = O.omega_array(Z, 15)
+    assert om.size == size
+    a = np.arange(size) + first_axis
+    ref = 8.0 * np.power(2.0, -(a.astype(np.float64) ** 2) / 2500.0)
+    assert np.allclose(om, np.maximum(ref, 0.2), rtol=1e-14, atol=0)
+    k0 = int(np.where(a == 0)[0][0])
+    assert om[k0] == 8.0
+    if Z == 11:   # 3R = 12: a runs -5 .. 6
+        assert om[0] == pytest.approx(8.0 * 2.0 ** -0.01, rel=1e-15)
+        assert om[-1] == pytest.approx(8.0 * 2.0 ** -(36 / 2500), rel=1e-15)
+    if Z == 51:   # R = ceil(51/3) = 17 -> even 18, 3R = 54: a runs -26 .. 27
+        assert om[0] == pytest.approx(8.0 * 2.0 ** -(676 / 2500), rel=1e-15)
+        assert om[-1] == pytest.approx(6.53596, abs=1e-5)    # 8 * 2^-0.2916
+    # a fifth of the Gaussian's half-maximum width: omega halves at |a| = 50
+    if Z == 101:
+        assert om[list(a).index(50)] == pytest.approx(4.0, rel=1e-15)
+        assert om[list(a).index(-50)] == pytest.approx(4.0, rel=1e-15)
+
+
+def test_filter_bank_is_analytic_gaussian_transform():
+    """recon:57-60: |fftshift(fft2(exp(-(x/w)^2 - (y/w)^2)))| / max is the
+    sampled transform of a Gaussian, exp(-(pi w f)^2) per axis (f in cycles
+    per pixel), when the Gaussian is narrow against the image and the
+    frequency grid covers its support."""
+    I = 128
+    bank = O.filter_bank(I, I, 11, 15)
+    om = O.omega_array(11, 15)
+    f = (np.arange(I) - I // 2) / I          # fftshift order
+    for k in (0, 5, 11):
+        g = np.exp(-(np.pi * om[k] * f) ** 2)
+        ref = np.outer(g, g)
+        assert np.allclose(bank[:, :, k], ref / ref.max(), atol=1e-9)
+
+
 # ------------------------------------------------------------ update rules
 def test_update_special_values():
     X = np.array([1.0, 2.0, 0.0, 3.0, 4.0, 5.0])
@@ -244,6 +324,23 @@ def test_rl_compact_equals_literal_small():
     cp = O.compact_from_dense(H, Ht)
     b = O.rl_compact(LF, cp, 1, views_limit=6)
     assert np.linalg.norm(a - b) <= 1e-13 * np.linalg.norm(a)
+
+
+@pytest.mark.slow
+def test_rl_compact_equals_literal_production_psf():
+    """Same with a random raw H and an independent random Ht (not the
+    adjoint; rb = rf + 1 in two planes) and unequal view weights: the compact
+    backward tables read Ht at the view's own slot (recon:121,123) exactly
+    as the full-resolution conv2 does."""
+    from tests.helpers import random_production_psf
+    rng = np.random.default_rng(9)
+    H, Ht, info = random_production_psf(rng)
+    LF = rng.poisson(50.0, (45, 60)).astype(np.float64)
+    a = O.rl_literal(LF, H, Ht, 1, views_limit=8)
+    cp = O.compact_from_dense(H, Ht)
+    assert cp.rb == info["rb"]
+    b = O.rl_compact(LF, cp, 1, views_limit=8)
+    assert np.linalg.norm(a - b) <= 1e-12 * np.linalg.norm(a)
 
 
 # ------------------------------------------------------------------- MEX op
@@ -293,6 +390,36 @@ def test_mex_op_batch_axis():
     out = O.mex_op(b, K)
     for n in range(3):
         assert np.allclose(out[:, :, n], O.mex_op(b[:, :, n], K))
+
+
+@pytest.mark.parametrize("W,Hh,kw", [(31, 40, 7), (20, 17, 31), (45, 33, 15)])
+def test_mex_op_sparse_equals_dense(W, Hh, kw):
+    """The scatter form over non-zero pixels equals the gather form
+    (including kernels wider than the bitmap, where both clip at every edge)."""
+    rng = np.random.default_rng(W * Hh + kw)
+    b = np.zeros((W, Hh))
+    xs = rng.integers(0, W, 25)
+    ys = rng.integers(0, Hh, 25)
+    b[xs, ys] = rng.random(25)
+    xs, ys = np.nonzero(b)
+    K = rng.random((kw, kw, 15, 15))
+    got = O.mex_op_sparse(W, Hh, xs, ys, b[xs, ys], K)
+    assert np.allclose(got, O.mex_op(b, K), rtol=1e-13, atol=1e-13)
+    with pytest.raises(ValueError):
+        O.mex_op_sparse(W, Hh, xs, ys, b[xs, ys], K[:-1, :-1])
+
+
+def test_vols_slice_fixture_matches_the_reference_file_summary():
+    """The decoded reference MEX input (src/cuda/vols_slice.mat): 376 x 376 x
+    77 single, 72,050 non-zeros, max 1.0858954, sum 3570.611 -- the figures
+    read from the file when the reference was surveyed."""
+    from tests.golden import load_vols_slice
+    a = load_vols_slice()
+    assert a.shape == (376, 376, 77) and a.dtype == np.float32
+    assert np.count_nonzero(a) == 72050
+    assert float(a.max()) == pytest.approx(1.0858954, abs=1e-7)
+    assert float(a.astype(np.float64).sum()) == pytest.approx(3570.611, abs=1e-3)
+    assert a.min() >= 0
 
 
 # --------------------------------------------------- post-reconstruction step
