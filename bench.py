@@ -9,8 +9,11 @@ steps + sample_filter.  Inputs (PSF tables, image) are resident in HBM
 before the timed region starts.
 
 Prints one JSON line.  ``roofline`` describes the dominant kernel, timed
-live with HIP events on the launch stream; ``cpu_baseline`` is the FP64
-numpy oracle timed on a bounded sample of the same workload on this host.
+live with HIP events on the launch stream, priced with the algorithmic bytes
+and flops of the path (3 volume passes of 2 B per voxel and view step for the
+16-bit paths, 4 B for FP32; 2 n1 n2 sum_z K_z^2 flops per projection);
+``cpu_baseline`` is the oracle's C/OpenMP FP32 restatement timed on a bounded
+sample of the same workload on this host's cores.
 """
 from __future__ import annotations
 
@@ -36,13 +39,20 @@ def algorithmic_flops_per_view(wl) -> float:
 
 
 def algorithmic_bytes_per_view(wl, precision: str, kernel: str) -> float:
-    """HBM bytes one view step of `kernel` must move at least.
+    """Algorithmic HBM bytes of one launch: a view step moves the volume three
+    times at the operand width (2 B per voxel for BF16/FP16, 4 B for FP32):
+    the forward reads it once, the backprojection + update reads and writes it
+    (2 of the 3 passes)."""
+    vox = wl.n1 * wl.n2 * wl.Z * 225
+    esz = 4 if precision == "fp32" else 2
+    return float((2 if kernel == "backward_update" else 1) * esz * vox)
 
-    backward_update: read X and write X (FP32, 225 phases per lenslet), write
-    the 16-bit shadow (225 phases; not kept for FP32), read the view's taps
-    (225 (2r+1)^2 per plane), its 1/B1 table (225 (2r+1)^2 FP32 per plane)
-    and the ratio image.  forward: read the volume (16-bit shadow, or FP32 X)
-    and the view's taps."""
+
+def design_bytes_per_view(wl, precision: str, kernel: str) -> float:
+    """Bytes this design's launch must move: the backprojection + update reads
+    and writes the FP32 master volume (the update stays FP32), writes the
+    16-bit shadow the next forward reads, and reads the view's taps, its 1/B1
+    table and the ratio image; the forward reads the shadow (or X) and taps."""
     vox = wl.n1 * wl.n2 * wl.Z * 225
     esz = 4 if precision == "fp32" else 2
     taps = 225 * sum((2 * r + 1) ** 2 for r in wl.radii)
@@ -58,11 +68,16 @@ def roofline_for(wl, precision: str, kernel: str, avg_s: float) -> dict:
     balance), and `frac` is that time over the measured launch time."""
     flops = algorithmic_flops_per_view(wl)
     nbytes = algorithmic_bytes_per_view(wl, precision, kernel)
+    dbytes = design_bytes_per_view(wl, precision, kernel)
     t_mfma, t_hbm = flops / PEAK[precision], nbytes / HBM_PEAK
     common = {"kernel": kernel, "avg_launch_ms": avg_s * 1e3, "traffic": None,
               "algorithmic_per_launch": f"{flops:.4e} flop (2*n1*n2*sum_z(30r_z+1)^2), "
-                                        f"{nbytes:.4e} B",
-              "time_at_peak_ms": {"mfma": t_mfma * 1e3, "hbm": t_hbm * 1e3}}
+                                        f"{nbytes:.4e} B ({2 if kernel == 'backward_update' else 1} "
+                                        f"volume pass(es) x {4 if precision == 'fp32' else 2} B/voxel)",
+              "time_at_peak_ms": {"mfma": t_mfma * 1e3, "hbm": t_hbm * 1e3},
+              "mfma_TFLOPs": flops / avg_s / 1e12, "mfma_frac": flops / avg_s / PEAK[precision],
+              "design_bytes_per_launch": dbytes,
+              "design_GBs": dbytes / avg_s / 1e9, "design_frac": dbytes / avg_s / HBM_PEAK}
     if t_mfma >= t_hbm:
         return {"bound": "mfma", "achieved": flops / avg_s / 1e12, "peak": PEAK[precision] / 1e12,
                 "unit": "TFLOP/s", "frac": t_mfma / avg_s, **common}
@@ -75,50 +90,110 @@ def log(msg):
 
 
 def cpu_baseline(wl, tabs, LF, nviews_sample: int):
-    """FP64 numpy oracle (BLAS threads = host cores) on `nviews_sample` view
-    steps of the workload plus one filtered plane, extrapolated to a volume."""
+    """The oracle's FP32 restatements (oracle/cpu_baseline.py) on this host's
+    cores, on bounded samples -- the strided form on numpy's BLAS (sgemm),
+    the literal full-resolution form in C/OpenMP (oracle/cpu_rl.c):
+
+    * the workload itself: `nviews_sample` strided view steps and one filtered
+      plane at its shape, extrapolated to a volume (views x iterations);
+    * 33 x 33 lenslets x 11 planes: one whole strided iteration (177 view
+      steps + sample_filter) timed, x 8 iterations;
+    * 33 x 33 x 11, the reference CPU branch as written: one view step of
+      full-resolution conv2(..., 'same') (recon:191-208) timed, x 177 views
+      x 8 iterations (+ the strided iteration's filter time)."""
+    from mesolf_amd import synth, tables as T
+    from oracle import cpu_baseline as CB
     from oracle import mesolf_oracle as O
     from tests.helpers import oracle_compact_from_abi
 
-    sub = dict(tabs)
+    cores = CB.threads()
     per_view = sum((2 * r + 1) ** 2 * 225 for r in wl.radii)
+    sub = dict(tabs)
     sub["uv"] = tabs["uv"][:nviews_sample]
     sub["weights"] = tabs["weights"][:nviews_sample]
     sub["fwd"] = tabs["fwd"][:per_view * nviews_sample]
-    cp = oracle_compact_from_abi(wl, sub)
-    X = O.initial_volume(LF.astype(np.float64), wl.Z)
-    Xl = O.to_lenslet(X, 15)
-    ones = np.ones((wl.n1, wl.n2))
+    st = CB.StridedTables(oracle_compact_from_abi(wl, sub))
+    Xl = O.to_lenslet(O.initial_volume(LF.astype(np.float64), wl.Z), 15).astype(np.float32)
     t0 = time.perf_counter()
-    for k in range(nviews_sample):
-        u, v = cp.views[k]
-        y = O.forward_compact(Xl, cp.fwd[k], cp.rf)
-        R = O.ratio_update(LF[u::15, v::15].astype(np.float64), y)
-        B = O.backward_compact(R, cp.bwd[k], cp.rb)
-        B1 = O.backward_compact(ones, cp.bwd[k], cp.rb)
-        Xl = O.multiplicative_update(Xl, B, B1, cp.weights[k])
+    CB.view_steps_strided(Xl, LF, st)
     t_view = (time.perf_counter() - t0) / nviews_sample
-    # one plane of sample_filter (FFT form), scaled by the plane count
-    om = O.omega_array(wl.Z, 15)
-    k1 = np.fft.fft(O.filter_kernel_1d(wl.I1, om[0]))
-    k2 = np.fft.fft(O.filter_kernel_1d(wl.I2, om[0]))
-    img = O.from_lenslet(Xl, 15)[:, :, 0]
-    t1 = time.perf_counter()
-    buf = np.real(np.fft.ifft2(np.fft.fft2(img) * (k1[:, None] * k2[None, :])))
-    _ = img.sum() * buf / buf.sum()
-    t_plane = time.perf_counter() - t1
+    X2 = np.ones((wl.I1, wl.I2, 2), np.float32)
+    t0 = time.perf_counter()
+    CB.filter_planes(X2)
+    t_plane = (time.perf_counter() - t0) / 2
     t_volume = wl.iters * (len(tabs["uv"]) * t_view + wl.Z * t_plane)
-    cores = int(os.environ.get("OMP_NUM_THREADS", 0)) or len(os.sched_getaffinity(0))
+
+    # 33 x 33 x 11: one whole strided iteration
+    w1 = synth.CFG1
+    tabs1 = T.synthetic_compact(w1)
+    cp1 = oracle_compact_from_abi(w1, tabs1)
+    st1 = CB.StridedTables(cp1)
+    lf1 = np.random.default_rng(1).poisson(100.0, (w1.I1, w1.I2)).astype(np.float32)
+    X1 = O.initial_volume(lf1.astype(np.float64), w1.Z)
+    t0 = time.perf_counter()
+    Xl1 = CB.view_steps_strided(O.to_lenslet(X1, 15).astype(np.float32), lf1, st1)
+    t1_views = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    CB.filter_planes(O.from_lenslet(Xl1.astype(np.float64), 15))
+    t1_filter = time.perf_counter() - t0
+    t1_iter = t1_views + t1_filter
+    # ... and one literal view step
+    u, v = cp1.views[0]
+    Hv, Htv = CB.view_slices_from_synth(w1, u, v)
+    t0 = time.perf_counter()
+    CB.view_step_literal(X1.astype(np.float32), lf1, Hv, Htv, u, v, cp1.weights[0])
+    t1_lit_view = time.perf_counter() - t0
+    t1_lit_volume = w1.iters * (len(cp1.views) * t1_lit_view + t1_filter)
     return {
         "value": 1.0 / t_volume,
         "unit": "volumes/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"FP64 numpy oracle (BLAS, {cores} threads): {nviews_sample} view steps "
-                   f"({t_view:.2f} s each) + 1 filtered plane ({t_plane:.2f} s) of the same "
-                   f"workload, extrapolated x{len(tabs['uv'])} views x{wl.iters} iterations "
-                   f"(+{wl.Z} planes/iteration): {t_volume:.0f} s per volume"),
+        "sample": (f"FP32 strided form on numpy BLAS ({cores} threads): {nviews_sample} view "
+                   f"steps ({t_view:.3f} s each) + 1 filtered plane ({t_plane:.3f} s) of this workload, "
+                   f"x{len(tabs['uv'])} views x{wl.iters} iterations (+{wl.Z} planes per iteration): "
+                   f"{t_volume:.0f} s per volume"),
+        "cfg1_33x33x11": {
+            "strided_iteration_s": round(t1_iter, 3),
+            "strided_volume_s_x8": round(w1.iters * t1_iter, 2),
+            "literal_view_step_s": round(t1_lit_view, 3),
+            "literal_volume_s_x177x8": round(t1_lit_volume, 1),
+            "note": ("strided: one whole iteration timed (177 view steps + sample_filter), x8; literal: "
+                     "the reference CPU branch as written (full-resolution conv2 'same' of "
+                     f"{w1.I1}x{w1.I2} images with {w1.IMG}x{w1.IMG} kernels, recon:191-208), one view "
+                     "step timed, x177 views x8 iterations plus the strided filter time"),
+        },
     }
+
+
+def oracle_check(wl, tabs, LF, psf, precisions, nviews: int = 1):
+    """Relative L2 error, against the FP64 oracle, of each GPU path's volume
+    after the first `nviews` view steps from the initial volume (filter off)."""
+    import mesolf_amd
+    from oracle import mesolf_oracle as O
+    from tests.helpers import oracle_compact_from_abi
+
+    per_view = sum((2 * r + 1) ** 2 * 225 for r in wl.radii)
+    sub = dict(tabs)
+    sub["uv"] = tabs["uv"][:nviews]
+    sub["weights"] = tabs["weights"][:nviews]
+    sub["fwd"] = tabs["fwd"][:per_view * nviews]
+    cp = oracle_compact_from_abi(wl, sub)
+    t0 = time.perf_counter()
+    ref = O.rl_compact(LF.astype(np.float64), cp, 1, views_limit=nviews, filter_planes=False)
+    t_ref = time.perf_counter() - t0
+    out = {"view_steps": nviews, "oracle_s": round(t_ref, 1)}
+    nr = np.linalg.norm(ref.ravel())
+    for prec in precisions:
+        p = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision=prec)
+        p.set_image(LF)
+        p.init_volume()
+        p.run(1, views_limit=nviews, filter=False)
+        x = p.get_volume(f64=True)
+        out[f"rel_l2_{prec}"] = float(np.linalg.norm((x - ref).ravel()) / nr)
+        p.close()
+        del x
+    return out
 
 
 def _movie_patches(psf, grids, frames, rng, groups=4):
@@ -466,7 +541,9 @@ def main():
     ap.add_argument("--views-limit", type=int, default=0,
                     help="profiling only: visit the first N views (result flagged partial)")
     ap.add_argument("--iters", type=int, default=0, help="profiling only: RL iterations per volume")
-    ap.add_argument("--cpu-views", type=int, default=3)
+    ap.add_argument("--cpu-views", type=int, default=2)
+    ap.add_argument("--oracle-views", type=int, default=1,
+                    help="view steps checked against the FP64 oracle (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="keep the default tile shapes")
     ap.add_argument("--no-compare", action="store_true",
@@ -611,10 +688,40 @@ def main():
         log(f"fp32 path {1.0 / t_fp32:.4f} vol/s, rel L2 {err:.3e}")
         del Xb, Xf
 
+    fp32_kernels = None
+    if not args.no_compare and args.precision != "fp32" and not partial:
+        # the plain FP32 path's per-kernel times over one more volume
+        patch.set_image(LF)
+        patch.init_volume()
+        patch.set_timing(True, stride=8)
+        patch.kernel_times()
+        patch.run(iters, filter=True)
+        ctx.synchronize()
+        kt32 = patch.kernel_times()
+        patch.set_timing(False)
+        fp32_kernels = {}
+        for c in ("forward", "backward_update"):
+            ms, n = kt32[c]
+            if n:
+                a = ms / n * 1e-3
+                fp32_kernels[c] = {"avg_ms": round(a * 1e3, 4),
+                                   "algorithmic_GBs": round(algorithmic_bytes_per_view(wl, "fp32", c) / a / 1e9, 1),
+                                   "design_GBs": round(design_bytes_per_view(wl, "fp32", c) / a / 1e9, 1),
+                                   "TFLOPs": round(algorithmic_flops_per_view(wl) / a / 1e12, 1)}
+        if compare is not None:
+            compare["fp32_kernels"] = fp32_kernels
+
+    check = None
+    if args.oracle_views > 0 and not partial:
+        log(f"FP64 oracle check: {args.oracle_views} view step(s) ...")
+        check = oracle_check(wl, tabs, LF, psf, sorted({"fp32", args.precision}), args.oracle_views)
+        log(f"oracle check {check}")
+
     cpu = None
     if not args.no_cpu_baseline:
         log("cpu baseline ...")
         cpu = cpu_baseline(wl, tabs, LF, args.cpu_views)
+        log(f"cpu baseline {cpu['value']:.3e} vol/s ({cpu['sample']})")
 
     out = {
         "metric": "deconvolved volumes/s, 101x101-lenslet x 51-plane patch, 8 RL iterations",
@@ -638,6 +745,7 @@ def main():
         "kernels": per_kernel,
         "postprocess": post,
         "fp32_comparison": compare,
+        "oracle_check": check,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

@@ -369,13 +369,23 @@ mlf_status mlf_summary_image_global_device(mlf_context ctx, const float* d_movie
  * each slice; seg_ptr [nseg+1] non-decreasing with seg_ptr[0] = 0 and
  * seg_ptr[nseg] = nslices; kernels [kw, kw, 15, 15, Z] single; out
  * [width, height, nseg] single (a segment without slices gets zeros).  Each
- * slice's projection is formed on its own and then added in slice order, as
- * the reference adds per-plane results.  Host pointers, synchronous. */
+ * slice's projection is formed on its own and then added to its segment in
+ * ascending plane order (slices of one plane in slice order), the order in
+ * which the reference's loop over planes adds them (:263-318).  Odd kw runs
+ * on the MFMA form of the operator, one GEMM-operand build per plane.  Host
+ * pointers, synchronous. */
 mlf_status mlf_lfm_convolution_segments(const float* slices, int64_t width, int64_t height,
                                         int64_t nslices, const int32_t* plane,
                                         const int64_t* seg_ptr, int64_t nseg,
                                         const float* kernels, int64_t kw, int64_t nphase,
                                         int64_t Z, float* out, int32_t device);
+/* Same with slices, kernels and out in device memory (plane and seg_ptr stay
+ * host arrays), on ctx's stream; returns when done. */
+mlf_status mlf_lfm_convolution_segments_device(mlf_context ctx, const float* d_slices, int64_t width,
+                                               int64_t height, int64_t nslices, const int32_t* plane,
+                                               const int64_t* seg_ptr, int64_t nseg,
+                                               const float* d_kernels, int64_t kw, int64_t nphase,
+                                               int64_t Z, float* d_out);
 
 #ifdef __cplusplus
 }

@@ -6,7 +6,8 @@ host-side mirror of the reference's MATLAB-facing interface.
 """
 from .lib import (  # noqa: F401
     Batch, BudgetExceeded, Cancelled, Context, InvalidInput, MesolfError, Patch, PSF, Server, Unsupported,
-    compact_block_layout, exported_symbols, lfm_convolution_segments, load_library,
+    compact_block_layout, exported_symbols, lfm_convolution_segments, lfm_convolution_segments_device,
+    load_library,
     mex_lfm_convolution_vec, LFM_reconstruction_module, summary_image_generation_module,
     summary_image_pixelwise, summary_image_global, carpet_patch_info, crop_movie_device, img_rectify_mod, rectify_size,
     recon_theory_phase_space_LFM_with_filter,
@@ -16,6 +17,6 @@ __all__ = [
     "Context", "PSF", "Patch", "Batch", "Server", "MesolfError", "InvalidInput", "Unsupported", "BudgetExceeded", "Cancelled",
     "load_library", "exported_symbols", "compact_block_layout",
     "recon_theory_phase_space_LFM_with_filter", "mex_lfm_convolution_vec",
-    "lfm_convolution_segments", "LFM_reconstruction_module", "summary_image_generation_module",
+    "lfm_convolution_segments", "lfm_convolution_segments_device", "LFM_reconstruction_module", "summary_image_generation_module",
     "summary_image_pixelwise", "summary_image_global", "carpet_patch_info", "crop_movie_device", "img_rectify_mod", "rectify_size",
 ]

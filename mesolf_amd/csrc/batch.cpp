@@ -35,6 +35,7 @@ struct mlf_batch_s {
   int fwd_mt = 0, bwd_nt = 0;  // tile shape shared by every attached patch
   int grid_f = 0, grid_b = 0, grid_r = 0;
   size_t smem_f = 0, smem_b = 0;
+  int fwd_nbuf = 2;
   // one RL iteration's view loop as a graph
   bool use_graph = true;
   hipGraph_t graph = nullptr;
@@ -148,7 +149,7 @@ mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st, int nviews = 0) {
       MLF_LAUNCH(launch_fwd_tpz(b->kp, b->fwd_mt, Wk, g, b->grid_f, b->smem_f, st));
       MLF_LAUNCH(launch_ratio_gather_tpz(g, rg, b->grid_r, ps->Z, b->cw, u, v, 1, nullptr, st));
     } else {
-      MLF_LAUNCH(launch_fwd_mfma(b->kp, Wk, g, b->grid_f, b->smem_f, st));
+      MLF_LAUNCH(launch_fwd_mfma(b->kp, Wk, g, b->grid_f, b->smem_f, b->fwd_nbuf, st));
       MLF_LAUNCH(launch_ratio_gather(g, rg, b->grid_r, ps->Z, b->bm2, u, v, 1, nullptr, st));
     }
     const BwdGeo bg{b->d_slots.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, b->d_bsched.ptr,
@@ -290,6 +291,7 @@ mlf_status mlf_batch_attach(mlf_batch b, mlf_patch p) {
     b->bwd_nt = p->bwd_nt;
     b->smem_f = p->smem_f;
     b->smem_b = p->smem_b;
+    b->fwd_nbuf = p->fwd_nbuf;
   } else if (p->fwd_mt != b->fwd_mt || p->bwd_nt != b->bwd_nt) {
     // one launch serves every patch: the patch takes the batch's tile shape
     MLF_HIP(hipStreamSynchronize(b->psf->ctx->stream));
@@ -388,6 +390,7 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
     b->cw = p0->cw;
     b->smem_f = p0->smem_f;
     b->smem_b = p0->smem_b;
+    b->fwd_nbuf = p0->fwd_nbuf;
     drop_graph(b);
     b->dirty = true;
     return rebuild(b);

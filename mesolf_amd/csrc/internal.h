@@ -172,9 +172,16 @@ constexpr int kMexMfmaMaxKw = 511;
 size_t mex_kq_floats(int kw);
 hipError_t launch_mex_mfma(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
                            float* Kq, float* out, hipStream_t st);
-// out[s] = sum_{i in [seg_ptr[s], seg_ptr[s+1])} MEX(in[i], K[plane[i]]); K is [Z][15][15][kw][kw].
+// Kq of one kernel set K [kw][kw][15][15] (for launch_mex_mfma_list).
+hipError_t launch_mex_kq(const float* K, int kw, float* Kq, hipStream_t st);
+// out[dst[j]] += MEX(in[src[j]], K) for j < n (dst distinct within a launch).
+hipError_t launch_mex_mfma_list(const float* in, int64_t W, int64_t H, int64_t n, const float* Kq,
+                                int kw, const int32_t* src, const int32_t* dst, float* out,
+                                hipStream_t st);
+// out[s] = sum over j in [seg_ptr[s], seg_ptr[s+1]) of MEX(in[order[j]], K[plane[order[j]]]),
+// added in j order; K is [Z][15][15][kw][kw].
 hipError_t launch_mex_conv_segments(const float* in, int64_t W, int64_t H, const int32_t* plane,
-                                    const int64_t* seg_ptr, int64_t nseg, const float* K, int kw,
-                                    float* out, hipStream_t st);
+                                    const int64_t* seg_ptr, const int32_t* order, int64_t nseg,
+                                    const float* K, int kw, float* out, hipStream_t st);
 
 }  // namespace mlf

@@ -59,14 +59,15 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_mex_conv(const float* __rest
   if (valid) out[(size_t)img * W * H + (size_t)y * W + x] = acc;
 }
 
-// Segment-batched form: out[s] = sum over the slices i of segment s (in
-// order) of conv(slice i, kernel of plane[i]); each slice's sum is formed
-// separately and then added, as the per-plane MEX calls and additions of
+// Segment-batched form: out[s] = sum over the slices of segment s, taken in
+// the order order[seg_ptr[s]] .. order[seg_ptr[s+1]-1], of conv(slice i,
+// kernel of plane[i]); each slice's sum is formed separately and then added,
+// as the per-plane MEX calls and additions of
 // gen_seed_library_local_with_ring_bg_same_size.m:285-316 do.
 __global__ __launch_bounds__(kTileX* kTileY) void k_mex_conv_segments(
     const float* __restrict__ in, int W, int H, const int32_t* __restrict__ plane,
-    const int64_t* __restrict__ seg_ptr, const float* __restrict__ Kall, int kw,
-    float* __restrict__ out) {
+    const int64_t* __restrict__ seg_ptr, const int32_t* __restrict__ order,
+    const float* __restrict__ Kall, int kw, float* __restrict__ out) {
   extern __shared__ float sh[];
   const int tx = threadIdx.x % kTileX;
   const int ty = threadIdx.x / kTileX;
@@ -84,7 +85,8 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_mex_conv_segments(
   const size_t kplane = (size_t)kw * kw * kP;
   const size_t koff = (size_t)kw * kw * ((x % kN) + kN * (y % kN));
   float total = 0.f;
-  for (int64_t i = seg_ptr[seg]; i < seg_ptr[seg + 1]; ++i) {
+  for (int64_t j = seg_ptr[seg]; j < seg_ptr[seg + 1]; ++j) {
+    const int i = order[j];
     const float* src = in + (size_t)i * W * H;
     const float* Kp = Kall + (size_t)plane[i] * kplane + koff;
     float acc = 0.f;
@@ -140,9 +142,14 @@ __global__ void k_mex_kq(const float* __restrict__ K, int kw, int T4, float* __r
   }
 }
 
+// List mode (seed-library batches): image j of the launch is input image
+// src[j] and its projection is added to output image dst[j] (the dst of one
+// launch are distinct, so no two workgroups add to the same pixel).
 __global__ __launch_bounds__(256) void k_mex_mfma(const float* __restrict__ in, int W, int H, int n,
                                                   const float* __restrict__ Kq, int kw, int T4,
-                                                  float* __restrict__ out) {
+                                                  float* __restrict__ out,
+                                                  const int32_t* __restrict__ src,
+                                                  const int32_t* __restrict__ dst) {
   extern __shared__ float xs[];  // [kMexCK][kMexImg][XL]
   const int y = blockIdx.x, ng = blockIdx.y, ib = blockIdx.z;
   const int q = y % kN, h = kw / 2;
@@ -162,7 +169,7 @@ __global__ __launch_bounds__(256) void k_mex_mfma(const float* __restrict__ in, 
       const int r = y - h + c0 + kk, x = xlo + xi, nn = ng * kMexImg + im;
       float v = 0.f;
       if (c0 + kk < kw && r >= 0 && r < H && x >= 0 && x < W && nn < n)
-        v = in[(size_t)nn * W * H + (size_t)r * W + x];
+        v = in[(size_t)(src ? src[nn] : nn) * W * H + (size_t)r * W + x];
       xs[o] = v;
     }
     __syncthreads();
@@ -178,12 +185,16 @@ __global__ __launch_bounds__(256) void k_mex_mfma(const float* __restrict__ in, 
     }
   }
   if (img >= n || lr >= kN) return;
+  float* o = out + (size_t)(dst ? dst[img] : img) * W * H + (size_t)y * W;
 #pragma unroll
   for (int it = 0; it < 2; ++it)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int x = kN * (i0 + 16 * it + 4 * lg + j) + lr;
-      if (x < W) out[(size_t)img * W * H + (size_t)y * W + x] = acc[it][j];
+      if (x < W) {
+        if (dst) o[x] += acc[it][j];
+        else o[x] = acc[it][j];
+      }
     }
 }
 
@@ -192,6 +203,28 @@ __global__ __launch_bounds__(256) void k_mex_mfma(const float* __restrict__ in, 
 size_t mex_kq_floats(int kw) {
   const int T4 = (kw + kN - 1 + 3) / 4 * 4;
   return (size_t)kN * kw * T4 * 16;
+}
+
+hipError_t launch_mex_kq(const float* K, int kw, float* Kq, hipStream_t st) {
+  const int T4 = (kw + kN - 1 + 3) / 4 * 4;
+  hipLaunchKernelGGL(k_mex_kq, dim3(1024), dim3(256), 0, st, K, kw, T4, Kq);
+  return hipGetLastError();
+}
+
+hipError_t launch_mex_mfma_list(const float* in, int64_t W, int64_t H, int64_t n, const float* Kq,
+                                int kw, const int32_t* src, const int32_t* dst, float* out,
+                                hipStream_t st) {
+  const int T4 = (kw + kN - 1 + 3) / 4 * 4;
+  const int XL = kN * kMexCols + T4;
+  const size_t shmem = sizeof(float) * kMexCK * kMexImg * XL;
+  static hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mex_mfma),
+                                               hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+  if (attr != hipSuccess) return attr;
+  dim3 grid((unsigned)H, (unsigned)((n + kMexImg - 1) / kMexImg),
+            (unsigned)((W / kN + 1 + kMexCols - 1) / kMexCols));
+  hipLaunchKernelGGL(k_mex_mfma, grid, dim3(256), shmem, st, in, (int)W, (int)H, (int)n, Kq, kw, T4,
+                     out, src, dst);
+  return hipGetLastError();
 }
 
 hipError_t launch_mex_mfma(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
@@ -206,18 +239,18 @@ hipError_t launch_mex_mfma(const float* in, int64_t W, int64_t H, int64_t n, con
   dim3 grid((unsigned)H, (unsigned)((n + kMexImg - 1) / kMexImg),
             (unsigned)((W / kN + 1 + kMexCols - 1) / kMexCols));
   hipLaunchKernelGGL(k_mex_mfma, grid, dim3(256), shmem, st, in, (int)W, (int)H, (int)n, Kq, kw, T4,
-                     out);
+                     out, nullptr, nullptr);
   return hipGetLastError();
 }
 
 hipError_t launch_mex_conv_segments(const float* in, int64_t W, int64_t H, const int32_t* plane,
-                                    const int64_t* seg_ptr, int64_t nseg, const float* K, int kw,
-                                    float* out, hipStream_t st) {
+                                    const int64_t* seg_ptr, const int32_t* order, int64_t nseg,
+                                    const float* K, int kw, float* out, hipStream_t st) {
   dim3 grid((unsigned)((W + kTileX - 1) / kTileX), (unsigned)((H + kTileY - 1) / kTileY),
             (unsigned)nseg);
   const size_t shmem = sizeof(float) * kTileY * (kTileX + kw);
   hipLaunchKernelGGL(k_mex_conv_segments, grid, dim3(kTileX * kTileY), shmem, st, in, (int)W,
-                     (int)H, plane, seg_ptr, K, kw, out);
+                     (int)H, plane, seg_ptr, order, K, kw, out);
   return hipGetLastError();
 }
 

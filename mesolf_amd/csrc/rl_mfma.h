@@ -123,6 +123,9 @@ __host__ __device__ inline void norm_range(int c, int n, int r, int* lo, int* hi
 }
 
 size_t fwd_smem_bytes(Precision p, int rmax, int bm2);
+// tap-chunk buffers of the Q-scatter forward at this radius (2, or 1 when two
+// do not fit in LDS beside the window)
+int fwd_nbuf(Precision p, int rmax, int bm2);
 size_t tpz_smem_bytes(Precision p, int rmax, int mt);
 int tpz_max_radius();
 // mt: input columns per wave (f32: 1 or 2; bf16: 2 or 4)
@@ -142,7 +145,7 @@ hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo
                                hipStream_t st);
 size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt);
 
-hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem, int nbuf,
                            hipStream_t st);
 hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, int u, int v,
                                int use_lf, float* y_out, hipStream_t st);

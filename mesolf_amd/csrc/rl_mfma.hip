@@ -172,24 +172,17 @@ __device__ __forceinline__ void wait_vm_le(int n) {
 
 
 // ------------------------------------------------------------------ forward
-template <typename T>
-struct FwdCfg {
-  static constexpr int NBUF = 2;  // FP32: double-buffered chunks, 1 workgroup per CU
-};
-template <>
-struct FwdCfg<__bf16> {
-  static constexpr int NBUF = 2;  // BF16 (fallback kernel for r > 23)
-};
-
-template <typename T, int MT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FwdCfg<T>::NBUF == 1 ? 2 : 1)))
+// Q-scatter forward (the kernel for planes of radius > 23).  NBUF: tap chunks
+// in LDS -- 2 (the next chunk is copied while the current one is used) when
+// the window fits beside them, else 1 (FP32 beyond radius 22).
+template <typename T, int MT, int NBUF>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(NBUF == 1 ? 2 : 1)))
 void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
   using M = Mma<T>;
   constexpr int KSTEP = M::KSTEP, KPL = M::KPL, PHI = M::PHI, LDB = M::LDB;
   constexpr int NKS = PHI / KSTEP;
   constexpr int BM2 = 4 * MT;
   constexpr int CH = kFwdChunk;  // taps per chunk (4 N-tiles of 16)
-  constexpr int NBUF = FwdCfg<T>::NBUF;
   extern __shared__ __align__(16) char smem[];
   T* Bs = reinterpret_cast<T*>(smem);                                  // [NBUF][CH][LDB]
   float* ywin = reinterpret_cast<float*>(smem + sizeof(T) * NBUF * CH * LDB);
@@ -1233,11 +1226,18 @@ __global__ void k_norm_pass2(const double* __restrict__ S, float* __restrict__ T
 }  // namespace
 
 // ---------------------------------------------------------------- launchers
-size_t fwd_smem_bytes(Precision p, int rmax, int bm2) {
-  const size_t b = (p == Precision::kF32)
-                       ? sizeof(float) * FwdCfg<float>::NBUF * kFwdChunk * Mma<float>::LDB
-                       : sizeof(__bf16) * FwdCfg<__bf16>::NBUF * kFwdChunk * Mma<__bf16>::LDB;
+static size_t fwd_smem_nbuf(Precision p, int rmax, int bm2, int nbuf) {
+  const size_t b = (p == Precision::kF32) ? sizeof(float) * nbuf * kFwdChunk * Mma<float>::LDB
+                                          : sizeof(__bf16) * nbuf * kFwdChunk * Mma<__bf16>::LDB;
   return b + sizeof(float) * 4 * (16 + 2 * rmax) * (bm2 / 4 + 2 * rmax);
+}
+
+int fwd_nbuf(Precision p, int rmax, int bm2) {
+  return fwd_smem_nbuf(p, rmax, bm2, 2) <= 160 * 1024 ? 2 : 1;
+}
+
+size_t fwd_smem_bytes(Precision p, int rmax, int bm2) {
+  return fwd_smem_nbuf(p, rmax, bm2, fwd_nbuf(p, rmax, bm2));
 }
 
 size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt) {
@@ -1246,13 +1246,20 @@ size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt) {
   return bwd_ring_bytes(Wd, nt, 2) + 16 * (size_t)Wd * Wd;
 }
 
-hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem,
+hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem, int nbuf,
                            hipStream_t st) {
   if (p == Precision::kF32) {
-    hipLaunchKernelGGL((k_fwd_mfma<float, kFwdMTF32>), dim3(grid), dim3(256), smem, st,
-                       (const float*)W, g);
+    if (nbuf == 1)
+      hipLaunchKernelGGL((k_fwd_mfma<float, kFwdMTF32, 1>), dim3(grid), dim3(256), smem, st,
+                         (const float*)W, g);
+    else
+      hipLaunchKernelGGL((k_fwd_mfma<float, kFwdMTF32, 2>), dim3(grid), dim3(256), smem, st,
+                         (const float*)W, g);
+  } else if (nbuf == 1) {
+    hipLaunchKernelGGL((k_fwd_mfma<__bf16, kFwdMTBF16, 1>), dim3(grid), dim3(256), smem, st,
+                       (const __bf16*)W, g);
   } else {
-    hipLaunchKernelGGL((k_fwd_mfma<__bf16, kFwdMTBF16>), dim3(grid), dim3(256), smem, st,
+    hipLaunchKernelGGL((k_fwd_mfma<__bf16, kFwdMTBF16, 2>), dim3(grid), dim3(256), smem, st,
                        (const __bf16*)W, g);
   }
   return hipGetLastError();
@@ -1431,8 +1438,10 @@ namespace mlf {
 // Allow the forward kernels more than 64 KB of dynamic LDS (gfx950 has 160 KB).
 hipError_t mfma_kernel_attributes() {
   const void* kernels[] = {
-      reinterpret_cast<const void*>(&k_fwd_mfma<float, kFwdMTF32>),
-      reinterpret_cast<const void*>(&k_fwd_mfma<__bf16, kFwdMTBF16>),
+      reinterpret_cast<const void*>(&k_fwd_mfma<float, kFwdMTF32, 1>),
+      reinterpret_cast<const void*>(&k_fwd_mfma<float, kFwdMTF32, 2>),
+      reinterpret_cast<const void*>(&k_fwd_mfma<__bf16, kFwdMTBF16, 1>),
+      reinterpret_cast<const void*>(&k_fwd_mfma<__bf16, kFwdMTBF16, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<float, 1>),
       reinterpret_cast<const void*>(&k_fwd_tpz<float, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 2>),

@@ -822,7 +822,7 @@ static mlf_status patch_forward_ratio(mlf_patch_s* p, int k, const float* LF, fl
   if (p->tpz) {
     MLF_LAUNCH(launch_fwd_tpz(p->kp, p->fwd_mt, Wk, g, p->nblk_f, p->smem_f, st));
   } else {
-    MLF_LAUNCH(launch_fwd_mfma(p->kp, Wk, g, p->nblk_f, p->smem_f, st));
+    MLF_LAUNCH(launch_fwd_mfma(p->kp, Wk, g, p->nblk_f, p->smem_f, p->fwd_nbuf, st));
   }
   MLF_CHECK(tim_end(p));
   MLF_CHECK(tim_begin(p, 1));
@@ -1005,6 +1005,7 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   sd.xs_inv = 1.f / p->xs;
   MLF_CHECK(upload_vec(p->d_slot, std::vector<SlotDesc>{sd}));
   p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax, p->fwd_mt) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
+  p->fwd_nbuf = fwd_nbuf(p->kp, rfmax, p->bm2);
   p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax, p->bwd_nt);
   if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
     return set_error(MLF_ERR_UNSUPPORTED,
@@ -1859,19 +1860,16 @@ static mlf_status summary_host(const float* movie, int64_t height, int64_t width
   return MLF_OK;
 }
 
-mlf_status mlf_lfm_convolution_segments(const float* slices, int64_t width, int64_t height,
-                                        int64_t nslices, const int32_t* plane,
-                                        const int64_t* seg_ptr, int64_t nseg,
-                                        const float* kernels, int64_t kw, int64_t nphase,
-                                        int64_t Z, float* out, int32_t device) {
-  clear_error();
-  if (!plane || !seg_ptr || !kernels || !out || (nslices > 0 && !slices))
-    return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+static mlf_status segments_check(int64_t width, int64_t height, int64_t nslices, const int32_t* plane,
+                                 const int64_t* seg_ptr, int64_t nseg, int64_t kw, int64_t nphase,
+                                 int64_t Z) {
+  if (!plane || !seg_ptr) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
   if (width < 1 || height < 1 || nslices < 0 || nseg < 1 || Z < 1)
     return set_error(MLF_ERR_INVALID_ARGUMENT, "bad dimensions");
   if (nphase != kN) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernels must be [kw,kw,15,15,Z]");
   if (kw < 1 || kw > 2048) return set_error(MLF_ERR_INVALID_ARGUMENT, "kernel width out of [1,2048]");
   if (nseg > 65535) return set_error(MLF_ERR_INVALID_ARGUMENT, "too many segments (> 65535)");
+  if (nslices > INT32_MAX / 2) return set_error(MLF_ERR_INVALID_ARGUMENT, "too many slices");
   if (seg_ptr[0] != 0 || seg_ptr[nseg] != nslices)
     return set_error(MLF_ERR_INVALID_ARGUMENT, "seg_ptr must start at 0 and end at nslices");
   for (int64_t s = 0; s < nseg; ++s)
@@ -1880,30 +1878,129 @@ mlf_status mlf_lfm_convolution_segments(const float* slices, int64_t width, int6
     if (plane[i] < 0 || plane[i] >= Z)
       return set_error(MLF_ERR_INVALID_ARGUMENT, "slice %lld: plane %d outside [0,%lld)", (long long)i,
                        (int)plane[i], (long long)Z);
+  return MLF_OK;
+}
+
+// The seed-library batch on device buffers (host plane / seg_ptr tables).
+// Every segment's contributions are added in ascending plane order, slices of
+// one plane in slice order: the order of the reference's loop over planes
+// (gen_seed_library_local_with_ring_bg_same_size.m:263-318).  Odd kw: per
+// plane, the kernel's GEMM operand is built once and the plane's slices run
+// through the MFMA operator in rounds whose output segments are distinct,
+// each adding its projection into its segment.  Even kw: the scalar kernel
+// walks each segment's slices in the same order.
+static mlf_status segments_device(mlf_context ctx, const float* d_in, int64_t W, int64_t H,
+                                  int64_t nslices, const int32_t* plane, const int64_t* seg_ptr,
+                                  int64_t nseg, const float* d_K, int kw, int64_t Z, float* d_out) {
+  hipStream_t st = ctx->stream;
+  const size_t img = (size_t)W * H;
+  std::vector<int32_t> seg_of(nslices);
+  for (int64_t s = 0; s < nseg; ++s)
+    for (int64_t i = seg_ptr[s]; i < seg_ptr[s + 1]; ++i) seg_of[i] = (int32_t)s;
+  const size_t kplane = (size_t)kw * kw * kP;
+  if (kw % 2 == 1 && kw <= kMexMfmaMaxKw) {
+    // launches: (plane, round) -> lists of (slice, segment)
+    std::vector<std::vector<int32_t>> by_plane(Z);
+    for (int64_t i = 0; i < nslices; ++i) by_plane[plane[i]].push_back((int32_t)i);
+    std::vector<int32_t> src, dst;
+    struct Launch { int64_t z, off, n; };
+    std::vector<Launch> launches;
+    std::vector<int32_t> round_of_seg(nseg, 0);
+    for (int64_t z = 0; z < Z; ++z) {
+      if (by_plane[z].empty()) continue;
+      std::vector<std::vector<int32_t>> rounds;
+      for (int32_t i : by_plane[z]) {
+        const int r = round_of_seg[seg_of[i]]++;
+        if ((int)rounds.size() <= r) rounds.resize(r + 1);
+        rounds[r].push_back(i);
+      }
+      for (int32_t i : by_plane[z]) round_of_seg[seg_of[i]] = 0;
+      for (const auto& rd : rounds) {
+        launches.push_back(Launch{z, (int64_t)src.size(), (int64_t)rd.size()});
+        for (int32_t i : rd) {
+          src.push_back(i);
+          dst.push_back(seg_of[i]);
+        }
+      }
+    }
+    DevBuf<int32_t> d_src, d_dst;
+    MLF_CHECK(upload_vec(d_src, src));
+    MLF_CHECK(upload_vec(d_dst, dst));
+    MLF_LAUNCH(launch_fill(d_out, img * nseg, 0.f, st));
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    const size_t need = mex_kq_floats(kw);
+    if (ctx->mex_kq.count < need) {
+      MLF_HIP(hipStreamSynchronize(st));  // earlier calls may still read it
+      MLF_CHECK(ctx->mex_kq.alloc(need));
+    }
+    int64_t zprev = -1;
+    for (const Launch& l : launches) {
+      if (l.z != zprev) MLF_LAUNCH(launch_mex_kq(d_K + (size_t)l.z * kplane, kw, ctx->mex_kq.ptr, st));
+      zprev = l.z;
+      MLF_LAUNCH(launch_mex_mfma_list(d_in, W, H, l.n, ctx->mex_kq.ptr, kw, d_src.ptr + l.off,
+                                      d_dst.ptr + l.off, d_out, st));
+    }
+    // the index lists are freed on return: wait for the launches that read them
+    MLF_HIP(hipStreamSynchronize(st));
+    return MLF_OK;
+  }
+  std::vector<int32_t> order(nslices);
+  for (int64_t s = 0; s < nseg; ++s) {
+    for (int64_t i = seg_ptr[s]; i < seg_ptr[s + 1]; ++i) order[i] = (int32_t)i;
+    std::stable_sort(order.begin() + seg_ptr[s], order.begin() + seg_ptr[s + 1],
+                     [&](int32_t a, int32_t b) { return plane[a] < plane[b]; });
+  }
+  DevBuf<int32_t> dplane, dorder;
+  DevBuf<int64_t> dptr;
+  MLF_CHECK(upload_vec(dplane, std::vector<int32_t>(plane, plane + nslices)));
+  MLF_CHECK(upload_vec(dorder, order));
+  MLF_CHECK(upload_vec(dptr, std::vector<int64_t>(seg_ptr, seg_ptr + nseg + 1)));
+  MLF_LAUNCH(launch_mex_conv_segments(d_in, W, H, dplane.ptr, dptr.ptr, dorder.ptr, nseg, d_K, kw,
+                                      d_out, st));
+  MLF_HIP(hipStreamSynchronize(st));
+  return MLF_OK;
+}
+
+mlf_status mlf_lfm_convolution_segments(const float* slices, int64_t width, int64_t height,
+                                        int64_t nslices, const int32_t* plane,
+                                        const int64_t* seg_ptr, int64_t nseg,
+                                        const float* kernels, int64_t kw, int64_t nphase,
+                                        int64_t Z, float* out, int32_t device) {
+  clear_error();
+  if (!kernels || !out || (nslices > 0 && !slices))
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLF_CHECK(segments_check(width, height, nslices, plane, seg_ptr, nseg, kw, nphase, Z));
   mlf_context ctx = nullptr;
   MLF_CHECK(mlf_context_create(device, &ctx));
   std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> guard(ctx, mlf_context_destroy);
   hipStream_t st = ctx->stream;
   const size_t img = (size_t)width * height;
   DevBuf<float> din, dk, dout;
-  DevBuf<int32_t> dplane;
-  DevBuf<int64_t> dptr;
   MLF_CHECK(din.alloc(std::max<size_t>(1, img * nslices)));
   MLF_CHECK(dk.alloc((size_t)kw * kw * kP * Z));
   MLF_CHECK(dout.alloc(img * nseg));
-  MLF_CHECK(dplane.alloc(std::max<int64_t>(1, nslices)));
-  MLF_CHECK(dptr.alloc(nseg + 1));
-  if (nslices > 0) {
+  if (nslices > 0)
     MLF_HIP(hipMemcpyAsync(din.ptr, slices, img * nslices * sizeof(float), hipMemcpyHostToDevice, st));
-    MLF_HIP(hipMemcpyAsync(dplane.ptr, plane, nslices * sizeof(int32_t), hipMemcpyHostToDevice, st));
-  }
   MLF_HIP(hipMemcpyAsync(dk.ptr, kernels, dk.bytes(), hipMemcpyHostToDevice, st));
-  MLF_HIP(hipMemcpyAsync(dptr.ptr, seg_ptr, dptr.bytes(), hipMemcpyHostToDevice, st));
-  MLF_LAUNCH(launch_mex_conv_segments(din.ptr, width, height, dplane.ptr, dptr.ptr, nseg, dk.ptr,
-                                      (int)kw, dout.ptr, st));
+  MLF_CHECK(segments_device(ctx, din.ptr, width, height, nslices, plane, seg_ptr, nseg, dk.ptr, (int)kw,
+                            Z, dout.ptr));
   MLF_HIP(hipMemcpyAsync(out, dout.ptr, dout.bytes(), hipMemcpyDeviceToHost, st));
   MLF_HIP(hipStreamSynchronize(st));
   return MLF_OK;
+}
+
+mlf_status mlf_lfm_convolution_segments_device(mlf_context ctx, const float* d_slices, int64_t width,
+                                               int64_t height, int64_t nslices, const int32_t* plane,
+                                               const int64_t* seg_ptr, int64_t nseg,
+                                               const float* d_kernels, int64_t kw, int64_t nphase,
+                                               int64_t Z, float* d_out) {
+  clear_error();
+  if (!ctx || !d_kernels || !d_out || (nslices > 0 && !d_slices))
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  MLF_CHECK(segments_check(width, height, nslices, plane, seg_ptr, nseg, kw, nphase, Z));
+  MLF_CHECK(ctx_activate(ctx));
+  return segments_device(ctx, d_slices, width, height, nslices, plane, seg_ptr, nseg, d_kernels, (int)kw,
+                         Z, d_out);
 }
 
 }  // extern "C"

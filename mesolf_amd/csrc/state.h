@@ -116,6 +116,7 @@ struct mlf_patch_s {
   mlf::DevBuf<mlf::SlotDesc> d_slot;
   int nblk_f = 0, nblk_b = 0, nblk_r = 0;
   size_t smem_f = 0, smem_b = 0;
+  int fwd_nbuf = 2;  // Q-scatter forward: tap-chunk buffers
   mlf::NormTable* norm = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;

@@ -150,6 +150,8 @@ _SIGNATURES = {
                                            _P]),
     "mlf_lfm_convolution_segments": (C.c_int, [_F32P, _I64, _I64, _I64, _I32P, _I64P, _I64, _F32P,
                                                _I64, _I64, _I64, _F32P, _I32]),
+    "mlf_lfm_convolution_segments_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I32P, _I64P, _I64, _P,
+                                                      _I64, _I64, _I64, _P]),
 }
 
 
@@ -815,7 +817,8 @@ def lfm_convolution_segments(slices, planes, seg_ptr, kernels, device: int = 0):
     slices: single [W, H, nslices]; planes: 0-based plane per slice;
     seg_ptr: [nseg+1] slice ranges per segment; kernels: single
     [kw, kw, 15, 15, Z].  Returns [W, H, nseg]: per segment, the sum over its
-    slices of ``mex_lfm_convolution_vec(slice, kernels[..., plane])``."""
+    slices of ``mex_lfm_convolution_vec(slice, kernels[..., plane])``, added
+    in ascending plane order (the reference's loop order)."""
     lib = load_library()
     s = _col(slices, np.float32)
     if s.ndim == 2:
@@ -834,3 +837,18 @@ def lfm_convolution_segments(slices, planes, seg_ptr, kernels, device: int = 0):
                                             _fptr(K), K.shape[0], K.shape[2], K.shape[4],
                                             _fptr(out), int(device)))
     return out
+
+
+def lfm_convolution_segments_device(ctx: "Context", d_slices: int, W: int, H: int, nslices: int, planes,
+                                    seg_ptr, d_kernels: int, kw: int, Z: int, d_out: int):
+    """``lfm_convolution_segments`` on device buffers given as raw pointers
+    (ints): slices [W, H, nslices], kernels [kw, kw, 15, 15, Z] and the output
+    [W, H, nseg], all single; planes and seg_ptr are host sequences."""
+    pl = np.ascontiguousarray(planes, dtype=np.int32)
+    sp = np.ascontiguousarray(seg_ptr, dtype=np.int64)
+    if pl.size != int(nslices) or sp.ndim != 1:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "bad argument shapes")
+    _check(ctx._lib.mlf_lfm_convolution_segments_device(
+        ctx.handle, C.c_void_p(d_slices), int(W), int(H), int(nslices), _fptr(pl, C.c_int32),
+        _fptr(sp, C.c_int64), sp.size - 1, C.c_void_p(d_kernels), int(kw), 15, int(Z),
+        C.c_void_p(d_out)))

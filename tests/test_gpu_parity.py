@@ -367,3 +367,31 @@ def test_handle_lifetimes_in_any_order():
     p.set_volume(X)
     assert np.isfinite(p.forward_view(0)).all()
     p.close()
+
+
+@pytest.mark.parametrize("kw", [9, 8])
+def test_segment_projection_order_and_rounds(kw):
+    """Segments whose slices repeat a plane (several launches per plane on
+    the MFMA path) or list planes out of order: each segment equals the sum of
+    its per-slice MEX calls added in ascending plane order (slice order within
+    a plane), bit for bit, and the FP64 oracle within 1e-5."""
+    rng = np.random.default_rng(21 + kw)
+    W, H, Z = 50, 41, 3
+    K = rng.random((kw, kw, 15, 15, Z)).astype(np.float32)
+    planes = [2, 0, 2, 1, 1, 1, 0]
+    seg_ptr = [0, 4, 6, 6, 7]                # segment 2 is empty
+    sl = rng.random((W, H, len(planes))).astype(np.float32)
+    got = mesolf_amd.lfm_convolution_segments(sl, planes, seg_ptr, K)
+    for sgi in range(4):
+        idx = list(range(seg_ptr[sgi], seg_ptr[sgi + 1]))
+        idx.sort(key=lambda i: planes[i])    # stable: slice order within a plane
+        acc = np.zeros((W, H), np.float32)
+        ref = np.zeros((W, H))
+        for i in idx:
+            acc += mesolf_amd.mex_lfm_convolution_vec(sl[:, :, i], np.asfortranarray(K[..., planes[i]]))
+            ref += O.mex_op(sl[:, :, i].astype(np.float64), K[..., planes[i]].astype(np.float64))
+        assert np.array_equal(got[:, :, sgi], acc), sgi
+        if idx:
+            assert rel_l2(got[:, :, sgi], ref) <= 1e-5
+        else:
+            assert np.all(got[:, :, sgi] == 0)

@@ -41,9 +41,11 @@ def ctx():
 def prod():
     rng = np.random.default_rng(9)
     H, Ht, info = random_production_psf(rng)
-    cp = O.compact_from_dense(H, Ht)
+    # the library takes single arrays: the oracle starts from the same values
+    H, Ht = H.astype(np.float32), Ht.astype(np.float32)
+    cp = O.compact_from_dense(H.astype(np.float64), Ht.astype(np.float64))
     LF = rng.poisson(60.0, (60, 75)).astype(np.float32)  # 4 x 5 lenslets
-    return H.astype(np.float32), Ht.astype(np.float32), info, cp, LF
+    return H, Ht, info, cp, LF
 
 
 @pytest.fixture(scope="module")
