@@ -286,6 +286,20 @@ mlf_status mlf_lfm_convolution_vec(const float* bitmap, int64_t width, int64_t h
 mlf_status mlf_lfm_convolution_vec_device(mlf_context ctx, const float* d_bitmap, int64_t width,
                                           int64_t height, int64_t n, const float* d_kernels,
                                           int64_t kw, int64_t nphase, float* d_out);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): mlf_lfm_convolution_vec_device with no allocation of
+ * any kind.  Its GEMM operand lives in a 64 MiB device array of the library
+ * that every GPU it runs on holds until the process ends; odd kernels are
+ * limited to kw <= 257 by that array (MLF_ERR_INVALID_ARGUMENT beyond), even
+ * kernels need no scratch.  Results equal mlf_lfm_convolution_vec_device's
+ * bit for bit.  Calls on one context run in its stream's order; calls that
+ * overlap on one GPU from different contexts race on the shared array.
+ * ctx must have been created beforehand (creation resolves the array's
+ * device address). */
+mlf_status mlf_lfm_convolution_vec_noalloc(mlf_context ctx, const float* d_bitmap, int64_t width,
+                                           int64_t height, int64_t n, const float* d_kernels,
+                                           int64_t kw, int64_t nphase, float* d_out);
 
 /* Replaces img_rectify_mod (image_load_module_aug.m:124-164): frames
  * [H, W] single (column-major, n = frames of them) resampled at a grid of

@@ -28,7 +28,7 @@ COMMON = [
 ]
 
 SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "post.hip", "summary.hip", "runtime.cpp",
-           "batch.cpp", "server.cpp", "tune.cpp"]
+           "batch.cpp", "server.cpp", "tune.cpp", "noalloc.hip"]
 
 
 def _compile(src: str) -> str:

@@ -172,6 +172,9 @@ constexpr int kMexMfmaMaxKw = 511;
 size_t mex_kq_floats(int kw);
 hipError_t launch_mex_mfma(const float* in, int64_t W, int64_t H, int64_t n, const float* K, int kw,
                            float* Kq, float* out, hipStream_t st);
+// Device address of the no-allocation MEX scratch on `device` (the device
+// must be current), resolved once per device at context creation.
+mlf_status noalloc_scratch_init(int device);
 // Kq of one kernel set K [kw][kw][15][15] (for launch_mex_mfma_list).
 hipError_t launch_mex_kq(const float* K, int kw, float* Kq, hipStream_t st);
 // out[dst[j]] += MEX(in[src[j]], K) for j < n (dst distinct within a launch).

@@ -235,6 +235,7 @@ mlf_status mlf_context_create(int32_t device, mlf_context* out) {
   auto ctx = std::make_unique<mlf_context_s>();
   ctx->device = device;
   MLF_HIP(hipSetDevice(device));
+  MLF_CHECK(noalloc_scratch_init(device));
   MLF_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   *out = ctx.release();
   return MLF_OK;

@@ -135,6 +135,7 @@ _SIGNATURES = {
                                         _I32, _I32, C.c_int, _F64P, _P]),
     "mlf_lfm_convolution_vec": (C.c_int, [_F32P, _I64, _I64, _I64, _F32P, _I64, _I64, _F32P, _I32]),
     "mlf_lfm_convolution_vec_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P]),
+    "mlf_lfm_convolution_vec_noalloc": (C.c_int, [_P, _P, _I64, _I64, _I64, _P, _I64, _I64, _P]),
     "mlf_rectify_size": (C.c_int, [_I64, _I64, C.c_double, C.c_double, _I32, C.c_double, _I64P, _I64P]),
     "mlf_rectify": (C.c_int, [_F32P, _I64, _I64, _I64, C.c_double, C.c_double, _I32, C.c_double, _F32P,
                               _I32]),
