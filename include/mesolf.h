@@ -326,6 +326,35 @@ mlf_status mlf_rectify_device(mlf_context ctx, const float* d_in, int64_t H, int
  * kept.  Host only. */
 mlf_status mlf_carpet_patches(int64_t global_h, int64_t global_w, int32_t n_h, int32_t n_w,
                               int32_t M, int32_t min_size, int64_t* corners, int32_t* count);
+/* Replaces determine_patch_info_aug.m:12-281 for a raw frame of raw_h x raw_w
+ * px (the reference reads the first TIFF of the recording for its size; the
+ * pixel values only feed a figure).  The whole frame's resample grid steps
+ * dx/M through (x_center, y_center) = (SI.frames_x_offset, SI.frames_y_offset)
+ * from pixel 1 to the frame size, starts at a whole lenslet and is cut to
+ * whole lenslets (:51-84); crop_border = SI.frames_crop_border_microlenses
+ * [left, right, up, down] microlenses is cut off (:87-103); the patches tile
+ * the result: mode MLF_PATCH_CARPET with n_patches [2] (SI.n_patches, as
+ * mlf_carpet_patches), or MLF_PATCH_ROI with roi_list [n_roi][4]
+ * (SI.ROI_list rows: top, left, bottom, right) and roi_patch_size
+ * (SI.ROI_patch_size, cut down to a multiple of M; :144-218).  For every
+ * patch, raw_cut is the raw-frame region to read (rows raw_cut[0]..[1],
+ * columns [2]..[3], 1-based inclusive) and (raw_cut_xCenter,
+ * raw_cut_yCenter) the microlens centre in it that mlf_rectify takes to give
+ * exactly the patch's size (:221-275, image_load_module_aug.m:88-104).
+ * out NULL: *count only.  Host only. */
+#define MLF_PATCH_CARPET 0
+#define MLF_PATCH_ROI 1
+typedef struct {
+  int64_t location[4];   /* top, left, bottom, right in the cropped rectified image (1-based) */
+  int64_t size[2];       /* height, width (multiples of M) */
+  int64_t raw_cut[4];    /* first row, last row, first column, last column of the raw frame */
+  double raw_cut_xCenter, raw_cut_yCenter;
+} mlf_patch_info;
+mlf_status mlf_determine_patch_info(int64_t raw_h, int64_t raw_w, double x_center, double y_center,
+                                   double dx, int32_t M, const int32_t* crop_border, int32_t mode,
+                                   const int32_t* n_patches, const int64_t* roi_list, int32_t n_roi,
+                                   int32_t roi_patch_size, mlf_patch_info* out, int32_t capacity,
+                                   int32_t* count);
 /* sensor_movie(top : top+h-1, left : left+w-1, :) of a single movie
  * [H, W, frames] on the device (column-major, 1-based corner) into d_out
  * [h, w, frames]: the patch movie of mesolf.m's per-patch loop, the input of

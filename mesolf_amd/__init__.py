@@ -10,6 +10,7 @@ from .lib import (  # noqa: F401
     load_library,
     mex_lfm_convolution_vec, LFM_reconstruction_module, summary_image_generation_module,
     summary_image_pixelwise, summary_image_global, carpet_patch_info, crop_movie_device, img_rectify_mod, rectify_size,
+    determine_patch_info_aug,
     recon_theory_phase_space_LFM_with_filter,
 )
 
@@ -19,4 +20,5 @@ __all__ = [
     "recon_theory_phase_space_LFM_with_filter", "mex_lfm_convolution_vec",
     "lfm_convolution_segments", "lfm_convolution_segments_device", "LFM_reconstruction_module", "summary_image_generation_module",
     "summary_image_pixelwise", "summary_image_global", "carpet_patch_info", "crop_movie_device", "img_rectify_mod", "rectify_size",
+    "determine_patch_info_aug",
 ]

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdarg>
@@ -1732,6 +1733,135 @@ mlf_status mlf_carpet_patches(int64_t global_h, int64_t global_w, int32_t n_h, i
     c[3] = left + w - 1;
   }
   *count = k;
+  return MLF_OK;
+}
+
+// 'roi' tiling (determine_patch_info_aug.m:144-218): each ROI [top, left,
+// bottom, right] is tiled with patches of the patch size cut down to a
+// multiple of M (:35-41); an ROI smaller than a patch along an axis gives one
+// patch of the ROI's size there (:170-183).
+static void roi_tiles(const int64_t* roi, int32_t n_roi, int32_t roi_patch_size, int M, int min_size,
+                      std::vector<std::array<int64_t, 4>>* out) {
+  const int64_t ps = roi_patch_size - roi_patch_size % M;
+  for (int32_t q = 0; q < n_roi; ++q) {
+    const int64_t* c = roi + 4 * (size_t)q;
+    const int64_t sh = c[2] - c[0] + 1, sw = c[3] - c[1] + 1;
+    int64_t nh = (sh + ps - 1) / ps, nw = (sw + ps - 1) / ps, ph = ps, pw = ps;
+    if (sh < ps) ph = sh, nh = 1;
+    if (sw < ps) pw = sw, nw = 1;
+    for (int64_t j = 0; j < nh * nw; ++j) {
+      const int64_t ci = j % nh + 1, cj = j / nh + 1;
+      const int64_t top = (ci - 1) * ph + c[0], left = (cj - 1) * pw + c[1];
+      const int64_t bottom = std::min(ci * ph, sh) + c[0] - 1, right = std::min(cj * pw, sw) + c[1] - 1;
+      int64_t h = bottom - top + 1, w = right - left + 1;
+      h -= h % M;
+      w -= w % M;
+      if (h < min_size || w < min_size) continue;  // :201-204, discarded
+      out->push_back({top, left, top + h - 1, left + w - 1});
+    }
+  }
+}
+
+// a:d:b as a + k*d (also rectify_axis's convention)
+static std::vector<double> colon_seq(double a, double d, double b) {
+  const int64_t n = (int64_t)std::floor((b - a) / d + 1e-10) + 1;
+  std::vector<double> v;
+  for (int64_t k = 0; k < n; ++k) v.push_back(a + d * (double)k);
+  return v;
+}
+
+static int64_t matlab_round_i(double x) { return (int64_t)(x < 0 ? -std::floor(-x + 0.5) : std::floor(x + 0.5)); }
+
+mlf_status mlf_determine_patch_info(int64_t raw_h, int64_t raw_w, double x_center, double y_center,
+                                   double dx, int32_t M, const int32_t* crop_border, int32_t mode,
+                                   const int32_t* n_patches, const int64_t* roi_list, int32_t n_roi,
+                                   int32_t roi_patch_size, mlf_patch_info* out, int32_t capacity,
+                                   int32_t* count) {
+  clear_error();
+  if (!crop_border || !count) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
+  if (raw_h < 2 || raw_w < 2 || M < 1 || !(dx > 0.0) || !std::isfinite(dx))
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "bad frame size, M or dx");
+  if (!(x_center >= 1.0 && x_center <= (double)raw_w && y_center >= 1.0 && y_center <= (double)raw_h))
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "frame centre outside the frame");
+  const double s = dx / M;
+  // reference grid of the whole frame (:51-52): step dx/M through the centre, 1 .. size
+  std::vector<double> xf = colon_seq(x_center, -s, 1.0), yf = colon_seq(y_center, -s, 1.0);
+  const int64_t pre_x = (int64_t)xf.size(), pre_y = (int64_t)yf.size();  // find(Xresample == xCenter)
+  std::reverse(xf.begin(), xf.end());
+  std::reverse(yf.begin(), yf.end());
+  std::vector<double> Xr(xf), Yr(yf);
+  const std::vector<double> xs = colon_seq(x_center + s, s, (double)raw_w), ys = colon_seq(y_center + s, s, (double)raw_h);
+  Xr.insert(Xr.end(), xs.begin(), xs.end());
+  Yr.insert(Yr.end(), ys.begin(), ys.end());
+  // started at a whole lenslet (:60-69), cut to whole lenslets (:82-84)
+  const int64_t Md = M / 2;
+  const int64_t xq = (pre_x - Md) - (int64_t)M * (int64_t)std::floor((double)(pre_x - Md) / M);
+  const int64_t yq = (pre_y - Md) - (int64_t)M * (int64_t)std::floor((double)(pre_y - Md) / M);
+  const int64_t w_res = (int64_t)M * (((int64_t)Xr.size() - xq) / M);
+  const int64_t h_res = (int64_t)M * (((int64_t)Yr.size() - yq) / M);
+  // border crop in whole microlenses (:87-103)
+  const int64_t cl = crop_border[0], cr = crop_border[1], cu = crop_border[2], cd = crop_border[3];
+  if (cl < 0 || cr < 0 || cu < 0 || cd < 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "negative crop");
+  if (cl + cr >= w_res / M) return set_error(MLF_ERR_INVALID_ARGUMENT, "X-cut range is larger than the x-size of image");
+  if (cu + cd >= h_res / M) return set_error(MLF_ERR_INVALID_ARGUMENT, "Y-cut range is larger than the y-size of image");
+  const int64_t x_first = 1 + cl, y_first = 1 + cu;
+  const int64_t gw = (w_res / M - cr - x_first + 1) * M, gh = (h_res / M - cd - y_first + 1) * M;
+  std::vector<std::array<int64_t, 4>> tiles;
+  if (mode == MLF_PATCH_CARPET) {
+    if (!n_patches) return set_error(MLF_ERR_INVALID_ARGUMENT, "n_patches is NULL");
+    std::vector<int64_t> c(4 * (size_t)std::max(1, n_patches[0]) * std::max(1, n_patches[1]));
+    int32_t n = 0;
+    MLF_CHECK(mlf_carpet_patches(gh, gw, n_patches[0], n_patches[1], M, 50, c.data(), &n));
+    for (int32_t q = 0; q < n; ++q) tiles.push_back({c[4 * q], c[4 * q + 1], c[4 * q + 2], c[4 * q + 3]});
+  } else if (mode == MLF_PATCH_ROI) {
+    if (!roi_list || n_roi < 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "ROI list missing");
+    if (roi_patch_size < M) return set_error(MLF_ERR_INVALID_ARGUMENT, "ROI patch size %d < M", (int)roi_patch_size);
+    for (int32_t q = 0; q < n_roi; ++q) {
+      const int64_t* c = roi_list + 4 * (size_t)q;
+      if (c[0] < 1 || c[1] < 1 || c[2] < c[0] || c[3] < c[1])
+        return set_error(MLF_ERR_INVALID_ARGUMENT, "ROI %d is not [top, left, bottom, right]", (int)q);
+    }
+    roi_tiles(roi_list, n_roi, roi_patch_size, M, 50, &tiles);
+  } else {
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "patch_mode should be either 'carpet' or 'roi'");
+  }
+  *count = (int32_t)tiles.size();
+  if (!out) return MLF_OK;  // count query
+  if ((int32_t)tiles.size() > capacity)
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "%zu patches, capacity %d", tiles.size(), (int)capacity);
+  // raw cut and microlens centre per patch (:221-275)
+  const int64_t y_top = (y_first - 1) * M + 1, x_left = (x_first - 1) * M + 1;
+  const int64_t margin = (M + 1) / 2 - 2;  // ceil(M/2) - 2
+  const int64_t half = (M + 1) / 2;
+  for (size_t q = 0; q < tiles.size(); ++q) {
+    const auto& t = tiles[q];
+    const int64_t pt = t[0] + y_top - 1 + yq - 1 - margin, pl = t[1] + x_left - 1 + xq - 1 - margin;
+    const int64_t pb = t[2] + y_top - 1 + yq - 1 + margin - 1, pr = t[3] + x_left - 1 + xq - 1 + margin - 1;
+    if (pt < 1 || pl < 1 || pb > (int64_t)Yr.size() || pr > (int64_t)Xr.size())
+      return set_error(MLF_ERR_INVALID_ARGUMENT,
+                       "patch %zu: its raw cut leaves the frame's resample grid (the reference stops "
+                       "there too, determine_patch_info_aug.m:245-248); crop more border microlenses",
+                       q + 1);
+    mlf_patch_info& o = out[q];
+    o.location[0] = t[0];
+    o.location[1] = t[1];
+    o.location[2] = t[2];
+    o.location[3] = t[3];
+    o.size[0] = t[2] - t[0] + 1;
+    o.size[1] = t[3] - t[1] + 1;
+    o.raw_cut[0] = std::max<int64_t>((int64_t)std::floor(Yr[pt - 1]), 1);
+    o.raw_cut[1] = std::min<int64_t>((int64_t)std::ceil(Yr[pb - 1]), raw_h);
+    o.raw_cut[2] = std::max<int64_t>((int64_t)std::floor(Xr[pl - 1]), 1);
+    o.raw_cut[3] = std::min<int64_t>((int64_t)std::ceil(Xr[pr - 1]), raw_w);
+    const int64_t cy = matlab_round_i((double)(pt + pb) / 2.0), cx = matlab_round_i((double)(pl + pr) / 2.0);
+    const int64_t my = ((pre_y - cy) % M + M) % M, mx = ((pre_x - cx) % M + M) % M;
+    const int64_t iy = my >= half ? cy - (M - my) : cy + my;
+    const int64_t ix = mx >= half ? cx - (M - mx) : cx + mx;
+    if (iy < 1 || ix < 1 || iy > (int64_t)Yr.size() || ix > (int64_t)Xr.size())
+      return set_error(MLF_ERR_INVALID_ARGUMENT, "patch %zu: centre outside the resample grid", q + 1);
+    o.raw_cut_yCenter = Yr[iy - 1] - Yr[pt - 1];
+    o.raw_cut_xCenter = Xr[ix - 1] - Xr[pl - 1];
+  }
   return MLF_OK;
 }
 

@@ -143,6 +143,8 @@ _SIGNATURES = {
                                      _P]),
     "mlf_carpet_patches": (C.c_int, [_I64, _I64, _I32, _I32, _I32, _I32, _I64P, _I32P]),
     "mlf_crop_movie_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I64, _I64, _I64, _I64, _P]),
+    "mlf_determine_patch_info": (C.c_int, [_I64, _I64, C.c_double, C.c_double, C.c_double, _I32, _I32P, _I32,
+                                           _I32P, _I64P, _I32, _I32, _P, _I32, _I32P]),
     "mlf_summary_image": (C.c_int, [_F32P, _I64, _I64, _I64, _I32, _I32, _I32, _I32, _F64P, _F64P,
                                     _F64P, _I32]),
     "mlf_summary_image_global": (C.c_int, [_F32P, _I64, _I64, _I64, _I32, _F64P, _F64P, _F64P, _I32]),
@@ -705,6 +707,61 @@ def carpet_patch_info(global_h: int, global_w: int, n_patches, M: int = 15, min_
     out = []
     for t, l, b, r in corners[:cnt.value].tolist():
         out.append({"location": [[t, l], [b, r]], "size": [b - t + 1, r - l + 1]})
+    return out
+
+
+class _PatchInfo(C.Structure):
+    _fields_ = [("location", C.c_int64 * 4), ("size", C.c_int64 * 2), ("raw_cut", C.c_int64 * 4),
+                ("raw_cut_xCenter", C.c_double), ("raw_cut_yCenter", C.c_double)]
+
+
+def determine_patch_info_aug(SI, raw_size):
+    """Drop-in for determine_patch_info_aug.m:1-281 given the raw frame size
+    (rows, columns) -- the reference reads the recording's first TIFF for it.
+
+    Reads SI.frames_x_offset, frames_y_offset, frames_dx,
+    frames_crop_border_microlenses (default [0 0 0 0]), Nnum, patch_mode
+    ('carpet' with n_patches, or 'roi' with ROI_list and ROI_patch_size).
+    Returns the patch_info list: dicts with "location" [[top, left],
+    [bottom, right]] and "size" [h, w] in the cropped rectified image,
+    "raw_cut" [[row0, row1], [col0, col1]] (1-based, inclusive) of the raw
+    frame and "raw_cut_xCenter" / "raw_cut_yCenter", the microlens centre in
+    the cut for img_rectify_mod."""
+    lib = load_library()
+    mode = _si_get(SI, "patch_mode", "carpet")
+    M = int(_si_get(SI, "Nnum", 15))
+    crop = np.ascontiguousarray(np.ravel(_si_get(SI, "frames_crop_border_microlenses", [0, 0, 0, 0])),
+                                dtype=np.int32)
+    if crop.size != 4:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "frames_crop_border_microlenses must have 4 entries")
+    npat = np.zeros(2, np.int32)
+    roi = np.zeros((1, 4), np.int64)
+    n_roi, roi_ps = 0, 0
+    if mode == "carpet":
+        n = list(np.atleast_1d(_si_get(SI, "n_patches")))
+        npat[:] = [n[0], n[0]] if len(n) == 1 else n[:2]
+        m = 0
+    elif mode == "roi":
+        if _si_get(SI, "ROI_patch_size") is None:
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "In ROI mode, SI.ROI_patch_size is required")
+        roi = np.ascontiguousarray(np.atleast_2d(_si_get(SI, "ROI_list")), dtype=np.int64)
+        n_roi, roi_ps = roi.shape[0], int(_si_get(SI, "ROI_patch_size"))
+        m = 1
+    else:
+        raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "patch_mode should be either 'carpet' or 'roi'")
+    args = (int(raw_size[0]), int(raw_size[1]), float(_si_get(SI, "frames_x_offset")),
+            float(_si_get(SI, "frames_y_offset")), float(_si_get(SI, "frames_dx")), M,
+            crop.ctypes.data_as(_I32P), m, npat.ctypes.data_as(_I32P), roi.ctypes.data_as(_I64P), n_roi, roi_ps)
+    cnt = C.c_int32()
+    _check(lib.mlf_determine_patch_info(*args, None, 0, C.byref(cnt)))
+    arr = (_PatchInfo * max(1, cnt.value))()
+    _check(lib.mlf_determine_patch_info(*args, C.cast(arr, C.c_void_p), cnt.value, C.byref(cnt)))
+    out = []
+    for p in arr[:cnt.value]:
+        t, l, b, r = list(p.location)
+        y0, y1, x0, x1 = list(p.raw_cut)
+        out.append({"location": [[t, l], [b, r]], "size": list(p.size), "raw_cut": [[y0, y1], [x0, x1]],
+                    "raw_cut_xCenter": p.raw_cut_xCenter, "raw_cut_yCenter": p.raw_cut_yCenter})
     return out
 
 

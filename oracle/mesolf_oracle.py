@@ -46,6 +46,10 @@ summary_global              rank-1 of movie/max equals the divisor form
 summary_pixelwise           a per-pixel loop composition of brute-force pieces
 carpet_patches              tilings of 1000x1000, 100x100 (a discarded patch)
                             and 700x451 worked out by hand
+roi_patches,                a frame, its crop and two ROI layouts worked out by
+determine_patch_info        hand; every patch's raw cut rectifies about its
+                            centre to exactly the patch size (the reference's
+                            own consistency check, image_load_module_aug.m:106)
 rectify_axis / rectify      a grid worked out by hand; scipy's
                             RegularGridInterpolator (linear, extrapolating);
                             planar images reproduced at every query point
@@ -847,6 +851,103 @@ def carpet_patches(global_h: int, global_w: int, n_patches, M: int = 15, min_siz
         if h < min_size or w < min_size:
             continue
         out.append(((top, left), (top + h - 1, left + w - 1)))
+    return out
+
+
+def roi_patches(roi_list, roi_patch_size: int, M: int = 15, min_size: int = 50):
+    """Patches of the 'roi' mode (determine_patch_info_aug.m:32-42, 144-218):
+    each ROI [top, left, bottom, right] (1-based, in the cropped rectified
+    image) is tiled with patches of roi_patch_size (cut down to a multiple of
+    M); an ROI smaller than a patch along an axis gives one patch of the ROI's
+    size there; patches below min_size after the cut to multiples of M are
+    discarded.  Same corner list format as carpet_patches."""
+    ps = roi_patch_size - roi_patch_size % M
+    out = []
+    for roi in roi_list:
+        t0, l0, b0, r0 = [int(v) for v in roi]
+        sh, sw = b0 - t0 + 1, r0 - l0 + 1
+        nh, nw = -(-sh // ps), -(-sw // ps)
+        ph, pw = ps, ps
+        if sh < ps:
+            ph, nh = sh, 1
+        if sw < ps:
+            pw, nw = sw, 1
+        for j in range(nh * nw):
+            ci, cj = j % nh + 1, j // nh + 1
+            top, left = (ci - 1) * ph + t0, (cj - 1) * pw + l0
+            bottom, right = min(ci * ph, sh) + t0 - 1, min(cj * pw, sw) + l0 - 1
+            h = bottom - top + 1
+            w = right - left + 1
+            h -= h % M
+            w -= w % M
+            if h < min_size or w < min_size:
+                continue
+            out.append(((top, left), (top + h - 1, left + w - 1)))
+    return out
+
+
+def determine_patch_info(raw_h: int, raw_w: int, x_center: float, y_center: float, dx: float, M: int,
+                         crop_border, patch_mode: str = "carpet", n_patches=None, roi_list=None,
+                         roi_patch_size=None):
+    """determine_patch_info_aug.m:12-281 on the frame size (the pixel values
+    only feed a figure there).  The reference grid of the whole frame steps
+    dx/M through (x_center, y_center) from 1 to the frame size (:51-52), is
+    started at a whole lenslet (:60-69), cut to whole lenslets (:82-84) and
+    cropped by crop_border = [left, right, up, down] lenslets (:87-103); the
+    patches tile that image (carpet or roi); for each patch, raw_cut is the
+    raw-frame region [y0, y1; x0, x1] (1-based, inclusive) to read, with
+    (raw_cut_xCenter, raw_cut_yCenter) the centre of a microlens in it, to be
+    rectified about (:221-275).  Returns a list of dicts."""
+    s = dx / M
+    Xr = np.concatenate([_colon(x_center, -s, 1.0)[::-1], _colon(x_center + s, s, float(raw_w))])
+    Yr = np.concatenate([_colon(y_center, -s, 1.0)[::-1], _colon(y_center + s, s, float(raw_h))])
+    Mdiff = M // 2
+    pre_x = len(_colon(x_center, -s, 1.0))          # find(Xresample == xCenter), 1-based
+    pre_y = len(_colon(y_center, -s, 1.0))
+    xq_init = (pre_x - Mdiff) - M * math.floor((pre_x - Mdiff) / M)
+    yq_init = (pre_y - Mdiff) - M * math.floor((pre_y - Mdiff) / M)
+    h_res = M * ((len(Yr) - yq_init) // M)
+    w_res = M * ((len(Xr) - xq_init) // M)
+    cl, cr, cu, cd = [int(v) for v in crop_border]
+    xs_ml, ys_ml = w_res // M, h_res // M
+    if cl + cr >= xs_ml:
+        raise ValueError("X-cut range is larger than the x-size of image")
+    if cu + cd >= ys_ml:
+        raise ValueError("Y-cut range is larger than the y-size of image")
+    x_first, x_last = 1 + cl, xs_ml - cr
+    y_first, y_last = 1 + cu, ys_ml - cd
+    gh, gw = (y_last - y_first + 1) * M, (x_last - x_first + 1) * M
+    if patch_mode == "carpet":
+        corners = carpet_patches(gh, gw, n_patches, M)
+    elif patch_mode == "roi":
+        corners = roi_patches(roi_list, roi_patch_size, M)
+    else:
+        raise ValueError("patch_mode should be either 'carpet' or 'roi'")
+    y_top, x_left = (y_first - 1) * M + 1, (x_first - 1) * M + 1
+    margin = -(-M // 2) - 2
+    out = []
+    for (t, l), (b, r) in corners:
+        pt = t + y_top - 1 + yq_init - 1 - margin
+        pl = l + x_left - 1 + xq_init - 1 - margin
+        pb = b + y_top - 1 + yq_init - 1 + margin - 1
+        pr = r + x_left - 1 + xq_init - 1 + margin - 1
+        if pt < 1 or pl < 1 or pb > len(Yr) or pr > len(Xr):
+            # MATLAB stops at Yresample(patch_top_in_global_after) (:245-248)
+            raise IndexError(f"patch {(t, l, b, r)}: raw cut outside the resample grid")
+        y0 = max(math.floor(Yr[pt - 1]), 1)
+        y1 = min(math.ceil(Yr[pb - 1]), raw_h)
+        x0 = max(math.floor(Xr[pl - 1]), 1)
+        x1 = min(math.ceil(Xr[pr - 1]), raw_w)
+        cy = matlab_round((pt + pb) / 2)
+        cx = matlab_round((pl + pr) / 2)
+        my = (pre_y - cy) % M
+        mx = (pre_x - cx) % M
+        ycp = Yr[cy - (M - my) - 1] if my >= -(-M // 2) else Yr[cy + my - 1]
+        xcp = Xr[cx - (M - mx) - 1] if mx >= -(-M // 2) else Xr[cx + mx - 1]
+        out.append({"location": [[t, l], [b, r]], "size": [b - t + 1, r - l + 1],
+                    "raw_cut": [[y0, y1], [x0, x1]],
+                    "raw_cut_xCenter": float(xcp - Xr[pl - 1]),
+                    "raw_cut_yCenter": float(ycp - Yr[pt - 1])})
     return out
 
 

@@ -291,6 +291,12 @@ def test_server_destroy_cancels_queued_requests(psf):
     ths = [threading.Thread(target=waiter, args=(t,)) for t in ts]
     for th in ths:
         th.start()
+    # every waiter must be inside mlf_server_wait before the handle is
+    # destroyed (waiting on a destroyed server is a caller error); the request
+    # in flight runs 30 iterations, far longer than this
+    import time
+    time.sleep(0.5)
+    assert srv.stats()["pending"] >= 1
     h = srv.handle
     srv.handle = None
     assert lib.mlf_server_destroy(h) == 0

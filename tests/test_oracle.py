@@ -610,3 +610,48 @@ def test_summary_global_background_is_divisor_form():
     _, s, t = O.summary_global(movie, 3)
     s2, t2 = O.rank_1_factorization(movie, 3, movie.max())
     assert np.allclose(s, s2, rtol=1e-12) and np.allclose(t, t2, rtol=1e-12)
+
+
+# ------------------------------------------------------ patch layout (raw cut)
+HAND_SI = {"frames_x_offset": 100.0, "frames_y_offset": 90.0, "frames_dx": 15.0, "Nnum": 15,
+           "frames_crop_border_microlenses": [1, 1, 1, 1]}
+
+
+def test_determine_patch_info_carpet_by_hand():
+    """A 200 x 230 frame, centre (x 100, y 90), dx = 15 (grid step 1 px, so
+    the grid is the integers 1..230 / 1..200), crop 1 microlens per side:
+    X grid: centre index 100, XqInit = 93 - 90 = 3 -> 227 columns -> 225 (15
+    lenslets) -> crop -> 195; Y: index 90, YqInit = 83 - 75 = 8 -> 192 rows
+    -> 180 -> 150.  Carpet [2, 1]: patch heights ceil(75/15)*15 = 75.
+    Raw cut of patch 1 (margin ceil(15/2) - 2 = 6): rows 1 + 15 + 7 - 6 = 17 ..
+    75 + 15 + 7 + 5 = 102, columns 1 + 15 + 2 - 6 = 12 .. 195 + 15 + 2 + 5 =
+    217; centre row round((17 + 102)/2) = 60 (mod(90 - 60, 15) = 0), column
+    round((12 + 217)/2) = 115 -> yCenter 60 - 17 = 43, xCenter 115 - 12 = 103."""
+    got = O.determine_patch_info(200, 230, 100.0, 90.0, 15.0, 15, [1, 1, 1, 1], "carpet", n_patches=[2, 1])
+    assert [p["location"] for p in got] == [[[1, 1], [75, 195]], [[76, 1], [150, 195]]]
+    assert [p["raw_cut"] for p in got] == [[[17, 102], [12, 217]], [[92, 177], [12, 217]]]
+    assert [(p["raw_cut_xCenter"], p["raw_cut_yCenter"]) for p in got] == [(103.0, 43.0), (103.0, 43.0)]
+    # the raw cut rectifies about its centre to exactly the patch (image_load_module_aug.m:106)
+    for p in got:
+        (y0, y1), (x0, x1) = p["raw_cut"]
+        assert len(O.rectify_axis(p["raw_cut_yCenter"], y1 - y0 + 1, 15, 15.0)) == p["size"][0]
+        assert len(O.rectify_axis(p["raw_cut_xCenter"], x1 - x0 + 1, 15, 15.0)) == p["size"][1]
+
+
+def test_roi_patches_by_hand():
+    """ROI rows 16..120, columns 31..170 (105 x 140) with ROI_patch_size 64
+    (-> 60): 2 x 3 patches in ind2sub order; second row 76..120 is 45 high
+    (< 50, discarded); third column 151..170 is 20 wide -> 15 (discarded)."""
+    got = O.roi_patches([[16, 31, 120, 170]], 64, 15)
+    assert got == [((16, 31), (75, 90)), ((16, 91), (75, 150))]
+    # an ROI lower than a patch gives one patch row of the ROI's own height
+    # (80 -> 75); columns 1..120, 121..240 and 241..300 (60 wide)
+    assert O.roi_patches([[1, 1, 80, 300]], 120, 15) == [((1, 1), (75, 120)), ((1, 121), (75, 240)),
+                                                          ((1, 241), (75, 300))]
+
+
+def test_determine_patch_info_stops_where_the_reference_does():
+    with pytest.raises(IndexError):   # no right crop: the last patch's margin leaves the grid
+        O.determine_patch_info(200, 230, 100.0, 90.0, 15.0, 15, [1, 0, 0, 1], "carpet", n_patches=[2, 1])
+    with pytest.raises(ValueError):
+        O.determine_patch_info(200, 230, 100.0, 90.0, 15.0, 15, [8, 8, 1, 1], "carpet", n_patches=[1, 1])
