@@ -639,7 +639,7 @@ def main():
     if args.workload == "cfg2" and os.path.exists(tfile):
         # measured bytes per launch of the kernel instance this run used
         tr = json.load(open(tfile))["bytes_per_launch"].get(args.precision, {}).get(dominant)
-        fmt, bnt = work.tile_shape()
+        fmt, bnt, _ = work.tile_shape()
         if tr and "read" not in tr:
             tr = tr.get(f"fwd_columns_per_wave={fmt}" if dominant == "forward" else f"bwd_phase_tiles={bnt}")
         if tr:

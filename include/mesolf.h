@@ -25,7 +25,7 @@
 extern "C" {
 #endif
 
-#define MLF_ABI_VERSION 1
+#define MLF_ABI_VERSION 2
 
 typedef enum {
   MLF_OK = 0,
@@ -35,7 +35,8 @@ typedef enum {
   MLF_ERR_OUT_OF_MEMORY = 4,
   MLF_ERR_BUDGET = 5,           /* request refused by the HBM admission budget */
   MLF_ERR_INTERNAL = 6,
-  MLF_ERR_CANCELLED = 7         /* queued request dropped before it ran (server destroyed) */
+  MLF_ERR_CANCELLED = 7,        /* queued request dropped before it ran (server destroyed) */
+  MLF_ERR_IO = 8                /* file missing, unreadable, or not a well-formed MAT-file */
 } mlf_status;
 
 typedef enum {
@@ -94,6 +95,48 @@ mlf_status mlf_psf_create_compact(mlf_context ctx, int64_t N, int64_t Z, int64_t
                                   const int32_t* view_uv, const double* view_weight,
                                   const int32_t* rf, const float* fwd,
                                   const int32_t* rb, const float* bwd, mlf_psf* out);
+/* ---------------------------------------------------- MAT-file (-v7.3) reader */
+/* MesoLF saves its PSFs with save(..., '-v7.3') and loads them whole with
+ * load(SI.psffile) (mesolf.m:20-21): HDF5 behind a 512-byte MATLAB user
+ * block.  These entry points read the real numeric and logical variables of
+ * such a file (double, single, [u]int8..64, logical, char codes; any byte
+ * order; contiguous, compact or chunked with deflate / shuffle / fletcher32)
+ * in MATLAB's column-major order.  Complex, sparse, cell, struct and object
+ * variables are reported by mlf_mat_info and refused by the readers with
+ * MLF_ERR_UNSUPPORTED.  A malformed file gives MLF_ERR_IO. */
+typedef struct mlf_mat_s* mlf_mat;
+#define MLF_MAT_MAX_DIMS 32
+typedef struct {
+  char matlab_class[32];           /* MATLAB_class attribute ("double", "single", "struct", ...) */
+  int32_t ndim;                    /* MATLAB rank (>= 2) */
+  int64_t dims[MLF_MAT_MAX_DIMS];  /* MATLAB dims, first index contiguous */
+  int64_t numel;                   /* 0 for an empty variable */
+  int32_t numeric;                 /* 1 when mlf_mat_read_f32 / _f64 can read it */
+  int32_t stored_bytes;            /* bytes per element in the file */
+} mlf_mat_var;
+mlf_status mlf_mat_open(const char* path, mlf_mat* out);
+mlf_status mlf_mat_close(mlf_mat m);
+/* Top-level variables: count, and the name of variable i (cap bytes incl. NUL). */
+mlf_status mlf_mat_count(mlf_mat m, int32_t* n);
+mlf_status mlf_mat_name(mlf_mat m, int32_t i, char* name, size_t cap);
+/* name may be a '/'-separated path into a struct (a group). */
+mlf_status mlf_mat_info(mlf_mat m, const char* name, mlf_mat_var* info);
+/* Convert the variable to float / double; numel must equal its element count. */
+mlf_status mlf_mat_read_f32(mlf_mat m, const char* name, float* out, int64_t numel);
+mlf_status mlf_mat_read_f64(mlf_mat m, const char* name, double* out, int64_t numel);
+
+typedef struct {
+  int64_t S, N, Z;     /* H is [S, S, N, N, Z] */
+  double x_pixel_um;   /* |x1objspace(2) - x1objspace(1)| * 1e6 (mesolf.m:231), NaN if absent */
+  double z_pixel_um;   /* |x3objspace(2) - x3objspace(1)| * 1e6 (mesolf.m:232), NaN if absent */
+} mlf_psf_file_info;
+/* load(SI.psffile) (mesolf.m:20-21) followed by mlf_psf_create_dense on its
+ * H and Ht: the PSF is read once and stays resident on the device, where the
+ * reference copies H(:,:,u,v,:) and Ht(:,:,u,v,:) to the GPU for every view
+ * of every iteration (recon:109, 121, 123).  Nnum, when present, must equal
+ * N.  meta may be NULL. */
+mlf_status mlf_psf_load_mat(mlf_context ctx, const char* path, mlf_psf* out, mlf_psf_file_info* meta);
+
 /* nviews, Z, and per-plane radii (arrays of length Z may be NULL). */
 mlf_status mlf_psf_info(mlf_psf psf, int64_t* nviews, int64_t* Z, int32_t* rf, int32_t* rb);
 /* Active views in visiting order: uv[2*k], uv[2*k+1] (0-based), weight[k]. */
@@ -184,8 +227,11 @@ mlf_status mlf_batch_last_run_ms(mlf_batch b, double* ms);
 /* -------------------------------------------------------------- autotuning */
 /* No counterpart in the reference.  The view-step kernels are built in
  * several tile shapes: the forward projection's input columns per wave
- * (BF16: 4 or 2, FP32: 2 or 1; a workgroup covers 16 x 4*columns lenslets)
- * and the backprojection's 16-phase tiles per workgroup (5 or 3).
+ * (BF16: 4 or 2, FP32: 2 or 1; a workgroup covers 16 x 4*columns lenslets),
+ * the backprojection's 16-phase tiles per phase group (5 or 3: 3 or 5 groups
+ * cover the 225 phases) and its phase groups per work entry (0: all groups
+ * of a lenslet tile in one entry, which builds the tile's ratio window once;
+ * 1: one group per entry).
  * Autotuning times every shape on `views` view steps of the patch (or of all
  * patches of the batch, in its shared launches), restores the volumes, and
  * keeps the fastest.  Results are cached per precision, lenslet grid(s) and
@@ -195,6 +241,7 @@ mlf_status mlf_batch_last_run_ms(mlf_batch b, double* ms);
 typedef struct {
   int32_t fwd_columns_per_wave;
   int32_t bwd_phase_tiles;
+  int32_t bwd_groups_per_entry;
   double ms_per_view;  /* measured view step (forward + ratio + backprojection) */
 } mlf_tile_shape;
 mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best);

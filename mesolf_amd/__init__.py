@@ -5,7 +5,8 @@ built in-tree by ``python -m mesolf_amd.build``.  This package is the thin
 host-side mirror of the reference's MATLAB-facing interface.
 """
 from .lib import (  # noqa: F401
-    Batch, BudgetExceeded, Cancelled, Context, InvalidInput, MesolfError, Patch, PSF, Server, Unsupported,
+    Batch, BudgetExceeded, Cancelled, Context, InvalidInput, MatFile, MatFileError, MesolfError, Patch, PSF, Server,
+    Unsupported,
     compact_block_layout, exported_symbols, lfm_convolution_segments, lfm_convolution_segments_device,
     load_library,
     mex_lfm_convolution_vec, LFM_reconstruction_module, summary_image_generation_module,
@@ -16,6 +17,7 @@ from .lib import (  # noqa: F401
 
 __all__ = [
     "Context", "PSF", "Patch", "Batch", "Server", "MesolfError", "InvalidInput", "Unsupported", "BudgetExceeded", "Cancelled",
+    "MatFile", "MatFileError",
     "load_library", "exported_symbols", "compact_block_layout",
     "recon_theory_phase_space_LFM_with_filter", "mex_lfm_convolution_vec",
     "lfm_convolution_segments", "lfm_convolution_segments_device", "LFM_reconstruction_module", "summary_image_generation_module",

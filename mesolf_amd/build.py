@@ -28,7 +28,7 @@ COMMON = [
 ]
 
 SOURCES = ["rl_kernels.hip", "rl_mfma.hip", "mex_kernels.hip", "post.hip", "summary.hip", "runtime.cpp",
-           "batch.cpp", "server.cpp", "tune.cpp", "noalloc.hip"]
+           "batch.cpp", "server.cpp", "tune.cpp", "noalloc.hip", "mat73.cpp"]
 
 
 def _compile(src: str) -> str:
@@ -46,7 +46,7 @@ def build(verbose: bool = False) -> str:
     with ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
         objs = list(ex.map(_compile, SOURCES))
     tmp = OUT + ".tmp"
-    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs]
+    cmd = [HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lz"]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"link failed:\n{' '.join(cmd)}\n{r.stdout}\n{r.stderr}")

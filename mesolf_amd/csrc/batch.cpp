@@ -33,6 +33,7 @@ struct mlf_batch_s {
   bool tpz = false;
   int cw = 16, bm2 = 8;
   int fwd_mt = 0, bwd_nt = 0;  // tile shape shared by every attached patch
+  int bwd_grp = 0;
   int grid_f = 0, grid_b = 0, grid_r = 0;
   size_t smem_f = 0, smem_b = 0;
   int fwd_nbuf = 2;
@@ -106,7 +107,7 @@ mlf_status rebuild(mlf_batch_s* b) {
   std::stable_sort(be.begin(), be.end(), by_r);
   std::vector<int32_t> fs, bs;
   std::vector<int> fg, bg;  // XCD groups: (slot, plane) and (slot, plane, phase group)
-  const int ngrp = kPhiPadBwd / (16 * std::max(1, b->bwd_nt));
+  const int ngrp = kPhiPadBwd / (16 * std::max(1, b->bwd_nt));  // XCD groups: entries start at a phase group
   for (const Ent& e : fe) {
     fs.insert(fs.end(), e.v, e.v + 4);
     fg.push_back(e.v[0] * ps->Z + e.v[1]);
@@ -153,7 +154,7 @@ mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st, int nviews = 0) {
       MLF_LAUNCH(launch_ratio_gather(g, rg, b->grid_r, ps->Z, b->bm2, u, v, 1, nullptr, st));
     }
     const BwdGeo bg{b->d_slots.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, b->d_bsched.ptr,
-                    b->d_counts.ptr + 1, k};
+                    b->d_counts.ptr + 1, k, bwd_groups_per_entry(b->bwd_nt, b->bwd_grp)};
     MLF_LAUNCH(launch_bwd_mfma(b->kp, b->bwd_nt, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg,
                                b->grid_b, b->smem_b, (float)ps->weight[k], st));
   }
@@ -283,20 +284,22 @@ mlf_status mlf_batch_attach(mlf_batch b, mlf_patch p) {
     return set_error(MLF_ERR_BUDGET, "batch is full (%d patches)", b->capacity);
   MLF_CHECK(ctx_activate(b->psf->ctx));
   if (b->slots.empty()) {
-    if (b->fwd_mt != p->fwd_mt || b->bwd_nt != p->bwd_nt) drop_graph(b);
+    if (b->fwd_mt != p->fwd_mt || b->bwd_nt != p->bwd_nt || b->bwd_grp != p->bwd_grp) drop_graph(b);
     b->tpz = p->tpz;
     b->cw = p->cw;
     b->bm2 = p->bm2;
     b->fwd_mt = p->fwd_mt;
     b->bwd_nt = p->bwd_nt;
+    b->bwd_grp = p->bwd_grp;
     b->smem_f = p->smem_f;
     b->smem_b = p->smem_b;
     b->fwd_nbuf = p->fwd_nbuf;
-  } else if (p->fwd_mt != b->fwd_mt || p->bwd_nt != b->bwd_nt) {
+  } else if (p->fwd_mt != b->fwd_mt || p->bwd_nt != b->bwd_nt || p->bwd_grp != b->bwd_grp) {
     // one launch serves every patch: the patch takes the batch's tile shape
     MLF_HIP(hipStreamSynchronize(b->psf->ctx->stream));
     p->fwd_mt = b->fwd_mt;
     p->bwd_nt = b->bwd_nt;
+    p->bwd_grp = b->bwd_grp;
     MLF_CHECK(patch_setup_tiles(p));
     if (p->volume_set) MLF_CHECK(patch_refresh_shadow(p));
   }
@@ -376,17 +379,19 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
     MLF_CHECK(save.back()->alloc(p->X.count));
     MLF_HIP(hipMemcpyAsync(save.back()->ptr, p->X.ptr, p->X.bytes(), hipMemcpyDeviceToDevice, st));
   }
-  auto apply = [&](int mt, int nt) -> mlf_status {
+  auto apply = [&](const TileShape& t) -> mlf_status {
     MLF_HIP(hipStreamSynchronize(st));
     for (mlf_patch_s* p : b->slots) {
-      p->fwd_mt = mt;
-      p->bwd_nt = nt;
+      p->fwd_mt = t.mt;
+      p->bwd_nt = t.nt;
+      p->bwd_grp = t.grp;
       MLF_CHECK(patch_setup_tiles(p));
       MLF_CHECK(patch_refresh_shadow(p));
     }
     mlf_patch_s* p0 = b->slots[0];
-    b->fwd_mt = mt;
-    b->bwd_nt = nt;
+    b->fwd_mt = t.mt;
+    b->bwd_nt = t.nt;
+    b->bwd_grp = t.grp;
     b->cw = p0->cw;
     b->smem_f = p0->smem_f;
     b->smem_b = p0->smem_b;
@@ -405,7 +410,7 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
   MLF_HIP(hipEventCreate(&e0));
   MLF_HIP(hipEventCreate(&e1));
   double best_ms = 1e300;
-  int bmt = b->fwd_mt, bnt = b->bwd_nt;
+  TileShape bt{b->fwd_mt, b->bwd_nt, b->bwd_grp};
   mlf_status s = MLF_OK;
   // two rounds over the candidates, each keeping its faster round (shapes a
   // few per cent apart are otherwise decided by clock and cache noise)
@@ -414,7 +419,7 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
   for (int round = 0; round < 2 && s == MLF_OK; ++round) {
     for (size_t ci = 0; ci < cands.size(); ++ci) {
       const auto& c = cands[ci];
-      if (apply(c.first, c.second) != MLF_OK) continue;  // shape does not fit this PSF
+      if (apply(c) != MLF_OK) continue;  // shape does not fit this PSF
       s = enqueue_views(b, st, 1);  // warm-up
       if (s == MLF_OK && hipEventRecord(e0, st) != hipSuccess) s = set_error(MLF_ERR_DEVICE, "event");
       if (s == MLF_OK) s = enqueue_views(b, st, nv);
@@ -430,23 +435,23 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
   for (size_t ci = 0; ci < cands.size(); ++ci) {
     if (cand_ms[ci] < best_ms) {
       best_ms = cand_ms[ci];
-      bmt = cands[ci].first;
-      bnt = cands[ci].second;
+      bt = cands[ci];
     }
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   MLF_CHECK(s);
-  MLF_CHECK(apply(bmt, bnt));
+  MLF_CHECK(apply(bt));
   std::vector<std::pair<int, int>> grids;
   for (mlf_patch_s* p : b->slots) grids.emplace_back(p->n1, p->n2);
   std::sort(grids.begin(), grids.end());
   std::string key = tune_key(b->slots[0], "batch");
   for (const auto& g : grids) key += "|" + std::to_string(g.first) + "x" + std::to_string(g.second);
-  tune_cache_put(key, bmt, bnt, best_ms);
+  tune_cache_put(key, bt, best_ms);
   if (best) {
-    best->fwd_columns_per_wave = bmt;
-    best->bwd_phase_tiles = bnt;
+    best->fwd_columns_per_wave = bt.mt;
+    best->bwd_phase_tiles = bt.nt;
+    best->bwd_groups_per_entry = bt.grp;
     best->ms_per_view = best_ms;
   }
   return MLF_OK;

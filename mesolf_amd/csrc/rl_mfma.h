@@ -81,9 +81,10 @@ struct BwdGeo {
   const int32_t* r;       // [Z] backward radius
   const int64_t* boff;    // [Z] element offset of the plane block in the packed view
   const int32_t* epad;    // [Z] padded tap count (multiple of kBwdTapPad / kBwdTapPad16)
-  const int32_t* sched;   // [entries][5] = (slot, z, b1, b2, phase group)
+  const int32_t* sched;   // [entries][5] = (slot, z, b1, b2, first phase group)
   const int32_t* count;
   int view;               // selects the normaliser table of the view
+  int groups;             // phase groups per entry (they share its ratio window)
 };
 
 struct PackGeo {

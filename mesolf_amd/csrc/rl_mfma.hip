@@ -157,6 +157,15 @@ __device__ __forceinline__ void wait_vm() {
   static_assert(N >= 0 && N < 64, "vmcnt range");
   __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
 }
+// Workgroup barrier that waits only for this wave's LDS operations.
+// __syncthreads() is a workgroup fence plus s_barrier, and the fence waits for
+// every outstanding vector-memory operation (vmcnt(0)) because LDS DMA writes
+// are counted there; a caller that has already waited for the copies other
+// waves will read (wait_vm_le) keeps its younger loads in flight.
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0); vmcnt and expcnt not waited for
+  __builtin_amdgcn_s_barrier();
+}
 __device__ __forceinline__ void wait_vm_le(int n) {
   switch (n) {
 #define MLF_W(k) case k: wait_vm<k>(); break;
@@ -658,9 +667,8 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
 
 // Ratio of one lenslet (recon:116-117): R = LF(u::15, v::15) ./ y with
 // non-finite -> 0; y is the forward projection summed in FP64.
-__device__ __forceinline__ void ratio_out(const SlotDesc& sd, int i, int i1, int i2, double y,
+__device__ __forceinline__ void ratio_out(const SlotDesc& sd, int i, int i1, int i2, float yf,
                                           int u, int v, int use_lf, float* y_out) {
-  const float yf = (float)y;
   if (use_lf) {
     const float lf = sd.LF[(size_t)(u + kN * i1) + (size_t)(kN * sd.n1) * (v + kN * i2)];
     float q = lf / yf;
@@ -671,14 +679,14 @@ __device__ __forceinline__ void ratio_out(const SlotDesc& sd, int i, int i1, int
 }
 
 // Gather for the Toeplitz partials: y[i] = sum_z sum_{b2} window(b1 = i1/16, b2)
-// in FP64, then the ratio.  A workgroup takes kRatioLens lenslets; its
+// in FP32, then the ratio (FP32, recon:116-117).  A workgroup takes kRatioLens lenslets; its
 // kRatioSlices thread groups sum disjoint plane ranges, and the slice sums
 // are added in slice order (a fixed order: the result does not depend on
 // timing or on the other patches of a batch).
 __global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int Z, int CW, int u,
                                                           int v, int use_lf, float* __restrict__ y_out) {
   static_assert(kRatioLens * kRatioSlices == 256, "one thread per (lenslet, slice)");
-  __shared__ double part[kRatioSlices][kRatioLens];
+  __shared__ float part[kRatioSlices][kRatioLens];
   const int li = threadIdx.x % kRatioLens, sl = threadIdx.x / kRatioLens;
   const int z0 = sl * Z / kRatioSlices, z1 = (sl + 1) * Z / kRatioSlices;
   const int nent = uni(*rg.count);
@@ -693,9 +701,9 @@ __global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg,
     // The slice's planes go in chunks of kZC: the radii and offsets of a chunk
     // are read together, then up to kW windows per plane (all of them for
     // 16-column blocks, r <= 23), and only then summed, in plane then window
-    // order (FP64; an unused slot adds +0.0, which changes no sum).  A plane
-    // with more windows (narrower blocks) reads the rest one by one.
-    double y = 0.0;
+    // order (an unused slot adds +0.0, which changes no sum).  A plane with
+    // more windows (narrower blocks) reads the rest one by one.
+    float y = 0.f;
     if (ok) {
       constexpr int kZC = 8, kW = 4;
       for (int zc = z0; zc < z1; zc += kZC) {
@@ -726,19 +734,19 @@ __global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg,
         for (int q = 0; q < kZC; ++q) {
           if (zc + q >= z1) break;
   #pragma unroll
-          for (int wi = 0; wi < kW; ++wi) y += (double)vv[q][wi];
+          for (int wi = 0; wi < kW; ++wi) y += vv[q][wi];
           const int r = rr[q], Wc2 = CW + 2 * r;
           const int b2h = min(nb2 - 1, (i2 + r) / CW);
           const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
           for (int b2 = b2l + kW; b2 <= b2h; ++b2)
-            y += (double)gload(pz[q] + (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r));
+            y += gload(pz[q] + (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r));
         }
       }
     }
     part[sl][li] = y;
     __syncthreads();
     if (sl == 0 && ok) {
-      double t = part[0][li];
+      float t = part[0][li];
   #pragma unroll
       for (int q = 1; q < kRatioSlices; ++q) t += part[q][li];
       ratio_out(sd, i, i1, i2, t, u, v, use_lf, y_out);
@@ -767,12 +775,12 @@ __global__ void k_pack_fwd_tpz(const float* __restrict__ src, T* __restrict__ ds
   }
 }
 
-// y[i] = sum over planes and covering blocks of the window partials (FP64,
+// y[i] = sum over planes and covering blocks of the window partials (FP32,
 // fixed order: plane slices as in k_ratio_gather_tpz), then the ratio
 // (Q-scatter forward).
 __global__ __launch_bounds__(256) void k_ratio_gather(FwdGeo g, RatioGeo rg, int Z, int BM2, int u, int v,
                                                       int use_lf, float* __restrict__ y_out) {
-  __shared__ double part[kRatioSlices][kRatioLens];
+  __shared__ float part[kRatioSlices][kRatioLens];
   const int li = threadIdx.x % kRatioLens, sl = threadIdx.x / kRatioLens;
   const int z0 = sl * Z / kRatioSlices, z1 = (sl + 1) * Z / kRatioSlices;
   const int nent = uni(*rg.count);
@@ -783,7 +791,7 @@ __global__ __launch_bounds__(256) void k_ratio_gather(FwdGeo g, RatioGeo rg, int
     const bool ok = i < n1 * n2;
     const int i1 = ok ? i % n1 : 0, i2 = ok ? i / n1 : 0;
     const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, BM2);
-    double y = 0.0;
+    float y = 0.f;
     if (ok) {
       for (int z = z0; z < z1; ++z) {
         const int r = g.r[z];  // the plane slice differs between the half-waves
@@ -795,14 +803,13 @@ __global__ __launch_bounds__(256) void k_ratio_gather(FwdGeo g, RatioGeo rg, int
         const float* pz = sd.partial + sd.poff[z];
         for (int b2 = b2l; b2 <= b2h; ++b2)
           for (int b1 = b1l; b1 <= b1h; ++b1)
-            y += (double)pz[(size_t)(b2 * nb1 + b1) * W1 * W2 + (i2 - BM2 * b2 + r) * W1 +
-                            (i1 - 16 * b1 + r)];
+            y += pz[(size_t)(b2 * nb1 + b1) * W1 * W2 + (i2 - BM2 * b2 + r) * W1 + (i1 - 16 * b1 + r)];
       }
     }
     part[sl][li] = y;
     __syncthreads();
     if (sl == 0 && ok) {
-      double t = part[0][li];
+      float t = part[0][li];
   #pragma unroll
       for (int q = 1; q < kRatioSlices; ++q) t += part[q][li];
       ratio_out(sd, i, i1, i2, t, u, v, use_lf, y_out);
@@ -860,7 +867,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     T* __restrict__ Xsh = reinterpret_cast<T*>(sd.Xb);
     const float* __restrict__ R = sd.R;
     const int n1 = sd.n1, n2 = sd.n2;
-    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]), grp = uni(s[4]);
+    // phase groups grp0 .. grp0 + groups - 1 share the entry's ratio window
+    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]), grp0 = uni(s[4]);
     const int r = uni(g.r[z]);
     const int K = 2 * r + 1, ntaps = K * K;
     const int epad = uni(g.epad[z]);
@@ -916,7 +924,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         }
         toff[e] = off;
       }
-      __syncthreads();
+      lds_barrier();
     } else {
       // 16-bit: taps ordered (e2, e1) with e1 padded to a multiple of 8, so a
       // lane's 8 taps of a K-step are 8 consecutive e1 in one row.  Cell (x, y)
@@ -924,7 +932,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // fragment is a single 16-byte LDS read.
       float* Rs = reinterpret_cast<float*>(smem);  // [Wd][Wd]
       gather_window(R, n1, n2, 16 * b1 - r, 16 * b2 - r, Wd, Rs);
-      __syncthreads();
+      lds_barrier();
       // four cells per thread per pass, their 32 reads issued together
       for (int c0 = 0; c0 < Wd * Wd; c0 += 4 * 256) {
         float rv[4][8];
@@ -950,10 +958,11 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         }
       }
-      __syncthreads();
+      lds_barrier();  // Rv written, Rs read by every wave (the ring copies overwrite Rs)
     }
 
-    {  // the entry's phase group: 16 * NT phases from 16 * NT * grp
+    for (int gi = 0; gi < g.groups; ++gi) {  // phase group grp: 16 * NT phases from 16 * NT * grp
+      const int grp = grp0 + gi;
       const T* Bz = Wb + uni64(g.boff[z]) + (size_t)(grp * 16 * NT) * epad;
       floatx4 acc[MT][NT];
   #pragma unroll
@@ -1045,7 +1054,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           // this wave's copies of step ks landed (the epilogue loads are
           // younger than the slices of steps 0 .. kBwdStages - 2)
           wait_vm_le((kBwdStages - 2) * ncopy + (ks < kBwdStages - 1 ? kEpiLoads : 0));
-          __syncthreads();  // everyone's copies landed; everyone is done with step ks - 1
+          lds_barrier();  // everyone's copies landed; everyone is done with step ks - 1
           issue_taps(ks + kBwdStages - 1, (ks + kBwdStages - 1) % kBwdStages);
           const T* Bs = Bq + (ks % kBwdStages) * kSlice;
           const unsigned t0 = 32u * ks + 8u * lg;
@@ -1125,8 +1134,13 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         }
       }
+      // The ring is refilled by the next group and LDS reused by the next
+      // entry.  No tap copy is in flight here: the epilogue waited for its
+      // last X and 1/B1 loads, which were issued after every copy, and
+      // vector-memory operations retire in order; only this group's stores
+      // may still be in flight, and they need not be waited for.
+      lds_barrier();
     }
-    __syncthreads();  // LDS is reused by the next entry
   }
 }
 

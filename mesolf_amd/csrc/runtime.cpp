@@ -14,6 +14,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.h"
@@ -217,6 +218,7 @@ const char* mlf_status_string(mlf_status s) {
     case MLF_ERR_BUDGET: return "HBM budget exceeded";
     case MLF_ERR_INTERNAL: return "internal error";
     case MLF_ERR_CANCELLED: return "cancelled";
+    case MLF_ERR_IO: return "file error";
   }
   return "unknown status";
 }
@@ -441,6 +443,49 @@ static mlf_status psf_upload(mlf_psf_s* p, const float* fwd, const float* bwd) {
   return MLF_OK;
 }
 
+// Run f(i) for i in [0, n) on up to hardware_concurrency threads (static
+// interleaved split, so every thread gets a mix of large and small items).
+template <typename F>
+static void parallel_for(int64_t n, F&& f) {
+  const int64_t nt = std::max<int64_t>(1, std::min<int64_t>(n, std::min(64u, std::max(1u, std::thread::hardware_concurrency()))));
+  if (nt == 1) {
+    for (int64_t i = 0; i < n; ++i) f(i);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int64_t t = 0; t < nt; ++t)
+    th.emplace_back([&, t] {
+      for (int64_t i = t; i < n; i += nt) f(i);
+    });
+  for (auto& x : th) x.join();
+}
+
+// 64-bit content hash of n floats: per-chunk hashes (parallel) folded in
+// chunk order.  Used only to recognise an identical PSF between calls.
+static uint64_t content_hash(const float* p, size_t n) {
+  constexpr size_t kChunk = (size_t)1 << 22;
+  const size_t nc = (n + kChunk - 1) / kChunk;
+  std::vector<uint64_t> hc(nc);
+  parallel_for((int64_t)nc, [&](int64_t c) {
+    const uint32_t* w = reinterpret_cast<const uint32_t*>(p) + (size_t)c * kChunk;
+    const size_t m = std::min(kChunk, n - (size_t)c * kChunk);
+    uint64_t h0 = 0x9E3779B97F4A7C15ull ^ (uint64_t)c, h1 = 0xC2B2AE3D27D4EB4Full, h2 = 0x165667B19E3779F9ull,
+             h3 = 0x27D4EB2F165667C5ull;
+    size_t i = 0;
+    for (; i + 4 <= m; i += 4) {
+      h0 = (h0 ^ w[i]) * 0x100000001B3ull;
+      h1 = (h1 ^ w[i + 1]) * 0x100000001B3ull;
+      h2 = (h2 ^ w[i + 2]) * 0x100000001B3ull;
+      h3 = (h3 ^ w[i + 3]) * 0x100000001B3ull;
+    }
+    for (; i < m; ++i) h0 = (h0 ^ w[i]) * 0x100000001B3ull;
+    hc[c] = h0 ^ (h1 << 1 | h1 >> 63) ^ (h2 << 2 | h2 >> 62) ^ (h3 << 3 | h3 >> 61);
+  });
+  uint64_t h = 0xCBF29CE484222325ull ^ n;
+  for (uint64_t x : hc) h = (h ^ x) * 0x100000001B3ull + (h >> 29);
+  return h;
+}
+
 // Exact adjoint of a forward table: bwd[d] = fwd[-d] per plane.
 static void adjoint_tables(const std::vector<int32_t>& rf, int V, const float* fwd,
                            std::vector<float>* bwd) {
@@ -532,6 +577,13 @@ mlf_status mlf_psf_create_compact(mlf_context ctx, int64_t N, int64_t Z, int64_t
 // (closed form of the de-shift + reshape/permute chain), and the backward
 // table reads the cropped Ht at the view's own (u, v) slot (recon:121):
 //   bwd[d1,d2,s1,s2] = Htn[i0 + 15(d1+R) + c0 + s1 - u, i0 + 15(d2+R) + c0 + s2 - v, u, v, z].
+// Every raw element Hcrop[x, y, a, b, z] therefore belongs to exactly one
+// view, u = (x + a - c0) mod 15, at lenslet offset d1 = (x + a - c0) / 15 - R
+// (same along y), so the view weights (recon:30: W(u,v) is the sum of
+// H_ps(:,:,u,v,:)) and each view's extent per plane come from one streaming
+// pass over H; likewise for Ht, whose (u, v) slot is the view.  The passes run
+// in parallel over planes (each plane's sums in a fixed order, planes added
+// in order), the table fills in parallel over views.
 mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht, int64_t S,
                                 int64_t N, int64_t Z, mlf_psf* out) {
   clear_error();
@@ -544,48 +596,70 @@ mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht
   const int64_t i0 = matlab_round(S / 2.0) - (int64_t)std::trunc(IMG / 2.0) - 1;
   const int R = (int)((IMG / N - 1) / 2);
   const size_t SS = (size_t)S * S;
-  auto at = [&](const float* A, int64_t x, int64_t y, int a, int b, int z) -> double {
-    return (double)A[(size_t)x + (size_t)S * y + SS * ((size_t)a + (size_t)N * ((size_t)b + (size_t)N * z))];
-  };
-  // per-depth sums (recon:13-16), FP64
-  std::vector<double> hs(Z, 0.0), hts(Z, 0.0);
   const size_t slab = SS * N * N;
-  for (int64_t z = 0; z < Z; ++z) {
-    double a = 0.0, b = 0.0;
-    for (size_t o = 0; o < slab; ++o) {
-      a += H[o + slab * z];
-      b += Ht[o + slab * z];
-    }
-    hs[z] = a;
-    hts[z] = b;
-  }
-  const int Kf = 2 * R + 1, Kb = 2 * R + 3;
-  // full forward tables for all 225 views: needed for the weights anyway
+  auto plane_of = [&](const float* A, int a, int b, int64_t z) {
+    return A + SS * ((size_t)a + (size_t)N * ((size_t)b + (size_t)N * (size_t)z));
+  };
+  // per-depth sums (recon:13-16), FP64, each plane in a fixed order
+  std::vector<double> hs(Z, 0.0), hts(Z, 0.0);
+  parallel_for(2 * Z, [&](int64_t q) {
+    const int64_t z = q % Z;
+    const float* A = (q < Z ? H : Ht) + slab * z;
+    double acc = 0.0;
+    for (size_t o = 0; o < slab; ++o) acc += A[o];
+    (q < Z ? hs : hts)[z] = acc;
+  });
+  // one pass per plane: weights (sum of H_ps(:,:,u,v,z)) and, per view, the
+  // largest lenslet offset holding a non-zero tap (forward and backward)
+  std::vector<double> wz((size_t)Z * kP, 0.0);
+  std::vector<int32_t> rfz((size_t)Z * kP, -1), rbz((size_t)Z * kP, -1);
+  parallel_for(Z, [&](int64_t z) {
+    double* w = wz.data() + (size_t)z * kP;
+    int32_t* rf_uv = rfz.data() + (size_t)z * kP;
+    int32_t* rb_uv = rbz.data() + (size_t)z * kP;
+    const double inv = 1.0 / hs[z];
+    for (int b = 0; b < kN; ++b)
+      for (int a = 0; a < kN; ++a) {
+        const float* P = plane_of(H, a, b, z);
+        for (int64_t y = 0; y < IMG; ++y) {
+          const int64_t yp = y + b - kC0;
+          if (yp < 0 || yp >= IMG) continue;
+          const int v = (int)(yp % kN), d2 = (int)(yp / kN) - R;
+          const float* row = P + (size_t)(i0 + y) * S + i0;
+          for (int64_t x = 0; x < IMG; ++x) {
+            const int64_t xp = x + a - kC0;
+            if (xp < 0 || xp >= IMG) continue;
+            const float val = row[x];
+            const int u = (int)(xp % kN);
+            w[u + kN * v] += (double)val * inv;
+            if (val != 0.f) {
+              const int d = std::max(std::abs((int)(xp / kN) - R), std::abs(d2));
+              if (d > rf_uv[u + kN * v]) rf_uv[u + kN * v] = d;
+            }
+          }
+        }
+      }
+    // Ht: slot (u, v) is the view; x = 15 (d1 + R) + c0 + s1 - u, so
+    // d1 + R = floor((x - c0 + u) / 15) with x - c0 + u >= -7
+    for (int v = 0; v < kN; ++v)
+      for (int u = 0; u < kN; ++u) {
+        const float* P = plane_of(Ht, u, v, z);
+        int32_t m = -1;
+        for (int64_t y = 0; y < IMG; ++y) {
+          const int d2 = (int)std::floor((double)(y - kC0 + v) / kN) - R;
+          const float* row = P + (size_t)(i0 + y) * S + i0;
+          for (int64_t x = 0; x < IMG; ++x) {
+            if (row[x] == 0.f) continue;
+            const int d1 = (int)std::floor((double)(x - kC0 + u) / kN) - R;
+            m = std::max(m, std::max(std::abs(d1), std::abs(d2)));
+          }
+        }
+        rb_uv[u + kN * v] = m;
+      }
+  });
   std::vector<double> wsum(kP, 0.0);
-  auto fwd_val = [&](int u, int v, int z, int d1, int d2, int s1, int s2) -> double {
-    const int64_t x = u + (int64_t)N * (d1 + R) - s1 + kC0;
-    const int64_t y = v + (int64_t)N * (d2 + R) - s2 + kC0;
-    if (x < 0 || x >= IMG || y < 0 || y >= IMG) return 0.0;
-    return at(H, i0 + x, i0 + y, s1, s2, z) / hs[z];
-  };
-  auto bwd_val = [&](int u, int v, int z, int d1, int d2, int s1, int s2) -> double {
-    const int64_t x = (int64_t)N * (d1 + R) + kC0 + s1 - u;
-    const int64_t y = (int64_t)N * (d2 + R) + kC0 + s2 - v;
-    if (x < 0 || x >= IMG || y < 0 || y >= IMG) return 0.0;
-    return at(Ht, i0 + x, i0 + y, u, v, z) / hts[z];
-  };
-  // W(u,v) = sum over H_ps(:,:,u,v,:) (recon:30): H_ps covers exactly the
-  // (d in [-R,R], s) grid of the forward table.
-  for (int u = 0; u < kN; ++u)
-    for (int v = 0; v < kN; ++v) {
-      double s = 0.0;
-      for (int z = 0; z < Z; ++z)
-        for (int d2 = -R; d2 <= R; ++d2)
-          for (int d1 = -R; d1 <= R; ++d1)
-            for (int s2 = 0; s2 < kN; ++s2)
-              for (int s1 = 0; s1 < kN; ++s1) s += fwd_val(u, v, z, d1, d2, s1, s2);
-      wsum[u + kN * v] = s;
-    }
+  for (int64_t z = 0; z < Z; ++z)
+    for (int k = 0; k < kP; ++k) wsum[k] += wz[(size_t)z * kP + k];
   // weights indexed [u + 15 v]
   std::vector<double> w = view_weights(wsum, 0.2);
   std::vector<int32_t> uv;
@@ -602,57 +676,59 @@ mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht
   }
   const int V = (int)wv.size();
   if (V == 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "no active view");
-  // per-plane radii from nonzero extents over active views
+  // per-plane radii: the widest active view
   std::vector<int32_t> rf(Z, 0), rb(Z, 0);
-  for (int z = 0; z < Z; ++z) {
+  for (int64_t z = 0; z < Z; ++z)
     for (int k = 0; k < V; ++k) {
-      const int u = uv[2 * k], v = uv[2 * k + 1];
-      for (int d2 = -R; d2 <= R; ++d2)
-        for (int d1 = -R; d1 <= R; ++d1) {
-          const int ad = std::max(std::abs(d1), std::abs(d2));
-          if (ad <= rf[z]) continue;
-          for (int s = 0; s < kP && ad > rf[z]; ++s)
-            if (fwd_val(u, v, z, d1, d2, s % kN, s / kN) != 0.0) rf[z] = ad;
-        }
-      for (int d2 = -R - 1; d2 <= R + 1; ++d2)
-        for (int d1 = -R - 1; d1 <= R + 1; ++d1) {
-          const int ad = std::max(std::abs(d1), std::abs(d2));
-          if (ad <= rb[z]) continue;
-          for (int s = 0; s < kP && ad > rb[z]; ++s)
-            if (bwd_val(u, v, z, d1, d2, s % kN, s / kN) != 0.0) rb[z] = ad;
-        }
+      const int q = uv[2 * k] + kN * uv[2 * k + 1];
+      rf[z] = std::max(rf[z], rfz[(size_t)z * kP + q]);
+      rb[z] = std::max(rb[z], rbz[(size_t)z * kP + q]);
     }
-  }
-  (void)Kf;
-  (void)Kb;
   int64_t pvf = 0, pvb = 0;
-  for (int z = 0; z < Z; ++z) {
+  std::vector<int64_t> offf(Z), offb(Z);
+  for (int64_t z = 0; z < Z; ++z) {
+    offf[z] = pvf;
+    offb[z] = pvb;
     pvf += plane_block(rf[z]);
     pvb += plane_block(rb[z]);
   }
   std::vector<float> tf((size_t)pvf * V), tb((size_t)pvb * V);
-  for (int k = 0; k < V; ++k) {
+  parallel_for(V, [&](int64_t k) {
     const int u = uv[2 * k], v = uv[2 * k + 1];
-    int64_t of = (int64_t)k * pvf, ob = (int64_t)k * pvb;
-    for (int z = 0; z < Z; ++z) {
+    for (int64_t z = 0; z < Z; ++z) {
+      // forward: rows of the raw (s1, s2) slab, d1 innermost (stride 15 px)
       const int r = rf[z], K = 2 * r + 1;
-      for (int d2 = -r; d2 <= r; ++d2)
-        for (int d1 = -r; d1 <= r; ++d1)
-          for (int s2 = 0; s2 < kN; ++s2)
-            for (int s1 = 0; s1 < kN; ++s1)
-              tf[of + ((int64_t)(d2 + r) * K + (d1 + r)) * kP + s2 * kN + s1] =
-                  (float)fwd_val(u, v, z, d1, d2, s1, s2);
-      of += plane_block(r);
+      float* F = tf.data() + (size_t)k * pvf + offf[z];
+      for (int s2 = 0; s2 < kN; ++s2)
+        for (int s1 = 0; s1 < kN; ++s1) {
+          const float* P = plane_of(H, s1, s2, z);
+          for (int d2 = -r; d2 <= r; ++d2) {
+            const int64_t y = v + (int64_t)kN * (d2 + R) - s2 + kC0;
+            for (int d1 = -r; d1 <= r; ++d1) {
+              const int64_t x = u + (int64_t)kN * (d1 + R) - s1 + kC0;
+              const bool in = x >= 0 && x < IMG && y >= 0 && y < IMG;
+              F[((int64_t)(d2 + r) * K + (d1 + r)) * kP + s2 * kN + s1] =
+                  in ? (float)((double)P[(size_t)(i0 + y) * S + (i0 + x)] / hs[z]) : 0.f;
+            }
+          }
+        }
+      // backward: the view's own Ht slab, s1 innermost (contiguous)
       const int q = rb[z], Q = 2 * q + 1;
+      float* B = tb.data() + (size_t)k * pvb + offb[z];
+      const float* P = plane_of(Ht, u, v, z);
       for (int d2 = -q; d2 <= q; ++d2)
-        for (int d1 = -q; d1 <= q; ++d1)
-          for (int s2 = 0; s2 < kN; ++s2)
-            for (int s1 = 0; s1 < kN; ++s1)
-              tb[ob + ((int64_t)(d2 + q) * Q + (d1 + q)) * kP + s2 * kN + s1] =
-                  (float)bwd_val(u, v, z, d1, d2, s1, s2);
-      ob += plane_block(q);
+        for (int s2 = 0; s2 < kN; ++s2) {
+          const int64_t y = (int64_t)kN * (d2 + R) + kC0 + s2 - v;
+          for (int d1 = -q; d1 <= q; ++d1)
+            for (int s1 = 0; s1 < kN; ++s1) {
+              const int64_t x = (int64_t)kN * (d1 + R) + kC0 + s1 - u;
+              const bool in = x >= 0 && x < IMG && y >= 0 && y < IMG;
+              B[((int64_t)(d2 + q) * Q + (d1 + q)) * kP + s2 * kN + s1] =
+                  in ? (float)((double)P[(size_t)(i0 + y) * S + (i0 + x)] / hts[z]) : 0.f;
+            }
+        }
     }
-  }
+  });
   return mlf_psf_create_compact(ctx, N, Z, V, uv.data(), wv.data(), rf.data(), tf.data(),
                                 rb.data(), tb.data(), out);
 }
@@ -796,7 +872,7 @@ static BwdGeo bwd_geo(mlf_patch_s* p, int k) {
   mlf_psf_s* ps = p->psf;
   const PrecPack& pk = ps->pack[(int)p->kp];
   return BwdGeo{p->d_slot.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, p->d_bsched.ptr,
-                p->d_counts.ptr + 1, k};
+                p->d_counts.ptr + 1, k, bwd_groups_per_entry(p->bwd_nt, p->bwd_grp)};
 }
 
 // Forward projection of view k, then the ratio (LF != null) and/or y.
@@ -958,11 +1034,12 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   std::stable_sort(border.begin(), border.end(), [&](int a, int b) { return ps->rb[a] > ps->rb[b]; });
   const int mb1 = (p->n1 + 15) / 16, mb2 = (p->n2 + 15) / 16;
   const int ngrp = kPhiPadBwd / (16 * p->bwd_nt);
+  const int gpe = bwd_groups_per_entry(p->bwd_nt, p->bwd_grp);  // groups per entry share its window
   p->bent.clear();
   for (int z : border)
     for (int b2 = 0; b2 < mb2; ++b2)
       for (int b1 = 0; b1 < mb1; ++b1)
-        for (int gq = 0; gq < ngrp; ++gq) {
+        for (int gq = 0; gq < ngrp; gq += gpe) {
           p->bent.push_back(z);
           p->bent.push_back(b1);
           p->bent.push_back(b2);
@@ -1115,12 +1192,15 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
   p->reference = ref && ref[0] == '1' && prec == MLF_PRECISION_FP32;
   p->kp = kernel_precision(prec);
   if (!p->reference) {
-    p->fwd_mt = p->kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
-    p->bwd_nt = kBwdNTiles;
-    int mt = 0, nt = 0;
-    if (tune_cache_get(tune_key(p.get(), "patch"), p->kp, &mt, &nt)) {
-      p->fwd_mt = mt;
-      p->bwd_nt = nt;
+    const TileShape t0 = tile_candidates(p->kp)[0];
+    p->fwd_mt = t0.mt;
+    p->bwd_nt = t0.nt;
+    p->bwd_grp = t0.grp;
+    TileShape t;
+    if (tune_cache_get(tune_key(p.get(), "patch"), p->kp, &t)) {
+      p->fwd_mt = t.mt;
+      p->bwd_nt = t.nt;
+      p->bwd_grp = t.grp;
     }
     MLF_CHECK(patch_setup_tiles(p.get()));
   }
@@ -1395,6 +1475,7 @@ mlf_status mlf_patch_tile_shape(mlf_patch p, mlf_tile_shape* cur) {
   if (!p || !cur) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
   cur->fwd_columns_per_wave = p->fwd_mt;
   cur->bwd_phase_tiles = p->bwd_nt;
+  cur->bwd_groups_per_entry = p->bwd_grp;
   cur->ms_per_view = 0.0;
   return MLF_OK;
 }
@@ -1404,22 +1485,25 @@ mlf_status mlf_patch_set_tile_shape(mlf_patch p, const mlf_tile_shape* shape) {
   if (!p || !shape) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
   if (p->reference) return set_error(MLF_ERR_UNSUPPORTED, "reference kernels have one shape");
   if (p->batch) return set_error(MLF_ERR_INVALID_ARGUMENT, "detach the patch from its batch first");
+  const TileShape want{shape->fwd_columns_per_wave, shape->bwd_phase_tiles, shape->bwd_groups_per_entry};
   bool ok = false;
-  for (const auto& c : tile_candidates(p->kp))
-    ok = ok || (c.first == shape->fwd_columns_per_wave && c.second == shape->bwd_phase_tiles);
+  for (const auto& c : tile_candidates(p->kp)) ok = ok || c == want;
   if (!ok)
-    return set_error(MLF_ERR_INVALID_ARGUMENT, "tile shape (%d, %d) is not built for this precision",
-                     (int)shape->fwd_columns_per_wave, (int)shape->bwd_phase_tiles);
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "tile shape (%d, %d, %d) is not built for this precision",
+                     (int)shape->fwd_columns_per_wave, (int)shape->bwd_phase_tiles,
+                     (int)shape->bwd_groups_per_entry);
   std::lock_guard<std::mutex> lk(p->mu);
   MLF_CHECK(ctx_activate(p->psf->ctx));
   MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
-  const int mt = p->fwd_mt, nt = p->bwd_nt;
-  p->fwd_mt = shape->fwd_columns_per_wave;
-  p->bwd_nt = shape->bwd_phase_tiles;
+  const int mt = p->fwd_mt, nt = p->bwd_nt, gr = p->bwd_grp;
+  p->fwd_mt = want.mt;
+  p->bwd_nt = want.nt;
+  p->bwd_grp = want.grp;
   if (patch_setup_tiles(p) != MLF_OK) {
     const std::string e = last_error();
     p->fwd_mt = mt;
     p->bwd_nt = nt;
+    p->bwd_grp = gr;
     MLF_CHECK(patch_setup_tiles(p));
     return set_error(MLF_ERR_UNSUPPORTED, "%s", e.c_str());
   }
@@ -1450,7 +1534,7 @@ mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) 
   MLF_HIP(hipEventCreate(&e0.e));
   MLF_HIP(hipEventCreate(&e1.e));
   double best_ms = 1e300;
-  int bmt = p->fwd_mt, bnt = p->bwd_nt;
+  TileShape bt{p->fwd_mt, p->bwd_nt, p->bwd_grp};
   // two rounds over the candidates, each candidate keeping its faster round:
   // shapes a few per cent apart are otherwise decided by clock and cache noise
   const auto cands = tile_candidates(p->kp);
@@ -1459,8 +1543,9 @@ mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) 
     for (size_t ci = 0; ci < cands.size(); ++ci) {
       const auto& c = cands[ci];
       MLF_HIP(hipStreamSynchronize(st));
-      p->fwd_mt = c.first;
-      p->bwd_nt = c.second;
+      p->fwd_mt = c.mt;
+      p->bwd_nt = c.nt;
+      p->bwd_grp = c.grp;
       if (patch_setup_tiles(p) != MLF_OK) continue;  // shape does not fit this PSF
       MLF_CHECK(patch_refresh_shadow(p));
       MLF_CHECK(patch_view_step(p, 0));  // warm-up
@@ -1477,19 +1562,20 @@ mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) 
   for (size_t ci = 0; ci < cands.size(); ++ci) {
     if (cand_ms[ci] < best_ms) {
       best_ms = cand_ms[ci];
-      bmt = cands[ci].first;
-      bnt = cands[ci].second;
+      bt = cands[ci];
     }
   }
   MLF_HIP(hipStreamSynchronize(st));
-  p->fwd_mt = bmt;
-  p->bwd_nt = bnt;
+  p->fwd_mt = bt.mt;
+  p->bwd_nt = bt.nt;
+  p->bwd_grp = bt.grp;
   MLF_CHECK(patch_setup_tiles(p));
   MLF_CHECK(patch_refresh_shadow(p));
-  tune_cache_put(tune_key(p, "patch"), bmt, bnt, best_ms);
+  tune_cache_put(tune_key(p, "patch"), bt, best_ms);
   if (best) {
-    best->fwd_columns_per_wave = bmt;
-    best->bwd_phase_tiles = bnt;
+    best->fwd_columns_per_wave = bt.mt;
+    best->bwd_phase_tiles = bt.nt;
+    best->bwd_groups_per_entry = bt.grp;
     best->ms_per_view = best_ms;
   }
   return MLF_OK;
@@ -1513,12 +1599,60 @@ mlf_status mlf_recon_phase_space(const float* lfimg, int64_t I1, int64_t I2, con
                      (long long)I2);
   if (max_iter < 0) return set_error(MLF_ERR_INVALID_ARGUMENT, "maxIter < 0");
   const auto t0 = std::chrono::steady_clock::now();
-  mlf_context ctx = nullptr;
-  MLF_CHECK(mlf_context_create(device, &ctx));
-  std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> ctx_guard(ctx, mlf_context_destroy);
+  // The reference pipeline calls recon once per patch with the same H and Ht
+  // (LFM_reconstruction_module.m:37 inside mesolf.m's patch loop).  The last
+  // prepared PSF stays resident with its context and is reused when device,
+  // sizes and the contents of H and Ht (a 64-bit content hash of each) match;
+  // MLF_RECON_PSF_CACHE=0 turns this off.
+  struct PsfCache {
+    std::mutex mu;
+    mlf_context ctx = nullptr;
+    mlf_psf psf = nullptr;
+    int32_t device = -1;
+    int64_t S = 0, Z = 0;
+    uint64_t hH = 0, hHt = 0;
+  };
+  static PsfCache cache;
+  const char* env = std::getenv("MLF_RECON_PSF_CACHE");
+  const bool use_cache = !(env && env[0] == '0');
+  const size_t nH = (size_t)S * S * N * N * Z;
+  std::unique_lock<std::mutex> cache_lk(cache.mu, std::defer_lock);
   mlf_psf psf = nullptr;
-  MLF_CHECK(mlf_psf_create_dense(ctx, H, Ht, S, N, Z, &psf));
-  std::unique_ptr<mlf_psf_s, mlf_status (*)(mlf_psf)> psf_guard(psf, mlf_psf_destroy);
+  mlf_context ctx = nullptr;
+  std::unique_ptr<mlf_context_s, mlf_status (*)(mlf_context)> ctx_guard(nullptr, mlf_context_destroy);
+  std::unique_ptr<mlf_psf_s, mlf_status (*)(mlf_psf)> psf_guard(nullptr, mlf_psf_destroy);
+  if (use_cache) {
+    cache_lk.lock();  // one drop-in reconstruction at a time shares the cached PSF
+    const uint64_t hH = content_hash(H, nH), hHt = content_hash(Ht, nH);
+    if (!(cache.psf && cache.device == device && cache.S == S && cache.Z == Z && cache.hH == hH &&
+          cache.hHt == hHt)) {
+      if (cache.psf) (void)mlf_psf_destroy(cache.psf);
+      if (cache.ctx) (void)mlf_context_destroy(cache.ctx);
+      cache.psf = nullptr;
+      cache.ctx = nullptr;
+      MLF_CHECK(mlf_context_create(device, &cache.ctx));
+      const mlf_status cs = mlf_psf_create_dense(cache.ctx, H, Ht, S, N, Z, &cache.psf);
+      if (cs != MLF_OK) {
+        const std::string e = last_error();
+        (void)mlf_context_destroy(cache.ctx);
+        cache.ctx = nullptr;
+        cache.psf = nullptr;
+        return set_error(cs, "%s", e.c_str());
+      }
+      cache.device = device;
+      cache.S = S;
+      cache.Z = Z;
+      cache.hH = hH;
+      cache.hHt = hHt;
+    }
+    ctx = cache.ctx;
+    psf = cache.psf;
+  } else {
+    MLF_CHECK(mlf_context_create(device, &ctx));
+    ctx_guard.reset(ctx);
+    MLF_CHECK(mlf_psf_create_dense(ctx, H, Ht, S, N, Z, &psf));
+    psf_guard.reset(psf);
+  }
   mlf_patch p = nullptr;
   MLF_CHECK(mlf_patch_create(psf, I1 / N, I2 / N, prec, &p));
   std::unique_ptr<mlf_patch_s, mlf_status (*)(mlf_patch)> patch_guard(p, mlf_patch_destroy);

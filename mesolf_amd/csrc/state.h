@@ -104,6 +104,7 @@ struct mlf_patch_s {
   bool tpz = false;                    // Toeplitz forward kernel
   int cw = 16;                         // Toeplitz: input columns per workgroup (4 * fwd_mt)
   int fwd_mt = 0, bwd_nt = 0;          // tile shape: forward columns per wave, backward N-tiles
+  int bwd_grp = 0;                     // phase groups per backprojection entry (0: all of them)
   float xs = 1.f;                      // FP16: shadow = X * xs (chosen so that X * xs ~ 1)
   mlf::DevBuf<float> partial;               // forward window partials
   mlf::DevBuf<uint16_t> Xb;                 // BF16 shadow of X, [lenslet][kShadowRow] (BF16 Toeplitz)
@@ -162,13 +163,22 @@ mlf_status batch_forget_patch(mlf_patch_s* p);  // detach before destruction
 constexpr int kXcds = 8;
 void xcd_interleave(std::vector<int32_t>& ent, int width, const std::vector<int>& group);
 // (Re)build a patch's work entries, partial buffers and slot descriptor for
-// its tile shape (fwd_mt, bwd_nt); the caller synchronises the stream first.
+// its tile shape (fwd_mt, bwd_nt, bwd_grp); the caller synchronises the
+// stream first.
 mlf_status patch_setup_tiles(mlf_patch_s* p);
+// A tile shape: forward input columns per wave, backprojection 16-phase
+// tiles per group, phase groups per backprojection entry (0: all).
+struct TileShape {
+  int mt = 0, nt = 0, grp = 0;
+  bool operator==(const TileShape& o) const { return mt == o.mt && nt == o.nt && grp == o.grp; }
+};
 // Candidate tile shapes of a precision, default first.
-std::vector<std::pair<int, int>> tile_candidates(Precision kp);
+std::vector<TileShape> tile_candidates(Precision kp);
+// Phase groups per backprojection entry that a shape asks for (0 -> all).
+int bwd_groups_per_entry(int nt, int grp);
 // Tile shape cache (per process, optionally persisted in the file named by
 // MLF_TUNE_CACHE): key = precision, grid(s), PSF radii.
-bool tune_cache_get(const std::string& key, Precision kp, int* mt, int* nt);
-void tune_cache_put(const std::string& key, int mt, int nt, double ms);
+bool tune_cache_get(const std::string& key, Precision kp, TileShape* shape);
+void tune_cache_put(const std::string& key, const TileShape& shape, double ms);
 std::string tune_key(const mlf_patch_s* p, const char* scope);
 }  // namespace mlf

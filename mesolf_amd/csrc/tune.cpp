@@ -4,7 +4,7 @@
 // fastest measured tile shape.  It lives for the process; if MLF_TUNE_CACHE
 // names a file, entries are loaded from it when the name is first seen and
 // appended to it when measured, one line per entry:
-//   "mlf-tune-v2 <key> <fwd_mt> <bwd_nt> <ms per view>".
+//   "mlf-tune-v3 <key> <fwd_mt> <bwd_nt> <bwd_groups> <ms per view>".
 // Lines without the format tag (older builds) are ignored, and a cached shape
 // is used only if it is one of the shapes this build has for the patch's
 // precision: a stale or edited entry falls back to the default shape.
@@ -25,9 +25,9 @@
 namespace mlf {
 
 namespace {
-constexpr const char* kTuneTag = "mlf-tune-v2";
+constexpr const char* kTuneTag = "mlf-tune-v3";
 std::mutex g_tune_mu;
-std::map<std::string, std::pair<int, int>> g_tune;
+std::map<std::string, TileShape> g_tune;
 std::string g_tune_loaded_path;  // cache file whose entries are loaded
 
 void load_locked() {
@@ -39,8 +39,8 @@ void load_locked() {
   while (std::getline(f, line)) {
     std::istringstream is(line);
     std::string tag, key;
-    int mt = 0, nt = 0;
-    if (is >> tag >> key >> mt >> nt && tag == kTuneTag) g_tune[key] = {mt, nt};
+    TileShape t;
+    if (is >> tag >> key >> t.mt >> t.nt >> t.grp && tag == kTuneTag) g_tune[key] = t;
   }
 }
 
@@ -49,12 +49,18 @@ const char* precision_name(Precision kp) {
 }
 }  // namespace
 
-std::vector<std::pair<int, int>> tile_candidates(Precision kp) {
+std::vector<TileShape> tile_candidates(Precision kp) {
   const int mt0 = kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
-  std::vector<std::pair<int, int>> c;
-  for (int mt : {mt0, mt0 / 2})
-    for (int nt : {kBwdNTiles, 3}) c.emplace_back(mt, nt);
+  std::vector<TileShape> c;
+  for (int grp : {0, 1})  // all phase groups per entry (one ratio window), or one
+    for (int mt : {mt0, mt0 / 2})
+      for (int nt : {kBwdNTiles, 3}) c.push_back(TileShape{mt, nt, grp});
   return c;
+}
+
+int bwd_groups_per_entry(int nt, int grp) {
+  const int ngrp = kPhiPadBwd / (16 * nt);
+  return grp <= 0 || grp > ngrp || ngrp % grp != 0 ? ngrp : grp;
 }
 
 std::string tune_key(const mlf_patch_s* p, const char* scope) {
@@ -65,8 +71,8 @@ std::string tune_key(const mlf_patch_s* p, const char* scope) {
   return os.str();
 }
 
-bool tune_cache_get(const std::string& key, Precision kp, int* mt, int* nt) {
-  std::pair<int, int> v;
+bool tune_cache_get(const std::string& key, Precision kp, TileShape* shape) {
+  TileShape v;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
     load_locked();
@@ -76,21 +82,20 @@ bool tune_cache_get(const std::string& key, Precision kp, int* mt, int* nt) {
   }
   for (const auto& c : tile_candidates(kp))
     if (c == v) {
-      *mt = v.first;
-      *nt = v.second;
+      *shape = v;
       return true;
     }
   return false;  // not a shape of this build: keep the default
 }
 
-void tune_cache_put(const std::string& key, int mt, int nt, double ms) {
+void tune_cache_put(const std::string& key, const TileShape& t, double ms) {
   std::lock_guard<std::mutex> lk(g_tune_mu);
   load_locked();
-  g_tune[key] = {mt, nt};
+  g_tune[key] = t;
   const char* path = std::getenv("MLF_TUNE_CACHE");
   if (!path || !*path) return;
   std::ofstream f(path, std::ios::app);
-  f << kTuneTag << " " << key << " " << mt << " " << nt << " " << ms << "\n";
+  f << kTuneTag << " " << key << " " << t.mt << " " << t.nt << " " << t.grp << " " << ms << "\n";
 }
 
 }  // namespace mlf

@@ -34,6 +34,7 @@ MLF_ERR_OUT_OF_MEMORY = 4
 MLF_ERR_BUDGET = 5
 MLF_ERR_INTERNAL = 6
 MLF_ERR_CANCELLED = 7
+MLF_ERR_IO = 8
 
 # host-forced server failures (mlf_server_inject_fault)
 MLF_FAULT_ADMIT = 1
@@ -69,6 +70,10 @@ class BudgetExceeded(MesolfError):
 
 class Cancelled(MesolfError):
     """A queued server request dropped before it ran (server destroyed)."""
+
+
+class MatFileError(MesolfError, OSError):
+    """MLF_ERR_IO: a MAT-file that is missing, unreadable or malformed."""
 
 
 _lock = threading.Lock()
@@ -155,6 +160,14 @@ _SIGNATURES = {
                                                _I64, _I64, _I64, _F32P, _I32]),
     "mlf_lfm_convolution_segments_device": (C.c_int, [_P, _P, _I64, _I64, _I64, _I32P, _I64P, _I64, _P,
                                                       _I64, _I64, _I64, _P]),
+    "mlf_mat_open": (C.c_int, [C.c_char_p, C.POINTER(_P)]),
+    "mlf_mat_close": (C.c_int, [_P]),
+    "mlf_mat_count": (C.c_int, [_P, _I32P]),
+    "mlf_mat_name": (C.c_int, [_P, _I32, C.c_char_p, C.c_size_t]),
+    "mlf_mat_info": (C.c_int, [_P, C.c_char_p, _P]),
+    "mlf_mat_read_f32": (C.c_int, [_P, C.c_char_p, _F32P, _I64]),
+    "mlf_mat_read_f64": (C.c_int, [_P, C.c_char_p, _F64P, _I64]),
+    "mlf_psf_load_mat": (C.c_int, [_P, C.c_char_p, C.POINTER(_P), _P]),
 }
 
 
@@ -195,6 +208,8 @@ def _check(status: int):
         raise BudgetExceeded(status, msg)
     if status == MLF_ERR_CANCELLED:
         raise Cancelled(status, msg)
+    if status == MLF_ERR_IO:
+        raise MatFileError(status, msg)
     raise MesolfError(status, msg)
 
 
@@ -242,6 +257,77 @@ class Context:
             pass
 
 
+class _MatVar(C.Structure):
+    _fields_ = [("matlab_class", C.c_char * 32), ("ndim", C.c_int32), ("dims", C.c_int64 * 32),
+                ("numel", C.c_int64), ("numeric", C.c_int32), ("stored_bytes", C.c_int32)]
+
+
+class _PsfFileInfo(C.Structure):
+    _fields_ = [("S", C.c_int64), ("N", C.c_int64), ("Z", C.c_int64), ("x_pixel_um", C.c_double),
+                ("z_pixel_um", C.c_double)]
+
+
+class MatFile:
+    """A MATLAB -v7.3 MAT-file opened by the native reader (no HDF5 library).
+
+    ``read(name)`` returns the variable as a Fortran-ordered numpy array of
+    MATLAB's shape (float64 by default, float32 with ``dtype=np.float32``);
+    ``info(name)`` returns its class, shape and element count.  Names may be
+    '/'-separated paths into structs."""
+
+    def __init__(self, path: str):
+        self._lib = load_library()
+        h = _P()
+        _check(self._lib.mlf_mat_open(os.fsencode(path), C.byref(h)))
+        self.handle = h
+        self.path = path
+
+    def variables(self):
+        n = C.c_int32()
+        _check(self._lib.mlf_mat_count(self.handle, C.byref(n)))
+        out = []
+        buf = C.create_string_buffer(4096)
+        for i in range(n.value):
+            _check(self._lib.mlf_mat_name(self.handle, i, buf, len(buf)))
+            out.append(buf.value.decode())
+        return out
+
+    def info(self, name: str):
+        v = _MatVar()
+        _check(self._lib.mlf_mat_info(self.handle, name.encode(), C.byref(v)))
+        return {"class": v.matlab_class.decode(), "shape": tuple(v.dims[:v.ndim]), "numel": v.numel,
+                "numeric": bool(v.numeric), "stored_bytes": v.stored_bytes}
+
+    def read(self, name: str, dtype=np.float64):
+        inf = self.info(name)
+        dtype = np.dtype(dtype)
+        out = np.empty(inf["shape"] if inf["numel"] else (0,), dtype=dtype, order="F")
+        if dtype == np.float64:
+            _check(self._lib.mlf_mat_read_f64(self.handle, name.encode(), _fptr(out, C.c_double), inf["numel"]))
+        elif dtype == np.float32:
+            _check(self._lib.mlf_mat_read_f32(self.handle, name.encode(), _fptr(out), inf["numel"]))
+        else:
+            raise InvalidInput(MLF_ERR_INVALID_ARGUMENT, "dtype must be float32 or float64")
+        return out
+
+    def close(self):
+        if getattr(self, "handle", None):
+            self._lib.mlf_mat_close(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 def compact_block_layout(tab: np.ndarray) -> np.ndarray:
     """[d1+r, d2+r, s1, s2] table -> flat ABI order ((d2*K + d1)*225 + s2*15 + s1)."""
     return np.ascontiguousarray(np.transpose(tab, (1, 0, 3, 2)), dtype=np.float32).ravel()
@@ -272,6 +358,20 @@ class PSF:
         _check(ctx._lib.mlf_psf_create_dense(ctx.handle, _fptr(H), _fptr(Ht), H.shape[0],
                                              H.shape[2], H.shape[4], C.byref(h)))
         return cls(ctx, h)
+
+    @classmethod
+    def load_mat(cls, ctx: Context, path: str):
+        """load(SI.psffile) (mesolf.m:20-21) of a -v7.3 PSF file, then
+        :meth:`from_dense` on its H and Ht, in native code.  Returns the PSF;
+        ``psf.file_info`` holds S, N, Z and the pixel sizes in um
+        (mesolf.m:231-232; NaN when x1objspace / x3objspace are absent)."""
+        h = _P()
+        info = _PsfFileInfo()
+        _check(ctx._lib.mlf_psf_load_mat(ctx.handle, os.fsencode(path), C.byref(h), C.byref(info)))
+        psf = cls(ctx, h)
+        psf.file_info = {"S": info.S, "N": info.N, "Z": info.Z, "x_pixel_um": info.x_pixel_um,
+                         "z_pixel_um": info.z_pixel_um}
+        return psf
 
     @classmethod
     def from_compact(cls, ctx: Context, views_uv, weights, rf, fwd_flat, rb=None, bwd_flat=None):
@@ -386,12 +486,14 @@ class Patch:
         return _shape_dict(t)
 
     def tile_shape(self):
+        """(forward columns per wave, backprojection phase tiles, phase groups
+        per backprojection entry (0: all))."""
         t = _TileShape()
         _check(self._lib.mlf_patch_tile_shape(self.handle, C.byref(t)))
-        return (t.fwd_columns_per_wave, t.bwd_phase_tiles)
+        return (t.fwd_columns_per_wave, t.bwd_phase_tiles, t.bwd_groups_per_entry)
 
-    def set_tile_shape(self, fwd_columns_per_wave: int, bwd_phase_tiles: int):
-        t = _TileShape(int(fwd_columns_per_wave), int(bwd_phase_tiles), 0.0)
+    def set_tile_shape(self, fwd_columns_per_wave: int, bwd_phase_tiles: int, bwd_groups_per_entry: int = 0):
+        t = _TileShape(int(fwd_columns_per_wave), int(bwd_phase_tiles), int(bwd_groups_per_entry), 0.0)
         _check(self._lib.mlf_patch_set_tile_shape(self.handle, C.byref(t)))
 
     def forward_view(self, k: int):
@@ -443,12 +545,12 @@ class Patch:
 # -------------------------------------------------------------------- drop-ins
 class _TileShape(C.Structure):
     _fields_ = [("fwd_columns_per_wave", C.c_int32), ("bwd_phase_tiles", C.c_int32),
-                ("ms_per_view", C.c_double)]
+                ("bwd_groups_per_entry", C.c_int32), ("ms_per_view", C.c_double)]
 
 
 def _shape_dict(t):
     return {"fwd_columns_per_wave": t.fwd_columns_per_wave, "bwd_phase_tiles": t.bwd_phase_tiles,
-            "ms_per_view": t.ms_per_view}
+            "bwd_groups_per_entry": t.bwd_groups_per_entry, "ms_per_view": t.ms_per_view}
 
 
 class Batch:

@@ -1,9 +1,9 @@
 // MATLAB MEX gateway behind recon_theory_phase_space_LFM_with_filter.m:
 //   Xguess = recon_phase_space_mex(LFIMG, H, Ht, maxIter, gpuFlag, gpu_id)
 // Same arguments as src/recon/recon_theory_phase_space_LFM_with_filter.m:10.
-// LFIMG, H, Ht may be single or double (converted to single); gpu_id is
-// MATLAB's 1-based device index (recon:76).  Returns the double [I1,I2,Z]
-// volume (recon:147 gathers a double array).
+// LFIMG, H, Ht may be single (passed through without a copy) or double
+// (converted to single); gpu_id is MATLAB's 1-based device index (recon:76).
+// Returns the double [I1,I2,Z] volume (recon:147 gathers a double array).
 #include <cstdint>
 #include <vector>
 
@@ -12,17 +12,15 @@
 
 static const char* const kErrId = "mesolf:recon:InvalidInput";
 
-static std::vector<float> as_single(const mxArray* a) {
+// The single data of a (its own buffer when a is single, else a converted
+// copy kept in `store`).
+static const float* as_single(const mxArray* a, std::vector<float>* store) {
+  if (mxIsSingle(a)) return static_cast<const float*>(mxGetData(a));
   const size_t n = mxGetNumberOfElements(a);
-  std::vector<float> out(n);
-  if (mxIsSingle(a)) {
-    const float* p = static_cast<const float*>(mxGetData(a));
-    for (size_t i = 0; i < n; ++i) out[i] = p[i];
-  } else {
-    const double* p = static_cast<const double*>(mxGetData(a));
-    for (size_t i = 0; i < n; ++i) out[i] = static_cast<float>(p[i]);
-  }
-  return out;
+  store->resize(n);
+  const double* p = static_cast<const double*>(mxGetData(a));
+  for (size_t i = 0; i < n; ++i) (*store)[i] = static_cast<float>(p[i]);
+  return store->data();
 }
 
 void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
@@ -45,13 +43,14 @@ void mexFunction(int nlhs, mxArray* plhs[], int nrhs, const mxArray* prhs[]) {
   const int32_t max_iter = static_cast<int32_t>(mxGetScalar(prhs[3]));
   const int32_t gpu_flag = mxGetScalar(prhs[4]) != 0.0;
   const int32_t gpu_id = static_cast<int32_t>(mxGetScalar(prhs[5]));
-  const std::vector<float> lf = as_single(prhs[0]);
-  const std::vector<float> H = as_single(prhs[1]);
-  const std::vector<float> Ht = as_single(prhs[2]);
+  std::vector<float> lf_store, H_store, Ht_store;
+  const float* lf = as_single(prhs[0], &lf_store);
+  const float* H = as_single(prhs[1], &H_store);
+  const float* Ht = as_single(prhs[2], &Ht_store);
   const mwSize od[3] = {ld[0], ld[1], (mwSize)Z};
   plhs[0] = mxCreateNumericArray(3, od, mxDOUBLE_CLASS, mxREAL);
   const mlf_status st = mlf_recon_phase_space(
-      lf.data(), (int64_t)ld[0], (int64_t)ld[1], H.data(), Ht.data(), (int64_t)hd[0],
+      lf, (int64_t)ld[0], (int64_t)ld[1], H, Ht, (int64_t)hd[0],
       (int64_t)hd[2], Z, max_iter, gpu_flag, gpu_id - 1, MLF_PRECISION_FP32,
       static_cast<double*>(mxGetData(plhs[0])), nullptr);
   if (st == MLF_ERR_INVALID_ARGUMENT) mexErrMsgIdAndTxt(kErrId, "%s", mlf_last_error());

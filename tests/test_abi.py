@@ -32,7 +32,7 @@ def test_library_exports_every_header_symbol(root_dir):
 
 def test_abi_version_and_status_strings():
     lib = L.load_library()
-    assert lib.mlf_abi_version() == 1
+    assert lib.mlf_abi_version() == 2
     assert lib.mlf_status_string(1) == b"invalid argument"
 
 
