@@ -425,6 +425,165 @@ def bench_mex(args):
     }), flush=True)
 
 
+def bench_seedlib(args):
+    """The seed-library projection loop of
+    gen_seed_library_local_with_ring_bg_same_size.m:263-318 at the demo's
+    patch-1 shape: 51 planes x 4 arrays (sample, sample mask, ring, ring mask)
+    of 396 x 396 slices, N = 88-144 slices per plane, kw = 241, each slice's
+    projection added to its segment.  All four arrays, the kernels
+    (Ht(:,:,:,:,z), 241 x 241 x 15 x 15 x 51) and the outputs are resident in
+    HBM; one mlf_lfm_convolution_segments_device call per array."""
+    import torch
+
+    import mesolf_amd
+
+    W = H = 396
+    Z, kw, nseg = 51, 241, 144
+    # plane z is covered by the first n_z segments: 144 in the middle planes,
+    # 88 at the ends (the demo log's range of N per call)
+    n_z = [int(round(88 + 56 * (1 - abs(z - 25) / 25))) for z in range(Z)]
+    planes, seg_ptr = [], [0]
+    for s_ in range(nseg):
+        pl = [z for z in range(Z) if s_ < n_z[z]]
+        planes += pl
+        seg_ptr.append(len(planes))
+    ns = len(planes)
+    ctx = mesolf_amd.Context(0)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    # sparse segment stacks: a disc of ~40 px radius around each segment's centre
+    yy, xx = torch.meshgrid(torch.arange(W, device="cuda"), torch.arange(H, device="cuda"), indexing="ij")
+    arrays = []
+    for a in range(4):
+        x = torch.zeros((ns, H, W), device="cuda", dtype=torch.float32)  # [slice][y][x] = [W, H, n] F order
+        cx = torch.randint(60, W - 60, (nseg,), device="cuda", generator=g)
+        cy = torch.randint(60, H - 60, (nseg,), device="cuda", generator=g)
+        for s_ in range(nseg):
+            disc = (((xx - cx[s_]) ** 2 + (yy - cy[s_]) ** 2) < 1600).T.float()
+            vals = torch.rand((seg_ptr[s_ + 1] - seg_ptr[s_], H, W), device="cuda", generator=g)
+            x[seg_ptr[s_]:seg_ptr[s_ + 1]] = vals * disc if a % 2 == 0 else (vals * disc > 0).float()
+        arrays.append(x)
+    K = torch.rand((Z, 15, 15, kw, kw), device="cuda", generator=g, dtype=torch.float32)  # [kw,kw,15,15,Z] F
+    outs = [torch.empty((nseg, H, W), device="cuda", dtype=torch.float32) for _ in range(4)]
+    torch.cuda.synchronize()
+
+    def run():
+        for a in range(4):
+            mesolf_amd.lfm_convolution_segments_device(ctx, arrays[a].data_ptr(), W, H, ns, planes, seg_ptr,
+                                                       K.data_ptr(), kw, Z, outs[a].data_ptr())
+
+    # warm-up on a 2-segment prefix (code objects, operand tables)
+    mesolf_amd.lfm_convolution_segments_device(ctx, arrays[0].data_ptr(), W, H, seg_ptr[2], planes[:seg_ptr[2]],
+                                               seg_ptr[:3], K.data_ptr(), kw, Z, outs[0].data_ptr())
+    log(f"seed library: {ns} slices per array, N per plane {min(n_z)}-{max(n_z)}")
+    times = []
+    for i in range(max(1, args.steps)):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        run()
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t)
+        log(f"seed library step {i + 1} {times[-1]:.3f} s")
+    dt = min(times)
+    mac = 4.0 * ns * W * H * kw * kw
+    print(json.dumps({
+        "metric": "seed-library MEX projection loop, 396x396 slices, 51 planes x 4 arrays, N = 88-144 per "
+                  "plane, kw = 241 (gen_seed_library_local_with_ring_bg_same_size.m:263-318)",
+        "value": dt, "unit": "s", "n_gpus": 1, "steps": len(times), "warmup": 1, "ms_per_step": dt * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 (MFMA, f32 accumulate)", "data": "synthetic sparse segment stacks, random kernels",
+        "config": {"workload": "seedlib_396x396_51x4_kw241", "slices_per_array": ns, "segments": nseg,
+                   "n_per_plane": [min(n_z), max(n_z)]},
+        "roofline": {"bound": "mfma", "achieved": 2 * mac / dt / 1e12, "peak": PEAK["fp32"] / 1e12,
+                     "unit": "TFLOP/s", "frac": 2 * mac / dt / PEAK["fp32"], "traffic": None},
+        "reference_published_s": {"patch1_N88_144": 272.26, "patch2_N1_128": 73.19,
+                                  "note": "TITAN V, reference MEX; this run's N per plane is synthetic"},
+    }), flush=True)
+    ctx.close()
+
+
+def bench_prep(args):
+    """Host preparation of the recon drop-in at the benchmark patch's dense
+    PSF size: H, Ht [525, 525, 15, 15, 51] single (12.6 GB each), through
+    mlf_psf_create_dense (recon:13-40: normalisation, phase-space tables,
+    weights, upload) and through the drop-in twice (the second call reuses the
+    tables of identical H/Ht)."""
+    import mesolf_amd
+    from mesolf_amd import synth
+
+    wl = synth.CFG2
+    S, Z = wl.IMG, wl.Z
+    c = S // 2
+    x = np.arange(S)
+    d2 = (x[:, None] - c) ** 2 + (x[None, :] - c) ** 2
+    view = (1.0 + 0.01 * np.add.outer(np.arange(15), np.arange(15))).astype(np.float32)
+    t = time.perf_counter()
+    H = np.empty((S, S, 15, 15, Z), np.float32, order="F")
+    Ht = np.empty((S, S, 15, 15, Z), np.float32, order="F")
+    for z, r in enumerate(wl.radii):
+        m = (d2 <= (15 * r + 7) ** 2).astype(np.float32)
+        H[:, :, :, :, z] = m[:, :, None, None] * view[None, None, :, :]
+        mt = (d2 <= (15 * r + 14) ** 2).astype(np.float32)
+        Ht[:, :, :, :, z] = mt[:, :, None, None] * view[None, None, :, :]
+    log(f"dense H, Ht {H.nbytes / 1e9:.1f} GB each built in {time.perf_counter() - t:.1f}s")
+    ctx = mesolf_amd.Context(0)
+    t = time.perf_counter()
+    psf = mesolf_amd.PSF.from_dense(ctx, H, Ht)
+    ctx.synchronize()
+    t_prep = time.perf_counter() - t
+    rf, rb = psf.radii()
+    log(f"mlf_psf_create_dense {t_prep:.2f} s, {psf.nviews} views, max radius {rf.max()}/{rb.max()}")
+    psf.close()
+    LF = np.random.default_rng(3).poisson(50.0, (wl.I1, wl.I2)).astype(np.float32)
+    calls = []
+    for i in range(2):
+        t = time.perf_counter()
+        mesolf_amd.recon_theory_phase_space_LFM_with_filter(LF, H, Ht, 1, True, 1)
+        calls.append(time.perf_counter() - t)
+        log(f"recon drop-in call {i + 1} (1 iteration, FP32): {calls[-1]:.2f} s")
+    print(json.dumps({
+        "metric": "recon drop-in host preparation, dense H/Ht 525x525x15x15x51 single",
+        "value": t_prep, "unit": "s", "n_gpus": 1, "steps": 1, "warmup": 0, "ms_per_step": t_prep * 1e3,
+        "higher_is_better": False, "scaling": "weak", "vs_baseline": None, "dtype": "f32 in, f64 sums",
+        "data": "synthetic separable PSF supports of the benchmark's radii",
+        "config": {"workload": "prep_cfg2_dense_525x525x15x15x51", "host_threads": os.cpu_count()},
+        "recon_dropin_1_iteration_s": {"first_call": calls[0], "second_call_tables_reused": calls[1]},
+    }), flush=True)
+    ctx.close()
+
+
+def bench_hbm_copy(args):
+    """A plain device-to-device copy of 4 GiB (torch): the practical HBM
+    ceiling for the rooflines, and the reference kernel of known traffic
+    (4 GiB read + 4 GiB written per launch) for the HBM byte counters."""
+    import torch
+
+    n = 1 << 30  # floats
+    a = torch.ones(n, device="cuda", dtype=torch.float32)
+    b = torch.empty_like(a)
+    for _ in range(max(1, args.warmup)):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    st = torch.cuda.current_stream()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    times = []
+    for _ in range(max(1, args.steps)):
+        e0.record(st)
+        b.copy_(a)
+        e1.record(st)
+        e1.synchronize()
+        times.append(e0.elapsed_time(e1) * 1e-3)
+    dt = min(times)
+    nbytes = 8.0 * n
+    print(json.dumps({
+        "metric": "device copy bandwidth, 4 GiB fp32 (read + write)", "value": nbytes / dt / 1e9,
+        "unit": "GB/s", "n_gpus": 1, "steps": len(times), "warmup": args.warmup, "ms_per_step": dt * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "constant", "config": {"workload": "hbm_copy_4GiB"},
+        "roofline": {"bound": "hbm", "achieved": nbytes / dt / 1e9, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                     "frac": nbytes / dt / HBM_PEAK, "traffic": nbytes},
+    }), flush=True)
+
+
 def bench_summary(args):
     """Summary image + rank-1 background (summary_image_generation_module.m,
     'pixelwise' detrend, defaults 100 / 20 / 3 / 5) of a 1515 x 1515-px movie
@@ -527,7 +686,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", default="bf16", choices=["fp32", "bf16", "fp16"])
-    ap.add_argument("--workload", default="cfg2", choices=["cfg1", "cfg2", "cfg4", "cfg5", "mex", "summary"],
+    ap.add_argument("--workload", default="cfg2",
+                    choices=["cfg1", "cfg2", "cfg4", "cfg5", "mex", "summary", "seedlib", "prep", "hbm_copy"],
                     help="cfg4: batched movie of 12 uneven patches; cfg5: serving loop; "
                          "mex: the MEX operator at the demo shape")
     ap.add_argument("--patches", type=int, default=0, help="cfg4/cfg5: patch count (12 / 36)")
@@ -559,6 +719,12 @@ def main():
         return bench_summary(args)
     if args.workload == "mex":
         return bench_mex(args)
+    if args.workload == "seedlib":
+        return bench_seedlib(args)
+    if args.workload == "prep":
+        return bench_prep(args)
+    if args.workload == "hbm_copy":
+        return bench_hbm_copy(args)
     if args.workload == "cfg4":
         args.patches = args.patches or 12
         return bench_batched(args)
@@ -639,7 +805,7 @@ def main():
     if args.workload == "cfg2" and os.path.exists(tfile):
         # measured bytes per launch of the kernel instance this run used
         tr = json.load(open(tfile))["bytes_per_launch"].get(args.precision, {}).get(dominant)
-        fmt, bnt, _ = work.tile_shape()
+        fmt, bnt, _, _ = work.tile_shape()
         if tr and "read" not in tr:
             tr = tr.get(f"fwd_columns_per_wave={fmt}" if dominant == "forward" else f"bwd_phase_tiles={bnt}")
         if tr:

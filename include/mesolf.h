@@ -242,6 +242,7 @@ typedef struct {
   int32_t fwd_columns_per_wave;
   int32_t bwd_phase_tiles;
   int32_t bwd_groups_per_entry;
+  int32_t bwd_waves_per_simd;  /* 16-bit backprojection: 2, or 3 with bwd_phase_tiles = 3 */
   double ms_per_view;  /* measured view step (forward + ratio + backprojection) */
 } mlf_tile_shape;
 mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best);

@@ -33,8 +33,9 @@ struct mlf_batch_s {
   bool tpz = false;
   int cw = 16, bm2 = 8;
   int fwd_mt = 0, bwd_nt = 0;  // tile shape shared by every attached patch
-  int bwd_grp = 0;
+  int bwd_grp = 0, bwd_occ = 2;
   int grid_f = 0, grid_b = 0, grid_r = 0;
+  int cus = 0;
   size_t smem_f = 0, smem_b = 0;
   int fwd_nbuf = 2;
   // one RL iteration's view loop as a graph
@@ -155,7 +156,7 @@ mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st, int nviews = 0) {
     }
     const BwdGeo bg{b->d_slots.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, b->d_bsched.ptr,
                     b->d_counts.ptr + 1, k, bwd_groups_per_entry(b->bwd_nt, b->bwd_grp)};
-    MLF_LAUNCH(launch_bwd_mfma(b->kp, b->bwd_nt, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg,
+    MLF_LAUNCH(launch_bwd_mfma(b->kp, b->bwd_nt, b->bwd_occ, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg,
                                b->grid_b, b->smem_b, (float)ps->weight[k], st));
   }
   return MLF_OK;
@@ -243,8 +244,9 @@ mlf_status mlf_batch_create(mlf_psf psf, mlf_precision prec, int32_t capacity, m
   MLF_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, psf->ctx->device));
   // two workgroups per CU for the MFMA kernels (their LDS and register
   // budgets allow two), more for the light ratio kernel
+  b->cus = cus;
   b->grid_f = 2 * cus;
-  b->grid_b = 2 * cus;
+  b->grid_b = 2 * cus;  // bwd_occ workgroups per CU once a shape is chosen
   b->grid_r = 4 * cus;
   MLF_HIP(hipEventCreate(&b->ev0));
   MLF_HIP(hipEventCreate(&b->ev1));
@@ -284,22 +286,27 @@ mlf_status mlf_batch_attach(mlf_batch b, mlf_patch p) {
     return set_error(MLF_ERR_BUDGET, "batch is full (%d patches)", b->capacity);
   MLF_CHECK(ctx_activate(b->psf->ctx));
   if (b->slots.empty()) {
-    if (b->fwd_mt != p->fwd_mt || b->bwd_nt != p->bwd_nt || b->bwd_grp != p->bwd_grp) drop_graph(b);
+    if (b->fwd_mt != p->fwd_mt || b->bwd_nt != p->bwd_nt || b->bwd_grp != p->bwd_grp || b->bwd_occ != p->bwd_occ)
+      drop_graph(b);
     b->tpz = p->tpz;
     b->cw = p->cw;
     b->bm2 = p->bm2;
     b->fwd_mt = p->fwd_mt;
     b->bwd_nt = p->bwd_nt;
     b->bwd_grp = p->bwd_grp;
+    b->bwd_occ = p->bwd_occ;
+    b->grid_b = b->bwd_occ * b->cus;
     b->smem_f = p->smem_f;
     b->smem_b = p->smem_b;
     b->fwd_nbuf = p->fwd_nbuf;
-  } else if (p->fwd_mt != b->fwd_mt || p->bwd_nt != b->bwd_nt || p->bwd_grp != b->bwd_grp) {
+  } else if (p->fwd_mt != b->fwd_mt || p->bwd_nt != b->bwd_nt || p->bwd_grp != b->bwd_grp ||
+             p->bwd_occ != b->bwd_occ) {
     // one launch serves every patch: the patch takes the batch's tile shape
     MLF_HIP(hipStreamSynchronize(b->psf->ctx->stream));
     p->fwd_mt = b->fwd_mt;
     p->bwd_nt = b->bwd_nt;
     p->bwd_grp = b->bwd_grp;
+    p->bwd_occ = b->bwd_occ;
     MLF_CHECK(patch_setup_tiles(p));
     if (p->volume_set) MLF_CHECK(patch_refresh_shadow(p));
   }
@@ -385,6 +392,7 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
       p->fwd_mt = t.mt;
       p->bwd_nt = t.nt;
       p->bwd_grp = t.grp;
+      p->bwd_occ = t.occ;
       MLF_CHECK(patch_setup_tiles(p));
       MLF_CHECK(patch_refresh_shadow(p));
     }
@@ -392,6 +400,8 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
     b->fwd_mt = t.mt;
     b->bwd_nt = t.nt;
     b->bwd_grp = t.grp;
+    b->bwd_occ = t.occ;
+    b->grid_b = b->bwd_occ * b->cus;
     b->cw = p0->cw;
     b->smem_f = p0->smem_f;
     b->smem_b = p0->smem_b;
@@ -410,7 +420,7 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
   MLF_HIP(hipEventCreate(&e0));
   MLF_HIP(hipEventCreate(&e1));
   double best_ms = 1e300;
-  TileShape bt{b->fwd_mt, b->bwd_nt, b->bwd_grp};
+  TileShape bt{b->fwd_mt, b->bwd_nt, b->bwd_grp, b->bwd_occ};
   mlf_status s = MLF_OK;
   // two rounds over the candidates, each keeping its faster round (shapes a
   // few per cent apart are otherwise decided by clock and cache noise)
@@ -452,6 +462,7 @@ mlf_status mlf_batch_autotune(mlf_batch b, int32_t views, mlf_tile_shape* best) 
     best->fwd_columns_per_wave = bt.mt;
     best->bwd_phase_tiles = bt.nt;
     best->bwd_groups_per_entry = bt.grp;
+    best->bwd_waves_per_simd = bt.occ;
     best->ms_per_view = best_ms;
   }
   return MLF_OK;

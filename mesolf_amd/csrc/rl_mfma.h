@@ -18,13 +18,18 @@ constexpr int kFwdMTF32 = 2;      // M-tiles per wave, forward, f32 (block 16 x 
 constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x 16 lenslets)
 constexpr int kBwdNTiles = 5;     // default N-tiles per backward workgroup (80 phases; 3 also built)
 constexpr int kPhiPadBwd = 240;   // 3 groups of 80 phases (or 5 of 48)
-constexpr int kBwdStages = 6;     // 16-bit backprojection: tap ring depth (copies run 5 K-steps ahead)
+// 16-bit backprojection variants by waves per SIMD: 2 (X of three M-tiles
+// loaded before the K loop, 6-slice tap ring, copies 5 K-steps ahead) or 3
+// (X loaded one M-tile ahead in the epilogue, 4-slice ring) -- fewer
+// registers and LDS per workgroup, so three workgroups share a CU.
+constexpr int kBwdStages = 6;
+inline __host__ __device__ constexpr int bwd_stages(int occ) { return occ >= 3 ? 4 : kBwdStages; }
 // Bytes before the 16-bit backprojection's shifted window: the FP32 staging
-// window [Wd][Wd] and the tap ring (kBwdStages slices of [16 nt][32] taps),
+// window [Wd][Wd] and the tap ring (bwd_stages slices of [16 nt][32] taps),
 // which alias (the staging window is dead once the shifted one is built).
-inline __host__ __device__ int bwd_ring_bytes(int Wd, int nt, int elem_bytes) {
+inline __host__ __device__ int bwd_ring_bytes(int Wd, int nt, int elem_bytes, int occ) {
   const int rs = 4 * ((Wd * Wd + 3) & ~3);
-  const int ring = kBwdStages * nt * 16 * 32 * elem_bytes;
+  const int ring = bwd_stages(occ) * nt * 16 * 32 * elem_bytes;
   return rs > ring ? rs : ring;
 }
 constexpr int kRatioLens = 32;    // lenslets per ratio work entry
@@ -144,14 +149,15 @@ hipError_t launch_ratio_gather_tpz(FwdGeo g, RatioGeo rg, int grid, int Z, int c
                                    int use_lf, float* y_out, hipStream_t st);
 hipError_t launch_pack_fwd_tpz(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                                hipStream_t st);
-size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt);
+size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt, int occ);
 
 hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem, int nbuf,
                            hipStream_t st);
 hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, int u, int v,
                                int use_lf, float* y_out, hipStream_t st);
-// nt: 16-phase N-tiles per workgroup (3 or 5)
-hipError_t launch_bwd_mfma(Precision p, int nt, const void* Wb, BwdGeo g, int grid, size_t smem,
+// nt: 16-phase N-tiles per workgroup (3 or 5); occ: waves per SIMD of the
+// 16-bit variants (2, or 3 with nt = 3; FP32 has only 2)
+hipError_t launch_bwd_mfma(Precision p, int nt, int occ, const void* Wb, BwdGeo g, int grid, size_t smem,
                            float w, hipStream_t st);
 hipError_t launch_pack_fwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                            hipStream_t st);

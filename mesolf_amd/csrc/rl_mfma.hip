@@ -852,12 +852,16 @@ __device__ __forceinline__ int norm_class(int a, int n, int r) {
 }
 
 
-template <typename T, int NT>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+template <typename T, int NT, int OCC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   using M = Mma<T>;
   constexpr int KSTEP = M::KSTEP, KPL = M::KPL;
   constexpr int MT = 4;  // NT: N-tiles (16 phases each) per workgroup
+  constexpr int STG = bwd_stages(OCC);  // 16-bit tap ring depth
+  // M-tiles whose X (and the first 1/B1 rows) load before the K loop; with
+  // none, the epilogue loads each tile one tile ahead
+  constexpr int PRE = OCC >= 3 ? 0 : MT - 1;
   extern __shared__ __align__(16) char smem[];
   const int nent = uni(*g.count);
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
@@ -912,7 +916,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // FP32: [epad]
     using frag_t = typename M::frag;
     // 16-bit: [Wd][Wd] after the staging window Rs / tap ring (which alias)
-    frag_t* Rv = reinterpret_cast<frag_t*>(smem + bwd_ring_bytes(Wd, NT, (int)sizeof(T)));
+    frag_t* Rv = reinterpret_cast<frag_t*>(smem + bwd_ring_bytes(Wd, NT, (int)sizeof(T), OCC));
     if constexpr (KPL == 1) {
       // FP32: scalar im2col reads, taps e1-fastest
       gather_window(R, n1, n2, 16 * b1 - r, 16 * b2 - r, Wd, Rw);
@@ -977,25 +981,27 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // of tile 0 too, those of later tiles (L2-resident) when the tile is
       // reached
       floatx4_t xv[MT][NT], bv[NT];
-      auto load_x = [&](int mt0, int mt1) {
+      auto ldx = [&](int mt, floatx4_t* dst) {
+        const int a2 = 16 * b2 + 4 * w + mt;
+        const int xo = ((a2 < n2 ? a2 : n2 - 1) * n1 + a1c) * kXRow;
   #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          if (mt < mt0 || mt >= mt1) continue;
-          const int a2 = 16 * b2 + 4 * w + mt;
-          const int xo = ((a2 < n2 ? a2 : n2 - 1) * n1 + a1c) * kXRow;
-  #pragma unroll
-          for (int nt = 0; nt < NT; ++nt)
-            xv[mt][nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0) + 64 * nt, 0, 0));
-        }
+        for (int nt = 0; nt < NT; ++nt)
+          dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0) + 64 * nt, 0, 0));
       };
-      auto load_t = [&](int mt) {
+      auto ldt = [&](int mt, floatx4_t* dst) {
         const int a2 = 16 * b2 + 4 * w + mt;
         const int to = (norm_class(a2 < n2 ? a2 : n2 - 1, n2, r) * C1 + c1) * kP;
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
-          bv[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0) + 64 * nt, 0, 0));
+          dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0) + 64 * nt, 0, 0));
       };
-      constexpr int kEpiLoads = (MT - 1) * NT + NT;  // load_x(0, MT - 1) + load_t(0)
+      auto load_x = [&](int mt0, int mt1) {
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt)
+          if (mt >= mt0 && mt < mt1) ldx(mt, xv[mt]);
+      };
+      auto load_t = [&](int mt) { ldt(mt, bv); };
+      constexpr int kEpiLoads = PRE > 0 ? PRE * NT + NT : 0;  // load_x(0, PRE) + load_t(0)
 
       if constexpr (KPL == 1) {
         load_x(0, MT - 1);
@@ -1023,8 +1029,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       } else {
         // Taps go global -> LDS by DMA (global_load_lds, one copy per
         // workgroup instead of one per wave through the L1) into a ring of
-        // kBwdStages K-step slices [16 NT rows][32 taps], kBwdStages - 1 steps
-        // ahead; slices past the last step re-load the last one, so every wave
+        // STG K-step slices [16 NT rows][32 taps], STG - 1 steps ahead;
+        // slices past the last step re-load the last one, so every wave
         // issues the same number of copies per step and waits with a fixed
         // count.  The window offset of a lane's 8 taps is computed, not looked
         // up: t0 = 32 ks + 8 lg -> (e2, e1) = divmod(t0, Kp8) with a 20-bit
@@ -1034,8 +1040,13 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         T* Bq = reinterpret_cast<T*>(smem);  // ring (aliases the dead staging window Rs)
         constexpr int kSlice = NT * 16 * 32;  // elements per K-step slice
         const int ncopy = w < NT ? (NT - w + 3) / 4 : 0;  // copies per step issued by this wave
-        // the lane's byte offset within a copy: row lane / 4, 8 taps at (lane % 4) * 8
-        const int tap_lane = ((lane >> 2) * epad + (lane & 3) * 8) * (int)sizeof(T);
+        // A copy moves one N-tile's slice, 16 rows x 32 taps, as 64 16-byte
+        // pieces; DMA lane s lands at byte 16 s.  Piece (row, 8-tap group q)
+        // sits at s = 4 row + ((q + row / 4) % 4), so the 16 lanes that read
+        // one group q at rows 0..15 together hit 16 distinct 4-bank groups
+        // (row-major pieces, 64 bytes per row, would be a 4-way conflict).
+        const int tap_lane = ((lane >> 2) * epad + (((lane & 3) - (lane >> 4)) & 3) * 8) * (int)sizeof(T);
+        const int tap_slot = 8 * (4 * lr + ((lg + (lr >> 2)) & 3));  // element offset of (row lr, group lg)
         auto issue_taps = [&](int ks, int slot) {
           const int kc = min(ks, nks - 1);
           char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
@@ -1047,23 +1058,25 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         };
   #pragma unroll
-        for (int q = 0; q < kBwdStages - 1; ++q) issue_taps(q, q);
-        load_x(0, MT - 1);  // after the first slices: their waits need not cover these loads
-        load_t(0);
+        for (int q = 0; q < STG - 1; ++q) issue_taps(q, q);
+        if constexpr (PRE > 0) {
+          load_x(0, PRE);  // after the first slices: their waits need not cover these loads
+          load_t(0);
+        }
         for (int ks = 0; ks < nks; ++ks) {
           // this wave's copies of step ks landed (the epilogue loads are
-          // younger than the slices of steps 0 .. kBwdStages - 2)
-          wait_vm_le((kBwdStages - 2) * ncopy + (ks < kBwdStages - 1 ? kEpiLoads : 0));
+          // younger than the slices of steps 0 .. STG - 2)
+          wait_vm_le((STG - 2) * ncopy + (ks < STG - 1 ? kEpiLoads : 0));
           lds_barrier();  // everyone's copies landed; everyone is done with step ks - 1
-          issue_taps(ks + kBwdStages - 1, (ks + kBwdStages - 1) % kBwdStages);
-          const T* Bs = Bq + (ks % kBwdStages) * kSlice;
+          issue_taps(ks + STG - 1, (ks + STG - 1) % STG);
+          const T* Bs = Bq + (ks % STG) * kSlice;
           const unsigned t0 = 32u * ks + 8u * lg;
           const int e2 = (int)((t0 * rcp) >> 20);
           const int e10 = (int)t0 - e2 * Kp8;
           const int o = ((r - e10) - (e2 - r) * Wd) & -(int)(e2 < K);
           frag_t bf[NT];
   #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) bf[nt] = *reinterpret_cast<const frag_t*>(Bs + (nt * 16 + lr) * 32 + lg * 8);
+          for (int nt = 0; nt < NT; ++nt) bf[nt] = *reinterpret_cast<const frag_t*>(Bs + nt * 512 + tap_slot);
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
             const frag_t af = Rv[base + mt * Wd + o];
@@ -1073,16 +1086,13 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         }
       }
 
-      // fused multiplicative update (recon:130-137), continued from the loads
-      // issued before the K loop.
+      // fused multiplicative update (recon:130-137) of M-tile mt from its X
+      // rows x and 1/B1 rows b.
       // Stores take no branch except in the one N-tile that reaches past phase
       // 224 (the last of 225): lanes outside the grid store at an offset past
       // the plane, which the buffer store drops (the plane is below 2^31
       // bytes, so 2^31 + a row offset is out of range and does not wrap).
-      load_x(MT - 1, MT);
-  #pragma unroll
-      for (int mt = 0; mt < MT; ++mt) {
-        if (mt > 0) load_t(mt);
+      auto update_tile = [&](int mt, const floatx4_t* x4, const floatx4_t* b4) {
         const int a2 = 16 * b2 + 4 * w + mt;
         const bool ok = ok1 && a2 < n2;
         const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
@@ -1093,8 +1103,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           floatx4_t xn;
   #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const float x = xv[mt][nt][j];
-            float C = x * acc[mt][nt][j] * bv[nt][j];  // bv: 1 / B1
+            const float x = x4[nt][j];
+            float C = x * acc[mt][nt][j] * b4[nt][j];  // b4: 1 / B1
             // NaN, +-Inf and negatives -> 0: keep C only in the classes
             // -0, +0, +denormal, +normal (one class test and a select)
             C = __builtin_amdgcn_classf(C, 0x1E0) ? C : 0.f;
@@ -1131,6 +1141,36 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
             if (Xsh)
               __builtin_amdgcn_raw_buffer_store_b16(to_bits16<T>(is_f16<T>() ? xn[0] * sd.xs : xn[0]), sr,
                                                     so, 0, 0);
+          }
+        }
+      };
+      if constexpr (PRE > 0) {
+        // continued from the loads issued before the K loop
+        load_x(PRE, MT);
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          if (mt > 0) load_t(mt);
+          update_tile(mt, xv[mt], bv);
+        }
+      } else {
+        // X and 1/B1 rows one M-tile ahead, in two register sets
+        floatx4_t xa[NT], ba[NT], xb[NT], bb[NT];
+        ldx(0, xa);
+        ldt(0, ba);
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt) {
+          if (mt % 2 == 0) {
+            if (mt + 1 < MT) {
+              ldx(mt + 1, xb);
+              ldt(mt + 1, bb);
+            }
+            update_tile(mt, xa, ba);
+          } else {
+            if (mt + 1 < MT) {
+              ldx(mt + 1, xa);
+              ldt(mt + 1, ba);
+            }
+            update_tile(mt, xb, bb);
           }
         }
       }
@@ -1254,10 +1294,10 @@ size_t fwd_smem_bytes(Precision p, int rmax, int bm2) {
   return fwd_smem_nbuf(p, rmax, bm2, fwd_nbuf(p, rmax, bm2));
 }
 
-size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt) {
+size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt, int occ) {
   const int Wd = 16 + 2 * rmax;
   if (p == Precision::kF32) return sizeof(float) * ((Wd * Wd + 7) & ~7) + sizeof(int) * epad_max;
-  return bwd_ring_bytes(Wd, nt, 2) + 16 * (size_t)Wd * Wd;
+  return bwd_ring_bytes(Wd, nt, 2, occ) + 16 * (size_t)Wd * Wd;
 }
 
 hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem, int nbuf,
@@ -1390,23 +1430,31 @@ hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, 
   return hipGetLastError();
 }
 
-hipError_t launch_bwd_mfma(Precision p, int nt, const void* Wb, BwdGeo g, int grid, size_t smem,
+template <typename T>
+static void launch_bwd_t(int nt, int occ, const void* Wb, BwdGeo g, int grid, size_t smem, float w,
+                         hipStream_t st) {
+  const T* W = (const T*)Wb;
+  // three waves per SIMD only with 3 N-tiles (with 5 the accumulators and
+  // operands exceed 168 registers)
+  if (nt == 3 && occ >= 3)
+    hipLaunchKernelGGL((k_bwd_mfma<T, 3, 3>), dim3(grid), dim3(256), smem, st, W, g, w);
+  else if (nt == 3)
+    hipLaunchKernelGGL((k_bwd_mfma<T, 3, 2>), dim3(grid), dim3(256), smem, st, W, g, w);
+  else
+    hipLaunchKernelGGL((k_bwd_mfma<T, 5, 2>), dim3(grid), dim3(256), smem, st, W, g, w);
+}
+
+hipError_t launch_bwd_mfma(Precision p, int nt, int occ, const void* Wb, BwdGeo g, int grid, size_t smem,
                            float w, hipStream_t st) {
   if (p == Precision::kF32) {
     if (nt == 3)
-      hipLaunchKernelGGL((k_bwd_mfma<float, 3>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
+      hipLaunchKernelGGL((k_bwd_mfma<float, 3, 2>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
     else
-      hipLaunchKernelGGL((k_bwd_mfma<float, 5>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
+      hipLaunchKernelGGL((k_bwd_mfma<float, 5, 2>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
   } else if (p == Precision::kF16) {
-    if (nt == 3)
-      hipLaunchKernelGGL((k_bwd_mfma<_Float16, 3>), dim3(grid), dim3(256), smem, st, (const _Float16*)Wb, g, w);
-    else
-      hipLaunchKernelGGL((k_bwd_mfma<_Float16, 5>), dim3(grid), dim3(256), smem, st, (const _Float16*)Wb, g, w);
+    launch_bwd_t<_Float16>(nt, occ, Wb, g, grid, smem, w, st);
   } else {
-    if (nt == 3)
-      hipLaunchKernelGGL((k_bwd_mfma<__bf16, 3>), dim3(grid), dim3(256), smem, st, (const __bf16*)Wb, g, w);
-    else
-      hipLaunchKernelGGL((k_bwd_mfma<__bf16, 5>), dim3(grid), dim3(256), smem, st, (const __bf16*)Wb, g, w);
+    launch_bwd_t<__bf16>(nt, occ, Wb, g, grid, smem, w, st);
   }
   return hipGetLastError();
 }
@@ -1460,14 +1508,16 @@ hipError_t mfma_kernel_attributes() {
       reinterpret_cast<const void*>(&k_fwd_tpz<float, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 4>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<float, 3>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<float, 5>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<float, 3, 2>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<float, 5, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<_Float16, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<_Float16, 4>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 5>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3, 2>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 5, 2>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3, 3>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 2>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5, 2>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 3>),
   };
   for (const void* k : kernels) {
     const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

@@ -583,7 +583,8 @@ mlf_status mlf_psf_create_compact(mlf_context ctx, int64_t N, int64_t Z, int64_t
 // H_ps(:,:,u,v,:)) and each view's extent per plane come from one streaming
 // pass over H; likewise for Ht, whose (u, v) slot is the view.  The passes run
 // in parallel over planes (each plane's sums in a fixed order, planes added
-// in order), the table fills in parallel over views.
+// in order); the forward tables fill in parallel over (plane, v, d2), each
+// task writing whole table rows, the backward tables over (view, plane).
 mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht, int64_t S,
                                 int64_t N, int64_t Z, mlf_psf* out) {
   clear_error();
@@ -600,63 +601,93 @@ mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht
   auto plane_of = [&](const float* A, int a, int b, int64_t z) {
     return A + SS * ((size_t)a + (size_t)N * ((size_t)b + (size_t)N * (size_t)z));
   };
-  // per-depth sums (recon:13-16), FP64, each plane in a fixed order
+  // One streaming pass per (array, plane): the plane's sum over the whole
+  // S x S x 15 x 15 slab (recon:13-16, FP64, in a fixed order) and, over the
+  // cropped region, per view the raw sum of H_ps(:,:,u,v,z) and the largest
+  // lenslet offset holding a non-zero tap (forward from H, backward from Ht).
+  // Each row is read once; the cropped work reuses it from cache.
   std::vector<double> hs(Z, 0.0), hts(Z, 0.0);
-  parallel_for(2 * Z, [&](int64_t q) {
-    const int64_t z = q % Z;
-    const float* A = (q < Z ? H : Ht) + slab * z;
-    double acc = 0.0;
-    for (size_t o = 0; o < slab; ++o) acc += A[o];
-    (q < Z ? hs : hts)[z] = acc;
-  });
-  // one pass per plane: weights (sum of H_ps(:,:,u,v,z)) and, per view, the
-  // largest lenslet offset holding a non-zero tap (forward and backward)
-  std::vector<double> wz((size_t)Z * kP, 0.0);
+  std::vector<double> wz((size_t)Z * kP, 0.0);  // raw (unnormalised) view sums of H per plane
   std::vector<int32_t> rfz((size_t)Z * kP, -1), rbz((size_t)Z * kP, -1);
-  parallel_for(Z, [&](int64_t z) {
+  auto floor15 = [](int64_t t) { return t >= 0 ? t / kN : -((-t + kN - 1) / kN); };
+  parallel_for(2 * Z, [&](int64_t q) {
+    const bool isH = q < Z;
+    const int64_t z = q % Z;
+    const float* A = isH ? H : Ht;
     double* w = wz.data() + (size_t)z * kP;
-    int32_t* rf_uv = rfz.data() + (size_t)z * kP;
-    int32_t* rb_uv = rbz.data() + (size_t)z * kP;
-    const double inv = 1.0 / hs[z];
+    int32_t* ext = (isH ? rfz : rbz).data() + (size_t)z * kP;
+    double total = 0.0;
     for (int b = 0; b < kN; ++b)
       for (int a = 0; a < kN; ++a) {
-        const float* P = plane_of(H, a, b, z);
-        for (int64_t y = 0; y < IMG; ++y) {
-          const int64_t yp = y + b - kC0;
-          if (yp < 0 || yp >= IMG) continue;
-          const int v = (int)(yp % kN), d2 = (int)(yp / kN) - R;
-          const float* row = P + (size_t)(i0 + y) * S + i0;
-          for (int64_t x = 0; x < IMG; ++x) {
-            const int64_t xp = x + a - kC0;
-            if (xp < 0 || xp >= IMG) continue;
-            const float val = row[x];
-            const int u = (int)(xp % kN);
-            w[u + kN * v] += (double)val * inv;
-            if (val != 0.f) {
-              const int d = std::max(std::abs((int)(xp / kN) - R), std::abs(d2));
-              if (d > rf_uv[u + kN * v]) rf_uv[u + kN * v] = d;
+        const float* P = plane_of(A, a, b, z);
+        for (int64_t y = 0; y < S; ++y) {
+          const float* row = P + (size_t)y * S;
+          double s4[4] = {0.0, 0.0, 0.0, 0.0};
+          int64_t x = 0;
+          for (; x + 4 <= S; x += 4)
+            for (int j = 0; j < 4; ++j) s4[j] += row[x + j];
+          for (; x < S; ++x) s4[0] += row[x];
+          total += (s4[0] + s4[1]) + (s4[2] + s4[3]);
+          const int64_t yc = y - i0;  // row of the cropped PSF
+          if (yc < 0 || yc >= IMG) continue;
+          const float* rc = row + i0;
+          if (isH) {
+            // H_ps(:,:,u,v,z) takes Hcrop(x, y, a, b) with xp = x + a - c0 =
+            // 15 (d1 + R) + u, yp = y + b - c0 = 15 (d2 + R) + v
+            const int64_t yp = yc + b - kC0;
+            if (yp < 0 || yp >= IMG) continue;
+            const int v = (int)(yp % kN), ad2 = std::abs((int)(yp / kN) - R);
+            double acc[kN] = {0.0};
+            int32_t m[kN];
+            for (int u = 0; u < kN; ++u) m[u] = -1;
+            const int64_t xp_lo = std::max<int64_t>(0, a - kC0), xp_hi = std::min<int64_t>(IMG, IMG + a - kC0);
+            const float* src = rc + (kC0 - a);  // src[xp] = Hcrop(xp - a + c0, y)
+            for (int64_t L = xp_lo / kN; L * kN < xp_hi; ++L) {
+              const int dl = std::abs((int)L - R);
+              const int64_t x0 = kN * L;
+              if (x0 >= xp_lo && x0 + kN <= xp_hi) {  // a whole lenslet: fixed trip count
+                unsigned nz = 0;
+                for (int u = 0; u < kN; ++u) {
+                  const float val = src[x0 + u];
+                  acc[u] += val;
+                  nz |= (unsigned)(val != 0.f) << u;
+                }
+                if (nz)
+                  for (int u = 0; u < kN; ++u)
+                    if (nz >> u & 1u) m[u] = std::max(m[u], dl);
+                continue;
+              }
+              for (int64_t xp = std::max(x0, xp_lo); xp < std::min(x0 + kN, xp_hi); ++xp) {
+                const int u = (int)(xp - x0);
+                const float val = src[xp];
+                acc[u] += val;
+                if (val != 0.f) m[u] = std::max(m[u], dl);
+              }
             }
+            for (int u = 0; u < kN; ++u) {
+              w[u + kN * v] += acc[u];
+              if (m[u] >= 0) ext[u + kN * v] = std::max(ext[u + kN * v], std::max(m[u], ad2));
+            }
+          } else {
+            // Ht's (u, v) slot is the view: d1 + R = floor((x - c0 + u) / 15),
+            // monotone in x, so the row's first and last non-zero bound |d1|
+            const int u = a, v = b;
+            int64_t x0 = 0, x1 = IMG - 1;
+            while (x0 < IMG && rc[x0] == 0.f) ++x0;
+            if (x0 == IMG) continue;
+            while (rc[x1] == 0.f) --x1;
+            const int d2 = (int)floor15(yc - kC0 + v) - R;
+            const int d1a = (int)floor15(x0 - kC0 + u) - R, d1b = (int)floor15(x1 - kC0 + u) - R;
+            const int mm = std::max(std::max(std::abs(d1a), std::abs(d1b)), std::abs(d2));
+            ext[u + kN * v] = std::max(ext[u + kN * v], mm);
           }
         }
       }
-    // Ht: slot (u, v) is the view; x = 15 (d1 + R) + c0 + s1 - u, so
-    // d1 + R = floor((x - c0 + u) / 15) with x - c0 + u >= -7
-    for (int v = 0; v < kN; ++v)
-      for (int u = 0; u < kN; ++u) {
-        const float* P = plane_of(Ht, u, v, z);
-        int32_t m = -1;
-        for (int64_t y = 0; y < IMG; ++y) {
-          const int d2 = (int)std::floor((double)(y - kC0 + v) / kN) - R;
-          const float* row = P + (size_t)(i0 + y) * S + i0;
-          for (int64_t x = 0; x < IMG; ++x) {
-            if (row[x] == 0.f) continue;
-            const int d1 = (int)std::floor((double)(x - kC0 + u) / kN) - R;
-            m = std::max(m, std::max(std::abs(d1), std::abs(d2)));
-          }
-        }
-        rb_uv[u + kN * v] = m;
-      }
+    (isH ? hs : hts)[z] = total;
   });
+  // weights (recon:30): W(u,v) = sum over z of the view's sum in H_ps / hs(z)
+  for (int64_t z = 0; z < Z; ++z)
+    for (int k = 0; k < kP; ++k) wz[(size_t)z * kP + k] /= hs[z];
   std::vector<double> wsum(kP, 0.0);
   for (int64_t z = 0; z < Z; ++z)
     for (int k = 0; k < kP; ++k) wsum[k] += wz[(size_t)z * kP + k];
@@ -692,45 +723,66 @@ mlf_status mlf_psf_create_dense(mlf_context ctx, const float* H, const float* Ht
     pvf += plane_block(rf[z]);
     pvb += plane_block(rb[z]);
   }
-  std::vector<float> tf((size_t)pvf * V), tb((size_t)pvb * V);
-  parallel_for(V, [&](int64_t k) {
-    const int u = uv[2 * k], v = uv[2 * k + 1];
-    for (int64_t z = 0; z < Z; ++z) {
-      // forward: rows of the raw (s1, s2) slab, d1 innermost (stride 15 px)
-      const int r = rf[z], K = 2 * r + 1;
-      float* F = tf.data() + (size_t)k * pvf + offf[z];
-      for (int s2 = 0; s2 < kN; ++s2)
-        for (int s1 = 0; s1 < kN; ++s1) {
-          const float* P = plane_of(H, s1, s2, z);
-          for (int d2 = -r; d2 <= r; ++d2) {
-            const int64_t y = v + (int64_t)kN * (d2 + R) - s2 + kC0;
-            for (int d1 = -r; d1 <= r; ++d1) {
-              const int64_t x = u + (int64_t)kN * (d1 + R) - s1 + kC0;
-              const bool in = x >= 0 && x < IMG && y >= 0 && y < IMG;
-              F[((int64_t)(d2 + r) * K + (d1 + r)) * kP + s2 * kN + s1] =
-                  in ? (float)((double)P[(size_t)(i0 + y) * S + (i0 + x)] / hs[z]) : 0.f;
-            }
+  // every element is written below (zeros for taps outside the crop), so the
+  // tables are not cleared first; pages are first touched by the fill threads
+  std::unique_ptr<float[]> tf(new float[(size_t)pvf * V]), tb(new float[(size_t)pvb * V]);
+  std::vector<int> kidx(kP, -1);
+  for (int k = 0; k < V; ++k) kidx[uv[2 * k] + kN * uv[2 * k + 1]] = k;
+  // forward tables, one task per (plane, view row v, lenslet row d2): the
+  // task writes rows d2 of the views (u, v) (K x 225 contiguous floats each)
+  // and reads the 225 raw rows y = v + 15 (d2 + R) + c0 - s2 of the planes
+  // (s1, s2) that feed them; phase (s1, s2) at lenslet offset d1 is pixel
+  // x = u + 15 (d1 + R) + c0 - s1 of its row
+  const int NR = 2 * R + 1;
+  parallel_for(Z * kN * NR, [&](int64_t q) {
+    const int64_t z = q / (kN * NR);
+    const int v = (int)(q / NR % kN), d2 = (int)(q % NR) - R;
+    const int r = rf[z], K = 2 * r + 1;
+    if (d2 < -r || d2 > r) return;
+    const double hz = hs[z];
+    const float* rowp[kN][kN];  // [s2][s1]; null for rows outside the crop
+    for (int s2 = 0; s2 < kN; ++s2) {
+      const int64_t y = v + (int64_t)kN * (d2 + R) + kC0 - s2;
+      for (int s1 = 0; s1 < kN; ++s1)
+        rowp[s2][s1] = y >= 0 && y < IMG ? plane_of(H, s1, s2, z) + (size_t)(i0 + y) * S + i0 : nullptr;
+    }
+    for (int u = 0; u < kN; ++u) {
+      const int k = kidx[u + kN * v];
+      if (k < 0) continue;
+      float* F = tf.get() + (size_t)k * pvf + offf[z] + (size_t)(d2 + r) * K * kP;
+      for (int d1 = -r; d1 <= r; ++d1) {
+        const int64_t xb = u + (int64_t)kN * (d1 + R) + kC0;
+        float* o = F + (size_t)(d1 + r) * kP;
+        for (int s2 = 0; s2 < kN; ++s2)
+          for (int s1 = 0; s1 < kN; ++s1) {
+            const int64_t x = xb - s1;
+            const float* rp = rowp[s2][s1];
+            o[s2 * kN + s1] = rp && x >= 0 && x < IMG ? (float)((double)rp[x] / hz) : 0.f;
           }
-        }
-      // backward: the view's own Ht slab, s1 innermost (contiguous)
-      const int q = rb[z], Q = 2 * q + 1;
-      float* B = tb.data() + (size_t)k * pvb + offb[z];
-      const float* P = plane_of(Ht, u, v, z);
-      for (int d2 = -q; d2 <= q; ++d2)
-        for (int s2 = 0; s2 < kN; ++s2) {
-          const int64_t y = (int64_t)kN * (d2 + R) + kC0 + s2 - v;
-          for (int d1 = -q; d1 <= q; ++d1)
-            for (int s1 = 0; s1 < kN; ++s1) {
-              const int64_t x = (int64_t)kN * (d1 + R) + kC0 + s1 - u;
-              const bool in = x >= 0 && x < IMG && y >= 0 && y < IMG;
-              B[((int64_t)(d2 + q) * Q + (d1 + q)) * kP + s2 * kN + s1] =
-                  in ? (float)((double)P[(size_t)(i0 + y) * S + (i0 + x)] / hts[z]) : 0.f;
-            }
-        }
+      }
     }
   });
-  return mlf_psf_create_compact(ctx, N, Z, V, uv.data(), wv.data(), rf.data(), tf.data(),
-                                rb.data(), tb.data(), out);
+  // backward tables: the view's own Ht slab, s1 innermost (contiguous)
+  parallel_for((int64_t)V * Z, [&](int64_t q) {
+    const int64_t k = q / Z, z = q % Z;
+    const int u = uv[2 * k], v = uv[2 * k + 1];
+    const int qr = rb[z], Q = 2 * qr + 1;
+    float* B = tb.get() + (size_t)k * pvb + offb[z];
+    const float* P = plane_of(Ht, u, v, z);
+    for (int d2 = -qr; d2 <= qr; ++d2)
+      for (int s2 = 0; s2 < kN; ++s2) {
+        const int64_t y = (int64_t)kN * (d2 + R) + kC0 + s2 - v;
+        for (int d1 = -qr; d1 <= qr; ++d1)
+          for (int s1 = 0; s1 < kN; ++s1) {
+            const int64_t x = (int64_t)kN * (d1 + R) + kC0 + s1 - u;
+            const bool in = x >= 0 && x < IMG && y >= 0 && y < IMG;
+            B[((int64_t)(d2 + qr) * Q + (d1 + qr)) * kP + s2 * kN + s1] =
+                in ? (float)((double)P[(size_t)(i0 + y) * S + (i0 + x)] / hts[z]) : 0.f;
+          }
+      }
+  });
+  return mlf_psf_create_compact(ctx, N, Z, V, uv.data(), wv.data(), rf.data(), tf.get(),
+                                rb.data(), tb.get(), out);
 }
 
 mlf_status mlf_psf_info(mlf_psf psf, int64_t* nviews, int64_t* Z, int32_t* rf, int32_t* rb) {
@@ -930,7 +982,7 @@ static mlf_status patch_view_step(mlf_patch_s* p, int k) {
                                            p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
   } else {
     const PrecPack& pk = ps->pack[(int)p->kp];
-    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->bwd_nt, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
+    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->bwd_nt, p->bwd_occ, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
                                bwd_geo(p, k), p->nblk_b, p->smem_b, (float)ps->weight[k], st));
   }
   MLF_CHECK(tim_end(p));
@@ -1085,7 +1137,7 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   MLF_CHECK(upload_vec(p->d_slot, std::vector<SlotDesc>{sd}));
   p->smem_f = p->tpz ? tpz_smem_bytes(p->kp, rfmax, p->fwd_mt) : fwd_smem_bytes(p->kp, rfmax, p->bm2);
   p->fwd_nbuf = fwd_nbuf(p->kp, rfmax, p->bm2);
-  p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax, p->bwd_nt);
+  p->smem_b = bwd_smem_bytes(p->kp, rbmax, epmax, p->bwd_nt, p->bwd_occ);
   if (p->smem_f > 160 * 1024 || p->smem_b > 160 * 1024)
     return set_error(MLF_ERR_UNSUPPORTED,
                      "PSF radius too large for the LDS tiles (rf_max=%d, rb_max=%d)", rfmax, rbmax);
@@ -1196,11 +1248,13 @@ mlf_status mlf_patch_create(mlf_psf psf, int64_t n1, int64_t n2, mlf_precision p
     p->fwd_mt = t0.mt;
     p->bwd_nt = t0.nt;
     p->bwd_grp = t0.grp;
+    p->bwd_occ = t0.occ;
     TileShape t;
     if (tune_cache_get(tune_key(p.get(), "patch"), p->kp, &t)) {
       p->fwd_mt = t.mt;
       p->bwd_nt = t.nt;
       p->bwd_grp = t.grp;
+      p->bwd_occ = t.occ;
     }
     MLF_CHECK(patch_setup_tiles(p.get()));
   }
@@ -1476,6 +1530,7 @@ mlf_status mlf_patch_tile_shape(mlf_patch p, mlf_tile_shape* cur) {
   cur->fwd_columns_per_wave = p->fwd_mt;
   cur->bwd_phase_tiles = p->bwd_nt;
   cur->bwd_groups_per_entry = p->bwd_grp;
+  cur->bwd_waves_per_simd = p->bwd_occ;
   cur->ms_per_view = 0.0;
   return MLF_OK;
 }
@@ -1485,25 +1540,28 @@ mlf_status mlf_patch_set_tile_shape(mlf_patch p, const mlf_tile_shape* shape) {
   if (!p || !shape) return set_error(MLF_ERR_INVALID_ARGUMENT, "NULL argument");
   if (p->reference) return set_error(MLF_ERR_UNSUPPORTED, "reference kernels have one shape");
   if (p->batch) return set_error(MLF_ERR_INVALID_ARGUMENT, "detach the patch from its batch first");
-  const TileShape want{shape->fwd_columns_per_wave, shape->bwd_phase_tiles, shape->bwd_groups_per_entry};
+  const TileShape want{shape->fwd_columns_per_wave, shape->bwd_phase_tiles, shape->bwd_groups_per_entry,
+                       shape->bwd_waves_per_simd};
   bool ok = false;
   for (const auto& c : tile_candidates(p->kp)) ok = ok || c == want;
   if (!ok)
-    return set_error(MLF_ERR_INVALID_ARGUMENT, "tile shape (%d, %d, %d) is not built for this precision",
+    return set_error(MLF_ERR_INVALID_ARGUMENT, "tile shape (%d, %d, %d, %d) is not built for this precision",
                      (int)shape->fwd_columns_per_wave, (int)shape->bwd_phase_tiles,
-                     (int)shape->bwd_groups_per_entry);
+                     (int)shape->bwd_groups_per_entry, (int)shape->bwd_waves_per_simd);
   std::lock_guard<std::mutex> lk(p->mu);
   MLF_CHECK(ctx_activate(p->psf->ctx));
   MLF_HIP(hipStreamSynchronize(p->psf->ctx->stream));
-  const int mt = p->fwd_mt, nt = p->bwd_nt, gr = p->bwd_grp;
+  const int mt = p->fwd_mt, nt = p->bwd_nt, gr = p->bwd_grp, oc = p->bwd_occ;
   p->fwd_mt = want.mt;
   p->bwd_nt = want.nt;
   p->bwd_grp = want.grp;
+  p->bwd_occ = want.occ;
   if (patch_setup_tiles(p) != MLF_OK) {
     const std::string e = last_error();
     p->fwd_mt = mt;
     p->bwd_nt = nt;
     p->bwd_grp = gr;
+    p->bwd_occ = oc;
     MLF_CHECK(patch_setup_tiles(p));
     return set_error(MLF_ERR_UNSUPPORTED, "%s", e.c_str());
   }
@@ -1534,7 +1592,7 @@ mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) 
   MLF_HIP(hipEventCreate(&e0.e));
   MLF_HIP(hipEventCreate(&e1.e));
   double best_ms = 1e300;
-  TileShape bt{p->fwd_mt, p->bwd_nt, p->bwd_grp};
+  TileShape bt{p->fwd_mt, p->bwd_nt, p->bwd_grp, p->bwd_occ};
   // two rounds over the candidates, each candidate keeping its faster round:
   // shapes a few per cent apart are otherwise decided by clock and cache noise
   const auto cands = tile_candidates(p->kp);
@@ -1546,6 +1604,7 @@ mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) 
       p->fwd_mt = c.mt;
       p->bwd_nt = c.nt;
       p->bwd_grp = c.grp;
+      p->bwd_occ = c.occ;
       if (patch_setup_tiles(p) != MLF_OK) continue;  // shape does not fit this PSF
       MLF_CHECK(patch_refresh_shadow(p));
       MLF_CHECK(patch_view_step(p, 0));  // warm-up
@@ -1569,6 +1628,7 @@ mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) 
   p->fwd_mt = bt.mt;
   p->bwd_nt = bt.nt;
   p->bwd_grp = bt.grp;
+  p->bwd_occ = bt.occ;
   MLF_CHECK(patch_setup_tiles(p));
   MLF_CHECK(patch_refresh_shadow(p));
   tune_cache_put(tune_key(p, "patch"), bt, best_ms);
@@ -1576,6 +1636,7 @@ mlf_status mlf_patch_autotune(mlf_patch p, int32_t views, mlf_tile_shape* best) 
     best->fwd_columns_per_wave = bt.mt;
     best->bwd_phase_tiles = bt.nt;
     best->bwd_groups_per_entry = bt.grp;
+    best->bwd_waves_per_simd = bt.occ;
     best->ms_per_view = best_ms;
   }
   return MLF_OK;

@@ -105,6 +105,7 @@ struct mlf_patch_s {
   int cw = 16;                         // Toeplitz: input columns per workgroup (4 * fwd_mt)
   int fwd_mt = 0, bwd_nt = 0;          // tile shape: forward columns per wave, backward N-tiles
   int bwd_grp = 0;                     // phase groups per backprojection entry (0: all of them)
+  int bwd_occ = 2;                     // backprojection waves per SIMD (16-bit: 2 or 3)
   float xs = 1.f;                      // FP16: shadow = X * xs (chosen so that X * xs ~ 1)
   mlf::DevBuf<float> partial;               // forward window partials
   mlf::DevBuf<uint16_t> Xb;                 // BF16 shadow of X, [lenslet][kShadowRow] (BF16 Toeplitz)
@@ -163,14 +164,16 @@ mlf_status batch_forget_patch(mlf_patch_s* p);  // detach before destruction
 constexpr int kXcds = 8;
 void xcd_interleave(std::vector<int32_t>& ent, int width, const std::vector<int>& group);
 // (Re)build a patch's work entries, partial buffers and slot descriptor for
-// its tile shape (fwd_mt, bwd_nt, bwd_grp); the caller synchronises the
+// its tile shape (fwd_mt, bwd_nt, bwd_grp, bwd_occ); the caller synchronises the
 // stream first.
 mlf_status patch_setup_tiles(mlf_patch_s* p);
 // A tile shape: forward input columns per wave, backprojection 16-phase
 // tiles per group, phase groups per backprojection entry (0: all).
 struct TileShape {
-  int mt = 0, nt = 0, grp = 0;
-  bool operator==(const TileShape& o) const { return mt == o.mt && nt == o.nt && grp == o.grp; }
+  int mt = 0, nt = 0, grp = 0, occ = 2;
+  bool operator==(const TileShape& o) const {
+    return mt == o.mt && nt == o.nt && grp == o.grp && occ == o.occ;
+  }
 };
 // Candidate tile shapes of a precision, default first.
 std::vector<TileShape> tile_candidates(Precision kp);

@@ -4,7 +4,7 @@
 // fastest measured tile shape.  It lives for the process; if MLF_TUNE_CACHE
 // names a file, entries are loaded from it when the name is first seen and
 // appended to it when measured, one line per entry:
-//   "mlf-tune-v3 <key> <fwd_mt> <bwd_nt> <bwd_groups> <ms per view>".
+//   "mlf-tune-v4 <key> <fwd_mt> <bwd_nt> <bwd_groups> <bwd_waves_per_simd> <ms per view>".
 // Lines without the format tag (older builds) are ignored, and a cached shape
 // is used only if it is one of the shapes this build has for the patch's
 // precision: a stale or edited entry falls back to the default shape.
@@ -25,7 +25,7 @@
 namespace mlf {
 
 namespace {
-constexpr const char* kTuneTag = "mlf-tune-v3";
+constexpr const char* kTuneTag = "mlf-tune-v4";
 std::mutex g_tune_mu;
 std::map<std::string, TileShape> g_tune;
 std::string g_tune_loaded_path;  // cache file whose entries are loaded
@@ -40,7 +40,7 @@ void load_locked() {
     std::istringstream is(line);
     std::string tag, key;
     TileShape t;
-    if (is >> tag >> key >> t.mt >> t.nt >> t.grp && tag == kTuneTag) g_tune[key] = t;
+    if (is >> tag >> key >> t.mt >> t.nt >> t.grp >> t.occ && tag == kTuneTag) g_tune[key] = t;
   }
 }
 
@@ -53,8 +53,11 @@ std::vector<TileShape> tile_candidates(Precision kp) {
   const int mt0 = kp == Precision::kF32 ? kTpzMTF32 : kTpzMTBF16;
   std::vector<TileShape> c;
   for (int grp : {0, 1})  // all phase groups per entry (one ratio window), or one
-    for (int mt : {mt0, mt0 / 2})
-      for (int nt : {kBwdNTiles, 3}) c.push_back(TileShape{mt, nt, grp});
+    for (int mt : {mt0, mt0 / 2}) {
+      for (int nt : {kBwdNTiles, 3}) c.push_back(TileShape{mt, nt, grp, 2});
+      // 16-bit: 3 N-tiles at three waves per SIMD
+      if (kp != Precision::kF32) c.push_back(TileShape{mt, 3, grp, 3});
+    }
   return c;
 }
 
@@ -95,7 +98,7 @@ void tune_cache_put(const std::string& key, const TileShape& t, double ms) {
   const char* path = std::getenv("MLF_TUNE_CACHE");
   if (!path || !*path) return;
   std::ofstream f(path, std::ios::app);
-  f << kTuneTag << " " << key << " " << t.mt << " " << t.nt << " " << t.grp << " " << ms << "\n";
+  f << kTuneTag << " " << key << " " << t.mt << " " << t.nt << " " << t.grp << " " << t.occ << " " << ms << "\n";
 }
 
 }  // namespace mlf

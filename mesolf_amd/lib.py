@@ -487,13 +487,15 @@ class Patch:
 
     def tile_shape(self):
         """(forward columns per wave, backprojection phase tiles, phase groups
-        per backprojection entry (0: all))."""
+        per backprojection entry (0: all), backprojection waves per SIMD)."""
         t = _TileShape()
         _check(self._lib.mlf_patch_tile_shape(self.handle, C.byref(t)))
-        return (t.fwd_columns_per_wave, t.bwd_phase_tiles, t.bwd_groups_per_entry)
+        return (t.fwd_columns_per_wave, t.bwd_phase_tiles, t.bwd_groups_per_entry, t.bwd_waves_per_simd)
 
-    def set_tile_shape(self, fwd_columns_per_wave: int, bwd_phase_tiles: int, bwd_groups_per_entry: int = 0):
-        t = _TileShape(int(fwd_columns_per_wave), int(bwd_phase_tiles), int(bwd_groups_per_entry), 0.0)
+    def set_tile_shape(self, fwd_columns_per_wave: int, bwd_phase_tiles: int, bwd_groups_per_entry: int = 0,
+                       bwd_waves_per_simd: int = 2):
+        t = _TileShape(int(fwd_columns_per_wave), int(bwd_phase_tiles), int(bwd_groups_per_entry),
+                       int(bwd_waves_per_simd), 0.0)
         _check(self._lib.mlf_patch_set_tile_shape(self.handle, C.byref(t)))
 
     def forward_view(self, k: int):
@@ -545,12 +547,14 @@ class Patch:
 # -------------------------------------------------------------------- drop-ins
 class _TileShape(C.Structure):
     _fields_ = [("fwd_columns_per_wave", C.c_int32), ("bwd_phase_tiles", C.c_int32),
-                ("bwd_groups_per_entry", C.c_int32), ("ms_per_view", C.c_double)]
+                ("bwd_groups_per_entry", C.c_int32), ("bwd_waves_per_simd", C.c_int32),
+                ("ms_per_view", C.c_double)]
 
 
 def _shape_dict(t):
     return {"fwd_columns_per_wave": t.fwd_columns_per_wave, "bwd_phase_tiles": t.bwd_phase_tiles,
-            "bwd_groups_per_entry": t.bwd_groups_per_entry, "ms_per_view": t.ms_per_view}
+            "bwd_groups_per_entry": t.bwd_groups_per_entry, "bwd_waves_per_simd": t.bwd_waves_per_simd,
+            "ms_per_view": t.ms_per_view}
 
 
 class Batch:

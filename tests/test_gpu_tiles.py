@@ -16,8 +16,10 @@ from tests.helpers import oracle_compact_from_abi, rel_l2
 pytestmark = pytest.mark.gpu
 
 WL = synth.Workload("odd_37x21", 37, 21, (-20, 0, 12), 1, 1)
-SHAPES = {"fp32": [(mt, nt, g) for g in (0, 1) for mt in (2, 1) for nt in (5, 3)],
-          "bf16": [(mt, nt, g) for g in (0, 1) for mt in (4, 2) for nt in (5, 3)]}
+# (forward columns per wave, backprojection N-tiles, phase groups per entry,
+# backprojection waves per SIMD), in the tuner's candidate order
+SHAPES = {"fp32": [(mt, nt, g, 2) for g in (0, 1) for mt in (2, 1) for nt in (5, 3)],
+          "bf16": [s for g in (0, 1) for mt in (4, 2) for s in ((mt, 5, g, 2), (mt, 3, g, 2), (mt, 3, g, 3))]}
 VOL_TOL = {"fp32": 1e-4, "bf16": 2e-3}
 
 
@@ -59,9 +61,15 @@ def test_every_tile_shape_matches_oracle(problem, prec):
     if prec == "fp32":
         for x in outs[1:]:
             assert rel_l2(x, outs[0]) <= 1e-6
-    # phase groups per entry only move work between workgroups: bit-identical
-    for i in range(4):
-        assert np.array_equal(outs[i], outs[i + 4]), SHAPES[prec][i]
+    # phase groups per entry and waves per SIMD only move work between
+    # workgroups and registers: bit-identical
+    half = len(SHAPES[prec]) // 2
+    for i in range(half):
+        assert np.array_equal(outs[i], outs[i + half]), SHAPES[prec][i]
+    for i, s in enumerate(SHAPES[prec]):
+        if s[3] == 3:
+            j = SHAPES[prec].index((s[0], s[1], s[2], 2))
+            assert np.array_equal(outs[i], outs[j]), (s, SHAPES[prec][j])
 
 
 def test_unknown_tile_shape_rejected(problem):
@@ -71,7 +79,13 @@ def test_unknown_tile_shape_rejected(problem):
         p.set_tile_shape(3, 5)
     with pytest.raises(mesolf_amd.InvalidInput):
         p.set_tile_shape(4, 5, 2)
+    with pytest.raises(mesolf_amd.InvalidInput):
+        p.set_tile_shape(4, 5, 0, 3)  # three waves per SIMD only with 3 N-tiles
     p.close()
+    q = mesolf_amd.Patch(psf, WL.n1, WL.n2, precision="fp32")
+    with pytest.raises(mesolf_amd.InvalidInput):
+        q.set_tile_shape(2, 3, 0, 3)  # FP32 has only the two-wave variant
+    q.close()
 
 
 def test_autotune_picks_a_shape_and_caches_it(problem):
@@ -81,7 +95,8 @@ def test_autotune_picks_a_shape_and_caches_it(problem):
     p.init_volume()
     before = p.get_volume()
     best = p.autotune(views=4)
-    shape = (best["fwd_columns_per_wave"], best["bwd_phase_tiles"], best["bwd_groups_per_entry"])
+    shape = (best["fwd_columns_per_wave"], best["bwd_phase_tiles"], best["bwd_groups_per_entry"],
+             best["bwd_waves_per_simd"])
     assert shape in SHAPES["bf16"] and best["ms_per_view"] > 0
     assert p.tile_shape() == shape
     assert np.array_equal(p.get_volume(), before)  # tuning restores the volume
@@ -107,7 +122,8 @@ def test_tuned_batch_matches_alone(problem):
         b.attach(p)
         ps.append(p)
     best = b.autotune(views=4)
-    shape = (best["fwd_columns_per_wave"], best["bwd_phase_tiles"], best["bwd_groups_per_entry"])
+    shape = (best["fwd_columns_per_wave"], best["bwd_phase_tiles"], best["bwd_groups_per_entry"],
+             best["bwd_waves_per_simd"])
     assert shape in SHAPES["bf16"]
     b.run(2)
     for p, g, im in zip(ps, grids, imgs):
@@ -132,10 +148,10 @@ def test_tune_cache_file_entries_are_validated(problem, tmp_path, monkeypatch):
     rf = ",".join(str(int(r)) for r in psf.radii()[0])
     path = tmp_path / "tune.txt"
     path.write_text(
-        f"mlf-tune-v3 patch|bf16|{WL.n1}x{WL.n2}|rf,{rf} 3 7 0 0.1\n"     # not a built shape
-        f"mlf-tune-v2 patch|bf16|16x30|rf,{rf} 2 3 0.1\n"                 # older format tag
-        f"mlf-tune-v3 patch|fp16|{WL.n1}x{WL.n2}|rf,{rf} 2 3 1 0.1\n"      # valid, fp16 key only
-        f"mlf-tune-v3 patch|fp32|16x30|rf,{rf} 1 3 0 0.1\n")               # valid fp32 entry
+        f"mlf-tune-v4 patch|bf16|{WL.n1}x{WL.n2}|rf,{rf} 3 7 0 2 0.1\n"   # not a built shape
+        f"mlf-tune-v3 patch|bf16|16x30|rf,{rf} 2 3 0 0.1\n"               # older format tag
+        f"mlf-tune-v4 patch|fp16|{WL.n1}x{WL.n2}|rf,{rf} 2 3 1 3 0.1\n"    # valid, fp16 key only
+        f"mlf-tune-v4 patch|fp32|16x30|rf,{rf} 1 3 0 2 0.1\n")             # valid fp32 entry
     monkeypatch.setenv("MLF_TUNE_CACHE", str(path))
     shapes = {}
     for prec, grid in (("bf16", (WL.n1, WL.n2)), ("fp16", (WL.n1, WL.n2)), ("bf16", (16, 30)),
@@ -143,7 +159,7 @@ def test_tune_cache_file_entries_are_validated(problem, tmp_path, monkeypatch):
         p = mesolf_amd.Patch(psf, *grid, precision=prec)
         shapes[(prec, grid)] = p.tile_shape()
         p.close()
-    assert shapes[("bf16", (WL.n1, WL.n2))] == (4, 5, 0)   # bogus entry ignored
-    assert shapes[("fp16", (WL.n1, WL.n2))] == (2, 3, 1)   # its own key
-    assert shapes[("bf16", (16, 30))] == (4, 5, 0)         # older-format line ignored
-    assert shapes[("fp32", (16, 30))] == (1, 3, 0)
+    assert shapes[("bf16", (WL.n1, WL.n2))] == (4, 5, 0, 2)   # bogus entry ignored
+    assert shapes[("fp16", (WL.n1, WL.n2))] == (2, 3, 1, 3)   # its own key
+    assert shapes[("bf16", (16, 30))] == (4, 5, 0, 2)         # older-format line ignored
+    assert shapes[("fp32", (16, 30))] == (1, 3, 0, 2)
