@@ -32,10 +32,28 @@ inline __host__ __device__ int bwd_ring_bytes(int Wd, int nt, int elem_bytes, in
   const int ring = bwd_stages(occ) * nt * 16 * 32 * elem_bytes;
   return rs > ring ? rs : ring;
 }
+// 16-bit backprojection tap order.  Row length K is padded to Kp8 (a multiple
+// of 8) and the row count to a multiple of 4; K-step ks (32 taps) holds the
+// 8-tap pieces (e2 = 4 (ks / nch) + q, e1 = 8 (ks % nch) + 0..7), q = 0..3,
+// nch = Kp8 / 8.  The four pieces of a step then sit at one window column in
+// four consecutive rows, so with window rows of a multiple of 16 cells the
+// four 16-lane groups of an A-fragment read hit the same LDS bank groups.
+inline __host__ __device__ int bwd_quad_steps(int K) { return ((K + 7) / 8) * ((K + 3) / 4); }
+inline __host__ __device__ void bwd_quad_tap(int t, int K, int* e2, int* e1) {
+  const int nch = (K + 7) / 8, ks = t / 32, w = t % 32, b = ks / nch;
+  *e2 = 4 * b + w / 8;
+  *e1 = 8 * (ks - b * nch) + w % 8;
+}
+// Window row stride (cells) of the 16-bit backprojection: a multiple of 16
+// when the workgroup's LDS still fits (otherwise Wd, with bank conflicts).
+inline __host__ __device__ int bwd_row_cells(int Wd, int nt, int occ) {
+  const int a = (Wd + 15) & ~15;
+  return bwd_ring_bytes(Wd, nt, 2, occ) + 16 * Wd * a <= 160 * 1024 ? a : Wd;
+}
 constexpr int kRatioLens = 32;    // lenslets per ratio work entry
 constexpr int kRatioSlices = 8;   // plane slices summed in parallel per entry (x kRatioLens = 256 threads)
 constexpr int kBwdTapPad = 32;    // FP32 backward taps padded to a multiple of 32
-constexpr int kBwdTapPad16 = 64;  // 16-bit: two K-steps of 32 (the K loop runs in pairs)
+constexpr int kBwdTapPad16 = 64;  // 16-bit: an even number of K-steps of 32 (the K loop runs in pairs)
 constexpr int kTpzMTF32 = 2;      // Toeplitz forward: default input columns per wave, f32 (1 also built)
 constexpr int kTpzMTBF16 = 4;     // Toeplitz forward: default input columns per wave, bf16 (2 also built)
 constexpr int kShadowRow = 256;   // BF16 volume shadow: phases per lenslet (zero beyond 225)
@@ -98,7 +116,8 @@ struct PackGeo {
   const int64_t* src_off;  // [Z] offset in the ABI view block
   const int64_t* dst_off;  // [Z] offset in the packed view block
   int64_t src_view, dst_view;
-  int row8;                // backward pack: tap rows padded to a multiple of 8 (16-bit types)
+  int quad;                // backward pack (16-bit types): taps in K-steps of four 8-tap pieces
+                           // from four consecutive rows (bwd_quad_tap)
   const float* scale;      // [V][Z] factor applied to the taps (FP16), or null
 };
 

@@ -562,11 +562,21 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
       // straight-line code, and the tile count is tested twice per step
       // rather than once per group of MFMAs (a branch makes the MFMAs after
       // it wait for every LDS read issued before it)
+      // B fragments are read two K-steps ahead (crossing into the next
+      // N-tile's first steps), so their LDS latency hides behind the MFMAs of
+      // the two steps before; reading a tile that the plane does not use is
+      // harmless (the buffer holds kTpzNT tiles).
+      auto load_seq = [&](int q) {  // q-th step of the tile sequence
+        return load_b(q % NKS, min(q / NKS, kTpzNT - 1));
+      };
+      typename M::frag bq0 = load_seq(0), bq1 = load_seq(1);
       auto tile = [&](auto ntc) __attribute__((always_inline)) {
         constexpr int nt = decltype(ntc)::value;
   #pragma unroll
         for (int ks = 0; ks < NKS; ++ks) {
-          const typename M::frag bf = load_b(ks, nt);
+          const typename M::frag bf = bq0;
+          bq0 = bq1;
+          bq1 = load_seq(nt * NKS + ks + 2);
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(a[mt][ks], bf, acc[mt][nt]);
           if constexpr (nt == 0) issue_piece(ks);
@@ -860,8 +870,11 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   constexpr int MT = 4;  // NT: N-tiles (16 phases each) per workgroup
   constexpr int STG = bwd_stages(OCC);  // 16-bit tap ring depth
   // M-tiles whose X (and the first 1/B1 rows) load before the K loop; with
-  // none, the epilogue loads each tile one tile ahead
-  constexpr int PRE = OCC >= 3 ? 0 : MT - 1;
+  // none, the epilogue loads each tile one tile ahead.  16-bit with 3 N-tiles
+  // has the registers to load every tile's X and 1/B1 rows before the K loop,
+  // so its epilogue waits for no load.
+  constexpr int PRE = OCC >= 3 ? 0 : (NT == 3 && M::KPL > 1 ? MT : MT - 1);
+  constexpr bool TPRE = PRE == MT;
   extern __shared__ __align__(16) char smem[];
   const int nent = uni(*g.count);
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
@@ -908,14 +921,16 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const bool ok1 = a1 < n1;
     const int a1c = ok1 ? a1 : n1 - 1;
     const int c1 = norm_class(a1c, n1, r);
-    // window cell of this lane's lenslet in M-tile mt: base + mt * Wd
-    const int base = (lr + r) + (4 * w + r) * Wd;
+    // window cell of this lane's lenslet in M-tile mt: base + mt * Wr (16-bit
+    // windows have rows of Wr = bwd_row_cells(Wd) cells; FP32: Wd)
+    const int Wr = KPL > 1 ? bwd_row_cells(Wd, NT, OCC) : Wd;
+    const int base = (lr + r) + (4 * w + r) * Wr;
 
     // ratio window
     float* Rw = reinterpret_cast<float*>(smem);  // FP32: [Wd][Wd]
     int* toff = reinterpret_cast<int*>(smem + sizeof(float) * ((Wd * Wd + 7) & ~7));  // FP32: [epad]
     using frag_t = typename M::frag;
-    // 16-bit: [Wd][Wd] after the staging window Rs / tap ring (which alias)
+    // 16-bit: [Wd][Wr] after the staging window Rs / tap ring (which alias)
     frag_t* Rv = reinterpret_cast<frag_t*>(smem + bwd_ring_bytes(Wd, NT, (int)sizeof(T), OCC));
     if constexpr (KPL == 1) {
       // FP32: scalar im2col reads, taps e1-fastest
@@ -930,10 +945,9 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       }
       lds_barrier();
     } else {
-      // 16-bit: taps ordered (e2, e1) with e1 padded to a multiple of 8, so a
-      // lane's 8 taps of a K-step are 8 consecutive e1 in one row.  Cell (x, y)
-      // of the pre-shifted window holds R(x-j, y) for j = 0..7, so each A
-      // fragment is a single 16-byte LDS read.
+      // 16-bit: a lane's 8 taps of a K-step are 8 consecutive e1 in one row
+      // (bwd_quad_tap).  Cell (x, y) of the pre-shifted window holds R(x-j, y)
+      // for j = 0..7, so each A fragment is a single 16-byte LDS read.
       float* Rs = reinterpret_cast<float*>(smem);  // [Wd][Wd]
       gather_window(R, n1, n2, 16 * b1 - r, 16 * b2 - r, Wd, Rs);
       lds_barrier();
@@ -958,7 +972,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   #pragma unroll
             for (int j = 0; j < 8; ++j)
               v[j] = (T)(is_f16<T>() ? fminf(rv[q][j], 65504.f) : rv[q][j]);  // FP16: clamp to the largest finite value
-            Rv[o] = v;
+            const int y = o / Wd;
+            Rv[o + y * (Wr - Wd)] = v;  // cell (o % Wd, y)
           }
         }
       }
@@ -980,7 +995,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // loop's operands and it is loaded when the loop ends); the 1/B1 rows
       // of tile 0 too, those of later tiles (L2-resident) when the tile is
       // reached
-      floatx4_t xv[MT][NT], bv[NT];
+      floatx4_t xv[MT][NT], bv[NT], bva[TPRE ? MT : 1][NT];
       auto ldx = [&](int mt, floatx4_t* dst) {
         const int a2 = 16 * b2 + 4 * w + mt;
         const int xo = ((a2 < n2 ? a2 : n2 - 1) * n1 + a1c) * kXRow;
@@ -1001,7 +1016,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           if (mt >= mt0 && mt < mt1) ldx(mt, xv[mt]);
       };
       auto load_t = [&](int mt) { ldt(mt, bv); };
-      constexpr int kEpiLoads = PRE > 0 ? PRE * NT + NT : 0;  // load_x(0, PRE) + load_t(0)
+      // loads issued before the K loop: load_x(0, PRE) and the first (or every) 1/B1 row set
+      constexpr int kEpiLoads = PRE > 0 ? PRE * NT + (TPRE ? MT * NT : NT) : 0;
 
       if constexpr (KPL == 1) {
         load_x(0, MT - 1);
@@ -1021,7 +1037,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           const int o = toff[ks * KSTEP + lg];
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
-            const typename M::frag af = Rw[base + mt * Wd + o];
+            const typename M::frag af = Rw[base + mt * Wr + o];
   #pragma unroll
             for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
           }
@@ -1032,21 +1048,19 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         // STG K-step slices [16 NT rows][32 taps], STG - 1 steps ahead;
         // slices past the last step re-load the last one, so every wave
         // issues the same number of copies per step and waits with a fixed
-        // count.  The window offset of a lane's 8 taps is computed, not looked
-        // up: t0 = 32 ks + 8 lg -> (e2, e1) = divmod(t0, Kp8) with a 20-bit
-        // reciprocal (exact for t0 * Kp8 < 2^20).
-        const int Kp8 = (K + 7) & ~7;
-        const unsigned rcp = ((1u << 20) + Kp8 - 1) / Kp8;
+        // count.  The window offset of a lane's 8 taps follows the step's row
+        // block and column chunk (bwd_quad_tap), stepped without divisions.
+        const int nch = (K + 7) / 8;
         T* Bq = reinterpret_cast<T*>(smem);  // ring (aliases the dead staging window Rs)
         constexpr int kSlice = NT * 16 * 32;  // elements per K-step slice
         const int ncopy = w < NT ? (NT - w + 3) / 4 : 0;  // copies per step issued by this wave
-        // A copy moves one N-tile's slice, 16 rows x 32 taps, as 64 16-byte
-        // pieces; DMA lane s lands at byte 16 s.  Piece (row, 8-tap group q)
-        // sits at s = 4 row + ((q + row / 4) % 4), so the 16 lanes that read
-        // one group q at rows 0..15 together hit 16 distinct 4-bank groups
-        // (row-major pieces, 64 bytes per row, would be a 4-way conflict).
-        const int tap_lane = ((lane >> 2) * epad + (((lane & 3) - (lane >> 4)) & 3) * 8) * (int)sizeof(T);
-        const int tap_slot = 8 * (4 * lr + ((lg + (lr >> 2)) & 3));  // element offset of (row lr, group lg)
+        // A copy moves one N-tile's slice (16 rows x 32 taps) as 64 16-byte
+        // pieces, DMA lane s landing at byte 16 s.  Piece (row, 8-tap group q)
+        // goes to s = 16 q + row, so the 16 lanes reading group q at rows
+        // 0..15 read 16 consecutive pieces and the four groups the same bank
+        // groups (the pattern ds_read_b128 serves without conflicts).
+        const int tap_lane = ((lane & 15) * epad + (lane >> 4) * 8) * (int)sizeof(T);
+        const int tap_slot = (16 * lg + lr) * 8;  // element offset of (row lr, 8-tap group lg) in an N-tile slice
         auto issue_taps = [&](int ks, int slot) {
           const int kc = min(ks, nks - 1);
           char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
@@ -1057,31 +1071,64 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
                                              16, 0, 0);
           }
         };
+        // SPB K-steps per barrier: a step's copies are waited for, and the
+        // ring refilled, once per SPB steps; the ring then runs STG - SPB
+        // steps ahead.  nks is even (taps padded to 64).
+        constexpr int SPB = OCC >= 3 ? 1 : 2;
+        static_assert(STG >= 2 * SPB, "ring too shallow");
   #pragma unroll
-        for (int q = 0; q < STG - 1; ++q) issue_taps(q, q);
+        for (int q = 0; q < STG - SPB; ++q) issue_taps(q, q);
         if constexpr (PRE > 0) {
           load_x(0, PRE);  // after the first slices: their waits need not cover these loads
-          load_t(0);
+          if constexpr (TPRE) {
+  #pragma unroll
+            for (int mt = 0; mt < MT; ++mt) ldt(mt, bva[mt]);
+          } else {
+            load_t(0);
+          }
         }
-        for (int ks = 0; ks < nks; ++ks) {
-          // this wave's copies of step ks landed (the epilogue loads are
-          // younger than the slices of steps 0 .. STG - 2)
-          wait_vm_le((STG - 2) * ncopy + (ks < STG - 1 ? kEpiLoads : 0));
-          lds_barrier();  // everyone's copies landed; everyone is done with step ks - 1
-          issue_taps(ks + STG - 1, (ks + STG - 1) % STG);
-          const T* Bs = Bq + (ks % STG) * kSlice;
-          const unsigned t0 = 32u * ks + 8u * lg;
-          const int e2 = (int)((t0 * rcp) >> 20);
-          const int e10 = (int)t0 - e2 * Kp8;
-          const int o = ((r - e10) - (e2 - r) * Wd) & -(int)(e2 < K);
-          frag_t bf[NT];
+        int qb = 0, qc = 0;  // row block and column chunk of step ks
+        for (int ks = 0; ks < nks; ks += SPB) {
+          // this wave's copies of steps ks .. ks + SPB - 1 landed: younger are
+          // the copies of steps up to ks + STG - SPB - 1 and, while those
+          // steps were copied before the loop, the epilogue loads
+          wait_vm_le((STG - 2 * SPB) * ncopy + (ks + SPB <= STG - SPB ? kEpiLoads : 0));
+          lds_barrier();  // everyone's copies landed; everyone is done with steps ks - SPB ..
   #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) bf[nt] = *reinterpret_cast<const frag_t*>(Bs + nt * 512 + tap_slot);
+          for (int h = 0; h < SPB; ++h) issue_taps(ks + STG - SPB + h, (ks + STG - SPB + h) % STG);
+          // one step's fragments live at a time (NT = 5 has no registers for two)
+  #pragma unroll 1
+          for (int h = 0; h < SPB; ++h) {
+            const int kk = ks + h;
+            const T* Bs = Bq + (kk % STG) * kSlice;
+            // piece lg: row e2 = 4 qb + lg, taps e1 = 8 qc .. 8 qc + 7; rows
+            // past K (zero weights) read row K - 1, which keeps the bank pattern
+            const int e2 = min(4 * qb + lg, K - 1);
+            const int o = (r - 8 * qc) - (e2 - r) * Wr;
+            if (++qc == nch) {
+              qc = 0;
+              ++qb;
+            }
+            // N-tile outer: the four A fragments are read first, then each
+            // N-tile's B fragment two tiles ahead of its MFMAs, so one LDS
+            // round trip per step is exposed instead of one per M-tile
+            // (scheduling fences keep the compiler from hoisting every B read
+            // to the top, which would leave no registers for the A fragments)
+            frag_t af[MT];
   #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) {
-            const frag_t af = Rv[base + mt * Wd + o];
+            for (int mt = 0; mt < MT; ++mt) af[mt] = Rv[base + mt * Wr + o];
+            frag_t bq[3];
+            bq[0] = *reinterpret_cast<const frag_t*>(Bs + tap_slot);
+            if (NT > 1) bq[1] = *reinterpret_cast<const frag_t*>(Bs + 512 + tap_slot);
+            __builtin_amdgcn_sched_barrier(0);
   #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
+            for (int nt = 0; nt < NT; ++nt) {
+              if (nt + 2 < NT) bq[(nt + 2) % 3] = *reinterpret_cast<const frag_t*>(Bs + (nt + 2) * 512 + tap_slot);
+              __builtin_amdgcn_sched_barrier(0);
+  #pragma unroll
+              for (int mt = 0; mt < MT; ++mt) acc[mt][nt] = M::mma(bq[nt % 3], af[mt], acc[mt][nt]);
+              __builtin_amdgcn_sched_barrier(0);
+            }
           }
         }
       }
@@ -1144,7 +1191,11 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         }
       };
-      if constexpr (PRE > 0) {
+      if constexpr (TPRE) {
+        // every operand was loaded before the K loop
+  #pragma unroll
+        for (int mt = 0; mt < MT; ++mt) update_tile(mt, xv[mt], bva[mt]);
+      } else if constexpr (PRE > 0) {
         // continued from the loads issued before the K loop
         load_x(PRE, MT);
   #pragma unroll
@@ -1218,11 +1269,9 @@ __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, P
        o += (size_t)gridDim.x * blockDim.x) {
     const int ph = (int)(o / epad), t = (int)(o % epad);
     int e2, e1;
-    if (pg.row8) {  // (e2, e1) with e1 padded to a multiple of 8
-      const int Kp8 = (K + 7) & ~7;
-      e2 = t / Kp8;
-      e1 = t - e2 * Kp8;
-    } else {        // e1-fastest, dense
+    if (pg.quad) {
+      bwd_quad_tap(t, K, &e2, &e1);
+    } else {  // e1-fastest, dense
       e2 = t / K;
       e1 = t - e2 * K;
     }
@@ -1297,7 +1346,7 @@ size_t fwd_smem_bytes(Precision p, int rmax, int bm2) {
 size_t bwd_smem_bytes(Precision p, int rmax, int epad_max, int nt, int occ) {
   const int Wd = 16 + 2 * rmax;
   if (p == Precision::kF32) return sizeof(float) * ((Wd * Wd + 7) & ~7) + sizeof(int) * epad_max;
-  return bwd_ring_bytes(Wd, nt, 2, occ) + 16 * (size_t)Wd * Wd;
+  return bwd_ring_bytes(Wd, nt, 2, occ) + 16 * (size_t)Wd * bwd_row_cells(Wd, nt, occ);
 }
 
 hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_t smem, int nbuf,

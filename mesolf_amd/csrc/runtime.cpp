@@ -310,7 +310,7 @@ static mlf_status psf_ensure_pack(mlf_psf_s* ps, Precision prec) {
     const int Kf = 2 * ps->rf[z] + 1;
     rows_f[z] = pk.tpz ? Kf * 16 * ((Kf + 15) / 16) : pk.nchunk[z] * kFwdChunk;
     const int Kb = 2 * ps->rb[z] + 1;
-    const int nt_b_pad = prec != Precision::kF32 ? Kb * ((Kb + 7) & ~7) : nt_b;
+    const int nt_b_pad = prec != Precision::kF32 ? 32 * bwd_quad_steps(Kb) : nt_b;
     const int tpad = prec != Precision::kF32 ? kBwdTapPad16 : kBwdTapPad;
     pk.epad[z] = (nt_b_pad + tpad - 1) / tpad * tpad;
     rows_b[z] = pk.epad[z];
@@ -416,6 +416,8 @@ static mlf_status psf_ensure_norm(mlf_psf_s* ps, Precision prec, int n1, int n2,
 }
 
 // Upload compact tables (host layout == device layout for the FP32 kernels).
+static mlf_status upload_large(void* dst, const void* src, size_t bytes, hipStream_t st);
+
 static mlf_status psf_upload(mlf_psf_s* p, const float* fwd, const float* bwd) {
   p->offf.assign(p->Z, 0);
   p->offb.assign(p->Z, 0);
@@ -430,8 +432,8 @@ static mlf_status psf_upload(mlf_psf_s* p, const float* fwd, const float* bwd) {
   p->per_view_b = ob;
   MLF_CHECK(p->fwd.alloc((size_t)of * p->V));
   MLF_CHECK(p->bwd.alloc((size_t)ob * p->V));
-  MLF_HIP(hipMemcpy(p->fwd.ptr, fwd, p->fwd.bytes(), hipMemcpyHostToDevice));
-  MLF_HIP(hipMemcpy(p->bwd.ptr, bwd, p->bwd.bytes(), hipMemcpyHostToDevice));
+  MLF_CHECK(upload_large(p->fwd.ptr, fwd, p->fwd.bytes(), p->ctx->stream));
+  MLF_CHECK(upload_large(p->bwd.ptr, bwd, p->bwd.bytes(), p->ctx->stream));
   MLF_CHECK(p->d_rf.alloc(p->Z));
   MLF_CHECK(p->d_rb.alloc(p->Z));
   MLF_CHECK(p->d_offf.alloc(p->Z));
@@ -458,6 +460,50 @@ static void parallel_for(int64_t n, F&& f) {
       for (int64_t i = t; i < n; i += nt) f(i);
     });
   for (auto& x : th) x.join();
+}
+
+// Host -> device copy of a large pageable buffer: 128 MB chunks are copied
+// into two pinned staging buffers by all host threads and sent by DMA while
+// the next chunk is staged (the runtime's own pageable path stages with one
+// thread).  Synchronous: returns when the data is on the device.
+static mlf_status upload_large(void* dst, const void* src, size_t bytes, hipStream_t st) {
+  constexpr size_t kStage = (size_t)128 << 20;
+  if (bytes < 2 * kStage) {
+    MLF_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return MLF_OK;
+  }
+  struct Staging {
+    void* buf[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    ~Staging() {
+      for (int i = 0; i < 2; ++i) {
+        if (ev[i]) (void)hipEventSynchronize(ev[i]);
+        if (ev[i]) (void)hipEventDestroy(ev[i]);
+        if (buf[i]) (void)hipHostFree(buf[i]);
+      }
+    }
+  } sg;
+  for (int i = 0; i < 2; ++i) {
+    MLF_HIP(hipHostMalloc(&sg.buf[i], kStage, hipHostMallocDefault));
+    MLF_HIP(hipEventCreateWithFlags(&sg.ev[i], hipEventDisableTiming));
+  }
+  const char* s = static_cast<const char*>(src);
+  char* d = static_cast<char*>(dst);
+  for (size_t off = 0, i = 0; off < bytes; off += kStage, ++i) {
+    const int b = (int)(i % 2);
+    const size_t len = std::min(kStage, bytes - off);
+    if (i >= 2) MLF_HIP(hipEventSynchronize(sg.ev[b]));  // its previous DMA finished
+    constexpr size_t kPiece = (size_t)8 << 20;
+    char* pb = static_cast<char*>(sg.buf[b]);
+    parallel_for((int64_t)((len + kPiece - 1) / kPiece), [&](int64_t q) {
+      const size_t o = (size_t)q * kPiece;
+      std::memcpy(pb + o, s + off + o, std::min(kPiece, len - o));
+    });
+    MLF_HIP(hipMemcpyAsync(d + off, pb, len, hipMemcpyHostToDevice, st));
+    MLF_HIP(hipEventRecord(sg.ev[b], st));
+  }
+  MLF_HIP(hipStreamSynchronize(st));
+  return MLF_OK;
 }
 
 // 64-bit content hash of n floats: per-chunk hashes (parallel) folded in
