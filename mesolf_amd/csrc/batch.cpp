@@ -156,7 +156,9 @@ mlf_status enqueue_views(mlf_batch_s* b, hipStream_t st, int nviews = 0) {
     }
     const BwdGeo bg{b->d_slots.ptr, ps->d_rb.ptr, pk.d_boff.ptr, pk.d_epad.ptr, b->d_bsched.ptr,
                     b->d_counts.ptr + 1, k, bwd_groups_per_entry(b->bwd_nt, b->bwd_grp)};
-    MLF_LAUNCH(launch_bwd_mfma(b->kp, b->bwd_nt, b->bwd_occ, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg,
+    // every slot of a BF16 Toeplitz batch holds its volume in the split form
+    const bool split = b->tpz && b->kp == Precision::kBF16;
+    MLF_LAUNCH(launch_bwd_mfma(b->kp, b->bwd_nt, b->bwd_occ, split, pk.bwd.ptr + (size_t)k * pk.pvb * eb, bg,
                                b->grid_b, b->smem_b, (float)ps->weight[k], st));
   }
   return MLF_OK;
@@ -360,6 +362,7 @@ mlf_status mlf_batch_run(mlf_batch b, int32_t iterations, int32_t filter) {
       } else {
         MLF_CHECK(enqueue_views(b, st));
       }
+      for (mlf_patch_s* p : b->slots) MLF_CHECK(patch_merge_split(p));
       if (filter)
         for (mlf_patch_s* p : b->slots) MLF_CHECK(patch_filter(p));
     }

@@ -67,8 +67,14 @@ inline size_t elem_bytes(Precision p) { return p == Precision::kF32 ? 4 : 2; }
 // launch covers the work entries of one or more slots; every entry names its
 // slot, so patches of different lenslet grids share one launch.
 struct SlotDesc {
-  float* X;               // FP32 volume [z][n2][n1][225]
+  float* X;               // FP32 volume [z][n2][n1][256] (phases >= 225 zero)
   void* Xb;               // 16-bit shadow [z][n2][n1][256] (BF16, or FP16 of X * xs), or null
+  // BF16 Toeplitz patches: the lower plane of the split volume, or null.  With
+  // it, X + 2^15 as a 32-bit pattern is split into halves: Xb holds the upper
+  // half (X rounded to BF16, ties away from zero: the forward's operand), Xlo
+  // the lower half.  The view loop keeps the volume in this form and the FP32
+  // array is rebuilt from it (launch_merge_split) before anything else reads X.
+  void* Xlo;
   const float* LF;        // measured image, column-major (15 n1) x (15 n2)
   float* R;               // ratio image [n2][n1]
   float* partial;         // forward partial windows
@@ -158,6 +164,10 @@ hipError_t launch_fwd_tpz(Precision p, int mt, const void* W, FwdGeo g, int grid
                           hipStream_t st);
 hipError_t launch_make_shadow(Precision p, const float* X, void* Xb, size_t nlens, float xs,
                               hipStream_t st);
+// FP32 volume rows [nlens][256] -> split form (upper plane Xb, lower plane
+// Xlo; SlotDesc::Xlo), and back.  Both are exact.
+hipError_t launch_make_split(const float* X, uint16_t* Xb, uint16_t* Xlo, size_t nlens, hipStream_t st);
+hipError_t launch_merge_split(const uint16_t* Xb, const uint16_t* Xlo, float* X, size_t nlens, hipStream_t st);
 // scale[v][z] = 1 / max|tap| of the (view, plane) block (1 if empty); inv = max.
 hipError_t launch_block_scale(const float* src, const int64_t* off, const int32_t* r,
                               int64_t per_view, int Z, int V, float* scale, float* inv,
@@ -175,9 +185,10 @@ hipError_t launch_fwd_mfma(Precision p, const void* W, FwdGeo g, int grid, size_
 hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, int u, int v,
                                int use_lf, float* y_out, hipStream_t st);
 // nt: 16-phase N-tiles per workgroup (3 or 5); occ: waves per SIMD of the
-// 16-bit variants (2, or 3 with nt = 3; FP32 has only 2)
-hipError_t launch_bwd_mfma(Precision p, int nt, int occ, const void* Wb, BwdGeo g, int grid, size_t smem,
-                           float w, hipStream_t st);
+// 16-bit variants (2, or 3 with nt = 3; FP32 has only 2); split (BF16 only):
+// every slot holds its volume in the split form (SlotDesc::Xlo)
+hipError_t launch_bwd_mfma(Precision p, int nt, int occ, bool split, const void* Wb, BwdGeo g, int grid,
+                           size_t smem, float w, hipStream_t st);
 hipError_t launch_pack_fwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,
                            hipStream_t st);
 hipError_t launch_pack_bwd(Precision p, const float* src, void* dst, PackGeo pg, int Z, int V,

@@ -94,6 +94,15 @@ template <typename V>
 __device__ __forceinline__ V gload(const V* p) {
   return *(const __attribute__((address_space(1))) V*)p;
 }
+// Read-only metadata (work entries, slot descriptors, per-plane geometry) is
+// written before a launch and never by a kernel, so it is read through the
+// constant address space: the compiler then uses scalar loads for uniform
+// addresses.  A vector load would wait (vector-memory operations complete in
+// order) for every store the wave still has in flight from its previous entry.
+template <typename V>
+__device__ __forceinline__ V cload(const V* p) {
+  return *(const __attribute__((address_space(4))) V*)p;
+}
 template <typename V>
 __device__ __forceinline__ void gstore(V* p, V v) {
   *(__attribute__((address_space(1))) V*)p = v;
@@ -132,22 +141,23 @@ __device__ __forceinline__ P* uni_ptr(P* p) {
 }
 // Slot descriptor with every field wave-uniform.
 __device__ __forceinline__ SlotDesc load_slot(const SlotDesc* slots, int i) {
-  const SlotDesc& d = slots[i];
+  const SlotDesc* d = slots + i;
   SlotDesc o;
-  o.X = uni_ptr(d.X);
-  o.Xb = uni_ptr(d.Xb);
-  o.LF = uni_ptr(d.LF);
-  o.R = uni_ptr(d.R);
-  o.partial = uni_ptr(d.partial);
-  o.poff = uni_ptr(d.poff);
-  o.Tn = uni_ptr(d.Tn);
-  o.pvt = uni64(d.pvt);
-  o.toffT = uni_ptr(d.toffT);
-  o.C1 = uni_ptr(d.C1);
-  o.n1 = uni(d.n1);
-  o.n2 = uni(d.n2);
-  o.xs = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, d.xs)));
-  o.xs_inv = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, d.xs_inv)));
+  o.X = uni_ptr(cload(&d->X));
+  o.Xb = uni_ptr(cload(&d->Xb));
+  o.Xlo = uni_ptr(cload(&d->Xlo));
+  o.LF = uni_ptr(cload(&d->LF));
+  o.R = uni_ptr(cload(&d->R));
+  o.partial = uni_ptr(cload(&d->partial));
+  o.poff = uni_ptr(cload(&d->poff));
+  o.Tn = uni_ptr(cload(&d->Tn));
+  o.pvt = uni64(cload(&d->pvt));
+  o.toffT = uni_ptr(cload(&d->toffT));
+  o.C1 = uni_ptr(cload(&d->C1));
+  o.n1 = uni(cload(&d->n1));
+  o.n2 = uni(cload(&d->n2));
+  o.xs = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, cload(&d->xs))));
+  o.xs_inv = __builtin_bit_cast(float, uni(__builtin_bit_cast(int, cload(&d->xs_inv))));
   return o;
 }
 // s_waitcnt vmcnt(<= n) without waiting on LDS/export counters (gfx9 encoding:
@@ -196,16 +206,16 @@ void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
   T* Bs = reinterpret_cast<T*>(smem);                                  // [NBUF][CH][LDB]
   float* ywin = reinterpret_cast<float*>(smem + sizeof(T) * NBUF * CH * LDB);
 
-  const int nent = uni(*g.count);
+  const int nent = uni(cload(g.count));
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
     const int* s = g.sched + 4 * ent;
-    const SlotDesc sd = load_slot(g.slots, uni(s[0]));
+    const SlotDesc sd = load_slot(g.slots, uni(cload(s)));
     const float* __restrict__ X = sd.X;
     const int n1 = sd.n1, n2 = sd.n2;
-    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]);
-    const int r = uni(g.r[z]);
+    const int z = uni(cload(s + 1)), b1 = uni(cload(s + 2)), b2 = uni(cload(s + 3));
+    const int r = uni(cload(g.r + z));
     const int K = 2 * r + 1, ntaps = K * K;
-    const int nch = uni(g.nchunk[z]);
+    const int nch = uni(cload(g.nchunk + z));
     const int W1 = 16 + 2 * r, W2 = BM2 + 2 * r;
     const int WW = MT + 2 * r;  // rows of one wave's window
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
@@ -239,7 +249,7 @@ void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
       }
     }
 
-    const T* Wz = W + uni64(g.woff[z]);
+    const T* Wz = W + uni64(cload(g.woff + z));
     // Chunk c (CH rows of PHI elements) is copied global -> LDS with
     // global_load_lds_dwordx4 (no staging registers), into the buffer the
     // waves are not reading, while the MFMAs of chunk c-1 run.  Each wave pass
@@ -319,7 +329,7 @@ void k_fwd_mfma(const T* __restrict__ W, FwdGeo g) {
     }
     __syncthreads();
     const int nb1 = ceil_div(n1, 16);
-    float* out = sd.partial + uni64(sd.poff[z]) + (size_t)(b2 * nb1 + b1) * W1 * W2;
+    float* out = sd.partial + uni64(cload(sd.poff + z)) + (size_t)(b2 * nb1 + b1) * W1 * W2;
     for (int o = tid; o < W1 * W2; o += 256) {
       const int y = o / W1, x = o - y * W1;
       float acc = 0.f;
@@ -377,15 +387,15 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
   T* ring = reinterpret_cast<T*>(smem + (size_t)NB * WSLOT * 16);            // [NB][CW][kRingRow]
   float* ywin = reinterpret_cast<float*>(ring + NB * CW * kRingRow);         // [4 waves][16][MT + 2r]
 
-  const int nent = uni(*g.count);
+  const int nent = uni(cload(g.count));
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
     const int* s = g.sched + 4 * ent;
-    const SlotDesc sd = load_slot(g.slots, uni(s[0]));
+    const SlotDesc sd = load_slot(g.slots, uni(cload(s)));
     const float* __restrict__ X = sd.X;
     const T* __restrict__ Xb = reinterpret_cast<const T*>(sd.Xb);
     const int n1 = sd.n1, n2 = sd.n2;
-    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]);
-    const int r = uni(g.r[z]);
+    const int z = uni(cload(s + 1)), b1 = uni(cload(s + 2)), b2 = uni(cload(s + 3));
+    const int r = uni(cload(g.r + z));
     const int K = 2 * r + 1;
     const int NT = (K + 15) / 16;
     const int nrow = 16 * NT;
@@ -396,9 +406,9 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     // partial-window output of the entry and (FP16) the unscaling factor, read
     // here so that their latency stays off the flush
     float* const out =
-        sd.partial + uni64(gload(sd.poff + z)) + (size_t)(b2 * ceil_div(n1, 16) + b1) * 16 * (CW + 2 * r);
+        sd.partial + uni64(cload(sd.poff + z)) + (size_t)(b2 * ceil_div(n1, 16) + b1) * 16 * (CW + 2 * r);
     float unscale = 1.f;
-    if constexpr (is_f16<T>()) unscale = gload(g.finv + z) * sd.xs_inv;
+    if constexpr (is_f16<T>()) unscale = cload(g.finv + z) * sd.xs_inv;
     const float* Xz = X + (size_t)z * n2 * n1 * kXRow;
     const T* Xbz = KPL > 1 ? Xb + (size_t)z * n2 * n1 * kShadowRow : nullptr;  // BF16 / FP16 shadow plane
     // the shadow plane as a buffer (a plane is below 2^31 bytes, checked at patch creation)
@@ -431,7 +441,7 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
     }
     __syncthreads();  // ring zeroed before any DMA lands
 
-    const T* Wz = W + uni64(g.woff[z]);  // [K][nrow][PHI]
+    const T* Wz = W + uni64(cload(g.woff + z));  // [K][nrow][PHI]
     // Every wave issues the same number of DMA instructions per step (W: the
     // entry's nw_inst; ring: CW*RPASS passes / 4 waves), so a step can wait
     // for an earlier batch with a fixed vmcnt.  Only the K tap rows e2 < K
@@ -699,11 +709,11 @@ __global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg,
   __shared__ float part[kRatioSlices][kRatioLens];
   const int li = threadIdx.x % kRatioLens, sl = threadIdx.x / kRatioLens;
   const int z0 = sl * Z / kRatioSlices, z1 = (sl + 1) * Z / kRatioSlices;
-  const int nent = uni(*rg.count);
+  const int nent = uni(cload(rg.count));
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
-    const SlotDesc sd = load_slot(g.slots, uni(rg.sched[2 * ent]));
+    const SlotDesc sd = load_slot(g.slots, uni(cload(rg.sched + 2 * ent)));
     const int n1 = sd.n1, n2 = sd.n2;
-    const int i = uni(rg.sched[2 * ent + 1]) + li;
+    const int i = uni(cload(rg.sched + 2 * ent + 1)) + li;
     const bool ok = i < n1 * n2;
     const int i1 = ok ? i % n1 : 0, i2 = ok ? i / n1 : 0;
     const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, CW);
@@ -793,11 +803,11 @@ __global__ __launch_bounds__(256) void k_ratio_gather(FwdGeo g, RatioGeo rg, int
   __shared__ float part[kRatioSlices][kRatioLens];
   const int li = threadIdx.x % kRatioLens, sl = threadIdx.x / kRatioLens;
   const int z0 = sl * Z / kRatioSlices, z1 = (sl + 1) * Z / kRatioSlices;
-  const int nent = uni(*rg.count);
+  const int nent = uni(cload(rg.count));
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
-    const SlotDesc sd = load_slot(g.slots, uni(rg.sched[2 * ent]));
+    const SlotDesc sd = load_slot(g.slots, uni(cload(rg.sched + 2 * ent)));
     const int n1 = sd.n1, n2 = sd.n2;
-    const int i = uni(rg.sched[2 * ent + 1]) + li;
+    const int i = uni(cload(rg.sched + 2 * ent + 1)) + li;
     const bool ok = i < n1 * n2;
     const int i1 = ok ? i % n1 : 0, i2 = ok ? i / n1 : 0;
     const int nb1 = ceil_div(n1, 16), nb2 = ceil_div(n2, BM2);
@@ -862,7 +872,10 @@ __device__ __forceinline__ int norm_class(int a, int n, int r) {
 }
 
 
-template <typename T, int NT, int OCC>
+// SPL (BF16 only): the volume is held as its split form during the view loop
+// (SlotDesc::Xlo): the epilogue reads and writes the two 16-bit planes and
+// leaves the FP32 array alone.
+template <typename T, int NT, int OCC, bool SPL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(OCC)))
 void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   using M = Mma<T>;
@@ -875,20 +888,23 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   // so its epilogue waits for no load.
   constexpr int PRE = OCC >= 3 ? 0 : (NT == 3 && M::KPL > 1 ? MT : MT - 1);
   constexpr bool TPRE = PRE == MT;
+  static_assert(!SPL || __is_same(T, __bf16), "the split volume is a BF16 form");
+  static_assert(NT % 2 == 1, "16-bit N-tiles: pairs and one unpaired tile");
   extern __shared__ __align__(16) char smem[];
-  const int nent = uni(*g.count);
+  const int nent = uni(cload(g.count));
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
     const int* s = g.sched + 5 * ent;
-    const SlotDesc sd = load_slot(g.slots, uni(s[0]));
+    const SlotDesc sd = load_slot(g.slots, uni(cload(s)));
     float* __restrict__ X = sd.X;
     T* __restrict__ Xsh = reinterpret_cast<T*>(sd.Xb);
     const float* __restrict__ R = sd.R;
     const int n1 = sd.n1, n2 = sd.n2;
     // phase groups grp0 .. grp0 + groups - 1 share the entry's ratio window
-    const int z = uni(s[1]), b1 = uni(s[2]), b2 = uni(s[3]), grp0 = uni(s[4]);
-    const int r = uni(g.r[z]);
+    const int z = uni(cload(s + 1)), b1 = uni(cload(s + 2)), b2 = uni(cload(s + 3)), grp0 = uni(cload(s + 4));
+    const int r = uni(cload(g.r + z));
     const int K = 2 * r + 1, ntaps = K * K;
-    const int epad = uni(g.epad[z]);
+    const int epad = uni(cload(g.epad + z));
+    const int64_t boffz = uni64(cload(g.boff + z));
     const int Wd = 16 + 2 * r;
     // w: wave index, wave-uniform (readfirstlane), so loops and bases over it run on the scalar unit
     const int tid = threadIdx.x, lane = tid & 63, w = uni(tid >> 6), lr = lane & 15, lg = lane >> 4;
@@ -904,16 +920,20 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     // ahead; only the stores are predicated.  Buffer resources span one plane
     // (a plane is below 2^31 bytes, checked at patch creation) and return
     // zero past it.
-    const int C1 = uni(sd.C1[z]), C2 = norm_classes(n2, r);
+    const int C1 = uni(cload(sd.C1 + z)), C2 = norm_classes(n2, r);
     const size_t nlp = (size_t)n2 * n1;  // lenslets per plane
     const __amdgpu_buffer_rsrc_t xr =
         __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kXRow, 0, (int)(nlp * kXRow * 4), kBufferDword3);
     const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(sd.toffT[z])), 0, C2 * C1 * kP * 4,
+        const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(cload(sd.toffT + z))), 0, C2 * C1 * kP * 4,
         kBufferDword3);
     const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc(
         Xsh ? Xsh + z * nlp * kShadowRow : nullptr, 0, Xsh ? (int)(nlp * kShadowRow * 2) : 0,
         kBufferDword3);
+    // SPL: the lower plane of the split volume, rows like the shadow's
+    const __amdgpu_buffer_rsrc_t lor = __builtin_amdgcn_make_buffer_rsrc(
+        SPL ? reinterpret_cast<uint16_t*>(sd.Xlo) + z * nlp * kShadowRow : nullptr, 0,
+        SPL ? (int)(nlp * kShadowRow * 2) : 0, kBufferDword3);
     typedef float floatx4_t __attribute__((ext_vector_type(4)));
     typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
     typedef unsigned uintx2 __attribute__((ext_vector_type(2)));
@@ -982,33 +1002,81 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
 
     for (int gi = 0; gi < g.groups; ++gi) {  // phase group grp: 16 * NT phases from 16 * NT * grp
       const int grp = grp0 + gi;
-      const T* Bz = Wb + uni64(g.boff[z]) + (size_t)(grp * 16 * NT) * epad;
+      const T* Bz = Wb + boffz + (size_t)(grp * 16 * NT) * epad;
       floatx4 acc[MT][NT];
   #pragma unroll
       for (int mt = 0; mt < MT; ++mt)
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = floatx4{0.f, 0.f, 0.f, 0.f};
-      // first of this lane's 4 phases in N-tile nt: ph0 + 16 nt
-      const int ph0 = grp * 16 * NT + 4 * lg;
+      // First of the 4 consecutive phases this lane holds in N-tile nt.
+      // SPL: N-tiles nt and nt + 1 (nt even, < NT - 1) form a pair that
+      // covers the 32 phases from a 32-aligned phase P, row j of half h being
+      // phase P + 8 (j / 4) + 4 h + j % 4, so that a lane's rows in the two
+      // halves are 8 consecutive phases, 16 bytes of each 16-bit plane (with
+      // FP32 X, two 16-byte pieces 16 bytes apart per instruction would touch
+      // twice the lines, so the other paths keep phases 16 nt + 4 lg + j);
+      // N-tile NT - 1 covers 16 phases in order, after the pairs in even
+      // groups and before them in odd ones (whose first phase is 16 past a
+      // 32-aligned one).  The tap copies below place the rows to match.
+      constexpr bool PAIR = SPL;
+      const int gbase = grp * 16 * NT;
+      const int odd = PAIR ? (grp & 1) : 0;
+      auto lph = [&](int nt) -> int {
+        if constexpr (PAIR) {
+          if (nt < NT - 1) return gbase + 16 * odd + 16 * (nt & ~1) + 8 * lg + 4 * (nt & 1);
+          return gbase + (odd ? 0 : 16 * (NT - 1)) + 4 * lg;
+        } else {
+          return gbase + 16 * nt + 4 * lg;
+        }
+      };
+      // every phase of N-tile nt is below 225 (only the unpaired tile of the
+      // last group reaches past phase 224)
+      auto tfull = [&](int nt) -> bool {
+        if constexpr (PAIR) {
+          if (nt < NT - 1) return true;
+          return gbase + (odd ? 0 : 16 * (NT - 1)) + 16 <= kP;
+        } else {
+          return gbase + 16 * nt + 16 <= kP;
+        }
+      };
       // X of M-tiles 0 .. MT-2 is loaded before the K loop (its HBM latency
       // hides under the MFMAs; the last tile's registers are taken by the
       // loop's operands and it is loaded when the loop ends); the 1/B1 rows
       // of tile 0 too, those of later tiles (L2-resident) when the tile is
       // reached
       floatx4_t xv[MT][NT], bv[NT], bva[TPRE ? MT : 1][NT];
+      // SPL: dst holds raw split words, decoded where they are used so that
+      // no load is waited for early: for a pair, the upper plane's 8 phases
+      // in dst[nt] and the lower plane's in dst[nt + 1]; for the unpaired
+      // tile, (upper pair, lower pair) of its 4 phases
       auto ldx = [&](int mt, floatx4_t* dst) {
         const int a2 = 16 * b2 + 4 * w + mt;
-        const int xo = ((a2 < n2 ? a2 : n2 - 1) * n1 + a1c) * kXRow;
+        const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
   #pragma unroll
-        for (int nt = 0; nt < NT; ++nt)
-          dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (xo + ph0) + 64 * nt, 0, 0));
+        for (int nt = 0; nt < NT; ++nt) {
+          if constexpr (SPL) {
+            const int so = 2 * (l * kShadowRow + lph(nt));
+            if (nt < NT - 1) {
+              if (nt & 1) continue;  // loaded with its pair
+              dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(sr, so, 0, 0));
+              dst[nt + 1] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(lor, so, 0, 0));
+            } else {
+              const uintx2 h = __builtin_amdgcn_raw_buffer_load_b64(sr, so, 0, 0);
+              const uintx2 q = __builtin_amdgcn_raw_buffer_load_b64(lor, so, 0, 0);
+              dst[nt] = __builtin_bit_cast(floatx4_t, uintx4{h[0], h[1], q[0], q[1]});
+            }
+          } else {
+            dst[nt] = __builtin_bit_cast(floatx4_t,
+                                         __builtin_amdgcn_raw_buffer_load_b128(xr, 4 * (l * kXRow + lph(nt)), 0, 0));
+          }
+        }
       };
       auto ldt = [&](int mt, floatx4_t* dst) {
         const int a2 = 16 * b2 + 4 * w + mt;
         const int to = (norm_class(a2 < n2 ? a2 : n2 - 1, n2, r) * C1 + c1) * kP;
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
-          dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + ph0) + 64 * nt, 0, 0));
+          dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + lph(nt)), 0, 0));
       };
       auto load_x = [&](int mt0, int mt1) {
   #pragma unroll
@@ -1017,7 +1085,9 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       };
       auto load_t = [&](int mt) { ldt(mt, bv); };
       // loads issued before the K loop: load_x(0, PRE) and the first (or every) 1/B1 row set
-      constexpr int kEpiLoads = PRE > 0 ? PRE * NT + (TPRE ? MT * NT : NT) : 0;
+      constexpr int kEpiLoads = PRE > 0 ? PRE * (SPL ? NT + 1 : NT) + (TPRE ? MT * NT : NT) : 0;
+      static_assert(kEpiLoads + (bwd_stages(OCC) - 2 * (OCC >= 3 ? 1 : 2)) * ((NT + 3) / 4) <= 40,
+                    "wait_vm_le covers at most 40 loads in flight");
 
       if constexpr (KPL == 1) {
         load_x(0, MT - 1);
@@ -1059,13 +1129,22 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         // goes to s = 16 q + row, so the 16 lanes reading group q at rows
         // 0..15 read 16 consecutive pieces and the four groups the same bank
         // groups (the pattern ds_read_b128 serves without conflicts).
-        const int tap_lane = ((lane & 15) * epad + (lane >> 4) * 8) * (int)sizeof(T);
+        // SPL: slice row j of a paired N-tile (half h) is phase row
+        // 8 (j / 4) + 4 h + j % 4 of the pair (lph).
+        const int jr = lane & 15, tq = (lane >> 4) * 8;
+        const int tap_lane_s = (jr * epad + tq) * (int)sizeof(T);                      // unpaired tile
+        const int tap_lane_p = ((8 * (jr >> 2) + (jr & 3)) * epad + tq) * (int)sizeof(T);  // pair, half 0
+        const int tap_lane_h = 4 * epad * (int)sizeof(T);                                // half 1: 4 rows on
         const int tap_slot = (16 * lg + lr) * 8;  // element offset of (row lr, 8-tap group lg) in an N-tile slice
         auto issue_taps = [&](int ks, int slot) {
           const int kc = min(ks, nks - 1);
           char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
           for (int nt = w; nt < NT; nt += 4) {
-            const char* src = reinterpret_cast<const char*>(Bz + (size_t)nt * 16 * epad + kc * 32) + tap_lane;
+            const bool pr = PAIR && nt < NT - 1;
+            // first packed row relative to the group
+            const int row0 = !PAIR ? 16 * nt : pr ? 16 * odd + 16 * (nt & ~1) : (odd ? 0 : 16 * (NT - 1));
+            const int tl = pr ? tap_lane_p + (nt & 1) * tap_lane_h : tap_lane_s;
+            const char* src = reinterpret_cast<const char*>(Bz + (size_t)row0 * epad + kc * 32) + tl;
             __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                              (__attribute__((address_space(3))) void*)(dst + nt * 1024),
                                              16, 0, 0);
@@ -1145,12 +1224,32 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
         const unsigned xbase = ok ? 4u * (unsigned)(l * kXRow) : 0x80000000u;
         const unsigned sbase = ok ? 2u * (unsigned)(l * kShadowRow) : 0x80000000u;
+        uintx2 hi_prev, lo_prev;  // SPL: the first half of a pair, stored with the second
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt) {
+          float xo4[4];
+          if constexpr (SPL) {
+            // split words -> X: (upper << 16 | lower) - 2^15, per phase
+            uintx4 raw;
+            if (nt < NT - 1) {
+              const uintx4 hv = __builtin_bit_cast(uintx4, x4[nt & ~1]), lv = __builtin_bit_cast(uintx4, x4[nt | 1]);
+              const int h = 2 * (nt & 1);
+              raw = uintx4{hv[h], hv[h + 1], lv[h], lv[h + 1]};
+            } else {
+              raw = __builtin_bit_cast(uintx4, x4[nt]);
+            }
+            xo4[0] = __builtin_bit_cast(float, ((raw[0] << 16) | (raw[2] & 0xFFFFu)) - 0x8000u);
+            xo4[1] = __builtin_bit_cast(float, ((raw[0] & 0xFFFF0000u) | (raw[2] >> 16)) - 0x8000u);
+            xo4[2] = __builtin_bit_cast(float, ((raw[1] << 16) | (raw[3] & 0xFFFFu)) - 0x8000u);
+            xo4[3] = __builtin_bit_cast(float, ((raw[1] & 0xFFFF0000u) | (raw[3] >> 16)) - 0x8000u);
+          } else {
+  #pragma unroll
+            for (int j = 0; j < 4; ++j) xo4[j] = x4[nt][j];
+          }
           floatx4_t xn;
   #pragma unroll
           for (int j = 0; j < 4; ++j) {
-            const float x = x4[nt][j];
+            const float x = xo4[j];
             float C = x * acc[mt][nt][j] * b4[nt][j];  // b4: 1 / B1
             // NaN, +-Inf and negatives -> 0: keep C only in the classes
             // -0, +0, +denormal, +normal (one class test and a select)
@@ -1158,36 +1257,62 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
             const float v = C * wgt + (1.f - wgt) * x;
             xn[j] = v < 0.f ? 0.f : v;
           }
-          const int ph = ph0 + 16 * nt;
-          if (grp * 16 * NT + 16 * nt + 16 <= kP) {  // wave-uniform: the whole N-tile is in the row
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, (int)(xbase + 4u * ph), 0, 0);
-            if constexpr (KPL > 1) {  // the 16-bit paths always carry the shadow
-              floatx4_t sv = xn;
-              if constexpr (is_f16<T>()) sv *= sd.xs;
-              uintx2 h;
-              h[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
-              h[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
-              __builtin_amdgcn_raw_buffer_store_b64(h, sr, (int)(sbase + 2u * ph), 0, 0);
+          const int ph = lph(nt);
+          // SPL: X + 2^15 as a bit pattern, split into its upper half (X
+          // rounded to BF16, ties away from zero) and its lower half
+          uintx2 hi, lo;
+          if constexpr (SPL) {
+            const uintx4 q = __builtin_bit_cast(uintx4, xn) + 0x8000u;
+            hi = uintx2{(q[0] >> 16) | (q[1] & 0xFFFF0000u), (q[2] >> 16) | (q[3] & 0xFFFF0000u)};
+            lo = uintx2{(q[0] & 0xFFFFu) | (q[1] << 16), (q[2] & 0xFFFFu) | (q[3] << 16)};
+          } else {
+            floatx4_t sv = xn;
+            if constexpr (is_f16<T>()) sv *= sd.xs;
+            hi[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
+            hi[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
+          }
+          if (tfull(nt)) {  // wave-uniform: the whole N-tile is in the row
+            if constexpr (SPL) {
+              if (nt < NT - 1) {
+                if ((nt & 1) == 0) {
+                  hi_prev = hi;
+                  lo_prev = lo;
+                  continue;
+                }
+                // the pair's 8 phases start 4 before this half's
+                const int o = (int)(sbase + 2u * (ph - 4));
+                __builtin_amdgcn_raw_buffer_store_b128(uintx4{hi_prev[0], hi_prev[1], hi[0], hi[1]}, sr, o, 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b128(uintx4{lo_prev[0], lo_prev[1], lo[0], lo[1]}, lor, o, 0, 0);
+              } else {
+                __builtin_amdgcn_raw_buffer_store_b64(hi, sr, (int)(sbase + 2u * ph), 0, 0);
+                __builtin_amdgcn_raw_buffer_store_b64(lo, lor, (int)(sbase + 2u * ph), 0, 0);
+              }
+            } else {
+              __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, (int)(xbase + 4u * ph), 0, 0);
+              // the 16-bit paths always carry the shadow
+              if constexpr (KPL > 1) __builtin_amdgcn_raw_buffer_store_b64(hi, sr, (int)(sbase + 2u * ph), 0, 0);
             }
             continue;
           }
+          // the tile that reaches past phase 224
           const int xo = 4 * (l * kXRow + ph);
           const int so = 2 * (l * kShadowRow + ph);
           if (ok && ph + 4 <= kP) {
-            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
-            if (Xsh) {
-              floatx4_t sv = xn;
-              if constexpr (is_f16<T>()) sv *= sd.xs;
-              uintx2 h;
-              h[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
-              h[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
-              __builtin_amdgcn_raw_buffer_store_b64(h, sr, so, 0, 0);
+            if constexpr (SPL) {
+              __builtin_amdgcn_raw_buffer_store_b64(hi, sr, so, 0, 0);
+              __builtin_amdgcn_raw_buffer_store_b64(lo, lor, so, 0, 0);
+            } else {
+              __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
+              if (Xsh) __builtin_amdgcn_raw_buffer_store_b64(hi, sr, so, 0, 0);
             }
           } else if (ok && ph < kP) {  // phase 224: the last of 225
-            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
-            if (Xsh)
-              __builtin_amdgcn_raw_buffer_store_b16(to_bits16<T>(is_f16<T>() ? xn[0] * sd.xs : xn[0]), sr,
-                                                    so, 0, 0);
+            if constexpr (SPL) {
+              __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(hi[0] & 0xFFFFu), sr, so, 0, 0);
+              __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(lo[0] & 0xFFFFu), lor, so, 0, 0);
+            } else {
+              __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
+              if (Xsh) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(hi[0] & 0xFFFFu), sr, so, 0, 0);
+            }
           }
         }
       };
@@ -1196,12 +1321,26 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   #pragma unroll
         for (int mt = 0; mt < MT; ++mt) update_tile(mt, xv[mt], bva[mt]);
       } else if constexpr (PRE > 0) {
-        // continued from the loads issued before the K loop
+        // continued from the loads issued before the K loop.  The 1/B1 rows
+        // of tile mt + 1 are requested before tile mt's stores, so the wait
+        // for them does not include those stores (vector-memory operations
+        // complete in order), only the stores of tile mt - 1
+        // (SPL; the other paths have no registers for the second row set
+        // and request each tile's rows when they reach it)
         load_x(PRE, MT);
+        if constexpr (SPL) {
+          floatx4_t bw[NT];
   #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          if (mt > 0) load_t(mt);
-          update_tile(mt, xv[mt], bv);
+          for (int mt = 0; mt < MT; ++mt) {
+            if (mt + 1 < MT) ldt(mt + 1, mt % 2 == 0 ? bw : bv);
+            update_tile(mt, xv[mt], mt % 2 == 0 ? bv : bw);
+          }
+        } else {
+  #pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            if (mt > 0) load_t(mt);
+            update_tile(mt, xv[mt], bv);
+          }
         }
       } else {
         // X and 1/B1 rows one M-tile ahead, in two register sets
@@ -1424,6 +1563,52 @@ hipError_t launch_make_shadow(Precision p, const float* X, void* Xb, size_t nlen
   return hipGetLastError();
 }
 
+// Split form of the FP32 volume (SlotDesc::Xlo): p = bits(X) + 2^15, upper
+// half to Xb, lower half to Xlo; X = bits((upper << 16 | lower) - 2^15).  Four
+// phases per thread; rows are 256 phases in all three arrays.
+__global__ void k_make_split(const float* __restrict__ X, uint16_t* __restrict__ Xb,
+                             uint16_t* __restrict__ Xlo, size_t nq) {
+  typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
+  typedef unsigned uintx2 __attribute__((ext_vector_type(2)));
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nq; i += (size_t)gridDim.x * blockDim.x) {
+    const uintx4 q = reinterpret_cast<const uintx4*>(X)[i] + 0x8000u;
+    reinterpret_cast<uintx2*>(Xb)[i] =
+        uintx2{(q[0] >> 16) | (q[1] & 0xFFFF0000u), (q[2] >> 16) | (q[3] & 0xFFFF0000u)};
+    reinterpret_cast<uintx2*>(Xlo)[i] = uintx2{(q[0] & 0xFFFFu) | (q[1] << 16), (q[2] & 0xFFFFu) | (q[3] << 16)};
+  }
+}
+
+__global__ void k_merge_split(const uint16_t* __restrict__ Xb, const uint16_t* __restrict__ Xlo,
+                              float* __restrict__ X, size_t nq) {
+  typedef unsigned uintx4 __attribute__((ext_vector_type(4)));
+  typedef unsigned uintx2 __attribute__((ext_vector_type(2)));
+  for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nq; i += (size_t)gridDim.x * blockDim.x) {
+    const uintx2 h = reinterpret_cast<const uintx2*>(Xb)[i];
+    const uintx2 l = reinterpret_cast<const uintx2*>(Xlo)[i];
+    const uintx4 p{(h[0] << 16) | (l[0] & 0xFFFFu), (h[0] & 0xFFFF0000u) | (l[0] >> 16),
+                   (h[1] << 16) | (l[1] & 0xFFFFu), (h[1] & 0xFFFF0000u) | (l[1] >> 16)};
+    reinterpret_cast<uintx4*>(X)[i] = p - 0x8000u;
+  }
+}
+
+static unsigned stream_grid(size_t n) {
+  const size_t g = (n + 255) / 256;
+  return (unsigned)(g > 8192 ? 8192 : (g ? g : 1));
+}
+
+hipError_t launch_make_split(const float* X, uint16_t* Xb, uint16_t* Xlo, size_t nlens, hipStream_t st) {
+  const size_t nq = nlens * (kXRow / 4);
+  static_assert(kXRow == kShadowRow, "the split planes share the volume's row length");
+  hipLaunchKernelGGL(k_make_split, dim3(stream_grid(nq)), dim3(256), 0, st, X, Xb, Xlo, nq);
+  return hipGetLastError();
+}
+
+hipError_t launch_merge_split(const uint16_t* Xb, const uint16_t* Xlo, float* X, size_t nlens, hipStream_t st) {
+  const size_t nq = nlens * (kXRow / 4);
+  hipLaunchKernelGGL(k_merge_split, dim3(stream_grid(nq)), dim3(256), 0, st, Xb, Xlo, X, nq);
+  return hipGetLastError();
+}
+
 // Largest |tap| of each (view, plane) block of an ABI table -> 1 / max (1 for
 // an empty block): the FP16 pre-scaling factors.
 __global__ void k_block_scale(const float* __restrict__ src, const int64_t* __restrict__ off,
@@ -1479,31 +1664,34 @@ hipError_t launch_ratio_gather(FwdGeo g, RatioGeo rg, int grid, int Z, int bm2, 
   return hipGetLastError();
 }
 
-template <typename T>
+template <typename T, bool SPL>
 static void launch_bwd_t(int nt, int occ, const void* Wb, BwdGeo g, int grid, size_t smem, float w,
                          hipStream_t st) {
   const T* W = (const T*)Wb;
   // three waves per SIMD only with 3 N-tiles (with 5 the accumulators and
   // operands exceed 168 registers)
   if (nt == 3 && occ >= 3)
-    hipLaunchKernelGGL((k_bwd_mfma<T, 3, 3>), dim3(grid), dim3(256), smem, st, W, g, w);
+    hipLaunchKernelGGL((k_bwd_mfma<T, 3, 3, SPL>), dim3(grid), dim3(256), smem, st, W, g, w);
   else if (nt == 3)
-    hipLaunchKernelGGL((k_bwd_mfma<T, 3, 2>), dim3(grid), dim3(256), smem, st, W, g, w);
+    hipLaunchKernelGGL((k_bwd_mfma<T, 3, 2, SPL>), dim3(grid), dim3(256), smem, st, W, g, w);
   else
-    hipLaunchKernelGGL((k_bwd_mfma<T, 5, 2>), dim3(grid), dim3(256), smem, st, W, g, w);
+    hipLaunchKernelGGL((k_bwd_mfma<T, 5, 2, SPL>), dim3(grid), dim3(256), smem, st, W, g, w);
 }
 
-hipError_t launch_bwd_mfma(Precision p, int nt, int occ, const void* Wb, BwdGeo g, int grid, size_t smem,
-                           float w, hipStream_t st) {
+hipError_t launch_bwd_mfma(Precision p, int nt, int occ, bool split, const void* Wb, BwdGeo g, int grid,
+                           size_t smem, float w, hipStream_t st) {
+  if (split && p != Precision::kBF16) return hipErrorInvalidValue;
   if (p == Precision::kF32) {
     if (nt == 3)
-      hipLaunchKernelGGL((k_bwd_mfma<float, 3, 2>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
+      hipLaunchKernelGGL((k_bwd_mfma<float, 3, 2, false>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
     else
-      hipLaunchKernelGGL((k_bwd_mfma<float, 5, 2>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
+      hipLaunchKernelGGL((k_bwd_mfma<float, 5, 2, false>), dim3(grid), dim3(256), smem, st, (const float*)Wb, g, w);
   } else if (p == Precision::kF16) {
-    launch_bwd_t<_Float16>(nt, occ, Wb, g, grid, smem, w, st);
+    launch_bwd_t<_Float16, false>(nt, occ, Wb, g, grid, smem, w, st);
+  } else if (split) {
+    launch_bwd_t<__bf16, true>(nt, occ, Wb, g, grid, smem, w, st);
   } else {
-    launch_bwd_t<__bf16>(nt, occ, Wb, g, grid, smem, w, st);
+    launch_bwd_t<__bf16, false>(nt, occ, Wb, g, grid, smem, w, st);
   }
   return hipGetLastError();
 }
@@ -1557,16 +1745,19 @@ hipError_t mfma_kernel_attributes() {
       reinterpret_cast<const void*>(&k_fwd_tpz<float, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<__bf16, 4>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<float, 3, 2>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<float, 5, 2>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<float, 3, 2, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<float, 5, 2, false>),
       reinterpret_cast<const void*>(&k_fwd_tpz<_Float16, 2>),
       reinterpret_cast<const void*>(&k_fwd_tpz<_Float16, 4>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3, 2>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 5, 2>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3, 3>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 2>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5, 2>),
-      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 3>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3, 2, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 5, 2, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<_Float16, 3, 3, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 2, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5, 2, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 3, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 2, true>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5, 2, true>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 3, true>),
   };
   for (const void* k : kernels) {
     const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

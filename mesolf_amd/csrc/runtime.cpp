@@ -1028,8 +1028,9 @@ static mlf_status patch_view_step(mlf_patch_s* p, int k) {
                                            p->R.ptr, p->n1, p->n2, p->Z, (float)ps->weight[k], st));
   } else {
     const PrecPack& pk = ps->pack[(int)p->kp];
-    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->bwd_nt, p->bwd_occ, pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp),
-                               bwd_geo(p, k), p->nblk_b, p->smem_b, (float)ps->weight[k], st));
+    MLF_LAUNCH(launch_bwd_mfma(p->kp, p->bwd_nt, p->bwd_occ, p->Xlo.ptr != nullptr,
+                               pk.bwd.ptr + (size_t)k * pk.pvb * elem_bytes(p->kp), bwd_geo(p, k), p->nblk_b,
+                               p->smem_b, (float)ps->weight[k], st));
   }
   MLF_CHECK(tim_end(p));
   return MLF_OK;
@@ -1165,9 +1166,15 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   MLF_CHECK(p->partial.alloc((size_t)po));
   if (p->tpz && p->kp != Precision::kF32)
     MLF_CHECK(p->Xb.alloc((size_t)p->n1 * p->n2 * Z * kShadowRow));
+  // BF16: the view loop keeps the volume split into the shadow (upper half)
+  // and a lower plane, so the backprojection writes 4 bytes per voxel instead
+  // of FP32 X plus the shadow
+  if (p->tpz && p->kp == Precision::kBF16)
+    MLF_CHECK(p->Xlo.alloc((size_t)p->n1 * p->n2 * Z * kShadowRow));
   SlotDesc& sd = p->slot;
   sd.X = p->X.ptr;
   sd.Xb = reinterpret_cast<__bf16*>(p->Xb.ptr);
+  sd.Xlo = p->Xlo.ptr;
   sd.LF = p->LF.ptr;
   sd.R = p->R.ptr;
   sd.partial = p->partial.ptr;
@@ -1190,10 +1197,22 @@ mlf_status mlf::patch_setup_tiles(mlf_patch_s* p) {
   return MLF_OK;
 }
 
-// Re-derive the BF16 shadow after X changed outside the backprojection.
+// Re-derive the 16-bit shadow (BF16 Toeplitz: the split form) after X changed
+// outside the backprojection.
 mlf_status mlf::patch_refresh_shadow(mlf_patch_s* p) {
   if (!p->Xb.ptr) return MLF_OK;
-  MLF_LAUNCH(launch_make_shadow(p->kp, p->X.ptr, p->Xb.ptr, (size_t)p->n1 * p->n2 * p->Z, p->slot.xs,
+  const size_t nl = (size_t)p->n1 * p->n2 * p->Z;
+  if (p->Xlo.ptr)
+    MLF_LAUNCH(launch_make_split(p->X.ptr, p->Xb.ptr, p->Xlo.ptr, nl, p->psf->ctx->stream));
+  else
+    MLF_LAUNCH(launch_make_shadow(p->kp, p->X.ptr, p->Xb.ptr, nl, p->slot.xs, p->psf->ctx->stream));
+  return MLF_OK;
+}
+
+// Rebuild the FP32 volume from the split form after view steps (exact).
+mlf_status mlf::patch_merge_split(mlf_patch_s* p) {
+  if (!p->Xlo.ptr) return MLF_OK;
+  MLF_LAUNCH(launch_merge_split(p->Xb.ptr, p->Xlo.ptr, p->X.ptr, (size_t)p->n1 * p->n2 * p->Z,
                                 p->psf->ctx->stream));
   return MLF_OK;
 }
@@ -1214,7 +1233,8 @@ mlf_status mlf::patch_filter(mlf_patch_s* p) {
 
 size_t mlf::patch_device_bytes(const mlf_patch_s* p) {
   size_t b = p->X.bytes() + p->LF.bytes() + p->part.bytes() + p->R.bytes() + p->y.bytes() +
-             p->tmp.bytes() + p->pb.bytes() + p->pa.bytes() + p->partial.bytes() + p->Xb.bytes();
+             p->tmp.bytes() + p->pb.bytes() + p->pa.bytes() + p->partial.bytes() + p->Xb.bytes() +
+             p->Xlo.bytes();
   b += p->d_fsched.bytes() + p->d_bsched.bytes() + p->d_rsched.bytes() + p->f_taps_1.bytes() +
        p->f_taps_2.bytes();
   return b;
@@ -1234,6 +1254,7 @@ size_t mlf::patch_bytes_estimate(mlf_psf_s* ps, mlf_precision prec, int n1, int 
   b += 4 * n * kP + 4 * n * Z + 8 * n;
   b += 4 * (size_t)partial_elems(ps, tpz, cw, fwd_bm2(kp), n1, n2, nullptr);
   if (kp != Precision::kF32 && tpz) b += 2 * n * Z * kShadowRow;
+  if (kp == Precision::kBF16 && tpz) b += 2 * n * Z * kShadowRow;  // lower plane of the split volume
   // entry lists and filter taps: < 1 MB
   return b + (1u << 20);
 }
@@ -1433,6 +1454,7 @@ mlf_status mlf_patch_run(mlf_patch p, int32_t iterations, int32_t views_limit, i
   MLF_HIP(hipEventRecord(p->ev0, st));
   for (int it = 0; it < iterations; ++it) {
     for (int k = 0; k < nv; ++k) MLF_CHECK(patch_view_step(p, k));
+    MLF_CHECK(patch_merge_split(p));
     if (filter) MLF_CHECK(patch_filter(p));
   }
   MLF_HIP(hipEventRecord(p->ev1, st));

@@ -109,6 +109,7 @@ struct mlf_patch_s {
   float xs = 1.f;                      // FP16: shadow = X * xs (chosen so that X * xs ~ 1)
   mlf::DevBuf<float> partial;               // forward window partials
   mlf::DevBuf<uint16_t> Xb;                 // BF16 shadow of X, [lenslet][kShadowRow] (BF16 Toeplitz)
+  mlf::DevBuf<uint16_t> Xlo;                // BF16 Toeplitz: lower plane of the split volume (SlotDesc::Xlo)
   mlf::DevBuf<int64_t> d_poff;
   // work entries of this patch: host (without slot index) for batch assembly,
   // device (slot 0) for single-patch launches
@@ -143,6 +144,7 @@ namespace mlf {
 void psf_free(mlf_psf psf);  // called with g_life_mu held
 mlf_status patch_filter(mlf_patch_s* p);
 mlf_status patch_refresh_shadow(mlf_patch_s* p);
+mlf_status patch_merge_split(mlf_patch_s* p);
 // Precision of the kernels for an ABI precision.
 inline Precision kernel_precision(mlf_precision p) {
   return p == MLF_PRECISION_BF16 ? Precision::kBF16

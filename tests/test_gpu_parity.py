@@ -104,6 +104,68 @@ def test_volume_roundtrip_layout(ctx):
     assert np.array_equal(p.get_volume(f64=True), X.astype(np.float64))
 
 
+def _edge_volume(wl, seed):
+    """FP32 volume at the rounding edges of a 16-bit split of X + 2^15: exact
+    ties of the upper half, mantissas that carry into the next binade, lower
+    halves just below the tie, zeros, and normal values from 1e-30 to 1e30."""
+    rng = np.random.default_rng(seed)
+    n = wl.I1 * wl.I2 * wl.Z
+    bits = rng.integers(0x0D800000, 0x71000000, n, dtype=np.uint64).astype(np.uint32)
+    kind = rng.integers(0, 5, n)
+    bits = np.where(kind == 0, (bits & 0xFFFF0000) | 0x8000, bits)
+    bits = np.where(kind == 1, bits | 0x007FFFFF, bits)
+    bits = np.where(kind == 2, (bits & 0xFFFF0000) | 0x7FFF, bits)
+    bits = np.where(kind == 3, 0, bits).astype(np.uint32)
+    return bits.view(np.float32).reshape((wl.I1, wl.I2, wl.Z), order="F")
+
+
+def _decayed(X, w):
+    """X after view steps with every ratio 0: X <- fl((1 - w_k) X) per view
+    (recon:130-137 with the backprojected ratio 0)."""
+    x = X.copy()
+    for wk in w:
+        x = (np.float32(1) - np.float32(wk)) * x
+    return x
+
+
+@pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
+def test_view_steps_keep_the_fp32_volume_exact(ctx, prec):
+    """With a zero image every ratio is 0, so each view step multiplies X by
+    (1 - w) in FP32, one rounding per step.  The BF16 path holds the volume
+    between view steps as the two 16-bit halves of X + 2^15; the result must
+    still be the FP32 arithmetic bit for bit, at the split's rounding edges,
+    alone and in a batch (graph-captured view loop)."""
+    wl = synth.TINY
+    psf = _psf(ctx, tables.synthetic_compact(wl))
+    _, w = psf.views()
+    X = _edge_volume(wl, 11)
+    p = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision=prec)
+    p.set_image(np.zeros((wl.I1, wl.I2), np.float32))
+    p.set_volume(X)
+    p.run(1, views_limit=3, filter=False)
+    got = p.get_volume()
+    want = _decayed(X, w[:3])
+    assert np.array_equal(got.view(np.uint32), want.view(np.uint32))
+    p.run(1, views_limit=2, filter=False)  # continues from the merged volume
+    want = _decayed(want, w[:2])
+    assert np.array_equal(p.get_volume().view(np.uint32), want.view(np.uint32))
+
+    X2 = _edge_volume(wl, 12)
+    q = mesolf_amd.Patch(psf, wl.n1, wl.n2, precision=prec)
+    q.set_image(np.zeros((wl.I1, wl.I2), np.float32))
+    q.set_volume(X2)
+    b = mesolf_amd.Batch(psf, 2, precision=prec)
+    b.attach(q)
+    b.run(2, filter=False)
+    want = _decayed(_decayed(X2, w), w)
+    assert np.array_equal(q.get_volume().view(np.uint32), want.view(np.uint32))
+    b.detach(q)
+    b.close()
+    for x in (q, p):
+        x.close()
+    psf.close()
+
+
 @pytest.mark.parametrize("prec", ["fp32", "bf16", "fp16"])
 def test_view_steps_parity(ctx, small_problem, prec):
     tabs, cp, truth, LF = small_problem
