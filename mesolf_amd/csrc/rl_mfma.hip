@@ -1000,6 +1000,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       lds_barrier();  // Rv written, Rs read by every wave (the ring copies overwrite Rs)
     }
 
+    bool pref = false;  // the group's first tap slices were issued by the previous group
     for (int gi = 0; gi < g.groups; ++gi) {  // phase group grp: 16 * NT phases from 16 * NT * grp
       const int grp = grp0 + gi;
       const T* Bz = Wb + boffz + (size_t)(grp * 16 * NT) * epad;
@@ -1089,6 +1090,65 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       static_assert(kEpiLoads + (bwd_stages(OCC) - 2 * (OCC >= 3 ? 1 : 2)) * ((NT + 3) / 4) <= 40,
                     "wait_vm_le covers at most 40 loads in flight");
 
+      // SPB K-steps per barrier: a step's copies are waited for, and the
+      // ring refilled, once per SPB steps; the ring then runs STG - SPB
+      // steps ahead.  nks is even (taps padded to 64).
+      constexpr int SPB = OCC >= 3 ? 1 : 2;
+      static_assert(STG >= 2 * SPB, "ring too shallow");
+      // Taps go global -> LDS by DMA (global_load_lds, one copy per
+      // workgroup instead of one per wave through the L1) into a ring of
+      // STG K-step slices [16 NT rows][32 taps], STG - SPB steps ahead;
+      // no slice past the last step is copied, and the waits count only
+      // the copies issued.  The window offset of a lane's 8 taps follows the step's row
+      // block and column chunk (bwd_quad_tap), stepped without divisions.
+      const int nch = (K + 7) / 8;
+      T* Bq = reinterpret_cast<T*>(smem);  // ring (aliases the dead staging window Rs)
+      constexpr int kSlice = NT * 16 * 32;  // elements per K-step slice
+      const int ncopy = w < NT ? (NT - w + 3) / 4 : 0;  // copies per step issued by this wave
+      // A copy moves one N-tile's slice (16 rows x 32 taps) as 64 16-byte
+      // pieces, DMA lane s landing at byte 16 s.  Piece (row, 8-tap group q)
+      // goes to s = 16 q + row, so the 16 lanes reading group q at rows
+      // 0..15 read 16 consecutive pieces and the four groups the same bank
+      // groups (the pattern ds_read_b128 serves without conflicts).
+      // SPL: slice row j of a paired N-tile (half h) is phase row
+      // 8 (j / 4) + 4 h + j % 4 of the pair (lph).
+      const int jr = lane & 15, tq = (lane >> 4) * 8;
+      const int tap_lane_s = (jr * epad + tq) * (int)sizeof(T);                      // unpaired tile
+      const int tap_lane_p = ((8 * (jr >> 2) + (jr & 3)) * epad + tq) * (int)sizeof(T);  // pair, half 0
+      const int tap_lane_h = 4 * epad * (int)sizeof(T);                                // half 1: 4 rows on
+      const int tap_slot = (16 * lg + lr) * 8;  // element offset of (row lr, 8-tap group lg) in an N-tile slice
+      // copies of K-step ks of phase group gx into ring slot `slot`
+      auto issue_taps_g = [&](int gx, int ks, int slot) {
+        if (ks >= nks) return;  // past the last step (wave-uniform)
+        const int kc = ks;
+        const int oddx = PAIR ? (gx & 1) : 0;
+        const T* Bx = Wb + boffz + (size_t)(gx * 16 * NT) * epad;
+        char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
+        for (int nt = w; nt < NT; nt += 4) {
+          const bool pr = PAIR && nt < NT - 1;
+          // first packed row relative to the group
+          const int row0 = !PAIR ? 16 * nt : pr ? 16 * oddx + 16 * (nt & ~1) : (oddx ? 0 : 16 * (NT - 1));
+          const int tl = pr ? tap_lane_p + (nt & 1) * tap_lane_h : tap_lane_s;
+          const char* src = reinterpret_cast<const char*>(Bx + (size_t)row0 * epad + kc * 32) + tl;
+          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
+                                           (__attribute__((address_space(3))) void*)(dst + nt * 1024),
+                                           16, 0, 0);
+        }
+      };
+      auto issue_taps = [&](int ks, int slot) { issue_taps_g(grp, ks, slot); };
+      // the first STG - SPB slices of the next phase group of the entry,
+      // issued once every wave is done with the ring, so that they land
+      // while this group's update runs (16-bit)
+      auto prefetch_next = [&]() {
+        if constexpr (KPL > 1) {
+          if (gi + 1 < g.groups) {
+            lds_barrier();
+  #pragma unroll
+            for (int q = 0; q < STG - SPB; ++q) issue_taps_g(grp + 1, q, q);
+            pref = true;
+          }
+        }
+      };
       if constexpr (KPL == 1) {
         load_x(0, MT - 1);
         load_t(0);
@@ -1113,50 +1173,11 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         }
       } else {
-        // Taps go global -> LDS by DMA (global_load_lds, one copy per
-        // workgroup instead of one per wave through the L1) into a ring of
-        // STG K-step slices [16 NT rows][32 taps], STG - 1 steps ahead;
-        // slices past the last step re-load the last one, so every wave
-        // issues the same number of copies per step and waits with a fixed
-        // count.  The window offset of a lane's 8 taps follows the step's row
-        // block and column chunk (bwd_quad_tap), stepped without divisions.
-        const int nch = (K + 7) / 8;
-        T* Bq = reinterpret_cast<T*>(smem);  // ring (aliases the dead staging window Rs)
-        constexpr int kSlice = NT * 16 * 32;  // elements per K-step slice
-        const int ncopy = w < NT ? (NT - w + 3) / 4 : 0;  // copies per step issued by this wave
-        // A copy moves one N-tile's slice (16 rows x 32 taps) as 64 16-byte
-        // pieces, DMA lane s landing at byte 16 s.  Piece (row, 8-tap group q)
-        // goes to s = 16 q + row, so the 16 lanes reading group q at rows
-        // 0..15 read 16 consecutive pieces and the four groups the same bank
-        // groups (the pattern ds_read_b128 serves without conflicts).
-        // SPL: slice row j of a paired N-tile (half h) is phase row
-        // 8 (j / 4) + 4 h + j % 4 of the pair (lph).
-        const int jr = lane & 15, tq = (lane >> 4) * 8;
-        const int tap_lane_s = (jr * epad + tq) * (int)sizeof(T);                      // unpaired tile
-        const int tap_lane_p = ((8 * (jr >> 2) + (jr & 3)) * epad + tq) * (int)sizeof(T);  // pair, half 0
-        const int tap_lane_h = 4 * epad * (int)sizeof(T);                                // half 1: 4 rows on
-        const int tap_slot = (16 * lg + lr) * 8;  // element offset of (row lr, 8-tap group lg) in an N-tile slice
-        auto issue_taps = [&](int ks, int slot) {
-          const int kc = min(ks, nks - 1);
-          char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
-          for (int nt = w; nt < NT; nt += 4) {
-            const bool pr = PAIR && nt < NT - 1;
-            // first packed row relative to the group
-            const int row0 = !PAIR ? 16 * nt : pr ? 16 * odd + 16 * (nt & ~1) : (odd ? 0 : 16 * (NT - 1));
-            const int tl = pr ? tap_lane_p + (nt & 1) * tap_lane_h : tap_lane_s;
-            const char* src = reinterpret_cast<const char*>(Bz + (size_t)row0 * epad + kc * 32) + tl;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                             (__attribute__((address_space(3))) void*)(dst + nt * 1024),
-                                             16, 0, 0);
-          }
-        };
-        // SPB K-steps per barrier: a step's copies are waited for, and the
-        // ring refilled, once per SPB steps; the ring then runs STG - SPB
-        // steps ahead.  nks is even (taps padded to 64).
-        constexpr int SPB = OCC >= 3 ? 1 : 2;
-        static_assert(STG >= 2 * SPB, "ring too shallow");
+        if (!pref) {
   #pragma unroll
-        for (int q = 0; q < STG - SPB; ++q) issue_taps(q, q);
+          for (int q = 0; q < STG - SPB; ++q) issue_taps(q, q);
+        }
+        pref = false;
         if constexpr (PRE > 0) {
           load_x(0, PRE);  // after the first slices: their waits need not cover these loads
           if constexpr (TPRE) {
@@ -1169,9 +1190,9 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         int qb = 0, qc = 0;  // row block and column chunk of step ks
         for (int ks = 0; ks < nks; ks += SPB) {
           // this wave's copies of steps ks .. ks + SPB - 1 landed: younger are
-          // the copies of steps up to ks + STG - SPB - 1 and, while those
-          // steps were copied before the loop, the epilogue loads
-          wait_vm_le((STG - 2 * SPB) * ncopy + (ks + SPB <= STG - SPB ? kEpiLoads : 0));
+          // the copies of steps up to min(ks + STG - SPB, nks) - 1 and, while
+          // those steps were copied before the loop, the epilogue loads
+          wait_vm_le(max(0, min(ks + STG - SPB, nks) - ks - SPB) * ncopy + (ks + SPB <= STG - SPB ? kEpiLoads : 0));
           lds_barrier();  // everyone's copies landed; everyone is done with steps ks - SPB ..
   #pragma unroll
           for (int h = 0; h < SPB; ++h) issue_taps(ks + STG - SPB + h, (ks + STG - SPB + h) % STG);
@@ -1318,6 +1339,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       };
       if constexpr (TPRE) {
         // every operand was loaded before the K loop
+        prefetch_next();
   #pragma unroll
         for (int mt = 0; mt < MT; ++mt) update_tile(mt, xv[mt], bva[mt]);
       } else if constexpr (PRE > 0) {
@@ -1330,12 +1352,15 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         load_x(PRE, MT);
         if constexpr (SPL) {
           floatx4_t bw[NT];
+          ldt(1, bw);
+          prefetch_next();  // after this group's first loads: their waits need not cover the copies
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
-            if (mt + 1 < MT) ldt(mt + 1, mt % 2 == 0 ? bw : bv);
+            if (mt >= 1 && mt + 1 < MT) ldt(mt + 1, mt % 2 == 0 ? bw : bv);
             update_tile(mt, xv[mt], mt % 2 == 0 ? bv : bw);
           }
         } else {
+          prefetch_next();
   #pragma unroll
           for (int mt = 0; mt < MT; ++mt) {
             if (mt > 0) load_t(mt);
@@ -1347,6 +1372,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         floatx4_t xa[NT], ba[NT], xb[NT], bb[NT];
         ldx(0, xa);
         ldt(0, ba);
+        prefetch_next();
   #pragma unroll
         for (int mt = 0; mt < MT; ++mt) {
           if (mt % 2 == 0) {
@@ -1364,11 +1390,13 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         }
       }
-      // The ring is refilled by the next group and LDS reused by the next
-      // entry.  No tap copy is in flight here: the epilogue waited for its
-      // last X and 1/B1 loads, which were issued after every copy, and
-      // vector-memory operations retire in order; only this group's stores
-      // may still be in flight, and they need not be waited for.
+      // LDS is reused by the next entry (the staging window aliases the
+      // ring) once every wave is here.  After an entry's last group no tap
+      // copy is in flight: that group prefetched none, its epilogue waited
+      // for loads issued after its last copies, and vector-memory operations
+      // complete in order; only stores may still be in flight, and they need
+      // not be waited for.  Copies prefetched for the next group may be in
+      // flight after the other groups; that group's first wait covers them.
       lds_barrier();
     }
   }
