@@ -50,14 +50,16 @@ def algorithmic_bytes_per_view(wl, precision: str, kernel: str) -> float:
 
 def design_bytes_per_view(wl, precision: str, kernel: str) -> float:
     """Bytes this design's launch must move: the backprojection + update reads
-    and writes the FP32 master volume (the update stays FP32), writes the
-    16-bit shadow the next forward reads, and reads the view's taps, its 1/B1
-    table and the ratio image; the forward reads the shadow (or X) and taps."""
+    and writes the FP32 volume (the update stays FP32: for BF16 as the two
+    16-bit planes of its split form, whose upper plane is the next forward's
+    operand; for FP16 as FP32 X plus the FP16 shadow it writes), and reads the
+    view's taps, its 1/B1 table and the ratio image; the forward reads the
+    16-bit operand plane (or X) and the taps."""
     vox = wl.n1 * wl.n2 * wl.Z * 225
     esz = 4 if precision == "fp32" else 2
     taps = 225 * sum((2 * r + 1) ** 2 for r in wl.radii)
     if kernel == "backward_update":
-        shadow = 0 if precision == "fp32" else 2 * vox
+        shadow = 2 * vox if precision == "fp16" else 0
         return 8.0 * vox + shadow + esz * taps + 4.0 * taps + 4.0 * wl.n1 * wl.n2
     return float(esz * vox + esz * taps)
 
@@ -810,7 +812,8 @@ def main():
             tr = tr.get(f"fwd_columns_per_wave={fmt}" if dominant == "forward" else f"bwd_phase_tiles={bnt}")
         if tr:
             roof["traffic"] = tr["read"] + tr["write"]
-            roof["traffic_source"] = "rocprofv3 FETCH_SIZE+WRITE_SIZE, profiles/"
+            roof["traffic_source"] = ("rocprofv3 L2-to-memory request counters (" + tr.get("method", "FETCH_SIZE+WRITE_SIZE")
+                                      + "), profiles/hbm_traffic_cfg2_101x101x51.json")
     per_kernel = {c: {"total_ms": round(ms, 3), "launches": n,
                       "avg_ms": round(ms / n, 4) if n else None} for c, (ms, n) in ktimes.items()}
 
