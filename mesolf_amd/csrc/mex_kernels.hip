@@ -118,13 +118,20 @@ __global__ __launch_bounds__(kTileX* kTileY) void k_mex_conv_segments(
 //   out = sum_{ky, t} in[15 i + t - h, y - h + ky, n] * Kq[q][ky][t][p],
 //   Kq[q][ky][t][p] = K[t - p, ky, p, q] for 0 <= t - p < kw (else 0),
 // i.e. M = lenslet columns i, N = the 15 phases p (padded to 16), K = (ky, t).
-// A workgroup takes one output row and kMexImg images (a wave each) over 32
-// lenslet columns (two M-tiles); input rows are staged in LDS kMexCK at a
-// time, Kq fragments stream from L2 (every workgroup of the row's q reads the
-// same 64 consecutive floats per K-step).
+// A workgroup takes one output row and kMexImg images (kMexImgW per wave) over
+// 32 lenslet columns (two M-tiles).  Per chunk of kMexCK input rows it stages
+// the images' rows and the chunk's Kq rows (T4 x 16 floats per row, read with
+// 16-byte loads, shared by every wave) in LDS; each Kq fragment read from LDS
+// feeds the MFMAs of both M-tiles.  Measured on the seed-library loop
+// (396 x 396, kw = 241): 7.37 s with these settings (11.8 + 16 KB of LDS per
+// workgroup at kw = 241); Kq fragments from L2 instead: 9.60 s with one image
+// per wave, 8.27 s with two (three 9.52, four 10.76 s); two input rows per
+// chunk, Kq from L2: 10.59 s.
 typedef float mex_floatx4 __attribute__((ext_vector_type(4)));
-constexpr int kMexImg = 4;
-constexpr int kMexCK = 2;
+constexpr int kMexImgW = 1;
+constexpr int kMexImg = 8 * kMexImgW;
+constexpr int kMexThreads = 64 * kMexImg / kMexImgW;  // one wave per kMexImgW images
+constexpr int kMexCK = 1;
 constexpr int kMexCols = 32;  // lenslet columns per workgroup (2 M-tiles)
 
 __global__ void k_mex_kq(const float* __restrict__ K, int kw, int T4, float* __restrict__ Kq) {
@@ -145,24 +152,26 @@ __global__ void k_mex_kq(const float* __restrict__ K, int kw, int T4, float* __r
 // List mode (seed-library batches): image j of the launch is input image
 // src[j] and its projection is added to output image dst[j] (the dst of one
 // launch are distinct, so no two workgroups add to the same pixel).
-__global__ __launch_bounds__(256) void k_mex_mfma(const float* __restrict__ in, int W, int H, int n,
+__global__ __launch_bounds__(kMexThreads) void k_mex_mfma(const float* __restrict__ in, int W, int H, int n,
                                                   const float* __restrict__ Kq, int kw, int T4,
                                                   float* __restrict__ out,
                                                   const int32_t* __restrict__ src,
                                                   const int32_t* __restrict__ dst) {
-  extern __shared__ float xs[];  // [kMexCK][kMexImg][XL]
+  extern __shared__ float xs[];  // [kMexCK][kMexImg][XL], then the chunk's Kq rows [kMexCK][T4][16]
   const int y = blockIdx.x, ng = blockIdx.y, ib = blockIdx.z;
   const int q = y % kN, h = kw / 2;
   const int i0 = kMexCols * ib;
   const int xlo = kN * i0 - h;
   const int XL = kN * kMexCols + T4;
   const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6, lr = lane & 15, lg = lane >> 4;
-  const int img = ng * kMexImg + w;
-  mex_floatx4 acc[2] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+  const int img0 = ng * kMexImg + kMexImgW * w;
+  mex_floatx4 acc[kMexImgW][2];
+#pragma unroll
+  for (int m = 0; m < kMexImgW; ++m) acc[m][0] = acc[m][1] = mex_floatx4{0.f, 0.f, 0.f, 0.f};
   const float* Kqq = Kq + (size_t)q * kw * T4 * 16;
   for (int c0 = 0; c0 < kw; c0 += kMexCK) {
     __syncthreads();  // previous chunk consumed
-    for (int o = tid; o < kMexCK * kMexImg * XL; o += 256) {
+    for (int o = tid; o < kMexCK * kMexImg * XL; o += kMexThreads) {
       const int xi = o % XL;
       const int im = (o / XL) % kMexImg;
       const int kk = o / (XL * kMexImg);
@@ -172,30 +181,44 @@ __global__ __launch_bounds__(256) void k_mex_mfma(const float* __restrict__ in, 
         v = in[(size_t)(src ? src[nn] : nn) * W * H + (size_t)r * W + x];
       xs[o] = v;
     }
+    {
+      float* Bs = xs + kMexCK * kMexImg * XL;
+      const mex_floatx4* src4 = reinterpret_cast<const mex_floatx4*>(Kqq + (size_t)c0 * T4 * 16);
+      for (int o = tid; o < kMexCK * T4 * 4; o += kMexThreads)
+        reinterpret_cast<mex_floatx4*>(Bs)[o] = (c0 + o / (T4 * 4) < kw) ? src4[o] : mex_floatx4{0.f, 0.f, 0.f, 0.f};
+    }
     __syncthreads();
     for (int kk = 0; kk < kMexCK && c0 + kk < kw; ++kk) {
-      const float* Bq = Kqq + (size_t)(c0 + kk) * T4 * 16 + lg * 16 + lr;
-      const float* Ar = xs + (kk * kMexImg + w) * XL + kN * lr + lg;
+      const float* Bq = xs + kMexCK * kMexImg * XL + (size_t)kk * T4 * 16 + lg * 16 + lr;
+      const float* Ar = xs + (kk * kMexImg + kMexImgW * w) * XL + kN * lr + lg;
 #pragma unroll 8
       for (int s4 = 0; s4 < T4; s4 += 4) {
         const float b = Bq[s4 * 16];
-        acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[s4], b, acc[0], 0, 0, 0);
-        acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[s4 + kN * 16], b, acc[1], 0, 0, 0);
+#pragma unroll
+        for (int m = 0; m < kMexImgW; ++m) {
+          acc[m][0] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[m * XL + s4], b, acc[m][0], 0, 0, 0);
+          acc[m][1] = __builtin_amdgcn_mfma_f32_16x16x4f32(Ar[m * XL + s4 + kN * 16], b, acc[m][1], 0, 0, 0);
+        }
       }
     }
   }
-  if (img >= n || lr >= kN) return;
-  float* o = out + (size_t)(dst ? dst[img] : img) * W * H + (size_t)y * W;
+  if (lr >= kN) return;
 #pragma unroll
-  for (int it = 0; it < 2; ++it)
+  for (int m = 0; m < kMexImgW; ++m) {
+    const int img = img0 + m;
+    if (img >= n) break;
+    float* o = out + (size_t)(dst ? dst[img] : img) * W * H + (size_t)y * W;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int x = kN * (i0 + 16 * it + 4 * lg + j) + lr;
-      if (x < W) {
-        if (dst) o[x] += acc[it][j];
-        else o[x] = acc[it][j];
+    for (int it = 0; it < 2; ++it)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int x = kN * (i0 + 16 * it + 4 * lg + j) + lr;
+        if (x < W) {
+          if (dst) o[x] += acc[m][it][j];
+          else o[x] = acc[m][it][j];
+        }
       }
-    }
+  }
 }
 
 }  // namespace
@@ -216,13 +239,13 @@ hipError_t launch_mex_mfma_list(const float* in, int64_t W, int64_t H, int64_t n
                                 hipStream_t st) {
   const int T4 = (kw + kN - 1 + 3) / 4 * 4;
   const int XL = kN * kMexCols + T4;
-  const size_t shmem = sizeof(float) * kMexCK * kMexImg * XL;
+  const size_t shmem = sizeof(float) * kMexCK * (kMexImg * XL + T4 * 16);
   static hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mex_mfma),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (attr != hipSuccess) return attr;
   dim3 grid((unsigned)H, (unsigned)((n + kMexImg - 1) / kMexImg),
             (unsigned)((W / kN + 1 + kMexCols - 1) / kMexCols));
-  hipLaunchKernelGGL(k_mex_mfma, grid, dim3(256), shmem, st, in, (int)W, (int)H, (int)n, Kq, kw, T4,
+  hipLaunchKernelGGL(k_mex_mfma, grid, dim3(kMexThreads), shmem, st, in, (int)W, (int)H, (int)n, Kq, kw, T4,
                      out, src, dst);
   return hipGetLastError();
 }
@@ -232,13 +255,13 @@ hipError_t launch_mex_mfma(const float* in, int64_t W, int64_t H, int64_t n, con
   const int T4 = (kw + kN - 1 + 3) / 4 * 4;
   hipLaunchKernelGGL(k_mex_kq, dim3(1024), dim3(256), 0, st, K, kw, T4, Kq);
   const int XL = kN * kMexCols + T4;
-  const size_t shmem = sizeof(float) * kMexCK * kMexImg * XL;
+  const size_t shmem = sizeof(float) * kMexCK * (kMexImg * XL + T4 * 16);
   static hipError_t attr = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_mex_mfma),
                                                hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
   if (attr != hipSuccess) return attr;
   dim3 grid((unsigned)H, (unsigned)((n + kMexImg - 1) / kMexImg),
             (unsigned)((W / kN + 1 + kMexCols - 1) / kMexCols));
-  hipLaunchKernelGGL(k_mex_mfma, grid, dim3(256), shmem, st, in, (int)W, (int)H, (int)n, Kq, kw, T4,
+  hipLaunchKernelGGL(k_mex_mfma, grid, dim3(kMexThreads), shmem, st, in, (int)W, (int)H, (int)n, Kq, kw, T4,
                      out, nullptr, nullptr);
   return hipGetLastError();
 }
