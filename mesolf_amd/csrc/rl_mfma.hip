@@ -1152,24 +1152,34 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       if constexpr (KPL == 1) {
         load_x(0, MT - 1);
         load_t(0);
-        typename M::frag bn[NT];
+        // taps in chunks of four K-steps (the pack's chunk order): a lane's
+        // taps of a chunk are one 16-byte load, issued a chunk ahead
+        static_assert(KSTEP == 4, "FP32 K-step of 4 taps");
+        const float* Bl = Bz + (size_t)lr * epad + 4 * lg;
+        floatx4_t bc[NT];
   #pragma unroll
-        for (int nt = 0; nt < NT; ++nt) bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + lg * KPL);
-        for (int ks = 0; ks < nks; ++ks) {
-          typename M::frag bf[NT];
-  #pragma unroll
-          for (int nt = 0; nt < NT; ++nt) bf[nt] = bn[nt];
-          if (ks + 1 < nks) {
+        for (int nt = 0; nt < NT; ++nt) bc[nt] = gload(reinterpret_cast<const floatx4_t*>(Bl + (size_t)nt * 16 * epad));
+        const int nc4 = nks / 4;  // epad is a multiple of 32
+        for (int c = 0; c < nc4; ++c) {
+          floatx4_t bx[NT];
+          if (c + 1 < nc4) {
   #pragma unroll
             for (int nt = 0; nt < NT; ++nt)
-              bn[nt] = M::load(Bz + (size_t)(nt * 16 + lr) * epad + (ks + 1) * KSTEP + lg * KPL);
+              bx[nt] = gload(reinterpret_cast<const floatx4_t*>(Bl + (size_t)nt * 16 * epad + 16 * (c + 1)));
           }
-          const int o = toff[ks * KSTEP + lg];
   #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) {
-            const typename M::frag af = Rw[base + mt * Wr + o];
+          for (int s4 = 0; s4 < 4; ++s4) {
+            const int o = toff[(4 * c + s4) * KSTEP + lg];
   #pragma unroll
-            for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bf[nt], af, acc[mt][nt]);
+            for (int mt = 0; mt < MT; ++mt) {
+              const typename M::frag af = Rw[base + mt * Wr + o];
+  #pragma unroll
+              for (int nt = 0; nt < NT; ++nt) acc[mt][nt] = M::mma(bc[nt][s4], af, acc[mt][nt]);
+            }
+          }
+          if (c + 1 < nc4) {
+  #pragma unroll
+            for (int nt = 0; nt < NT; ++nt) bc[nt] = bx[nt];
           }
         }
       } else {
@@ -1438,9 +1448,13 @@ __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, P
     int e2, e1;
     if (pg.quad) {
       bwd_quad_tap(t, K, &e2, &e1);
-    } else {  // e1-fastest, dense
-      e2 = t / K;
-      e1 = t - e2 * K;
+    } else {
+      // FP32: taps e1-fastest, dense, stored in chunks of 16 (four K-steps of
+      // 4) lane-group-major: position 16 c + 4 g + s holds tap 16 c + 4 s + g,
+      // so lane group g reads its taps of the four steps with one 16-byte load
+      const int lt = (t & ~15) | ((t & 3) << 2) | ((t >> 2) & 3);
+      e2 = lt / K;
+      e1 = lt - e2 * K;
     }
     const bool ok = ph < kP && e2 < K && e1 < K;
     d[o] = (T)(ok ? s[((size_t)e2 * K + e1) * kP + ph] * sc : 0.f);
