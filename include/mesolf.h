@@ -228,8 +228,9 @@ mlf_status mlf_batch_last_run_ms(mlf_batch b, double* ms);
 /* No counterpart in the reference.  The view-step kernels are built in
  * several tile shapes: the forward projection's input columns per wave
  * (BF16: 4 or 2, FP32: 2 or 1; a workgroup covers 16 x 4*columns lenslets),
- * the backprojection's 16-phase tiles per phase group (5 or 3: 3 or 5 groups
- * cover the 225 phases) and its phase groups per work entry (0: all groups
+ * the backprojection's 16-phase tiles per phase group (5 or 3, and 4 for
+ * BF16: 3, 5 or 4 groups of 80, 48 or 64 phases cover the 225 phases) and its
+ * phase groups per work entry (0: all groups
  * of a lenslet tile in one entry, which builds the tile's ratio window once;
  * 1: one group per entry).
  * Autotuning times every shape on `views` view steps of the patch (or of all

@@ -889,7 +889,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   constexpr int PRE = OCC >= 3 ? 0 : (NT == 3 && M::KPL > 1 ? MT : MT - 1);
   constexpr bool TPRE = PRE == MT;
   static_assert(!SPL || __is_same(T, __bf16), "the split volume is a BF16 form");
-  static_assert(NT % 2 == 1, "16-bit N-tiles: pairs and one unpaired tile");
+  // SPL N-tiles: NPR pairs, then (odd NT) one unpaired tile
+  constexpr int NPR = NT / 2;
   extern __shared__ __align__(16) char smem[];
   const int nent = uni(cload(g.count));
   for (int ent = blockIdx.x; ent < nent; ent += gridDim.x) {
@@ -1021,10 +1022,12 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // 32-aligned one).  The tap copies below place the rows to match.
       constexpr bool PAIR = SPL;
       const int gbase = grp * 16 * NT;
-      const int odd = PAIR ? (grp & 1) : 0;
+      // with an odd tile count, a group starts 16 phases past a 32-aligned
+      // phase when its index is odd
+      const int odd = PAIR && NT % 2 == 1 ? (grp & 1) : 0;
       auto lph = [&](int nt) -> int {
         if constexpr (PAIR) {
-          if (nt < NT - 1) return gbase + 16 * odd + 16 * (nt & ~1) + 8 * lg + 4 * (nt & 1);
+          if (nt < 2 * NPR) return gbase + 16 * odd + 16 * (nt & ~1) + 8 * lg + 4 * (nt & 1);
           return gbase + (odd ? 0 : 16 * (NT - 1)) + 4 * lg;
         } else {
           return gbase + 16 * nt + 4 * lg;
@@ -1034,7 +1037,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // last group reaches past phase 224)
       auto tfull = [&](int nt) -> bool {
         if constexpr (PAIR) {
-          if (nt < NT - 1) return true;
+          if (nt < 2 * NPR) return gbase + 16 * odd + 16 * (nt & ~1) + 32 <= kP;
           return gbase + (odd ? 0 : 16 * (NT - 1)) + 16 <= kP;
         } else {
           return gbase + 16 * nt + 16 <= kP;
@@ -1057,7 +1060,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         for (int nt = 0; nt < NT; ++nt) {
           if constexpr (SPL) {
             const int so = 2 * (l * kShadowRow + lph(nt));
-            if (nt < NT - 1) {
+            if (nt < 2 * NPR) {
               if (nt & 1) continue;  // loaded with its pair
               dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(sr, so, 0, 0));
               dst[nt + 1] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(lor, so, 0, 0));
@@ -1086,7 +1089,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       };
       auto load_t = [&](int mt) { ldt(mt, bv); };
       // loads issued before the K loop: load_x(0, PRE) and the first (or every) 1/B1 row set
-      constexpr int kEpiLoads = PRE > 0 ? PRE * (SPL ? NT + 1 : NT) + (TPRE ? MT * NT : NT) : 0;
+      // SPL: two loads per pair of N-tiles and two for an unpaired one
+      constexpr int kEpiLoads = PRE > 0 ? PRE * (SPL ? 2 * ((NT + 1) / 2) : NT) + (TPRE ? MT * NT : NT) : 0;
       static_assert(kEpiLoads + (bwd_stages(OCC) - 2 * (OCC >= 3 ? 1 : 2)) * ((NT + 3) / 4) <= 40,
                     "wait_vm_le covers at most 40 loads in flight");
 
@@ -1121,11 +1125,11 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       auto issue_taps_g = [&](int gx, int ks, int slot) {
         if (ks >= nks) return;  // past the last step (wave-uniform)
         const int kc = ks;
-        const int oddx = PAIR ? (gx & 1) : 0;
+        const int oddx = PAIR && NT % 2 == 1 ? (gx & 1) : 0;
         const T* Bx = Wb + boffz + (size_t)(gx * 16 * NT) * epad;
         char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
         for (int nt = w; nt < NT; nt += 4) {
-          const bool pr = PAIR && nt < NT - 1;
+          const bool pr = PAIR && nt < 2 * NPR;
           // first packed row relative to the group
           const int row0 = !PAIR ? 16 * nt : pr ? 16 * oddx + 16 * (nt & ~1) : (oddx ? 0 : 16 * (NT - 1));
           const int tl = pr ? tap_lane_p + (nt & 1) * tap_lane_h : tap_lane_s;
@@ -1262,7 +1266,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           if constexpr (SPL) {
             // split words -> X: (upper << 16 | lower) - 2^15, per phase
             uintx4 raw;
-            if (nt < NT - 1) {
+            if (nt < 2 * NPR) {
               const uintx4 hv = __builtin_bit_cast(uintx4, x4[nt & ~1]), lv = __builtin_bit_cast(uintx4, x4[nt | 1]);
               const int h = 2 * (nt & 1);
               raw = uintx4{hv[h], hv[h + 1], lv[h], lv[h + 1]};
@@ -1304,7 +1308,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
           if (tfull(nt)) {  // wave-uniform: the whole N-tile is in the row
             if constexpr (SPL) {
-              if (nt < NT - 1) {
+              if (nt < 2 * NPR) {
                 if ((nt & 1) == 0) {
                   hi_prev = hi;
                   lo_prev = lo;
@@ -1433,7 +1437,7 @@ __global__ void k_pack_fwd(const float* __restrict__ src, T* __restrict__ dst, P
   }
 }
 
-// ABI block [e][phase] -> backward pack [phase_pad = 240][e_pad] (taps contiguous).
+// ABI block [e][phase] -> backward pack [kPhiPadBwd phase rows][e_pad] (taps contiguous).
 template <typename T>
 __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, PackGeo pg) {
   const int z = blockIdx.y, v = blockIdx.z;
@@ -1712,6 +1716,13 @@ static void launch_bwd_t(int nt, int occ, const void* Wb, BwdGeo g, int grid, si
   const T* W = (const T*)Wb;
   // three waves per SIMD only with 3 N-tiles (with 5 the accumulators and
   // operands exceed 168 registers)
+  if constexpr (__is_same(T, __bf16)) {
+    // BF16 only: 4 N-tiles, phase groups of 64 (128 bytes of each 16-bit plane)
+    if (nt == 4) {
+      hipLaunchKernelGGL((k_bwd_mfma<T, 4, 2, SPL>), dim3(grid), dim3(256), smem, st, W, g, w);
+      return;
+    }
+  }
   if (nt == 3 && occ >= 3)
     hipLaunchKernelGGL((k_bwd_mfma<T, 3, 3, SPL>), dim3(grid), dim3(256), smem, st, W, g, w);
   else if (nt == 3)
@@ -1800,6 +1811,8 @@ hipError_t mfma_kernel_attributes() {
       reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 2, true>),
       reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 5, 2, true>),
       reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 3, 3, true>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 4, 2, false>),
+      reinterpret_cast<const void*>(&k_bwd_mfma<__bf16, 4, 2, true>),
   };
   for (const void* k : kernels) {
     const hipError_t e = hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);

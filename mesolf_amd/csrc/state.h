@@ -42,7 +42,7 @@ mlf_status upload_vec(DevBuf<T>& d, const std::vector<T>& h) {
 struct PrecPack {
   bool built = false;
   bool tpz = false;                   // forward tables in the Toeplitz layout [z][e1][e2pad][PHI]
-  DevBuf<uint8_t> fwd, bwd;           // [V][z][e_pad64][PHI], [V][z][240][e_pad32]
+  DevBuf<uint8_t> fwd, bwd;           // [V][z][e_pad64][PHI], [V][z][kPhiPadBwd][e_pad32]
   int64_t pvf = 0, pvb = 0;           // elements per view
   std::vector<int64_t> woff, boff;    // per-plane element offsets inside a view
   std::vector<int32_t> nchunk, epad;

@@ -55,6 +55,8 @@ std::vector<TileShape> tile_candidates(Precision kp) {
   for (int grp : {0, 1})  // all phase groups per entry (one ratio window), or one
     for (int mt : {mt0, mt0 / 2}) {
       for (int nt : {kBwdNTiles, 3}) c.push_back(TileShape{mt, nt, grp, 2});
+      // BF16: 4 N-tiles, phase groups of 64 phases
+      if (kp == Precision::kBF16) c.push_back(TileShape{mt, 4, grp, 2});
       // 16-bit: 3 N-tiles at three waves per SIMD
       if (kp != Precision::kF32) c.push_back(TileShape{mt, 3, grp, 3});
     }

@@ -19,7 +19,8 @@ WL = synth.Workload("odd_37x21", 37, 21, (-20, 0, 12), 1, 1)
 # (forward columns per wave, backprojection N-tiles, phase groups per entry,
 # backprojection waves per SIMD), in the tuner's candidate order
 SHAPES = {"fp32": [(mt, nt, g, 2) for g in (0, 1) for mt in (2, 1) for nt in (5, 3)],
-          "bf16": [s for g in (0, 1) for mt in (4, 2) for s in ((mt, 5, g, 2), (mt, 3, g, 2), (mt, 3, g, 3))]}
+          "bf16": [s for g in (0, 1) for mt in (4, 2)
+                   for s in ((mt, 5, g, 2), (mt, 3, g, 2), (mt, 4, g, 2), (mt, 3, g, 3))]}
 VOL_TOL = {"fp32": 1e-4, "bf16": 2e-3}
 
 
@@ -85,7 +86,13 @@ def test_unknown_tile_shape_rejected(problem):
     q = mesolf_amd.Patch(psf, WL.n1, WL.n2, precision="fp32")
     with pytest.raises(mesolf_amd.InvalidInput):
         q.set_tile_shape(2, 3, 0, 3)  # FP32 has only the two-wave variant
+    with pytest.raises(mesolf_amd.InvalidInput):
+        q.set_tile_shape(2, 4)  # 4 N-tiles are built for BF16 only
     q.close()
+    f = mesolf_amd.Patch(psf, WL.n1, WL.n2, precision="fp16")
+    with pytest.raises(mesolf_amd.InvalidInput):
+        f.set_tile_shape(4, 4)
+    f.close()
 
 
 def test_autotune_picks_a_shape_and_caches_it(problem):
