@@ -885,9 +885,10 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
   // M-tiles whose X (and the first 1/B1 rows) load before the K loop; with
   // none, the epilogue loads each tile one tile ahead.  16-bit with 3 N-tiles
   // has the registers to load every tile's X and 1/B1 rows before the K loop,
-  // so its epilogue waits for no load.
-  constexpr int PRE = OCC >= 3 ? 0 : (NT == 3 && M::KPL > 1 ? MT : MT - 1);
-  constexpr bool TPRE = PRE == MT;
+  // so its epilogue waits for no load; with 4, every tile's X.
+  constexpr int PRE = OCC >= 3 ? 0 : ((NT == 3 || NT == 4) && M::KPL > 1 ? MT : MT - 1);
+  // 3 N-tiles: every tile's 1/B1 rows before the K loop too
+  constexpr bool TPRE = PRE == MT && NT == 3;
   static_assert(!SPL || __is_same(T, __bf16), "the split volume is a BF16 form");
   // SPL N-tiles: NPR pairs, then (odd NT) one unpaired tile
   constexpr int NPR = NT / 2;

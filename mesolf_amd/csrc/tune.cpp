@@ -54,9 +54,10 @@ std::vector<TileShape> tile_candidates(Precision kp) {
   std::vector<TileShape> c;
   for (int grp : {0, 1})  // all phase groups per entry (one ratio window), or one
     for (int mt : {mt0, mt0 / 2}) {
-      for (int nt : {kBwdNTiles, 3}) c.push_back(TileShape{mt, nt, grp, 2});
-      // BF16: 4 N-tiles, phase groups of 64 phases
+      // BF16 first tries 4 N-tiles (phase groups of 64 phases: one 128-byte
+      // line of each split-volume plane row), its default
       if (kp == Precision::kBF16) c.push_back(TileShape{mt, 4, grp, 2});
+      for (int nt : {kBwdNTiles, 3}) c.push_back(TileShape{mt, nt, grp, 2});
       // 16-bit: 3 N-tiles at three waves per SIMD
       if (kp != Precision::kF32) c.push_back(TileShape{mt, 3, grp, 3});
     }

@@ -20,7 +20,7 @@ WL = synth.Workload("odd_37x21", 37, 21, (-20, 0, 12), 1, 1)
 # backprojection waves per SIMD), in the tuner's candidate order
 SHAPES = {"fp32": [(mt, nt, g, 2) for g in (0, 1) for mt in (2, 1) for nt in (5, 3)],
           "bf16": [s for g in (0, 1) for mt in (4, 2)
-                   for s in ((mt, 5, g, 2), (mt, 3, g, 2), (mt, 4, g, 2), (mt, 3, g, 3))]}
+                   for s in ((mt, 4, g, 2), (mt, 5, g, 2), (mt, 3, g, 2), (mt, 3, g, 3))]}
 VOL_TOL = {"fp32": 1e-4, "bf16": 2e-3}
 
 
@@ -166,7 +166,7 @@ def test_tune_cache_file_entries_are_validated(problem, tmp_path, monkeypatch):
         p = mesolf_amd.Patch(psf, *grid, precision=prec)
         shapes[(prec, grid)] = p.tile_shape()
         p.close()
-    assert shapes[("bf16", (WL.n1, WL.n2))] == (4, 5, 0, 2)   # bogus entry ignored
+    assert shapes[("bf16", (WL.n1, WL.n2))] == (4, 4, 0, 2)   # bogus entry ignored: the default
     assert shapes[("fp16", (WL.n1, WL.n2))] == (2, 3, 1, 3)   # its own key
-    assert shapes[("bf16", (16, 30))] == (4, 5, 0, 2)         # older-format line ignored
+    assert shapes[("bf16", (16, 30))] == (4, 4, 0, 2)         # older-format line ignored
     assert shapes[("fp32", (16, 30))] == (1, 3, 0, 2)
