@@ -17,6 +17,7 @@ constexpr int kFwdChunk = 64;     // taps per forward B chunk (4 N-tiles)
 constexpr int kFwdMTF32 = 2;      // M-tiles per wave, forward, f32 (block 16 x 8 lenslets)
 constexpr int kFwdMTBF16 = 4;     // M-tiles per wave, forward, bf16 (block 16 x 16 lenslets)
 constexpr int kBwdNTiles = 5;     // default N-tiles per backward workgroup (80 phases; 3 also built)
+constexpr int kNormRow = 256;     // floats per 1/B1 class row: 225 used, 1 KB so rows start on a 128-byte line
 constexpr int kPhiPadBwd = 256;   // phase rows of a backward plane block: 3 groups of 80 phases, 5 of 48 or 4 of 64
 // 16-bit backprojection variants by waves per SIMD: 2 (X of three M-tiles
 // loaded before the K loop, 6-slice tap ring, copies 5 K-steps ahead) or 3

@@ -927,7 +927,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
     const __amdgpu_buffer_rsrc_t xr =
         __builtin_amdgcn_make_buffer_rsrc(X + z * nlp * kXRow, 0, (int)(nlp * kXRow * 4), kBufferDword3);
     const __amdgpu_buffer_rsrc_t tr = __builtin_amdgcn_make_buffer_rsrc(
-        const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(cload(sd.toffT + z))), 0, C2 * C1 * kP * 4,
+        const_cast<float*>(sd.Tn + (size_t)g.view * sd.pvt + uni64(cload(sd.toffT + z))), 0, C2 * C1 * kNormRow * 4,
         kBufferDword3);
     const __amdgpu_buffer_rsrc_t sr = __builtin_amdgcn_make_buffer_rsrc(
         Xsh ? Xsh + z * nlp * kShadowRow : nullptr, 0, Xsh ? (int)(nlp * kShadowRow * 2) : 0,
@@ -1078,7 +1078,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       };
       auto ldt = [&](int mt, floatx4_t* dst) {
         const int a2 = 16 * b2 + 4 * w + mt;
-        const int to = (norm_class(a2 < n2 ? a2 : n2 - 1, n2, r) * C1 + c1) * kP;
+        const int to = (norm_class(a2 < n2 ? a2 : n2 - 1, n2, r) * C1 + c1) * kNormRow;
   #pragma unroll
         for (int nt = 0; nt < NT; ++nt)
           dst[nt] = __builtin_bit_cast(floatx4_t, __builtin_amdgcn_raw_buffer_load_b128(tr, 4 * (to + lph(nt)), 0, 0));
@@ -1496,14 +1496,18 @@ __global__ void k_norm_pass2(const double* __restrict__ S, float* __restrict__ T
   const int z = blockIdx.y;
   const int r = ng.r[z];
   const int C1 = ng.C1[z], C2 = ng.C2[z];
-  const size_t total = (size_t)C2 * C1 * kP;
+  const size_t total = (size_t)C2 * C1 * kNormRow;
   const double* s = S + ng.s_off[z];
   float* d = Tn + ng.t_off[z];
   for (size_t o = blockIdx.x * (size_t)blockDim.x + threadIdx.x; o < total;
        o += (size_t)gridDim.x * blockDim.x) {
-    const int ph = (int)(o % kP);
-    const int c1 = (int)((o / kP) % C1);
-    const int c2 = (int)(o / ((size_t)kP * C1));
+    const int ph = (int)(o % kNormRow);
+    const int c1 = (int)((o / kNormRow) % C1);
+    const int c2 = (int)(o / ((size_t)kNormRow * C1));
+    if (ph >= kP) {  // row padding
+      d[o] = 0.f;
+      continue;
+    }
     int lo, hi;
     norm_range(c2, ng.n2, r, &lo, &hi);
     double acc = 0.0;

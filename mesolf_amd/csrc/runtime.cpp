@@ -386,7 +386,7 @@ static mlf_status psf_ensure_norm(mlf_psf_s* ps, Precision prec, int n1, int n2,
     soff[z] = so;
     so += (int64_t)(2 * r + 1) * nt->C1[z] * kP;
     toff[z] = to;
-    to += (int64_t)nt->C2[z] * nt->C1[z] * kP;
+    to += (int64_t)nt->C2[z] * nt->C1[z] * kNormRow;
   }
   nt->toff = toff;
   nt->pvt = to;
