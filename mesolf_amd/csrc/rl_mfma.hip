@@ -670,9 +670,12 @@ void k_fwd_tpz(const T* __restrict__ W, FwdGeo g) {
       }
     }
     __syncthreads();
+    // partial window stored column-major ([Wc2 columns][16 rows]): the
+    // ratio gather's lanes take consecutive rows i1, so they read
+    // consecutive floats
     const int Wc2 = CW + 2 * r;
     for (int o = tid; o < 16 * Wc2; o += 256) {
-      const int i = o / Wc2, c = o - i * Wc2;
+      const int c = o >> 4, i = o & 15;
       float sum = 0.f;
   #pragma unroll
       for (int q = 0; q < 4; ++q) {
@@ -745,7 +748,7 @@ __global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg,
           for (int wi = 0; wi < kW; ++wi) {
             const int b2 = b2l + wi;
             const bool use = zc + q < z1 && b2 <= b2h;
-            const size_t idx = use ? (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r) : 0;
+            const size_t idx = use ? (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i2 - CW * b2 + r) * 16 + (i1 - 16 * b1) : 0;
             const float t = gload(pz[q] + idx);
             vv[q][wi] = use ? t : 0.f;
           }
@@ -759,7 +762,7 @@ __global__ __launch_bounds__(256) void k_ratio_gather_tpz(FwdGeo g, RatioGeo rg,
           const int b2h = min(nb2 - 1, (i2 + r) / CW);
           const int b2l = (i2 - r - CW + 1 <= 0) ? 0 : (i2 - r - CW + 1 + CW - 1) / CW;
           for (int b2 = b2l + kW; b2 <= b2h; ++b2)
-            y += gload(pz[q] + (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i1 - 16 * b1) * Wc2 + (i2 - CW * b2 + r));
+            y += gload(pz[q] + (size_t)(b2 * nb1 + b1) * 16 * Wc2 + (i2 - CW * b2 + r) * 16 + (i1 - 16 * b1));
         }
       }
     }
