@@ -1120,10 +1120,15 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // groups (the pattern ds_read_b128 serves without conflicts).
       // SPL: slice row j of a paired N-tile (half h) is phase row
       // 8 (j / 4) + 4 h + j % 4 of the pair (lph).
+      // 16-bit tables are stored tile-major (bwd_tap_index): a 16-row tile's
+      // K-step is 1 KB of contiguous memory, [16 rows][32 taps].  Element
+      // offset of (row R of the tile pair starting at row0, tap tq of the step):
+      // (R / 16) * 16 epad + (R % 16) * 32 + tq, for R < 32.
       const int jr = lane & 15, tq = (lane >> 4) * 8;
-      const int tap_lane_s = (jr * epad + tq) * (int)sizeof(T);                      // unpaired tile
-      const int tap_lane_p = ((8 * (jr >> 2) + (jr & 3)) * epad + tq) * (int)sizeof(T);  // pair, half 0
-      const int tap_lane_h = 4 * epad * (int)sizeof(T);                                // half 1: 4 rows on
+      auto tap_off = [&](int R) { return ((R >> 4) * 16 * epad + (R & 15) * 32 + tq) * (int)sizeof(T); };
+      const int tap_lane_s = tap_off(jr);                                  // unpaired tile
+      const int tap_lane_p0 = tap_off(8 * (jr >> 2) + (jr & 3));           // pair, half 0
+      const int tap_lane_p1 = tap_off(8 * (jr >> 2) + 4 + (jr & 3));       // pair, half 1
       const int tap_slot = (16 * lg + lr) * 8;  // element offset of (row lr, 8-tap group lg) in an N-tile slice
       // copies of K-step ks of phase group gx into ring slot `slot`
       auto issue_taps_g = [&](int gx, int ks, int slot) {
@@ -1136,8 +1141,8 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           const bool pr = PAIR && nt < 2 * NPR;
           // first packed row relative to the group
           const int row0 = !PAIR ? 16 * nt : pr ? 16 * oddx + 16 * (nt & ~1) : (oddx ? 0 : 16 * (NT - 1));
-          const int tl = pr ? tap_lane_p + (nt & 1) * tap_lane_h : tap_lane_s;
-          const char* src = reinterpret_cast<const char*>(Bx + (size_t)row0 * epad + kc * 32) + tl;
+          const int tl = pr ? ((nt & 1) ? tap_lane_p1 : tap_lane_p0) : tap_lane_s;
+          const char* src = reinterpret_cast<const char*>(Bx + (size_t)row0 * epad + kc * 512) + tl;
           __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
                                            (__attribute__((address_space(3))) void*)(dst + nt * 1024),
                                            16, 0, 0);
@@ -1465,7 +1470,9 @@ __global__ void k_pack_bwd(const float* __restrict__ src, T* __restrict__ dst, P
       e1 = lt - e2 * K;
     }
     const bool ok = ph < kP && e2 < K && e1 < K;
-    d[o] = (T)(ok ? s[((size_t)e2 * K + e1) * kP + ph] * sc : 0.f);
+    // 16-bit: tile-major, [16-row tile][K-step of 32 taps][16 rows][32 taps]
+    const size_t od = pg.quad ? (size_t)(ph >> 4) * 16 * epad + (size_t)(t >> 5) * 512 + (ph & 15) * 32 + (t & 31) : o;
+    d[od] = (T)(ok ? s[((size_t)e2 * K + e1) * kP + ph] * sc : 0.f);
   }
 }
 
