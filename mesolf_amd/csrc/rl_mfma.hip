@@ -183,7 +183,9 @@ __device__ __forceinline__ void wait_vm_le(int n) {
     MLF_W(10) MLF_W(11) MLF_W(12) MLF_W(13) MLF_W(14) MLF_W(15) MLF_W(16) MLF_W(17) MLF_W(18)
     MLF_W(19) MLF_W(20) MLF_W(21) MLF_W(22) MLF_W(23) MLF_W(24) MLF_W(25) MLF_W(26) MLF_W(27)
     MLF_W(28) MLF_W(29) MLF_W(30) MLF_W(31) MLF_W(32) MLF_W(33) MLF_W(34) MLF_W(35) MLF_W(36)
-    MLF_W(37) MLF_W(38) MLF_W(39) MLF_W(40)
+    MLF_W(37) MLF_W(38) MLF_W(39) MLF_W(40) MLF_W(41) MLF_W(42) MLF_W(43) MLF_W(44) MLF_W(45)
+    MLF_W(46) MLF_W(47) MLF_W(48) MLF_W(49) MLF_W(50) MLF_W(51) MLF_W(52) MLF_W(53) MLF_W(54)
+    MLF_W(55) MLF_W(56) MLF_W(57) MLF_W(58) MLF_W(59) MLF_W(60) MLF_W(61) MLF_W(62) MLF_W(63)
 #undef MLF_W
     default: wait_vm<0>(); break;
   }
@@ -850,6 +852,9 @@ __device__ __forceinline__ void gather_window(const float* __restrict__ R, int n
                                               int a2_0, int Wd, float* __restrict__ dst) {
   constexpr int kPer = 16;
   const int tid = threadIdx.x, ncell = Wd * Wd;
+  // global (not flat) loads: a pending flat load makes the compiler wait for
+  // every memory operation at the next use of any loaded register
+  const __attribute__((address_space(1))) float* Rg = (const __attribute__((address_space(1))) float*)R;
   for (int c0 = 0; c0 < ncell; c0 += 256 * kPer) {
     float v[kPer];
   #pragma unroll
@@ -858,9 +863,13 @@ __device__ __forceinline__ void gather_window(const float* __restrict__ R, int n
       const int y = o / Wd, x = o - y * Wd;
       const int a1 = a1_0 + x, a2 = a2_0 + y;
       const bool in = a1 >= 0 && a1 < n1 && a2 >= 0 && a2 < n2;
-      const float t = R[min(max(a1, 0), n1 - 1) + n1 * min(max(a2, 0), n2 - 1)];
+      const float t = Rg[min(max(a1, 0), n1 - 1) + n1 * min(max(a2, 0), n2 - 1)];
       v[q] = in ? t : 0.f;
     }
+    // every load waited for here, on every path: a load the compiler still
+    // counts as in flight later (its register reused) would make it wait at
+    // the reuse for the memory operations issued since, stores included
+    wait_vm<0>();
   #pragma unroll
     for (int q = 0; q < kPer; ++q) {
       const int o = c0 + tid + 256 * q;
@@ -1095,8 +1104,19 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // loads issued before the K loop: load_x(0, PRE) and the first (or every) 1/B1 row set
       // SPL: two loads per pair of N-tiles and two for an unpaired one
       constexpr int kEpiLoads = PRE > 0 ? PRE * (SPL ? 2 * ((NT + 1) / 2) : NT) + (TPRE ? MT * NT : NT) : 0;
-      static_assert(kEpiLoads + (bwd_stages(OCC) - 2 * (OCC >= 3 ? 1 : 2)) * ((NT + 3) / 4) <= 40,
-                    "wait_vm_le covers at most 40 loads in flight");
+      // Vector-memory operations the previous group's update issues after it
+      // copied this group's first tap slices (prefetch_next), which this
+      // group's first waits count instead of waiting for them (the split
+      // form's update with X loaded before the K loop): every tile's stores
+      // -- a prefetching group ends below phase 225, so each of its tiles
+      // takes the full-tile stores, two per pair of N-tiles and two for an
+      // unpaired tile -- and, with 5 N-tiles, the 1/B1 rows of tiles
+      // 2 .. MT - 1 (requested after the copies).  A lower bound is safe:
+      // the waits then also cover the operations not counted.
+      constexpr int kUpdOps =
+          (SPL && PRE > 0) ? MT * 2 * ((NT + 1) / 2) + (TPRE || NT <= 4 ? 0 : (MT - 2) * NT) : 0;
+      static_assert(kEpiLoads + kUpdOps + (bwd_stages(OCC) - 2 * (OCC >= 3 ? 1 : 2)) * ((NT + 3) / 4) <= 63,
+                    "wait_vm_le covers at most 63 operations in flight");
 
       // SPB K-steps per barrier: a step's copies are waited for, and the
       // ring refilled, once per SPB steps; the ring then runs STG - SPB
@@ -1130,22 +1150,27 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       const int tap_lane_p0 = tap_off(8 * (jr >> 2) + (jr & 3));           // pair, half 0
       const int tap_lane_p1 = tap_off(8 * (jr >> 2) + 4 + (jr & 3));       // pair, half 1
       const int tap_slot = (16 * lg + lr) * 8;  // element offset of (row lr, 8-tap group lg) in an N-tile slice
-      // copies of K-step ks of phase group gx into ring slot `slot`
+      // copies of K-step ks of phase group gx into ring slot `slot`: buffer
+      // loads to LDS (MUBUF) over the plane's table, [kPhiPadBwd rows][epad]
+      // taps.  A FLAT-encoded global_load_lds would count as an access that
+      // may reach LDS out of order, and the compiler then turns every wait for
+      // a load issued before it into a wait for all memory operations,
+      // stores included.
+      const __amdgpu_buffer_rsrc_t wr = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<T*>(Wb + boffz), 0, kPhiPadBwd * epad * (int)sizeof(T), kBufferDword3);
       auto issue_taps_g = [&](int gx, int ks, int slot) {
         if (ks >= nks) return;  // past the last step (wave-uniform)
         const int kc = ks;
         const int oddx = PAIR && NT % 2 == 1 ? (gx & 1) : 0;
-        const T* Bx = Wb + boffz + (size_t)(gx * 16 * NT) * epad;
         char* dst = reinterpret_cast<char*>(Bq + slot * kSlice);
         for (int nt = w; nt < NT; nt += 4) {
           const bool pr = PAIR && nt < 2 * NPR;
           // first packed row relative to the group
           const int row0 = !PAIR ? 16 * nt : pr ? 16 * oddx + 16 * (nt & ~1) : (oddx ? 0 : 16 * (NT - 1));
           const int tl = pr ? ((nt & 1) ? tap_lane_p1 : tap_lane_p0) : tap_lane_s;
-          const char* src = reinterpret_cast<const char*>(Bx + (size_t)row0 * epad + kc * 512) + tl;
-          __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void*)src,
-                                           (__attribute__((address_space(3))) void*)(dst + nt * 1024),
-                                           16, 0, 0);
+          const int so = ((gx * 16 * NT + row0) * epad + kc * 512) * (int)sizeof(T);  // wave-uniform
+          __builtin_amdgcn_raw_ptr_buffer_load_lds(wr, (__attribute__((address_space(3))) void*)(dst + nt * 1024),
+                                                   16, tl, so, 0, 0);
         }
       };
       auto issue_taps = [&](int ks, int slot) { issue_taps_g(grp, ks, slot); };
@@ -1196,6 +1221,9 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           }
         }
       } else {
+        // the first slices came from the previous group's prefetch: the
+        // update's operations issued since are younger than them
+        const int upd = pref ? kUpdOps : 0;
         if (!pref) {
   #pragma unroll
           for (int q = 0; q < STG - SPB; ++q) issue_taps(q, q);
@@ -1214,8 +1242,10 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
         for (int ks = 0; ks < nks; ks += SPB) {
           // this wave's copies of steps ks .. ks + SPB - 1 landed: younger are
           // the copies of steps up to min(ks + STG - SPB, nks) - 1 and, while
-          // those steps were copied before the loop, the epilogue loads
-          wait_vm_le(max(0, min(ks + STG - SPB, nks) - ks - SPB) * ncopy + (ks + SPB <= STG - SPB ? kEpiLoads : 0));
+          // those steps were copied before the loop, the epilogue loads (and
+          // the previous group's update after a prefetch)
+          wait_vm_le(max(0, min(ks + STG - SPB, nks) - ks - SPB) * ncopy +
+                     (ks + SPB <= STG - SPB ? kEpiLoads + upd : 0));
           lds_barrier();  // everyone's copies landed; everyone is done with steps ks - SPB ..
   #pragma unroll
           for (int h = 0; h < SPB; ++h) issue_taps(ks + STG - SPB + h, (ks + STG - SPB + h) % STG);
@@ -1262,7 +1292,11 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
       // 224 (the last of 225): lanes outside the grid store at an offset past
       // the plane, which the buffer store drops (the plane is below 2^31
       // bytes, so 2^31 + a row offset is out of range and does not wrap).
-      auto update_tile = [&](int mt, const floatx4_t* x4, const floatx4_t* b4) {
+      // FULL (compile time): every N-tile of the group lies below phase 225,
+      // so each takes the full-tile stores without a branch, and the
+      // compiler's waits for the update's loads can count those stores
+      auto update_tile = [&](auto fullc, int mt, const floatx4_t* x4, const floatx4_t* b4) {
+        constexpr bool FULL = decltype(fullc)::value;
         const int a2 = 16 * b2 + 4 * w + mt;
         const bool ok = ok1 && a2 < n2;
         const int l = (a2 < n2 ? a2 : n2 - 1) * n1 + a1c;
@@ -1315,7 +1349,7 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
             hi[0] = (unsigned)to_bits16<T>(sv[0]) | ((unsigned)to_bits16<T>(sv[1]) << 16);
             hi[1] = (unsigned)to_bits16<T>(sv[2]) | ((unsigned)to_bits16<T>(sv[3]) << 16);
           }
-          if (tfull(nt)) {  // wave-uniform: the whole N-tile is in the row
+          if (FULL || tfull(nt)) {  // wave-uniform: the whole N-tile is in the row
             if constexpr (SPL) {
               if (nt < 2 * NPR) {
                 if ((nt & 1) == 0) {
@@ -1341,77 +1375,108 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           // the tile that reaches past phase 224
           const int xo = 4 * (l * kXRow + ph);
           const int so = 2 * (l * kShadowRow + ph);
-          if (ok && ph + 4 <= kP) {
-            if constexpr (SPL) {
-              __builtin_amdgcn_raw_buffer_store_b64(hi, sr, so, 0, 0);
-              __builtin_amdgcn_raw_buffer_store_b64(lo, lor, so, 0, 0);
-            } else {
-              __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
-              if (Xsh) __builtin_amdgcn_raw_buffer_store_b64(hi, sr, so, 0, 0);
-            }
+          if constexpr (SPL) {
+            // no branches: every lane issues all four stores, at an offset
+            // past the plane where it has nothing to store, so that the count
+            // of memory operations (which the compiler's waits follow) does
+            // not depend on the lane's phases
+            const unsigned o4 = ok && ph + 4 <= kP ? (unsigned)so : 0x80000000u;
+            const unsigned o1 = ok && ph < kP && ph + 4 > kP ? (unsigned)so : 0x80000000u;  // phase 224
+            __builtin_amdgcn_raw_buffer_store_b64(hi, sr, (int)o4, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b64(lo, lor, (int)o4, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(hi[0] & 0xFFFFu), sr, (int)o1, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(lo[0] & 0xFFFFu), lor, (int)o1, 0, 0);
+          } else if (ok && ph + 4 <= kP) {
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(uintx4, xn), xr, xo, 0, 0);
+            if (Xsh) __builtin_amdgcn_raw_buffer_store_b64(hi, sr, so, 0, 0);
           } else if (ok && ph < kP) {  // phase 224: the last of 225
-            if constexpr (SPL) {
-              __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(hi[0] & 0xFFFFu), sr, so, 0, 0);
-              __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(lo[0] & 0xFFFFu), lor, so, 0, 0);
+            __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
+            if (Xsh) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(hi[0] & 0xFFFFu), sr, so, 0, 0);
+          }
+        }
+      };
+      auto update = [&](auto fullc) {
+        if constexpr (TPRE) {
+          // every operand was loaded before the K loop
+          prefetch_next();
+  #pragma unroll
+          for (int mt = 0; mt < MT; ++mt) update_tile(fullc, mt, xv[mt], bva[mt]);
+        } else if constexpr (PRE > 0) {
+          // continued from the loads issued before the K loop.
+          // SPL, up to 4 N-tiles: the 1/B1 rows of every remaining tile are
+          // requested at once, then the next group's first tap slices, then
+          // the stores: every load the update uses is older than every store,
+          // so no wait for a load covers a store (vector-memory operations
+          // complete in order).  SPL, 5 N-tiles (no registers for four row
+          // sets): the rows of tile mt + 1 are requested before tile mt's
+          // stores, so the wait for them covers the stores of tile mt - 1
+          // only.  The other paths request each tile's rows when they reach it.
+          load_x(PRE, MT);
+          if constexpr (SPL && NT <= 4) {
+            floatx4_t bt[MT - 1][NT];
+  #pragma unroll
+            for (int mt = 1; mt < MT; ++mt) ldt(mt, bt[mt - 1]);
+            prefetch_next();
+  #pragma unroll
+            for (int mt = 0; mt < MT; ++mt) update_tile(fullc, mt, xv[mt], mt == 0 ? bv : bt[mt > 0 ? mt - 1 : 0]);
+          } else if constexpr (SPL) {
+            floatx4_t bw[NT];
+            ldt(1, bw);
+            prefetch_next();  // after this group's first loads: their waits need not cover the copies
+  #pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+              if (mt >= 1 && mt + 1 < MT) ldt(mt + 1, mt % 2 == 0 ? bw : bv);
+              update_tile(fullc, mt, xv[mt], mt % 2 == 0 ? bv : bw);
+            }
+          } else {
+            prefetch_next();
+  #pragma unroll
+            for (int mt = 0; mt < MT; ++mt) {
+              if (mt > 0) load_t(mt);
+              update_tile(fullc, mt, xv[mt], bv);
+            }
+          }
+        } else {
+          // X and 1/B1 rows one M-tile ahead, in two register sets
+          floatx4_t xa[NT], ba[NT], xb[NT], bb[NT];
+          ldx(0, xa);
+          ldt(0, ba);
+          prefetch_next();
+  #pragma unroll
+          for (int mt = 0; mt < MT; ++mt) {
+            if (mt % 2 == 0) {
+              if (mt + 1 < MT) {
+                ldx(mt + 1, xb);
+                ldt(mt + 1, bb);
+              }
+              update_tile(fullc, mt, xa, ba);
             } else {
-              __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, xn[0]), xr, xo, 0, 0);
-              if (Xsh) __builtin_amdgcn_raw_buffer_store_b16((unsigned short)(hi[0] & 0xFFFFu), sr, so, 0, 0);
+              if (mt + 1 < MT) {
+                ldx(mt + 1, xa);
+                ldt(mt + 1, ba);
+              }
+              update_tile(fullc, mt, xb, bb);
             }
           }
         }
       };
-      if constexpr (TPRE) {
-        // every operand was loaded before the K loop
-        prefetch_next();
-  #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) update_tile(mt, xv[mt], bva[mt]);
-      } else if constexpr (PRE > 0) {
-        // continued from the loads issued before the K loop.  The 1/B1 rows
-        // of tile mt + 1 are requested before tile mt's stores, so the wait
-        // for them does not include those stores (vector-memory operations
-        // complete in order), only the stores of tile mt - 1
-        // (SPL; the other paths have no registers for the second row set
-        // and request each tile's rows when they reach it)
-        load_x(PRE, MT);
-        if constexpr (SPL) {
-          floatx4_t bw[NT];
-          ldt(1, bw);
-          prefetch_next();  // after this group's first loads: their waits need not cover the copies
-  #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) {
-            if (mt >= 1 && mt + 1 < MT) ldt(mt + 1, mt % 2 == 0 ? bw : bv);
-            update_tile(mt, xv[mt], mt % 2 == 0 ? bv : bw);
-          }
-        } else {
-          prefetch_next();
-  #pragma unroll
-          for (int mt = 0; mt < MT; ++mt) {
-            if (mt > 0) load_t(mt);
-            update_tile(mt, xv[mt], bv);
-          }
-        }
+      // the split form with up to 4 N-tiles only: two copies of the update
+      // cost the other paths registers they do not have
+      constexpr bool kTwoUpdates = SPL && NT <= 4;
+      // Every X load of this group has landed (the update used them).  The
+      // compiler may still count some as in flight (a path that did not read
+      // a register) and then, when the next group reloads that register,
+      // waits for the memory operations issued since, the update's stores
+      // included.  A wait after the update tells it: with every tile full,
+      // one that leaves the stores and the prefetched copies in flight (free);
+      // in the last group (no copies), one that leaves the 1/B1 row loads'
+      // count in flight (the X loads are older than those on every path).
+      if (kTwoUpdates && gbase + 16 * NT <= kP) {
+        update(std::integral_constant<bool, kTwoUpdates>{});
+        if constexpr (kTwoUpdates) wait_vm<MT * 2 * ((NT + 1) / 2) + (STG - SPB) * ((NT + 3) / 4)>();
       } else {
-        // X and 1/B1 rows one M-tile ahead, in two register sets
-        floatx4_t xa[NT], ba[NT], xb[NT], bb[NT];
-        ldx(0, xa);
-        ldt(0, ba);
-        prefetch_next();
-  #pragma unroll
-        for (int mt = 0; mt < MT; ++mt) {
-          if (mt % 2 == 0) {
-            if (mt + 1 < MT) {
-              ldx(mt + 1, xb);
-              ldt(mt + 1, bb);
-            }
-            update_tile(mt, xa, ba);
-          } else {
-            if (mt + 1 < MT) {
-              ldx(mt + 1, xa);
-              ldt(mt + 1, ba);
-            }
-            update_tile(mt, xb, bb);
-          }
-        }
+        update(std::integral_constant<bool, false>{});
+        if constexpr (kTwoUpdates) wait_vm<MT * NT>();
       }
       // LDS is reused by the next entry (the staging window aliases the
       // ring) once every wave is here.  After an entry's last group no tap
