@@ -1244,8 +1244,16 @@ void k_bwd_mfma(const T* __restrict__ Wb, BwdGeo g, float wgt) {
           // the copies of steps up to min(ks + STG - SPB, nks) - 1 and, while
           // those steps were copied before the loop, the epilogue loads (and
           // the previous group's update after a prefetch)
-          wait_vm_le(max(0, min(ks + STG - SPB, nks) - ks - SPB) * ncopy +
-                     (ks + SPB <= STG - SPB ? kEpiLoads + upd : 0));
+          // In the steady state (past the first slices, before the last;
+          // one copy per wave and step) the count is STG - 2 SPB, waited for
+          // with one test of the step: the general wait compiles to a search
+          // over the 64 counts (tens of scalar branches).  Inline assembly,
+          // which the compiler cannot fold back into that search.
+          if (ks >= STG - SPB && ks + STG - SPB <= nks && ncopy == 1)
+            asm volatile("s_waitcnt vmcnt(%0)" ::"n"(STG - 2 * SPB) : "memory");
+          else
+            wait_vm_le(max(0, min(ks + STG - SPB, nks) - ks - SPB) * ncopy +
+                       (ks + SPB <= STG - SPB ? kEpiLoads + upd : 0));
           lds_barrier();  // everyone's copies landed; everyone is done with steps ks - SPB ..
   #pragma unroll
           for (int h = 0; h < SPB; ++h) issue_taps(ks + STG - SPB + h, (ks + STG - SPB + h) % STG);
